@@ -1,0 +1,136 @@
+/*
+ * netbc: exact Brandes betweenness on an AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in C ABI for knappnet's graph handle and betweenness entry point
+ * (crates/capi/include/netcapi.h), plus the device path's own entry
+ * points. Conventions follow netcapi.h:1-25:
+ * - Functions that take `err` return 0 on success and 1 on failure, after
+ *   writing a NUL-terminated message into `err` (at most `err_len` bytes;
+ *   `err` may be NULL or `err_len` 0). Accessors return -1 for NULL.
+ * - Node indices are 0-based; output buffers are allocated by the caller.
+ * - A handle belongs to the caller until released with net_graph_free,
+ *   which accepts NULL.
+ * - weights_as: 0 = distance, 1 = strength (path lengths 1/w), used only
+ *   when weighted != 0. `threads` is accepted for compatibility; the
+ *   device computes every source and the result never depends on it.
+ * Device failures (no GPU, out of memory, launch errors) return 1 with a
+ * message; nothing aborts the process.
+ */
+#ifndef NETBC_H
+#define NETBC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* An immutable graph; release with net_graph_free. The device copy of its
+ * CSR is built on first use and freed with the handle. */
+typedef struct GraphHandle GraphHandle;
+
+/* Replaces net_graph_new, netcapi.h:43-48 (crates/capi/src/lib.rs:133-164,
+ * crates/engine/src/graph.rs:142-234): builds a graph from m edges (src,
+ * tgt, optional weights) on n nodes; directed != 0 keeps arc directions. */
+int32_t net_graph_new(int64_t n, const int64_t *src, const int64_t *tgt,
+                      const double *weights, int64_t m, int32_t directed,
+                      GraphHandle **out, char *err, size_t err_len);
+
+/* Replaces net_graph_free, netcapi.h:50-51 (lib.rs:166-175). */
+void net_graph_free(GraphHandle *h);
+
+/* Replaces net_graph_node_count, netcapi.h:53-54 (lib.rs:177-184). */
+int64_t net_graph_node_count(const GraphHandle *h);
+
+/* Replaces net_graph_edge_count, netcapi.h:56-57
+ * (crates/capi/src/descriptive.rs:83-85). */
+int64_t net_graph_edge_count(const GraphHandle *h);
+
+/* Replaces net_betweenness, netcapi.h:386-391 (lib.rs:214-244,
+ * crates/engine/src/metrics/betweenness.rs:63-115). Shortest-path
+ * betweenness of every node, endpoints excluded, undirected pairs counted
+ * once; normalized != 0 divides by (n-1)(n-2) (directed) or
+ * (n-1)(n-2)/2. `out` holds n doubles. */
+int32_t net_betweenness(const GraphHandle *h, int32_t weighted,
+                        int32_t weights_as, int32_t normalized,
+                        int32_t threads, double *out, char *err,
+                        size_t err_len);
+
+/* ---- device-path entry points --------------------------------------- */
+
+/* Run options. Zero-initialised means: FP64 dependencies, automatic batch
+ * count and direction. */
+typedef struct netbc_options {
+    int32_t precision; /* 0: FP64 dependencies; 1: FP32 dependencies with
+                          Kahan-compensated sums (sigma and the cross-source
+                          sum stay FP64) */
+    int32_t slots;     /* 64-source batches per launch; 0 = automatic */
+    int32_t direction; /* 0 automatic, 1 top-down only, 2 bottom-up when
+                          the frontier allows */
+    int32_t reserved;
+    double alpha;      /* direction switch factor; 0 = default */
+} netbc_options;
+
+/* What a run saw. */
+typedef struct netbc_diagnostics {
+    double max_sigma;        /* largest shortest-path count */
+    int32_t sigma_overflow;  /* 1 if a count exceeded 2^53 (FP64 no longer
+                                exact; bit-exactness not guaranteed) */
+    int32_t path_kind;       /* 0 breadth-first, 1 integer (Dial), 2 real */
+    int64_t reached_nodes;   /* sum over sources of nodes reached */
+    int64_t reached_arcs;    /* sum over sources of out-arcs of those nodes */
+    int64_t levels;          /* search levels run */
+    double kernel_ms;        /* device time of the search and reduction */
+    double dominant_ms;      /* device time of the dependency sweep */
+    int64_t dominant_launches;
+    int32_t slots;           /* 64-source batches per launch used */
+    int32_t reserved;
+    double build_ms;         /* device CSR/CSC build time (first call) */
+} netbc_diagnostics;
+
+/* Betweenness restricted to a source list (NULL with k = n: every node,
+ * identical to net_betweenness). The per-source searches and the ordered
+ * group sum (crates/engine/src/parallel.rs:383-503) run over list
+ * positions; scaling is betweenness.rs:100-113's. Same error conventions;
+ * `diag` may be NULL. No counterpart in netcapi.h: it is the device path's
+ * extended entry (sampled sources, precision choice, diagnostics). */
+int32_t netbc_betweenness_sources(const GraphHandle *h, const int64_t *sources,
+                                  int64_t k, int32_t weighted,
+                                  int32_t weights_as, int32_t normalized,
+                                  const netbc_options *opt, double *out,
+                                  netbc_diagnostics *diag, char *err,
+                                  size_t err_len);
+
+/* Debug entry: the per-source state of k sources, each row n long:
+ * dist = hop count (breadth-first) or integer distance (Dial),
+ * UINT64_MAX when unreached; sigma = path counts; delta = dependencies
+ * (0 for unreached nodes and for the source itself). Follows the
+ * reference's Workspace state (betweenness.rs:161-295); not for real
+ * lengths. */
+int32_t netbc_state_dump(const GraphHandle *h, int32_t weighted,
+                         int32_t weights_as, int32_t precision,
+                         const int64_t *sources, int64_t k, uint64_t *dist,
+                         double *sigma, double *delta, char *err,
+                         size_t err_len);
+
+/* Number of stored arcs (m directed; 2m - loops undirected), or -1. */
+int64_t netbc_graph_arc_count(const GraphHandle *h);
+
+/* Copies the device CSR (incoming == 0) or CSC (incoming != 0) back:
+ * offsets n+1, targets and (if the graph is weighted and weights is not
+ * NULL) weights, one per arc. Mirrors crates/engine/src/graph.rs:18-119's
+ * Csr layout, for equality tests against the host build. */
+int32_t netbc_graph_csr(const GraphHandle *h, int32_t incoming,
+                        uint32_t *offsets, uint32_t *targets, double *weights,
+                        char *err, size_t err_len);
+
+/* Name of the device the library runs on (e.g. "AMD Instinct MI355X,
+ * gfx950, 256 CUs"). */
+int32_t netbc_device_info(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NETBC_H */

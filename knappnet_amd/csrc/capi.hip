@@ -1,0 +1,271 @@
+// The C ABI: graph handles and the betweenness entry points.
+//
+// Behaviour follows crates/capi/src/lib.rs: status 0 / 1, messages
+// truncated to err_len - 1 bytes plus NUL, NULL err tolerated (60-71),
+// failures caught and reported instead of unwinding (73-85), the same
+// validation order and texts as Graph::from_edges (graph.rs:142-234) and
+// error.rs:62-99.
+
+#include <chrono>
+#include <cstring>
+#include <new>
+
+#include "../../include/netbc.h"
+#include "netbc_internal.h"
+
+#define NETBC_EXPORT extern "C" __attribute__((visibility("default")))
+
+struct GraphHandle {
+    netbc::Graph g;
+};
+
+namespace {
+
+constexpr int32_t kOk = 0;
+constexpr int32_t kFailed = 1;
+
+void write_err(char* err, size_t err_len, const std::string& msg) {
+    if (!err || err_len == 0) return;
+    size_t n = std::min(msg.size(), err_len - 1);
+    memcpy(err, msg.data(), n);
+    err[n] = 0;
+}
+
+template <class F>
+int32_t guard(char* err, size_t err_len, F&& f) {
+    try {
+        f();
+        return kOk;
+    } catch (const netbc::Error& e) {
+        write_err(err, err_len, e.what());
+    } catch (const std::bad_alloc&) {
+        write_err(err, err_len, "out of host memory");
+    } catch (...) {
+        write_err(err, err_len, "internal error (panic) in the network engine");
+    }
+    return kFailed;
+}
+
+netbc::Graph& handle(const GraphHandle* h) {
+    if (!h) throw netbc::Error("graph handle is null");
+    return const_cast<GraphHandle*>(h)->g;
+}
+
+void check_kind(int32_t weights_as) {
+    if (weights_as != 0 && weights_as != 1)
+        throw netbc::Error("unknown weight kind code " + std::to_string(weights_as));
+}
+
+// Selects a device, failing with a message when there is none.
+void ensure_device() {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        throw netbc::Error("no HIP device is available for the betweenness engine");
+}
+
+double prepare(netbc::Graph& g) {
+    ensure_device();
+    auto t0 = std::chrono::steady_clock::now();
+    bool had = g.dev_ready;
+    netbc::build_device_graph(g);
+    if (had) return 0.0;
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double, std::milli>(t1 - t0).count();
+}
+
+void fill_diag(netbc_diagnostics* out, const netbc::RunDiag& d, double build_ms) {
+    if (!out) return;
+    out->max_sigma = d.max_sigma;
+    out->sigma_overflow = d.sigma_overflow;
+    out->path_kind = d.path_kind;
+    out->reached_nodes = d.reached_nodes;
+    out->reached_arcs = d.reached_arcs;
+    out->levels = d.levels;
+    out->kernel_ms = d.kernel_ms;
+    out->dominant_ms = d.dominant_ms;
+    out->dominant_launches = d.dominant_launches;
+    out->slots = d.slots;
+    out->reserved = 0;
+    out->build_ms = build_ms;
+}
+
+void run_sources(netbc::Graph& g, const int64_t* sources, int64_t k, int32_t weighted,
+                 int32_t weights_as, int32_t normalized, const netbc_options* opt, double* out,
+                 netbc_diagnostics* diag) {
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
+    double build_ms = prepare(g);
+    netbc::Lengths L;
+    netbc::classify_lengths(g, weighted != 0, weights_as, L);
+    netbc::RunOptions o;
+    if (opt) {
+        o.precision = opt->precision;
+        o.slots = opt->slots;
+        o.direction = opt->direction;
+        o.alpha = opt->alpha;
+    }
+    netbc::RunDiag d;
+    netbc::DevBuf<double> bc;
+    bc.alloc(g.n);
+    netbc::run_batched(g, L, sources, k, o, bc.p, d);
+    netbc::finish_scores(g, normalized != 0, bc.p, out);
+    fill_diag(diag, d, build_ms);
+}
+
+}  // namespace
+
+NETBC_EXPORT int32_t net_graph_new(int64_t n, const int64_t* src, const int64_t* tgt,
+                                   const double* weights, int64_t m, int32_t directed,
+                                   GraphHandle** out, char* err, size_t err_len) {
+    return guard(err, err_len, [&] {
+        if (n < 0 || m < 0) throw netbc::Error("negative node or edge count");
+        if ((uint64_t)n > 0xFFFFFFFFull)
+            throw netbc::Error("node count (" + std::to_string(n) +
+                               ") exceeds the supported maximum");
+        if (weights) {
+            for (int64_t e = 0; e < m; e++)
+                if (!std::isfinite(weights[e]))
+                    throw netbc::Error("edge " + std::to_string(e) +
+                                       " has a missing, NaN, or infinite weight");
+        }
+        if ((uint64_t)m > 0xFFFFFFFFull / 2)
+            throw netbc::Error("edge count (" + std::to_string(m) +
+                               ") exceeds the supported maximum");
+        auto* h = new GraphHandle();
+        netbc::Graph& g = h->g;
+        g.n = (uint64_t)n;
+        g.m = (uint64_t)m;
+        g.directed = directed != 0;
+        try {
+            g.src.resize((size_t)m);
+            g.tgt.resize((size_t)m);
+            for (int64_t e = 0; e < m; e++) {
+                int64_t ends[2] = {src[e], tgt[e]};
+                for (int64_t v : ends)
+                    if (v < 0 || (uint64_t)v >= (uint64_t)n)
+                        throw netbc::Error("edge " + std::to_string(e) +
+                                           " refers to node index " + std::to_string(v) +
+                                           ", but valid indices are 0.." + std::to_string(n));
+                g.src[e] = (uint32_t)src[e];
+                g.tgt[e] = (uint32_t)tgt[e];
+            }
+            if (weights) {
+                g.has_weights = true;
+                g.w.assign(weights, weights + m);
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+NETBC_EXPORT void net_graph_free(GraphHandle* h) { delete h; }
+
+NETBC_EXPORT int64_t net_graph_node_count(const GraphHandle* h) {
+    return h ? (int64_t)h->g.n : -1;
+}
+
+NETBC_EXPORT int64_t net_graph_edge_count(const GraphHandle* h) {
+    return h ? (int64_t)h->g.m : -1;
+}
+
+NETBC_EXPORT int32_t net_betweenness(const GraphHandle* h, int32_t weighted, int32_t weights_as,
+                                     int32_t normalized, int32_t threads, double* out, char* err,
+                                     size_t err_len) {
+    (void)threads;
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        check_kind(weights_as);
+        run_sources(g, nullptr, (int64_t)g.n, weighted, weights_as, normalized, nullptr, out,
+                    nullptr);
+    });
+}
+
+NETBC_EXPORT int32_t netbc_betweenness_sources(const GraphHandle* h, const int64_t* sources,
+                                               int64_t k, int32_t weighted, int32_t weights_as,
+                                               int32_t normalized, const netbc_options* opt,
+                                               double* out, netbc_diagnostics* diag, char* err,
+                                               size_t err_len) {
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        check_kind(weights_as);
+        if (sources) {
+            if (k < 0) throw netbc::Error("source count must be non-negative");
+            for (int64_t i = 0; i < k; i++)
+                if (sources[i] < 0 || (uint64_t)sources[i] >= g.n)
+                    throw netbc::Error("source " + std::to_string(i) + " is node index " +
+                                       std::to_string(sources[i]) +
+                                       ", but valid indices are 0.." + std::to_string(g.n));
+        } else {
+            k = (int64_t)g.n;
+        }
+        if (opt && (opt->precision < 0 || opt->precision > 1))
+            throw netbc::Error("unknown precision code " + std::to_string(opt->precision));
+        run_sources(g, sources, k, weighted, weights_as, normalized, opt, out, diag);
+    });
+}
+
+NETBC_EXPORT int32_t netbc_state_dump(const GraphHandle* h, int32_t weighted, int32_t weights_as,
+                                      int32_t precision, const int64_t* sources, int64_t k,
+                                      uint64_t* dist, double* sigma, double* delta, char* err,
+                                      size_t err_len) {
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        check_kind(weights_as);
+        for (int64_t i = 0; i < k; i++)
+            if (sources[i] < 0 || (uint64_t)sources[i] >= g.n)
+                throw netbc::Error("source out of range");
+        std::lock_guard<std::mutex> lock(g.mu);
+        if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
+        prepare(g);
+        netbc::Lengths L;
+        netbc::classify_lengths(g, weighted != 0, weights_as, L);
+        netbc::RunOptions o;
+        o.precision = precision;
+        netbc::run_dump(g, L, sources, k, o, dist, sigma, delta);
+    });
+}
+
+NETBC_EXPORT int64_t netbc_graph_arc_count(const GraphHandle* h) {
+    if (!h) return -1;
+    const netbc::Graph& g = h->g;
+    if (g.directed) return (int64_t)g.m;
+    int64_t loops = 0;
+    for (uint64_t e = 0; e < g.m; e++) loops += g.src[e] == g.tgt[e];
+    return (int64_t)(2 * g.m) - loops;
+}
+
+NETBC_EXPORT int32_t netbc_graph_csr(const GraphHandle* h, int32_t incoming, uint32_t* offsets,
+                                     uint32_t* targets, double* weights, char* err,
+                                     size_t err_len) {
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        std::lock_guard<std::mutex> lock(g.mu);
+        prepare(g);
+        const netbc::DeviceGraph& d = g.dev;
+        bool in = incoming && d.directed;
+        const uint32_t* off = in ? d.in_off.p : d.out_off.p;
+        const uint32_t* idx = in ? d.in_idx.p : d.out_idx.p;
+        const double* w = in ? d.in_w.p : d.out_w.p;
+        NETBC_HIP(hipMemcpy(offsets, off, (g.n + 1) * 4, hipMemcpyDeviceToHost));
+        if (d.arcs) NETBC_HIP(hipMemcpy(targets, idx, d.arcs * 4, hipMemcpyDeviceToHost));
+        if (weights && d.weighted && d.arcs)
+            NETBC_HIP(hipMemcpy(weights, w, d.arcs * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+NETBC_EXPORT int32_t netbc_device_info(char* buf, size_t len) {
+    return guard(buf, len, [&] {
+        ensure_device();
+        int dev = 0;
+        NETBC_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        NETBC_HIP(hipGetDeviceProperties(&p, dev));
+        std::string s = std::string(p.name) + ", " + p.gcnArchName + ", " +
+                        std::to_string(p.multiProcessorCount) + " CUs";
+        write_err(buf, len, s);
+    });
+}

@@ -1,0 +1,485 @@
+// Device CSR/CSC construction and path-length classification.
+//
+// The reference builds its out-CSR with two stable counting sorts (first
+// by target, then by source), so each row is sorted by target and parallel
+// arcs keep input order (crates/engine/src/graph.rs:72-118); undirected
+// edges become arcs (s,t),(t,s), a loop stays one arc (graph.rs:201-223);
+// directed graphs also keep the in-CSR (graph.rs:197-200). Here the same
+// order comes from one stable LSD radix sort of 64-bit keys
+// (row << b | column) carrying the edge index, so the device arrays equal
+// the reference's host arrays element for element.
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "netbc_internal.h"
+
+namespace netbc {
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kBlock * kSortItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) {
+        if (i < wid) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+}
+
+// Sum of each kScanTile chunk.
+__global__ void scan_reduce_kernel(const uint32_t* __restrict__ in, size_t len,
+                                   uint32_t* __restrict__ partial) {
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++)
+        if (base + i < len) s += in[base + i];
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// Exclusive scan of each chunk, offset by the scanned chunk sums.
+__global__ void scan_apply_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                  size_t len, const uint32_t* __restrict__ partial) {
+    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t vals[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        vals[i] = base + i < len ? in[base + i] : 0;
+        s += vals[i];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(s, &tot) + (partial ? partial[blockIdx.x] : 0);
+#pragma unroll
+    for (int i = 0; i < kScanItems; i++) {
+        if (base + i < len) out[base + i] = run;
+        run += vals[i];
+    }
+}
+
+// In-place-safe exclusive scan of len u32 values (out may alias in).
+void exclusive_scan(const uint32_t* in, uint32_t* out, size_t len, hipStream_t st) {
+    if (len == 0) return;
+    size_t blocks = (len + kScanTile - 1) / kScanTile;
+    if (blocks == 1) {
+        scan_apply_kernel<<<1, kBlock, 0, st>>>(in, out, len, nullptr);
+        NETBC_HIP(hipGetLastError());
+        return;
+    }
+    DevBuf<uint32_t> partial;
+    partial.alloc(blocks);
+    scan_reduce_kernel<<<blocks, kBlock, 0, st>>>(in, len, partial.p);
+    NETBC_HIP(hipGetLastError());
+    exclusive_scan(partial.p, partial.p, blocks, st);
+    scan_apply_kernel<<<blocks, kBlock, 0, st>>>(in, out, len, partial.p);
+    NETBC_HIP(hipGetLastError());
+    NETBC_HIP(hipStreamSynchronize(st));  // partial is freed on return
+}
+
+// Digit histogram per tile, digit-major: hist[d * tiles + tile].
+__global__ void radix_hist_kernel(const uint64_t* __restrict__ keys, size_t len, int shift,
+                                  uint32_t* __restrict__ hist, uint32_t tiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortItems; r++) {
+        size_t i = base + (size_t)r * kBlock + threadIdx.x;
+        if (i < len) atomicAdd(&cnt[(keys[i] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter of one tile: items are ranked in index order (round,
+// wave, lane), so equal digits keep their relative order.
+__global__ void radix_scatter_kernel(const uint64_t* __restrict__ kin,
+                                     const uint32_t* __restrict__ vin,
+                                     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                     size_t len, int shift, const uint32_t* __restrict__ hist,
+                                     uint32_t tiles) {
+    __shared__ uint32_t base_d[256];
+    __shared__ uint32_t wcnt[kBlock / 64][256];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    base_d[t] = hist[(size_t)t * tiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; w++) wcnt[w][t] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < kSortItems; r++) {
+        size_t i = (size_t)blockIdx.x * kSortTile + (size_t)r * kBlock + t;
+        bool valid = i < len;
+        uint64_t k = valid ? kin[i] : 0;
+        uint32_t v = valid ? vin[i] : 0;
+        uint32_t d = (uint32_t)(k >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            bool set = (d >> bit) & 1u;
+            uint64_t bb = __ballot(set);
+            peers &= set ? bb : ~bb;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[wid][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base_d[d] + rank;
+            for (int w = 0; w < wid; w++) pos += wcnt[w][d];
+            kout[pos] = k;
+            vout[pos] = v;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) {
+            add += wcnt[w][t];
+            wcnt[w][t] = 0;
+        }
+        base_d[t] += add;
+        __syncthreads();
+    }
+}
+
+// Stable sort of (key, value) pairs on the low `bits` bits of the keys.
+// Returns which buffer holds the result (0: k0/v0, 1: k1/v1).
+int radix_sort_pairs(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t len,
+                     int bits, hipStream_t st) {
+    if (len <= 1 || bits <= 0) return 0;
+    uint32_t tiles = (uint32_t)((len + kSortTile - 1) / kSortTile);
+    DevBuf<uint32_t> hist;
+    hist.alloc((size_t)256 * tiles);
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        uint64_t* ki = cur ? k1 : k0;
+        uint32_t* vi = cur ? v1 : v0;
+        uint64_t* ko = cur ? k0 : k1;
+        uint32_t* vo = cur ? v0 : v1;
+        radix_hist_kernel<<<tiles, kBlock, 0, st>>>(ki, len, shift, hist.p, tiles);
+        NETBC_HIP(hipGetLastError());
+        exclusive_scan(hist.p, hist.p, (size_t)256 * tiles, st);
+        radix_scatter_kernel<<<tiles, kBlock, 0, st>>>(ki, vi, ko, vo, len, shift, hist.p, tiles);
+        NETBC_HIP(hipGetLastError());
+        cur ^= 1;
+    }
+    NETBC_HIP(hipStreamSynchronize(st));
+    return cur;
+}
+
+__global__ void nonloop_flags_kernel(const uint32_t* __restrict__ s, const uint32_t* __restrict__ t,
+                                     size_t m, uint32_t* __restrict__ flag) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < m) flag[e] = s[e] != t[e];
+}
+
+// Arc list of the reference's order before sorting: directed, one arc per
+// edge; undirected, (s,t) then (t,s) unless a loop. Keys are
+// (row << b | column); values the edge index.
+__global__ void make_arcs_kernel(const uint32_t* __restrict__ s, const uint32_t* __restrict__ t,
+                                 size_t m, int directed, int reverse, int b,
+                                 const uint32_t* __restrict__ before,
+                                 uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    uint64_t a = s[e], c = t[e];
+    if (directed) {
+        if (reverse) { uint64_t x = a; a = c; c = x; }
+        key[e] = a << b | c;
+        val[e] = (uint32_t)e;
+        return;
+    }
+    size_t pos = e + before[e];
+    key[pos] = a << b | c;
+    val[pos] = (uint32_t)e;
+    if (a != c) {
+        key[pos + 1] = c << b | a;
+        val[pos + 1] = (uint32_t)e;
+    }
+}
+
+__global__ void row_count_kernel(const uint64_t* __restrict__ key, size_t arcs, int b,
+                                 uint32_t* __restrict__ deg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < arcs) atomicAdd(&deg[key[i] >> b], 1u);
+}
+
+__global__ void unpack_kernel(const uint64_t* __restrict__ key, const uint32_t* __restrict__ val,
+                              size_t arcs, uint64_t mask, const double* __restrict__ w,
+                              uint32_t* __restrict__ idx, double* __restrict__ wout) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= arcs) return;
+    idx[i] = (uint32_t)(key[i] & mask);
+    if (w) wout[i] = w[val[i]];
+}
+
+int bits_for(uint64_t n) {
+    int b = 1;
+    while (b < 64 && (1ull << b) < n) b++;
+    return b;
+}
+
+inline unsigned grid_for(size_t items, int block = kBlock) {
+    return (unsigned)((items + block - 1) / block);
+}
+
+void build_one(uint64_t n, uint64_t m, bool directed, bool reverse, const uint32_t* ds,
+               const uint32_t* dt, const double* dw, const uint32_t* before, uint64_t arcs,
+               DevBuf<uint32_t>& off, DevBuf<uint32_t>& idx, DevBuf<double>& wout,
+               hipStream_t st) {
+    int b = bits_for(n);
+    DevBuf<uint64_t> k0, k1;
+    DevBuf<uint32_t> v0, v1;
+    k0.alloc(arcs);
+    k1.alloc(arcs);
+    v0.alloc(arcs);
+    v1.alloc(arcs);
+    if (m) {
+        make_arcs_kernel<<<grid_for(m), kBlock, 0, st>>>(ds, dt, m, directed, reverse, b, before,
+                                                         k0.p, v0.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    int which = radix_sort_pairs(k0.p, v0.p, k1.p, v1.p, arcs, 2 * b, st);
+    const uint64_t* key = which ? k1.p : k0.p;
+    const uint32_t* val = which ? v1.p : v0.p;
+    off.alloc(n + 1);
+    NETBC_HIP(hipMemsetAsync(off.p, 0, (n + 1) * sizeof(uint32_t), st));
+    if (arcs) {
+        row_count_kernel<<<grid_for(arcs), kBlock, 0, st>>>(key, arcs, b, off.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    exclusive_scan(off.p, off.p, n + 1, st);
+    idx.alloc(arcs);
+    if (dw) wout.alloc(arcs);
+    if (arcs) {
+        unpack_kernel<<<grid_for(arcs), kBlock, 0, st>>>(key, val, arcs, (1ull << b) - 1, dw,
+                                                         idx.p, dw ? wout.p : nullptr);
+        NETBC_HIP(hipGetLastError());
+    }
+    NETBC_HIP(hipStreamSynchronize(st));
+}
+
+// Classification flags (weights.rs:89-131).
+struct ClassifyOut {
+    unsigned long long first_bad;  // first edge (input order) with weight <= 0
+    uint32_t not_uniform;
+    uint32_t not_integer;
+    uint32_t max;
+    uint32_t pad;
+};
+
+__global__ void first_nonpositive_kernel(const double* __restrict__ w, size_t m,
+                                         ClassifyOut* out) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < m && w[e] <= 0.0) atomicMin(&out->first_bad, (unsigned long long)e);
+}
+
+__device__ __forceinline__ double arc_length(double w, int strength) {
+    return strength ? 1.0 / w : w;
+}
+
+__global__ void classify_kernel(const double* __restrict__ w, size_t arcs, int strength,
+                                ClassifyOut* out) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= arcs) return;
+    double x = arc_length(w[a], strength);
+    double x0 = arc_length(w[0], strength);
+    if (!(x == x0)) out->not_uniform = 1;
+    if (x - trunc(x) == 0.0 && x >= 1.0 && x <= 4294967295.0)
+        atomicMax(&out->max, (uint32_t)x);
+    else
+        out->not_integer = 1;
+}
+
+__global__ void int_lengths_kernel(const double* __restrict__ w, size_t arcs, int strength,
+                                   uint32_t* __restrict__ len) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < arcs) len[a] = (uint32_t)arc_length(w[a], strength);
+}
+
+__global__ void real_lengths_kernel(const double* __restrict__ w, size_t arcs, int strength,
+                                    double* __restrict__ len) {
+    size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < arcs) len[a] = arc_length(w[a], strength);
+}
+
+// Rust's Display for f64 (error.rs:81-85 prints the weight this way):
+// shortest round-trip digits, positional notation.
+std::string rust_f64(double x) {
+    if (x != x) return "NaN";
+    std::string sign = std::signbit(x) ? "-" : "";
+    if (std::isinf(x)) return sign + "inf";
+    x = std::fabs(x);
+    if (x == 0.0) return sign + "0";
+    char buf[64];
+    for (int p = 1; p <= 17; p++) {
+        snprintf(buf, sizeof buf, "%.*e", p - 1, x);
+        if (strtod(buf, nullptr) == x) break;
+    }
+    std::string digits;
+    const char* e = strchr(buf, 'e');
+    for (const char* c = buf; c < e; c++)
+        if (*c >= '0' && *c <= '9') digits.push_back(*c);
+    int exp10 = atoi(e + 1);
+    while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+    std::string r;
+    int nd = (int)digits.size();
+    if (exp10 >= 0) {
+        for (int i = 0; i <= exp10; i++) r.push_back(i < nd ? digits[i] : '0');
+        if (nd > exp10 + 1) {
+            r.push_back('.');
+            r.append(digits, exp10 + 1, std::string::npos);
+        }
+    } else {
+        r = "0.";
+        r.append(-exp10 - 1, '0');
+        r += digits;
+    }
+    return sign + r;
+}
+
+}  // namespace
+
+Graph::~Graph() {
+    if (workspace && workspace_free) workspace_free(workspace);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void ensure_stream(Graph& g) {
+    int dev = 0;
+    NETBC_HIP(hipGetDevice(&dev));
+    if (g.stream && g.device == dev) return;
+    if (g.stream) {
+        (void)hipStreamDestroy(g.stream);
+        g.stream = nullptr;
+    }
+    NETBC_HIP(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    g.device = dev;
+}
+
+void build_device_graph(Graph& g) {
+    if (g.dev_ready) return;
+    ensure_stream(g);
+    hipStream_t st = g.stream;
+    DeviceGraph& d = g.dev;
+    d.n = g.n;
+    d.m = g.m;
+    d.directed = g.directed;
+    d.weighted = g.has_weights;
+    DevBuf<uint32_t> ds, dt, before;
+    DevBuf<double> dw;
+    ds.alloc(g.m);
+    dt.alloc(g.m);
+    if (g.m) {
+        NETBC_HIP(hipMemcpyAsync(ds.p, g.src.data(), g.m * 4, hipMemcpyHostToDevice, st));
+        NETBC_HIP(hipMemcpyAsync(dt.p, g.tgt.data(), g.m * 4, hipMemcpyHostToDevice, st));
+    }
+    if (g.has_weights) {
+        dw.alloc(g.m);
+        if (g.m)
+            NETBC_HIP(hipMemcpyAsync(dw.p, g.w.data(), g.m * 8, hipMemcpyHostToDevice, st));
+    }
+    uint64_t arcs = g.m;
+    if (!g.directed) {
+        before.alloc(g.m + 1);
+        if (g.m) {
+            nonloop_flags_kernel<<<grid_for(g.m), kBlock, 0, st>>>(ds.p, dt.p, g.m, before.p);
+            NETBC_HIP(hipGetLastError());
+            exclusive_scan(before.p, before.p, g.m, st);
+            uint32_t last = 0;
+            NETBC_HIP(hipMemcpyAsync(&last, before.p + g.m - 1, 4, hipMemcpyDeviceToHost, st));
+            NETBC_HIP(hipStreamSynchronize(st));
+            uint64_t nonloops = (uint64_t)last + (g.src[g.m - 1] != g.tgt[g.m - 1]);
+            arcs = g.m + nonloops;
+        }
+    }
+    d.arcs = arcs;
+    build_one(g.n, g.m, g.directed, false, ds.p, dt.p, g.has_weights ? dw.p : nullptr,
+              before.p, arcs, d.out_off, d.out_idx, d.out_w, st);
+    if (g.directed)
+        build_one(g.n, g.m, true, true, ds.p, dt.p, g.has_weights ? dw.p : nullptr, nullptr,
+                  arcs, d.in_off, d.in_idx, d.in_w, st);
+    NETBC_HIP(hipStreamSynchronize(st));
+    g.dev_ready = true;
+}
+
+void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L) {
+    L.kind = kUniform;
+    L.max = 0;
+    if (!weighted) return;
+    if (!g.has_weights) throw Error("the graph has no edge weights");
+    hipStream_t st = g.stream;
+    DeviceGraph& d = g.dev;
+    DevBuf<ClassifyOut> res;
+    res.alloc(1);
+    ClassifyOut init{~0ull, 0, 0, 0, 0};
+    NETBC_HIP(hipMemcpyAsync(res.p, &init, sizeof init, hipMemcpyHostToDevice, st));
+    // The raw weights in input order are the out-CSR weights permuted; the
+    // first offending edge index is found on the host copy's order by
+    // scanning the device copy of the input-order weights.
+    DevBuf<double> dw;
+    dw.alloc(g.m);
+    if (g.m) {
+        NETBC_HIP(hipMemcpyAsync(dw.p, g.w.data(), g.m * 8, hipMemcpyHostToDevice, st));
+        first_nonpositive_kernel<<<grid_for(g.m), kBlock, 0, st>>>(dw.p, g.m, res.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    int strength = weights_as == 1;
+    if (d.arcs) {
+        classify_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.out_w.p, d.arcs, strength, res.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    ClassifyOut out;
+    NETBC_HIP(hipMemcpyAsync(&out, res.p, sizeof out, hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipStreamSynchronize(st));
+    if (out.first_bad != ~0ull) {
+        uint64_t e = out.first_bad;
+        throw Error("edge " + std::to_string(e) + " has weight " + rust_f64(g.w[e]) +
+                    "; this algorithm treats weights as distances and requires them to be "
+                    "strictly positive");
+    }
+    if (!out.not_uniform) return;  // Uniform -> breadth-first search
+    if (!out.not_integer && out.max <= 1024u) {
+        L.kind = kInteger;
+        L.max = out.max;
+        L.out_len.alloc(d.arcs);
+        int_lengths_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.out_w.p, d.arcs, strength,
+                                                                L.out_len.p);
+        NETBC_HIP(hipGetLastError());
+        if (d.directed) {
+            L.in_len.alloc(d.arcs);
+            int_lengths_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.in_w.p, d.arcs, strength,
+                                                                    L.in_len.p);
+            NETBC_HIP(hipGetLastError());
+        }
+    } else {
+        L.kind = kReal;
+        L.out_real.alloc(d.arcs);
+        real_lengths_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.out_w.p, d.arcs, strength,
+                                                                 L.out_real.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    NETBC_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace netbc

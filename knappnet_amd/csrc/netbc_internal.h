@@ -1,0 +1,153 @@
+// Internal declarations shared by the device library's translation units.
+// Nothing here is part of the public C ABI (include/netbc.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace netbc {
+
+constexpr int kLanes = 64;        // sources per batch = wavefront width on gfx950
+constexpr int kBlock = 256;       // threads per block for the batched kernels
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// An error that reaches the C caller as status 1 and its message.
+struct Error : std::runtime_error {
+    explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define NETBC_HIP(call)                                                              \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            throw ::netbc::Error(std::string("device error: ") + hipGetErrorString(e_) + \
+                                 " (" #call ")");                                    \
+    } while (0)
+
+// A device allocation owned by host code.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        size_t bytes = (count ? count : 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error("device error: out of device memory allocating " +
+                        std::to_string(bytes) + " bytes");
+        }
+        n = count;
+    }
+};
+
+// Path-length classes, as the reference's Lengths enum (weights.rs:72-82).
+enum LengthKind : int32_t { kUniform = 0, kInteger = 1, kReal = 2 };
+
+// The device copy of a graph: out-CSR (rows sorted by target, parallel arcs
+// in input order) and, for directed graphs, the in-CSR (CSC). For
+// undirected graphs the in-CSR is the out-CSR.
+struct DeviceGraph {
+    uint64_t n = 0, m = 0, arcs = 0;
+    bool directed = false;
+    bool weighted = false;
+    DevBuf<uint32_t> out_off, out_idx, in_off, in_idx;
+    DevBuf<double> out_w, in_w;  // arc weights in CSR / CSC order
+    const uint32_t* in_off_p() const { return directed ? in_off.p : out_off.p; }
+    const uint32_t* in_idx_p() const { return directed ? in_idx.p : out_idx.p; }
+    const double* in_w_p() const { return directed ? in_w.p : out_w.p; }
+};
+
+// Host graph handle: the validated edge list (input order) plus a lazily
+// built device copy.
+struct Graph {
+    uint64_t n = 0, m = 0;
+    bool directed = false;
+    std::vector<uint32_t> src, tgt;
+    std::vector<double> w;  // empty when unweighted
+    bool has_weights = false;
+
+    std::mutex mu;  // guards dev, the stream and the cached workspace
+    bool dev_ready = false;
+    DeviceGraph dev;
+    hipStream_t stream = nullptr;
+    int device = -1;
+    void* workspace = nullptr;  // engine scratch, freed with the handle
+    void (*workspace_free)(void*) = nullptr;
+    ~Graph();
+};
+
+// Length classification result (weights.rs:89-131), computed on device.
+struct Lengths {
+    LengthKind kind = kUniform;
+    uint32_t max = 0;
+    DevBuf<uint32_t> out_len, in_len;  // integer lengths, CSR / CSC order
+    DevBuf<double> out_real;           // real lengths, CSR order
+};
+
+// Options and diagnostics of one betweenness run.
+struct RunOptions {
+    int precision = 0;      // 0: FP64 dependencies; 1: FP32 dependencies + Kahan
+    int slots = 0;          // concurrent 64-source batches; 0 = auto
+    int direction = 0;      // 0 auto, 1 top-down only, 2 bottom-up whenever possible
+    double alpha = 0.0;     // bottom-up switch: frontier arcs > unvisited arcs / alpha (0 = default)
+    int keep_sigma = 0;     // debug: keep per-source sigma and levels
+};
+
+struct RunDiag {
+    double max_sigma = 0.0;
+    int32_t sigma_overflow = 0;
+    int32_t path_kind = 0;
+    int64_t reached_nodes = 0;   // sum over sources of reached nodes
+    int64_t reached_arcs = 0;    // sum over sources of out-arcs of reached nodes
+    int64_t levels = 0;          // forward levels run (sum over super-batches)
+    double kernel_ms = 0.0;      // device time of the search (events)
+    double dominant_ms = 0.0;    // summed duration of the dominant kernel
+    int64_t dominant_launches = 0;
+    int64_t dominant_bytes = 0;  // algorithmic bytes of the dominant kernel, summed
+    int32_t slots = 0;
+    int64_t bottom_up_levels = 0;
+};
+
+// Builds the device CSR/CSC of g (graph.hip). Caller holds g.mu.
+void build_device_graph(Graph& g);
+
+// Classifies lengths on the device, as path_lengths (weights.rs:89-131).
+// Throws Error with the reference's message on bad weights.
+void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L);
+
+// Runs the batched engine over item positions [0, k) with sources
+// `sources` (nullptr: item i is source i) and writes unscaled sums into
+// d_bc (device, n doubles) in the reference's group order.
+void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
+                 const RunOptions& opt, double* d_bc, RunDiag& diag);
+
+// Debug: per-source distances (hops or integer distances; UINT64_MAX when
+// unreached), sigma and delta for k sources, each n long, host buffers.
+void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
+              const RunOptions& opt, uint64_t* dist, double* sigma, double* delta);
+
+// Scales d_bc in place as betweenness.rs:100-113 and copies it to host out.
+void finish_scores(Graph& g, bool normalized, double* d_bc, double* out);
+
+// Ensures g has a stream on the current device.
+void ensure_stream(Graph& g);
+
+}  // namespace netbc

@@ -1,0 +1,139 @@
+"""Python mirror of knappnet's ctypes harness over the C ABI.
+
+``Graph`` follows the reference's ``validation/engine.py:92-128`` (the
+harness its validation and benchmarks use): construct from an edge list,
+call ``betweenness(weighted, normalized, threads, weights_as)``, get an
+``EngineError`` carrying the library's message on failure. Every call goes
+through ``libnetbc.so``; there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+from . import _native
+
+ERR_LEN = 1024
+KIND = {"distance": 0, "strength": 1}
+
+
+class EngineError(RuntimeError):
+    """A non-zero status from the C ABI, with its message."""
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(POINTER(ctype))
+
+
+class Graph:
+    """A graph handle of the device library (``net_graph_new``)."""
+
+    def __init__(self, n: int, src, tgt, weights=None, directed: bool = False):
+        self.lib = _native.device()
+        self.n = int(n)
+        self._src = _i64(src)
+        self._tgt = _i64(tgt)
+        m = len(self._src)
+        self._w = None if weights is None else np.ascontiguousarray(
+            np.asarray(weights, dtype=np.float64))
+        self.handle = c_void_p()
+        err = ctypes.create_string_buffer(ERR_LEN)
+        status = self.lib.net_graph_new(
+            self.n, _ptr(self._src, c_int64), _ptr(self._tgt, c_int64),
+            None if self._w is None else _ptr(self._w, c_double), m, int(bool(directed)),
+            ctypes.byref(self.handle), err, ERR_LEN)
+        if status:
+            self.handle = c_void_p()
+            raise EngineError(err.value.decode())
+        self.directed = bool(directed)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.net_graph_free(self.handle)
+            self.handle = c_void_p()
+
+    def node_count(self) -> int:
+        return int(self.lib.net_graph_node_count(self.handle))
+
+    def edge_count(self) -> int:
+        return int(self.lib.net_graph_edge_count(self.handle))
+
+    def arc_count(self) -> int:
+        return int(self.lib.netbc_graph_arc_count(self.handle))
+
+    def betweenness(self, weighted: bool = False, normalized: bool = False, threads: int = 0,
+                    weights_as: str = "distance") -> np.ndarray:
+        """``net_betweenness``: scores of every node, in node order."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if self.lib.net_betweenness(self.handle, int(bool(weighted)), kind,
+                                    int(bool(normalized)), int(threads),
+                                    _ptr(out, c_double), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return out[: self.n]
+
+    def betweenness_sources(self, sources=None, weighted: bool = False,
+                            normalized: bool = False, weights_as: str = "distance",
+                            precision: int = 0, slots: int = 0, direction: int = 0,
+                            alpha: float = 0.0):
+        """``netbc_betweenness_sources``: returns (scores, diagnostics dict)."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        opt = _native.Options(precision, slots, direction, 0, alpha)
+        diag = _native.Diagnostics()
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if sources is None:
+            src_p, k = None, self.n
+        else:
+            srcs = _i64(sources)
+            src_p, k = _ptr(srcs, c_int64), len(srcs)
+        if self.lib.netbc_betweenness_sources(self.handle, src_p, k, int(bool(weighted)), kind,
+                                              int(bool(normalized)), ctypes.byref(opt),
+                                              _ptr(out, c_double), ctypes.byref(diag), err,
+                                              ERR_LEN):
+            raise EngineError(err.value.decode())
+        return out[: self.n], diag.as_dict()
+
+    def state_dump(self, sources, weighted: bool = False, weights_as: str = "distance",
+                   precision: int = 0):
+        """``netbc_state_dump``: (dist, sigma, delta), each k x n."""
+        srcs = _i64(sources)
+        k = len(srcs)
+        dist = np.zeros((k, self.n), dtype=np.uint64)
+        sigma = np.zeros((k, self.n), dtype=np.float64)
+        delta = np.zeros((k, self.n), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        if self.lib.netbc_state_dump(self.handle, int(bool(weighted)), KIND[weights_as],
+                                     int(precision), _ptr(srcs, c_int64), k,
+                                     _ptr(dist, c_uint64), _ptr(sigma, c_double),
+                                     _ptr(delta, c_double), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return dist, sigma, delta
+
+    def csr(self, incoming: bool = False):
+        """The device CSR (or CSC): (offsets, targets, weights or None)."""
+        arcs = self.arc_count()
+        off = np.zeros(self.n + 1, dtype=np.uint32)
+        tgt = np.zeros(max(arcs, 1), dtype=np.uint32)
+        w = np.zeros(max(arcs, 1), dtype=np.float64) if self._w is not None else None
+        err = ctypes.create_string_buffer(ERR_LEN)
+        if self.lib.netbc_graph_csr(self.handle, int(bool(incoming)), _ptr(off, c_uint32),
+                                    _ptr(tgt, c_uint32),
+                                    None if w is None else _ptr(w, c_double), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return off, tgt[:arcs], None if w is None else w[:arcs]
+
+
+def device_info() -> str:
+    lib = _native.device()
+    buf = ctypes.create_string_buffer(256)
+    if lib.netbc_device_info(buf, 256):
+        raise EngineError(buf.value.decode())
+    return buf.value.decode()

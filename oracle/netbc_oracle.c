@@ -1,0 +1,1007 @@
+/*
+ * CPU oracle for knappnet's exact Brandes betweenness (net_betweenness).
+ *
+ * TEST INFRASTRUCTURE ONLY. This file restates, in plain C11, the
+ * reference engine's CPU algorithm so that the device library can be
+ * checked against it. Only the tests, the smoke check and the benchmark's
+ * CPU-baseline leg load it; the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it follows (paths are
+ * relative to the knappnet source tree). Floating-point operations are
+ * written in the reference's order and the file is compiled with
+ * -ffp-contract=off and without fast-math, so that FP64 results follow the
+ * reference bit for bit on the same inputs.
+ *
+ * Pinning (what checks each part; see tests/):
+ *   graph build      graph.rs:350-412 known answers, host/device CSR equality
+ *   path lengths     weights.rs:290-321 known answers
+ *   BFS/Dial/Dijk.   betweenness.rs:387-517 known answers, the 28 committed
+ *                    definition/NetworkX goldens, brute-force exact-rational
+ *                    definition on random small graphs, the sum identity
+ *   sum_in_order     parallel.rs:653-665 thread-count invariance
+ */
+#define _GNU_SOURCE
+#include <math.h>
+#include <pthread.h>
+#include <sched.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <unistd.h>
+
+#define ORC_OK 0
+#define ORC_FAILED 1
+#define U32_MAX_ 0xFFFFFFFFu
+
+/* ---- errors: lib.rs:60-71 (write_err), error.rs:62-99 (texts) ---------- */
+
+static void write_err(char *err, size_t err_len, const char *msg) {
+    if (!err || err_len == 0) return;
+    size_t n = strlen(msg);
+    if (n > err_len - 1) n = err_len - 1;
+    memcpy(err, msg, n);
+    err[n] = 0;
+}
+
+/* Rust's `Display` for f64: the shortest digits that round-trip, written
+ * positionally (no exponent). Used in the NonPositiveWeight message,
+ * error.rs:81-85. */
+static void fmt_f64(double x, char *out, size_t cap) {
+    if (isnan(x)) { snprintf(out, cap, "NaN"); return; }
+    if (isinf(x)) { snprintf(out, cap, x < 0 ? "-inf" : "inf"); return; }
+    char sign[2] = {0, 0};
+    if (signbit(x)) { sign[0] = '-'; x = -x; }
+    if (x == 0.0) { snprintf(out, cap, "%s0", sign); return; }
+    char buf[64];
+    int p;
+    for (p = 1; p <= 17; p++) {
+        snprintf(buf, sizeof buf, "%.*e", p - 1, x);
+        if (strtod(buf, NULL) == x) break;
+    }
+    /* buf = d[.ddd]e[+-]XX */
+    char digits[32];
+    int nd = 0;
+    char *e = strchr(buf, 'e');
+    for (char *c = buf; c < e; c++)
+        if (*c >= '0' && *c <= '9') digits[nd++] = *c;
+    int exp10 = atoi(e + 1);
+    while (nd > 1 && digits[nd - 1] == '0') nd--;
+    char res[400];
+    int r = 0;
+    if (exp10 >= 0) {
+        for (int i = 0; i <= exp10; i++) res[r++] = i < nd ? digits[i] : '0';
+        if (nd > exp10 + 1) {
+            res[r++] = '.';
+            for (int i = exp10 + 1; i < nd; i++) res[r++] = digits[i];
+        }
+    } else {
+        res[r++] = '0';
+        res[r++] = '.';
+        for (int i = 0; i < -exp10 - 1; i++) res[r++] = '0';
+        for (int i = 0; i < nd; i++) res[r++] = digits[i];
+    }
+    res[r] = 0;
+    snprintf(out, cap, "%s%s", sign, res);
+}
+
+/* ---- graph: graph.rs:18-234 -------------------------------------------- */
+
+typedef struct {
+    uint64_t *offsets; /* n + 1 */
+    uint32_t *targets; /* arcs */
+    double *weights;   /* arcs or NULL */
+    uint64_t arcs;
+} OCsr;
+
+typedef struct OGraph {
+    uint64_t n, m;
+    int directed;
+    uint32_t *src, *tgt; /* input order */
+    double *w;           /* input order or NULL */
+    OCsr out, inc;       /* inc only when directed */
+} OGraph;
+
+/* Csr::build, graph.rs:72-118: two stable counting sorts, first by target
+ * then by source, so each row ends up sorted by target with parallel arcs
+ * in input order. */
+static void csr_build(uint64_t n, const uint32_t *from, const uint32_t *to,
+                      const double *weights, uint64_t m, OCsr *c) {
+    uint64_t *by_target = calloc(n + 1, sizeof *by_target);
+    for (uint64_t a = 0; a < m; a++) by_target[(uint64_t)to[a] + 1]++;
+    for (uint64_t i = 0; i < n; i++) by_target[i + 1] += by_target[i];
+    uint64_t *order = malloc((m ? m : 1) * sizeof *order);
+    uint64_t *cursor = malloc((n + 1) * sizeof *cursor);
+    memcpy(cursor, by_target, (n + 1) * sizeof *cursor);
+    for (uint64_t a = 0; a < m; a++) order[cursor[to[a]]++] = a;
+
+    c->offsets = calloc(n + 1, sizeof *c->offsets);
+    for (uint64_t a = 0; a < m; a++) c->offsets[(uint64_t)from[a] + 1]++;
+    for (uint64_t i = 0; i < n; i++) c->offsets[i + 1] += c->offsets[i];
+    memcpy(cursor, c->offsets, (n + 1) * sizeof *cursor);
+    c->targets = malloc((m ? m : 1) * sizeof *c->targets);
+    c->weights = weights ? malloc((m ? m : 1) * sizeof *c->weights) : NULL;
+    for (uint64_t i = 0; i < m; i++) {
+        uint64_t a = order[i];
+        uint64_t slot = cursor[from[a]]++;
+        c->targets[slot] = to[a];
+        if (weights) c->weights[slot] = weights[a];
+    }
+    c->arcs = m;
+    free(by_target);
+    free(order);
+    free(cursor);
+}
+
+static void csr_free(OCsr *c) {
+    free(c->offsets);
+    free(c->targets);
+    free(c->weights);
+    memset(c, 0, sizeof *c);
+}
+
+void orc_graph_free(OGraph *g) {
+    if (!g) return;
+    csr_free(&g->out);
+    csr_free(&g->inc);
+    free(g->src);
+    free(g->tgt);
+    free(g->w);
+    free(g);
+}
+
+/* net_graph_new, lib.rs:133-164, and Graph::from_edges, graph.rs:142-234,
+ * with the same order of checks and the same messages. */
+int32_t orc_graph_new(int64_t n, const int64_t *src, const int64_t *tgt,
+                      const double *weights, int64_t m, int32_t directed,
+                      OGraph **out, char *err, size_t err_len) {
+    char msg[512];
+    if (n < 0 || m < 0) {
+        write_err(err, err_len, "negative node or edge count");
+        return ORC_FAILED;
+    }
+    if ((uint64_t)n > (uint64_t)U32_MAX_) {
+        snprintf(msg, sizeof msg, "node count (%lld) exceeds the supported maximum", (long long)n);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (weights) {
+        for (int64_t e = 0; e < m; e++) {
+            if (!isfinite(weights[e])) {
+                snprintf(msg, sizeof msg, "edge %lld has a missing, NaN, or infinite weight",
+                         (long long)e);
+                write_err(err, err_len, msg);
+                return ORC_FAILED;
+            }
+        }
+    }
+    if ((uint64_t)m > (uint64_t)(U32_MAX_ / 2)) {
+        snprintf(msg, sizeof msg, "edge count (%lld) exceeds the supported maximum", (long long)m);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    OGraph *g = calloc(1, sizeof *g);
+    g->n = (uint64_t)n;
+    g->m = (uint64_t)m;
+    g->directed = directed != 0;
+    g->src = malloc((m ? m : 1) * sizeof *g->src);
+    g->tgt = malloc((m ? m : 1) * sizeof *g->tgt);
+    for (int64_t e = 0; e < m; e++) {
+        int64_t ends[2] = {src[e], tgt[e]};
+        for (int k = 0; k < 2; k++) {
+            if (ends[k] < 0 || (uint64_t)ends[k] >= (uint64_t)n) {
+                snprintf(msg, sizeof msg,
+                         "edge %lld refers to node index %lld, but valid indices are 0..%lld",
+                         (long long)e, (long long)ends[k], (long long)n);
+                write_err(err, err_len, msg);
+                orc_graph_free(g);
+                return ORC_FAILED;
+            }
+        }
+        g->src[e] = (uint32_t)src[e];
+        g->tgt[e] = (uint32_t)tgt[e];
+    }
+    if (weights) {
+        g->w = malloc((m ? m : 1) * sizeof *g->w);
+        memcpy(g->w, weights, (size_t)m * sizeof *g->w);
+    }
+    if (g->directed) {
+        csr_build(g->n, g->src, g->tgt, g->w, g->m, &g->out);
+        csr_build(g->n, g->tgt, g->src, g->w, g->m, &g->inc);
+    } else {
+        /* Each non-loop edge becomes two arcs; a loop stays one arc
+         * (graph.rs:201-223). */
+        uint64_t loops = 0;
+        for (uint64_t e = 0; e < g->m; e++) loops += g->src[e] == g->tgt[e];
+        uint64_t arcs = 2 * g->m - loops;
+        uint32_t *from = malloc((arcs ? arcs : 1) * sizeof *from);
+        uint32_t *to = malloc((arcs ? arcs : 1) * sizeof *to);
+        double *w = g->w ? malloc((arcs ? arcs : 1) * sizeof *w) : NULL;
+        uint64_t k = 0;
+        for (uint64_t e = 0; e < g->m; e++) {
+            from[k] = g->src[e];
+            to[k] = g->tgt[e];
+            if (w) w[k] = g->w[e];
+            k++;
+            if (g->src[e] != g->tgt[e]) {
+                from[k] = g->tgt[e];
+                to[k] = g->src[e];
+                if (w) w[k] = g->w[e];
+                k++;
+            }
+        }
+        csr_build(g->n, from, to, w, arcs, &g->out);
+        free(from);
+        free(to);
+        free(w);
+    }
+    *out = g;
+    return ORC_OK;
+}
+
+int64_t orc_graph_node_count(const OGraph *g) { return g ? (int64_t)g->n : -1; }
+int64_t orc_graph_edge_count(const OGraph *g) { return g ? (int64_t)g->m : -1; }
+int64_t orc_graph_arc_count(const OGraph *g, int32_t incoming) {
+    if (!g) return -1;
+    return (int64_t)((incoming && g->directed) ? g->inc.arcs : g->out.arcs);
+}
+
+/* Copies one CSR direction out (incoming == 0: out-arcs; otherwise the
+ * in-arcs, which for undirected graphs are the out-arcs, graph.rs:281-285). */
+void orc_graph_csr(const OGraph *g, int32_t incoming, uint64_t *offsets, uint32_t *targets,
+                   double *weights) {
+    const OCsr *c = (incoming && g->directed) ? &g->inc : &g->out;
+    memcpy(offsets, c->offsets, (g->n + 1) * sizeof *offsets);
+    memcpy(targets, c->targets, c->arcs * sizeof *targets);
+    if (weights && c->weights) memcpy(weights, c->weights, c->arcs * sizeof *weights);
+}
+
+/* ---- path lengths: weights.rs:43-51, 89-131, 151-153 ------------------- */
+
+enum { LEN_UNIFORM = 0, LEN_INTEGER = 1, LEN_REAL = 2 };
+#define DIAL_MAX 1024u
+
+typedef struct {
+    int kind;
+    uint32_t max;
+    uint32_t *ilens; /* arcs, CSR order (Integer) */
+    double *rlens;   /* arcs, CSR order (Real) */
+} Lengths;
+
+static void lengths_free(Lengths *L) {
+    free(L->ilens);
+    free(L->rlens);
+}
+
+/* algorithm: 0 = Auto, 1 = Bfs, 2 = Dial, 3 = Dijkstra (weights.rs:54-66). */
+static int path_lengths(const OGraph *g, int weighted, int kind, int algorithm, Lengths *L,
+                        char *err, size_t err_len) {
+    memset(L, 0, sizeof *L);
+    if (!weighted || algorithm == 1) { L->kind = LEN_UNIFORM; return ORC_OK; }
+    if (!g->w) { write_err(err, err_len, "the graph has no edge weights"); return ORC_FAILED; }
+    for (uint64_t e = 0; e < g->m; e++) {
+        if (g->w[e] <= 0.0) {
+            char num[400], msg[600];
+            fmt_f64(g->w[e], num, sizeof num);
+            snprintf(msg, sizeof msg,
+                     "edge %llu has weight %s; this algorithm treats weights as distances and "
+                     "requires them to be strictly positive",
+                     (unsigned long long)e, num);
+            write_err(err, err_len, msg);
+            return ORC_FAILED;
+        }
+    }
+    uint64_t A = g->out.arcs;
+    double *lens = malloc((A ? A : 1) * sizeof *lens);
+    for (uint64_t a = 0; a < A; a++)
+        lens[a] = kind == 1 ? 1.0 / g->out.weights[a] : g->out.weights[a];
+    int uniform = 1;
+    for (uint64_t a = 1; a < A; a++)
+        if (!(lens[a - 1] == lens[a])) { uniform = 0; break; }
+    int is_int = 1;
+    uint64_t imax = 0;
+    for (uint64_t a = 0; a < A; a++) {
+        double x = lens[a];
+        if (x - trunc(x) == 0.0 && x >= 1.0 && x <= 4294967295.0) {
+            if ((uint64_t)x > imax) imax = (uint64_t)x;
+        } else {
+            is_int = 0;
+            break;
+        }
+    }
+    int use_int = 0;
+    if (algorithm == 0) {
+        if (uniform) { free(lens); L->kind = LEN_UNIFORM; return ORC_OK; }
+        use_int = is_int && imax <= DIAL_MAX;
+    } else if (algorithm == 2) {
+        if (!is_int) {
+            free(lens);
+            write_err(err, err_len, "Dial's algorithm needs positive integer path lengths");
+            return ORC_FAILED;
+        }
+        use_int = 1;
+    }
+    if (use_int) {
+        L->kind = LEN_INTEGER;
+        L->max = (uint32_t)imax;
+        L->ilens = malloc((A ? A : 1) * sizeof *L->ilens);
+        for (uint64_t a = 0; a < A; a++) L->ilens[a] = (uint32_t)lens[a];
+        free(lens);
+    } else {
+        L->kind = LEN_REAL;
+        L->rlens = lens;
+    }
+    return ORC_OK;
+}
+
+int32_t orc_path_lengths(const OGraph *g, int32_t weighted, int32_t weights_as, int32_t algorithm,
+                         int32_t *kind, uint32_t *max, char *err, size_t err_len) {
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, algorithm, &L, err, err_len)) return ORC_FAILED;
+    *kind = L.kind;
+    *max = L.max;
+    lengths_free(&L);
+    return ORC_OK;
+}
+
+/* ---- indexed 4-ary heap: weights.rs:155-277 ---------------------------- */
+
+typedef struct { double d; uint32_t v; } HItem;
+typedef struct { HItem *items; uint64_t len; uint32_t *pos; } Heap;
+
+static int h_less(HItem a, HItem b) {
+    if (a.d < b.d) return 1;
+    if (a.d > b.d) return 0;
+    return a.v < b.v;
+}
+
+static void h_sift_up(Heap *h, uint64_t i) {
+    HItem item = h->items[i];
+    while (i > 0) {
+        uint64_t parent = (i - 1) / 4;
+        HItem p = h->items[parent];
+        if (!h_less(item, p)) break;
+        h->items[i] = p;
+        h->pos[p.v] = (uint32_t)i;
+        i = parent;
+    }
+    h->items[i] = item;
+    h->pos[item.v] = (uint32_t)i;
+}
+
+static void h_sift_down(Heap *h, uint64_t i) {
+    uint64_t len = h->len;
+    HItem item = h->items[i];
+    for (;;) {
+        uint64_t first = i * 4 + 1;
+        if (first >= len) break;
+        uint64_t best = first;
+        uint64_t end = first + 4 < len ? first + 4 : len;
+        for (uint64_t c = first + 1; c < end; c++)
+            if (h_less(h->items[c], h->items[best])) best = c;
+        HItem b = h->items[best];
+        if (!h_less(b, item)) break;
+        h->items[i] = b;
+        h->pos[b.v] = (uint32_t)i;
+        i = best;
+    }
+    h->items[i] = item;
+    h->pos[item.v] = (uint32_t)i;
+}
+
+static void h_push_or_decrease(Heap *h, uint32_t node, double dist) {
+    uint32_t p = h->pos[node];
+    uint64_t i;
+    if (p == U32_MAX_) {
+        h->items[h->len] = (HItem){dist, node};
+        i = h->len++;
+    } else {
+        i = p;
+        HItem cand = {dist, node};
+        if (!h_less(cand, h->items[i])) return;
+        h->items[i].d = dist;
+    }
+    h_sift_up(h, i);
+}
+
+static int h_pop(Heap *h, HItem *top) {
+    if (h->len == 0) return 0;
+    *top = h->items[0];
+    h->pos[top->v] = U32_MAX_;
+    HItem last = h->items[--h->len];
+    if (h->len > 0) {
+        h->items[0] = last;
+        h_sift_down(h, 0);
+    }
+    return 1;
+}
+
+/* ---- per-thread workspace: betweenness.rs:122-159 ---------------------- */
+
+typedef struct { uint32_t *a; uint64_t len, cap; } U32Vec;
+static void vpush(U32Vec *v, uint32_t x) {
+    if (v->len == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 16;
+        v->a = realloc(v->a, v->cap * sizeof *v->a);
+    }
+    v->a[v->len++] = x;
+}
+
+typedef struct {
+    uint64_t n;
+    double *bc; /* the accumulator lent for the current source */
+    double *sigma, *coef, *dist;
+    uint32_t *hops, *rank, *order;
+    uint64_t norder;
+    uint64_t *idist;
+    U32Vec touched;
+    U32Vec *buckets;
+    uint32_t nbuckets;
+    Heap heap;
+    /* optional per-source delta capture (debug entry) */
+    double *delta_out;
+} Workspace;
+
+static void ws_init(Workspace *ws, uint64_t n) {
+    memset(ws, 0, sizeof *ws);
+    ws->n = n;
+    uint64_t nn = n ? n : 1;
+    ws->sigma = calloc(nn, sizeof(double));
+    ws->coef = calloc(nn, sizeof(double));
+    ws->dist = malloc(nn * sizeof(double));
+    ws->hops = malloc(nn * sizeof(uint32_t));
+    ws->rank = malloc(nn * sizeof(uint32_t));
+    ws->order = malloc(nn * sizeof(uint32_t));
+    ws->idist = malloc(nn * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; i++) {
+        ws->dist[i] = INFINITY;
+        ws->hops[i] = U32_MAX_;
+        ws->rank[i] = U32_MAX_;
+        ws->idist[i] = UINT64_MAX;
+    }
+    ws->heap.items = malloc(nn * sizeof(HItem));
+    ws->heap.pos = malloc(nn * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; i++) ws->heap.pos[i] = U32_MAX_;
+}
+
+static void ws_free(Workspace *ws) {
+    free(ws->sigma); free(ws->coef); free(ws->dist); free(ws->hops);
+    free(ws->rank); free(ws->order); free(ws->idist); free(ws->touched.a);
+    for (uint32_t i = 0; i < ws->nbuckets; i++) free(ws->buckets[i].a);
+    free(ws->buckets);
+    free(ws->heap.items); free(ws->heap.pos);
+}
+
+/* Adds the dependency of v to the lent accumulator, or records it for the
+ * debug entry. */
+static inline void credit(Workspace *ws, uint32_t v, uint64_t s, double delta) {
+    if (ws->delta_out) ws->delta_out[v] = delta;
+    if (v != s && ws->bc) ws->bc[v] += delta;
+}
+
+/* Workspace::single_source_bfs, betweenness.rs:161-210. */
+static void single_source_bfs(Workspace *ws, const OGraph *g, uint64_t s) {
+    const OCsr *adj = &g->out;
+    ws->norder = 0;
+    ws->hops[s] = 0;
+    ws->sigma[s] = 1.0;
+    ws->order[ws->norder++] = (uint32_t)s;
+    uint64_t head = 0;
+    while (head < ws->norder) {
+        uint32_t v = ws->order[head++];
+        uint32_t next = ws->hops[v] + 1;
+        for (uint64_t a = adj->offsets[v]; a < adj->offsets[v + 1]; a++) {
+            uint32_t w = adj->targets[a];
+            if (ws->hops[w] == U32_MAX_) {
+                ws->hops[w] = next;
+                ws->order[ws->norder++] = w;
+            }
+            if (ws->hops[w] == next) ws->sigma[w] += ws->sigma[v];
+        }
+    }
+    for (uint64_t i = ws->norder; i-- > 0;) {
+        uint32_t v = ws->order[i];
+        uint32_t next = ws->hops[v] + 1;
+        double dv = 0.0;
+        for (uint64_t a = adj->offsets[v]; a < adj->offsets[v + 1]; a++) {
+            uint32_t w = adj->targets[a];
+            if (ws->hops[w] == next) dv += ws->coef[w];
+        }
+        double delta = ws->sigma[v] * dv;
+        ws->coef[v] = (1.0 + delta) / ws->sigma[v];
+        credit(ws, v, s, delta);
+    }
+}
+
+static void reset_bfs(Workspace *ws) {
+    for (uint64_t i = 0; i < ws->norder; i++) {
+        uint32_t v = ws->order[i];
+        ws->hops[v] = U32_MAX_;
+        ws->sigma[v] = 0.0;
+        ws->coef[v] = 0.0;
+    }
+}
+
+/* Workspace::single_source_dial, betweenness.rs:214-295: Dial's circular
+ * buckets of width max + 1, exact integer distances, LIFO buckets. */
+static void single_source_dial(Workspace *ws, const OGraph *g, const uint32_t *lens,
+                               uint32_t max, uint64_t s) {
+    const OCsr *adj = &g->out;
+    uint64_t width = (uint64_t)max + 1;
+    if (ws->nbuckets != width) {
+        for (uint32_t i = 0; i < ws->nbuckets; i++) free(ws->buckets[i].a);
+        free(ws->buckets);
+        ws->buckets = calloc(width, sizeof(U32Vec));
+        ws->nbuckets = (uint32_t)width;
+    }
+    ws->norder = 0;
+    ws->idist[s] = 0;
+    ws->sigma[s] = 1.0;
+    vpush(&ws->buckets[0], (uint32_t)s);
+    uint64_t pending = 1;
+    ws->touched.len = 0;
+    vpush(&ws->touched, (uint32_t)s);
+    uint64_t cur = 0;
+    while (pending > 0) {
+        U32Vec *b = &ws->buckets[cur % width];
+        if (b->len == 0) { cur++; continue; }
+        uint32_t v = b->a[--b->len];
+        pending--;
+        if (ws->rank[v] != U32_MAX_ || ws->idist[v] != cur) continue;
+        ws->rank[v] = (uint32_t)ws->norder;
+        ws->order[ws->norder++] = v;
+        for (uint64_t k = adj->offsets[v]; k < adj->offsets[v + 1]; k++) {
+            uint32_t w = adj->targets[k];
+            if (ws->rank[w] != U32_MAX_) continue;
+            uint64_t alt = cur + (uint64_t)lens[k];
+            uint64_t d = ws->idist[w];
+            if (d == UINT64_MAX || alt < d) {
+                if (d == UINT64_MAX) vpush(&ws->touched, w);
+                ws->idist[w] = alt;
+                ws->sigma[w] = ws->sigma[v];
+                vpush(&ws->buckets[alt % width], w);
+                pending++;
+            } else if (alt == d) {
+                ws->sigma[w] += ws->sigma[v];
+            }
+        }
+    }
+    for (uint64_t i = ws->norder; i-- > 0;) {
+        uint32_t v = ws->order[i];
+        uint32_t rank_v = ws->rank[v];
+        uint64_t dist_v = ws->idist[v];
+        double dv = 0.0;
+        for (uint64_t k = adj->offsets[v]; k < adj->offsets[v + 1]; k++) {
+            uint32_t w = adj->targets[k];
+            if (ws->rank[w] != U32_MAX_ && ws->rank[w] > rank_v &&
+                dist_v + (uint64_t)lens[k] == ws->idist[w])
+                dv += ws->coef[w];
+        }
+        double delta = ws->sigma[v] * dv;
+        ws->coef[v] = (1.0 + delta) / ws->sigma[v];
+        credit(ws, v, s, delta);
+    }
+}
+
+static void reset_touched_int(Workspace *ws) {
+    for (uint64_t i = 0; i < ws->touched.len; i++) {
+        uint32_t v = ws->touched.a[i];
+        ws->idist[v] = UINT64_MAX;
+        ws->sigma[v] = 0.0;
+        ws->coef[v] = 0.0;
+        ws->rank[v] = U32_MAX_;
+    }
+}
+
+/* approx_eq with EPSILON = 1e-10, betweenness.rs:44, 117-120. */
+static inline int approx_eq(double a, double b) {
+    double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+    return fabs(a - b) <= 1e-10 * m;
+}
+
+/* Workspace::single_source_dijkstra, betweenness.rs:297-368. */
+static void single_source_dijkstra(Workspace *ws, const OGraph *g, const double *lens,
+                                   uint64_t s) {
+    const OCsr *adj = &g->out;
+    ws->norder = 0;
+    ws->dist[s] = 0.0;
+    ws->sigma[s] = 1.0;
+    h_push_or_decrease(&ws->heap, (uint32_t)s, 0.0);
+    ws->touched.len = 0;
+    vpush(&ws->touched, (uint32_t)s);
+    HItem top;
+    while (h_pop(&ws->heap, &top)) {
+        uint32_t v = top.v;
+        double d = top.d;
+        ws->rank[v] = (uint32_t)ws->norder;
+        ws->order[ws->norder++] = v;
+        for (uint64_t k = adj->offsets[v]; k < adj->offsets[v + 1]; k++) {
+            uint32_t w = adj->targets[k];
+            if (ws->rank[w] != U32_MAX_) continue;
+            double alt = d + lens[k];
+            double cur = ws->dist[w];
+            if (isinf(cur)) {
+                vpush(&ws->touched, w);
+                ws->dist[w] = alt;
+                ws->sigma[w] = ws->sigma[v];
+                h_push_or_decrease(&ws->heap, w, alt);
+            } else if (approx_eq(alt, cur)) {
+                ws->sigma[w] += ws->sigma[v];
+            } else if (alt < cur) {
+                ws->dist[w] = alt;
+                ws->sigma[w] = ws->sigma[v];
+                h_push_or_decrease(&ws->heap, w, alt);
+            }
+        }
+    }
+    for (uint64_t i = ws->norder; i-- > 0;) {
+        uint32_t v = ws->order[i];
+        uint32_t rank_v = ws->rank[v];
+        double dist_v = ws->dist[v];
+        double dv = 0.0;
+        for (uint64_t k = adj->offsets[v]; k < adj->offsets[v + 1]; k++) {
+            uint32_t w = adj->targets[k];
+            if (ws->rank[w] > rank_v && ws->rank[w] != U32_MAX_ &&
+                approx_eq(dist_v + lens[k], ws->dist[w]))
+                dv += ws->coef[w];
+        }
+        double delta = ws->sigma[v] * dv;
+        ws->coef[v] = (1.0 + delta) / ws->sigma[v];
+        credit(ws, v, s, delta);
+    }
+}
+
+static void reset_touched_real(Workspace *ws) {
+    for (uint64_t i = 0; i < ws->touched.len; i++) {
+        uint32_t v = ws->touched.a[i];
+        ws->dist[v] = INFINITY;
+        ws->sigma[v] = 0.0;
+        ws->coef[v] = 0.0;
+        ws->rank[v] = U32_MAX_;
+    }
+}
+
+static void run_source(Workspace *ws, const OGraph *g, const Lengths *L, uint64_t s) {
+    switch (L->kind) {
+    case LEN_UNIFORM:
+        single_source_bfs(ws, g, s);
+        reset_bfs(ws);
+        break;
+    case LEN_INTEGER:
+        single_source_dial(ws, g, L->ilens, L->max, s);
+        reset_touched_int(ws);
+        break;
+    default:
+        single_source_dijkstra(ws, g, L->rlens, s);
+        reset_touched_real(ws);
+        break;
+    }
+}
+
+/* ---- threads: parallel.rs:28-60 ---------------------------------------- */
+
+static uint64_t available_cores(void) {
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof set, &set) == 0) {
+        int c = CPU_COUNT(&set);
+        if (c > 0) return (uint64_t)c;
+    }
+    long c = sysconf(_SC_NPROCESSORS_ONLN);
+    return c > 0 ? (uint64_t)c : 1;
+}
+
+/* resolve_threads, parallel.rs:29-40. */
+static uint64_t resolve_threads(uint64_t requested, uint64_t items) {
+    uint64_t t = requested == 0 ? available_cores() : requested;
+    uint64_t hi = items > 1 ? items : 1;
+    if (t < 1) t = 1;
+    if (t > hi) t = hi;
+    return t;
+}
+
+#define GRAIN (1ull << 17)
+/* threads_for, parallel.rs:51-54; search_work, parallel.rs:58-60. */
+static uint64_t threads_for(uint64_t requested, uint64_t items, uint64_t work) {
+    uint64_t by_work = work / GRAIN;
+    if (by_work < 1) by_work = 1;
+    uint64_t t = resolve_threads(requested, items);
+    return t < by_work ? t : by_work;
+}
+
+static uint64_t search_work(uint64_t searches, uint64_t n, uint64_t arcs) {
+    unsigned __int128 w = (unsigned __int128)searches * (n + arcs);
+    return w > UINT64_MAX ? UINT64_MAX : (uint64_t)w;
+}
+
+int64_t orc_threads_for(int64_t requested, int64_t items, uint64_t work) {
+    return (int64_t)threads_for((uint64_t)requested, (uint64_t)items, work);
+}
+
+/* ---- sum_in_order: parallel.rs:379-503 --------------------------------- */
+
+#define SUM_GROUPS 64
+
+typedef struct {
+    const OGraph *g;
+    const Lengths *L;
+    const int64_t *sources; /* NULL: item i is source i */
+    uint64_t items, size, groups, window;
+    /* shared */
+    pthread_mutex_t mu;
+    pthread_cond_t turn;
+    uint64_t claim;
+    uint64_t next;
+    double **done;
+    double **pool;
+    uint64_t npool;
+    double *bc;
+} SumCtx;
+
+static void merge_group(SumCtx *c, double *acc) {
+    /* merge closure, betweenness.rs:91-96 */
+    for (uint64_t v = 0; v < c->g->n; v++) {
+        c->bc[v] += acc[v];
+        acc[v] = 0.0;
+    }
+}
+
+static void work_group(SumCtx *c, Workspace *ws, double *acc, uint64_t grp) {
+    uint64_t lo = grp * c->size, hi = (grp + 1) * c->size;
+    if (hi > c->items) hi = c->items;
+    ws->bc = acc;
+    for (uint64_t i = lo; i < hi; i++) {
+        uint64_t s = c->sources ? (uint64_t)c->sources[i] : i;
+        run_source(ws, c->g, c->L, s);
+    }
+    ws->bc = NULL;
+}
+
+static void *sum_worker(void *arg) {
+    SumCtx *c = arg;
+    Workspace ws;
+    ws_init(&ws, c->g->n);
+    for (;;) {
+        pthread_mutex_lock(&c->mu);
+        uint64_t grp = c->claim++;
+        if (grp >= c->groups) { pthread_mutex_unlock(&c->mu); break; }
+        while (grp >= c->next + c->window) pthread_cond_wait(&c->turn, &c->mu);
+        double *acc = c->npool ? c->pool[--c->npool] : NULL;
+        pthread_mutex_unlock(&c->mu);
+        if (!acc) acc = calloc(c->g->n ? c->g->n : 1, sizeof(double));
+        work_group(c, &ws, acc, grp);
+        pthread_mutex_lock(&c->mu);
+        c->done[grp] = acc;
+        int merged = 0;
+        while (c->next < c->groups && c->done[c->next]) {
+            double *a = c->done[c->next];
+            c->done[c->next] = NULL;
+            merge_group(c, a);
+            c->pool[c->npool++] = a;
+            c->next++;
+            merged = 1;
+        }
+        pthread_mutex_unlock(&c->mu);
+        if (merged) pthread_cond_broadcast(&c->turn);
+    }
+    ws_free(&ws);
+    return NULL;
+}
+
+static void sum_in_order(SumCtx *c, uint64_t threads) {
+    if (c->items == 0) return;
+    c->size = (c->items + SUM_GROUPS - 1) / SUM_GROUPS;
+    c->groups = (c->items + c->size - 1) / c->size;
+    threads = resolve_threads(threads, c->groups);
+    if (threads == 1) {
+        Workspace ws;
+        ws_init(&ws, c->g->n);
+        double *acc = calloc(c->g->n ? c->g->n : 1, sizeof(double));
+        for (uint64_t grp = 0; grp < c->groups; grp++) {
+            work_group(c, &ws, acc, grp);
+            merge_group(c, acc);
+        }
+        free(acc);
+        ws_free(&ws);
+        return;
+    }
+    c->window = 2 * threads;
+    pthread_mutex_init(&c->mu, NULL);
+    pthread_cond_init(&c->turn, NULL);
+    c->claim = 0;
+    c->next = 0;
+    c->done = calloc(c->groups, sizeof(double *));
+    c->pool = calloc(c->groups + threads + 1, sizeof(double *));
+    c->npool = 0;
+    pthread_t *tid = malloc(threads * sizeof *tid);
+    for (uint64_t t = 0; t < threads; t++) pthread_create(&tid[t], NULL, sum_worker, c);
+    for (uint64_t t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    for (uint64_t i = 0; i < c->npool; i++) free(c->pool[i]);
+    free(c->pool);
+    free(c->done);
+    free(tid);
+    pthread_mutex_destroy(&c->mu);
+    pthread_cond_destroy(&c->turn);
+}
+
+/* ---- betweenness: betweenness.rs:63-115 -------------------------------- */
+
+static void scale_scores(const OGraph *g, int normalized, double *bc) {
+    uint64_t n = g->n;
+    double scale = g->directed ? 1.0 : 0.5;
+    if (normalized && n > 2) {
+        double pairs = (double)((n - 1) * (n - 2));
+        scale *= g->directed ? 1.0 / pairs : 2.0 / pairs;
+    }
+    if (scale != 1.0)
+        for (uint64_t v = 0; v < n; v++) bc[v] *= scale;
+}
+
+static int check_kind(int32_t weights_as, char *err, size_t err_len) {
+    if (weights_as == 0 || weights_as == 1) return ORC_OK;
+    char msg[128];
+    snprintf(msg, sizeof msg, "unknown weight kind code %d", weights_as);
+    write_err(err, err_len, msg);
+    return ORC_FAILED;
+}
+
+/* net_betweenness, lib.rs:214-244 -> betweenness(), betweenness.rs:63-115.
+ * algorithm: 0 Auto, 1 Bfs, 2 Dial, 3 Dijkstra (Auto through the C ABI). */
+int32_t orc_betweenness_alg(const OGraph *g, int32_t weighted, int32_t weights_as,
+                            int32_t normalized, int32_t threads, int32_t algorithm, double *out,
+                            char *err, size_t err_len) {
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, algorithm, &L, err, err_len)) return ORC_FAILED;
+    uint64_t n = g->n;
+    memset(out, 0, n * sizeof *out);
+    SumCtx c;
+    memset(&c, 0, sizeof c);
+    c.g = g;
+    c.L = &L;
+    c.items = n;
+    c.bc = out;
+    uint64_t t = threads_for(threads > 0 ? (uint64_t)threads : 0, n,
+                             search_work(n, n, g->out.arcs));
+    sum_in_order(&c, t);
+    scale_scores(g, normalized, out);
+    lengths_free(&L);
+    return ORC_OK;
+}
+
+int32_t orc_betweenness(const OGraph *g, int32_t weighted, int32_t weights_as,
+                        int32_t normalized, int32_t threads, double *out, char *err,
+                        size_t err_len) {
+    return orc_betweenness_alg(g, weighted, weights_as, normalized, threads, 0, out, err, err_len);
+}
+
+/* The same computation restricted to a list of k sources: the per-source
+ * searches and the ordered group sum run over the list (items are list
+ * positions), and the result gets the same scaling as betweenness(). It is
+ * the comparator for sampled-source runs. */
+int32_t orc_betweenness_sources(const OGraph *g, const int64_t *sources, int64_t k,
+                                int32_t weighted, int32_t weights_as, int32_t normalized,
+                                int32_t threads, double *out, char *err, size_t err_len) {
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    for (int64_t i = 0; i < k; i++) {
+        if (sources[i] < 0 || (uint64_t)sources[i] >= g->n) {
+            write_err(err, err_len, "source out of range");
+            return ORC_FAILED;
+        }
+    }
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, 0, &L, err, err_len)) return ORC_FAILED;
+    memset(out, 0, g->n * sizeof *out);
+    SumCtx c;
+    memset(&c, 0, sizeof c);
+    c.g = g;
+    c.L = &L;
+    c.items = (uint64_t)k;
+    c.sources = sources;
+    c.bc = out;
+    uint64_t t = threads_for(threads > 0 ? (uint64_t)threads : 0, (uint64_t)k,
+                             search_work((uint64_t)k, g->n, g->out.arcs));
+    sum_in_order(&c, t);
+    scale_scores(g, normalized, out);
+    lengths_free(&L);
+    return ORC_OK;
+}
+
+/* One search from s, exposing its state: hops (BFS) or integer distance
+ * (Dial) in dist_u64 (UINT64_MAX when unreached), real distance in
+ * dist_f64 (Dijkstra, INFINITY when unreached), path counts in sigma and
+ * the dependency of every reached node in delta (0 elsewhere). Returns the
+ * number of reached nodes in *reached and the length kind in *kind. */
+int32_t orc_single_source(const OGraph *g, int32_t weighted, int32_t weights_as, int64_t s,
+                          int64_t *reached, int32_t *kind, uint64_t *dist_u64, double *dist_f64,
+                          double *sigma, double *delta, char *err, size_t err_len) {
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    if (s < 0 || (uint64_t)s >= g->n) { write_err(err, err_len, "source out of range"); return ORC_FAILED; }
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, 0, &L, err, err_len)) return ORC_FAILED;
+    uint64_t n = g->n;
+    Workspace ws;
+    ws_init(&ws, n);
+    for (uint64_t v = 0; v < n; v++) {
+        delta[v] = 0.0;
+        sigma[v] = 0.0;
+        if (dist_u64) dist_u64[v] = UINT64_MAX;
+        if (dist_f64) dist_f64[v] = INFINITY;
+    }
+    ws.delta_out = delta;
+    switch (L.kind) {
+    case LEN_UNIFORM: single_source_bfs(&ws, g, (uint64_t)s); break;
+    case LEN_INTEGER: single_source_dial(&ws, g, L.ilens, L.max, (uint64_t)s); break;
+    default: single_source_dijkstra(&ws, g, L.rlens, (uint64_t)s); break;
+    }
+    for (uint64_t i = 0; i < ws.norder; i++) {
+        uint32_t v = ws.order[i];
+        sigma[v] = ws.sigma[v];
+        if (L.kind == LEN_UNIFORM && dist_u64) dist_u64[v] = ws.hops[v];
+        if (L.kind == LEN_INTEGER && dist_u64) dist_u64[v] = ws.idist[v];
+        if (L.kind == LEN_REAL && dist_f64) dist_f64[v] = ws.dist[v];
+    }
+    *reached = (int64_t)ws.norder;
+    *kind = L.kind;
+    ws_free(&ws);
+    lengths_free(&L);
+    return ORC_OK;
+}
+
+/* Test hook for sum_in_order's determinism: sums 1/(1+i) (x1e8 when
+ * i % 3 == 0) over items in 64 ordered groups, as parallel.rs:567-585. */
+typedef struct { uint64_t items, size, groups, window, claim, next, npool; double total;
+                 double *done_v; int *done_f; pthread_mutex_t mu; pthread_cond_t turn; } TCtx;
+static void *t_worker(void *arg) {
+    TCtx *c = arg;
+    for (;;) {
+        pthread_mutex_lock(&c->mu);
+        uint64_t g = c->claim++;
+        if (g >= c->groups) { pthread_mutex_unlock(&c->mu); break; }
+        while (g >= c->next + c->window) pthread_cond_wait(&c->turn, &c->mu);
+        pthread_mutex_unlock(&c->mu);
+        double acc = 0.0;
+        uint64_t lo = g * c->size, hi = (g + 1) * c->size;
+        if (hi > c->items) hi = c->items;
+        for (uint64_t i = lo; i < hi; i++)
+            acc += 1.0 / (1.0 + (double)i) * (i % 3 == 0 ? 1e8 : 1.0);
+        pthread_mutex_lock(&c->mu);
+        c->done_v[g] = acc;
+        c->done_f[g] = 1;
+        int merged = 0;
+        while (c->next < c->groups && c->done_f[c->next]) {
+            c->total += c->done_v[c->next];
+            c->next++;
+            merged = 1;
+        }
+        pthread_mutex_unlock(&c->mu);
+        if (merged) pthread_cond_broadcast(&c->turn);
+    }
+    return NULL;
+}
+
+double orc_ordered_sum_test(uint64_t items, uint64_t threads) {
+    if (items == 0) return 0.0;
+    TCtx c;
+    memset(&c, 0, sizeof c);
+    c.items = items;
+    c.size = (items + SUM_GROUPS - 1) / SUM_GROUPS;
+    c.groups = (items + c.size - 1) / c.size;
+    threads = resolve_threads(threads, c.groups);
+    c.window = 2 * threads;
+    c.done_v = calloc(c.groups, sizeof(double));
+    c.done_f = calloc(c.groups, sizeof(int));
+    pthread_mutex_init(&c.mu, NULL);
+    pthread_cond_init(&c.turn, NULL);
+    pthread_t *tid = malloc(threads * sizeof *tid);
+    for (uint64_t t = 0; t < threads; t++) pthread_create(&tid[t], NULL, t_worker, &c);
+    for (uint64_t t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+    free(tid); free(c.done_v); free(c.done_f);
+    pthread_mutex_destroy(&c.mu);
+    pthread_cond_destroy(&c.turn);
+    return c.total;
+}
+
+uint64_t orc_available_cores(void) { return available_cores(); }

@@ -1,0 +1,91 @@
+"""Writes tests/golden/fixtures.json from the reference's validation data.
+
+The reference keeps 28 small networks (validation/fixtures/<name>/edges.csv
+and meta.json) and the values its oracles produced for them
+(validation/golden/{definition,networkx}/<name>/<variant>.csv, one
+``%.17g`` value per line under a ``value`` header; see
+validation/common.py:341-352). This script copies the edge lists, the
+metadata and the betweenness goldens into one JSON file, as data. Nothing
+of the reference is imported or run.
+
+Run once in a tree where the reference is checked out:
+
+    python tests/golden/make_fixtures.py /root/reference
+
+The output is committed; tests read only the JSON file.
+"""
+import csv
+import json
+import sys
+from pathlib import Path
+
+VARIANTS = [
+    "betweenness",
+    "betweenness_normalized",
+    "betweenness_weighted",
+    "betweenness_weighted_strength",
+    "betweenness_networkx",
+    "betweenness_networkx_weighted",
+    "edge_betweenness",
+    "edge_betweenness_normalized",
+    "edge_betweenness_weighted",
+    "edge_betweenness_weighted_strength",
+]
+ORACLES = ["definition", "networkx"]
+
+
+def read_values(path: Path):
+    with open(path, newline="") as fh:
+        rows = list(csv.reader(fh))
+    assert rows[0] == ["value"], path
+    out = []
+    for r in rows[1:]:
+        v = r[0]
+        out.append(None if v in ("NA", "") else float(v))
+    return out
+
+
+def main(ref: Path, dest: Path) -> None:
+    val = ref / "validation"
+    fixtures = {}
+    for d in sorted((val / "fixtures").iterdir()):
+        if not (d / "meta.json").exists():
+            continue
+        meta = json.loads((d / "meta.json").read_text())
+        src, tgt, w = [], [], []
+        with open(d / "edges.csv", newline="") as fh:
+            for row in csv.DictReader(fh):
+                src.append(int(row["from"]))
+                tgt.append(int(row["to"]))
+                if meta["weighted"]:
+                    w.append(float(row["weight"]))
+        golden = {}
+        for oracle in ORACLES:
+            for var in VARIANTS:
+                p = val / "golden" / oracle / d.name / f"{var}.csv"
+                if p.exists():
+                    golden.setdefault(var, {})[oracle] = read_values(p)
+        fixtures[d.name] = {
+            "n": meta["n"],
+            "directed": meta["directed"],
+            "weighted": meta["weighted"],
+            "tags": meta["tags"],
+            "src": src,
+            "tgt": tgt,
+            "weights": w if meta["weighted"] else None,
+            "golden": golden,
+        }
+    version = (val / "golden" / "networkx" / "VERSION").read_text().strip()
+    doc = {
+        "about": "knappnet validation fixtures and betweenness goldens, copied as data",
+        "networkx_version": version,
+        "tolerance": {"rel": 1e-9, "abs": 1e-9, "abs_normalized": 1e-12},
+        "fixtures": fixtures,
+    }
+    dest.write_text(json.dumps(doc, separators=(",", ":")) + "\n")
+    print(f"wrote {len(fixtures)} fixtures to {dest}")
+
+
+if __name__ == "__main__":
+    ref = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    main(ref, Path(__file__).resolve().parent / "fixtures.json")

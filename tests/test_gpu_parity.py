@@ -1,0 +1,253 @@
+"""Parity of the HIP path with the CPU oracle, through the C ABI.
+
+The bar: device CSR, levels/distances and sigma bit-exact; FP64 scores
+bit-exact against the oracle's restatement of the reference (same operation
+order; the reference's own tolerance for the committed goldens is rel 1e-9,
+abs 1e-9 / 1e-12 normalised). FP32-dependency runs: max relative error
+<= 1e-6 against FP64, stated in the test.
+"""
+import sys
+
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import breadth_first_order
+
+from _common import VARIANTS, close, random_case, tolerance
+from knappnet_amd import generate as gen
+from knappnet_amd.engine import EngineError, Graph
+from oracle.oracle import OracleGraph
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pair(n, s, t, w=None, directed=False):
+    return Graph(n, s, t, w, directed), OracleGraph(n, s, t, w, directed)
+
+
+def assert_same(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    if not np.array_equal(a, b):
+        i = int(np.argmax(a != b))
+        raise AssertionError(f"{what}: first difference at {i}: {a[i]!r} vs {b[i]!r}; "
+                             f"{int(np.sum(a != b))} differ")
+
+
+def test_device_info():
+    from knappnet_amd import device_info
+    info = device_info()
+    say("device:", info)
+    assert "gfx" in info
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_device_csr_matches_host_build(directed):
+    rng = np.random.default_rng(11)
+    for case in range(10):
+        n = int(rng.integers(1, 3000))
+        m = int(rng.integers(0, 5 * n))
+        s = rng.integers(0, n, m)
+        t = rng.integers(0, n, m)
+        w = rng.random(m) + 0.5
+        say("csr case", case, n, m, directed)
+        g, o = pair(n, s, t, w, directed)
+        for incoming in (False, True):
+            off, tgt, ww = g.csr(incoming)
+            ooff, otgt, ow = o.csr(incoming)
+            assert_same(off.astype(np.uint64), ooff, "offsets")
+            assert_same(tgt, otgt, "targets")
+            assert_same(ww, ow, "weights")
+
+
+def test_fixtures_bitexact_and_goldens(fixtures):
+    for name, fx in fixtures.items():
+        g, o = pair(fx["n"], fx["src"], fx["tgt"], fx["weights"], fx["directed"])
+        for variant, spec in VARIANTS.items():
+            if variant not in fx["golden"]:
+                continue
+            say("fixture", name, variant)
+            got = g.betweenness(**spec)
+            want = o.betweenness(**spec)
+            assert_same(got, want, f"{name} {variant}")
+            rel, ab = tolerance(variant)
+            assert close(got, fx["golden"][variant]["definition"], rel, ab), (name, variant)
+
+
+def test_reference_known_answers():
+    cases = [
+        (dict(n=4, s=[0, 1, 2], t=[1, 2, 3]), {}, [0.0, 2.0, 2.0, 0.0]),
+        (dict(n=3, s=[0, 1, 2], t=[1, 2, 0], directed=True), {}, [1.0, 1.0, 1.0]),
+        (dict(n=4, s=[0, 0, 1, 2], t=[1, 2, 3, 3]), {}, [0.5] * 4),
+        (dict(n=3, s=[0, 1, 0], t=[1, 2, 2], w=[1.0, 1.0, 5.0]), {"weighted": True},
+         [0.0, 1.0, 0.0]),
+        (dict(n=3, s=[0, 1, 0], t=[1, 2, 2], w=[1.0, 1.0, 4.0]),
+         {"weighted": True, "weights_as": "strength"}, [0.0, 0.0, 0.0]),
+    ]
+    for graph, kw, want in cases:
+        g = Graph(graph["n"], graph["s"], graph["t"], graph.get("w"),
+                  graph.get("directed", False))
+        assert list(g.betweenness(**kw)) == want
+    g = Graph(5, [0, 0, 0, 0], [1, 2, 3, 4])
+    r = g.betweenness(normalized=True)
+    assert abs(r[0] - 1.0) < 1e-12 and all(x == 0.0 for x in r[1:])
+    g = Graph(4, [0, 1, 0, 2], [1, 3, 2, 3], [0.1, 0.2, 0.15, 0.15], directed=True)
+    r = g.betweenness(weighted=True)
+    assert abs(r[1] - 0.5) < 1e-12 and abs(r[2] - 0.5) < 1e-12
+
+
+def test_device_error_paths():
+    g = Graph(2, [0], [1], [0.0], True)
+    with pytest.raises(EngineError, match=r"^edge 0 has weight 0; this algorithm treats "
+                                          r"weights as distances and requires them to be "
+                                          r"strictly positive$"):
+        g.betweenness(weighted=True)
+    g = Graph(3, [0, 1], [1, 2], [2.5, -1e-7], True)
+    with pytest.raises(EngineError, match=r"^edge 1 has weight -0.0000001;"):
+        g.betweenness(weighted=True, weights_as="strength")
+    g = Graph(0, [], [], None, False)
+    assert len(g.betweenness()) == 0
+    g = Graph(3, [], [], None, True)
+    assert list(g.betweenness(normalized=True)) == [0.0, 0.0, 0.0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_multigraphs_bitexact(seed):
+    rng = np.random.default_rng(500 + seed)
+    for case in range(12):
+        n, s, t, directed = random_case(rng, max_n=60)
+        w = (1 + rng.integers(0, 9, len(s))).astype(np.float64)
+        say("random", seed, case, n, len(s), directed)
+        g, o = pair(n, s, t, w, directed)
+        for kw in ({}, {"normalized": True}, {"weighted": True},
+                   {"weighted": True, "weights_as": "strength"}):
+            assert_same(g.betweenness(**kw), o.betweenness(**kw), f"{seed}/{case} {kw}")
+
+
+def test_real_lengths_bitexact():
+    rng = np.random.default_rng(3)
+    for case in range(8):
+        n, s, t, directed = random_case(rng, max_n=80)
+        w = rng.random(len(s)) * 3.5 + 0.5
+        say("real", case, n, len(s), directed)
+        g, o = pair(n, s, t, w, directed)
+        assert_same(g.betweenness(weighted=True), o.betweenness(weighted=True), "real")
+
+
+def test_er_2000_bitexact():
+    el = gen.er_2000()
+    say("er_2000")
+    g, o = pair(el.n, el.src, el.tgt, None, False)
+    got, diag = g.betweenness_sources()
+    want = o.betweenness(threads=0)
+    assert_same(got, want, "er_2000")
+    assert_same(g.betweenness(normalized=True), o.betweenness(normalized=True, threads=0),
+                "er_2000 normalized")
+    assert diag["sigma_overflow"] == 0
+    # reached nodes / arcs: every node's component, summed over sources
+    A = csr_matrix((np.ones(el.m), (el.src, el.tgt)), shape=(el.n, el.n))
+    A = A + A.T
+    deg = np.diff(A.indptr)
+    from scipy.sparse.csgraph import connected_components
+    ncomp, lab = connected_components(A, directed=False)
+    size = np.bincount(lab)
+    arcs = np.bincount(lab, weights=deg)
+    assert diag["reached_nodes"] == int(np.sum(size[lab]))
+    assert diag["reached_arcs"] == int(np.sum(arcs[lab]))
+
+
+@pytest.mark.parametrize("directed", [True, False])
+def test_rmat_bitexact(directed):
+    el = gen.rmat(12, 16, directed=directed, seed=5)
+    say("rmat12", directed, el.m)
+    g, o = pair(el.n, el.src, el.tgt, None, directed)
+    assert_same(g.betweenness(), o.betweenness(threads=0), "rmat12")
+
+
+def test_dial_ba_bitexact():
+    el = gen.ba_weighted(n=3000, m=4, seed=5)
+    say("ba3000", el.m)
+    g, o = pair(el.n, el.src, el.tgt, el.weights, False)
+    got, diag = g.betweenness_sources(weighted=True)
+    assert diag["path_kind"] == 1
+    assert_same(got, o.betweenness(weighted=True, threads=0), "ba3000")
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_state_dump_bitexact(weighted):
+    rng = np.random.default_rng(77)
+    for case in range(6):
+        n, s, t, directed = random_case(rng, max_n=120)
+        w = (1 + rng.integers(0, 7, len(s))).astype(np.float64)
+        g, o = pair(n, s, t, w, directed)
+        srcs = np.arange(n)
+        say("dump", weighted, case, n, len(s), directed)
+        dist, sig, dl = g.state_dump(srcs, weighted=weighted)
+        for src in srcs:
+            kind, du, _, osig, odl, _ = o.single_source(src, weighted=weighted)
+            assert_same(dist[src], du, f"dist {case}/{src}")
+            assert_same(sig[src], osig, f"sigma {case}/{src}")
+            odl = odl.copy()
+            odl[src] = 0.0
+            assert_same(dl[src], odl, f"delta {case}/{src}")
+
+
+def test_slots_do_not_change_bits():
+    el = gen.rmat(11, 8, directed=False, seed=2)
+    g = Graph(el.n, el.src, el.tgt, None, False)
+    base, _ = g.betweenness_sources(slots=1)
+    for slots in (2, 3, 7, 32):
+        say("slots", slots)
+        got, d = g.betweenness_sources(slots=slots)
+        assert_same(got, base, f"slots={slots}")
+
+
+def test_source_list_matches_oracle():
+    el = gen.rmat(11, 8, directed=True, seed=3)
+    srcs = gen.sample_sources(el.n, 300, 7)
+    g, o = pair(el.n, el.src, el.tgt, None, True)
+    say("source list")
+    got, _ = g.betweenness_sources(srcs)
+    assert_same(got, o.betweenness_sources(srcs, threads=0), "sources")
+    full, _ = g.betweenness_sources(np.arange(el.n))
+    assert_same(full, g.betweenness(), "full list")
+
+
+def test_fp32_dependencies_within_1e6():
+    # FP32 coefficients with Kahan-compensated successor sums; sigma and the
+    # cross-source sum in FP64. Tolerance: max relative error <= 1e-6
+    # against the FP64 result on nodes with non-zero score.
+    for directed, scale in ((False, 12), (True, 12)):
+        el = gen.rmat(scale, 16, directed=directed, seed=9)
+        g = Graph(el.n, el.src, el.tgt, None, directed)
+        say("fp32", directed)
+        f64, _ = g.betweenness_sources(precision=0)
+        f32, _ = g.betweenness_sources(precision=1)
+        nz = f64 > 0
+        rel = np.abs(f32[nz] - f64[nz]) / f64[nz]
+        say("fp32 max rel err", float(rel.max()), "mean", float(rel.mean()))
+        assert rel.max() <= 1e-6
+        assert np.all(f32[~nz] == 0)
+
+
+def test_sigma_overflow_is_flagged():
+    # 60 diamonds in series: 2^60 shortest paths end to end.
+    k = 60
+    s, t = [], []
+    for i in range(k):
+        a, b, c, d = 3 * i, 3 * i + 1, 3 * i + 2, 3 * i + 3
+        s += [a, a, b, c]
+        t += [b, c, d, d]
+    n = 3 * k + 1
+    g = Graph(n, s, t, None, True)
+    _, diag = g.betweenness_sources()
+    assert diag["sigma_overflow"] == 1 and diag["max_sigma"] == 2.0**60
+    g = Graph(n, s[: 4 * 20], t[: 4 * 20], None, True)
+    _, diag = g.betweenness_sources()
+    assert diag["sigma_overflow"] == 0 and diag["max_sigma"] == 2.0**20

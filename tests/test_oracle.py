@@ -1,0 +1,328 @@
+"""Pins the CPU oracle (oracle/netbc_oracle.c) before it is trusted.
+
+Checks, in order of strength:
+- the reference's committed goldens for all 28 fixtures (definition oracle
+  and NetworkX, with the reference's documented NetworkX differences);
+- the reference's unit known answers (betweenness.rs:387-517,
+  graph.rs:354-412, weights.rs:290-321, parallel.rs:644-665);
+- an exact-rational evaluation of the definition (tests/definition.py) on
+  random multigraphs, weighted and unweighted;
+- closed forms and the distance-sum identity;
+- the reference's randomised invariants (tests/properties.rs:61-250).
+"""
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import shortest_path
+
+import definition
+from _common import VARIANTS, close, networkx_documented, random_case, tolerance
+from oracle.oracle import (OracleError, OracleGraph, available_cores, ordered_sum_bits,
+                           threads_for)
+
+
+def og(fx):
+    return OracleGraph(fx["n"], fx["src"], fx["tgt"], fx["weights"], fx["directed"])
+
+
+# ---- committed goldens -------------------------------------------------------
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_goldens_definition(fixtures, variant):
+    rel, ab = tolerance(variant)
+    checked = 0
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"]:
+            continue
+        g = og(fx)
+        for threads in (1, 4):
+            got = g.betweenness(threads=threads, **VARIANTS[variant])
+            assert close(got, fx["golden"][variant]["definition"], rel, ab), (name, variant)
+        checked += 1
+    assert checked >= 7
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_goldens_networkx(fixtures, variant):
+    rel, ab = tolerance(variant)
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"] or "networkx" not in fx["golden"][variant]:
+            continue
+        got = og(fx).betweenness(**VARIANTS[variant])
+        ok = close(got, fx["golden"][variant]["networkx"], rel, ab)
+        if networkx_documented(fx, variant):
+            continue  # documented difference; the definition golden rules
+        assert ok, (name, variant)
+
+
+# ---- unit known answers: betweenness.rs:387-517 -------------------------------
+
+def bc(n, s, t, w=None, directed=False, **kw):
+    return OracleGraph(n, s, t, w, directed).betweenness(**kw)
+
+
+def test_path_graph():
+    assert list(bc(4, [0, 1, 2], [1, 2, 3])) == [0.0, 2.0, 2.0, 0.0]
+
+
+def test_star_normalized_is_one():
+    r = bc(5, [0, 0, 0, 0], [1, 2, 3, 4], normalized=True)
+    assert abs(r[0] - 1.0) < 1e-12 and all(x == 0.0 for x in r[1:])
+
+
+def test_directed_cycle():
+    assert list(bc(3, [0, 1, 2], [1, 2, 0], directed=True)) == [1.0, 1.0, 1.0]
+
+
+def test_split_paths_share_credit():
+    assert list(bc(4, [0, 0, 1, 2], [1, 2, 3, 3])) == [0.5, 0.5, 0.5, 0.5]
+
+
+def test_weights_change_paths():
+    assert list(bc(3, [0, 1, 0], [1, 2, 2], [1.0, 1.0, 5.0], weighted=True)) == [0.0, 1.0, 0.0]
+
+
+def test_tied_float_paths_are_detected():
+    r = bc(4, [0, 1, 0, 2], [1, 3, 2, 3], [0.1, 0.2, 0.15, 0.15], directed=True, weighted=True)
+    assert abs(r[1] - 0.5) < 1e-12 and abs(r[2] - 0.5) < 1e-12
+
+
+def test_rejects_non_positive_weights():
+    g = OracleGraph(2, [0], [1], [0.0], True)
+    with pytest.raises(OracleError, match=r"^edge 0 has weight 0; this algorithm treats "
+                                          r"weights as distances and requires them to be "
+                                          r"strictly positive$"):
+        g.betweenness(weighted=True)
+    g = OracleGraph(3, [0, 1], [1, 2], [2.5, -1e-7], True)
+    with pytest.raises(OracleError, match=r"^edge 1 has weight -0.0000001;"):
+        g.betweenness(weighted=True)
+
+
+def test_every_algorithm_agrees_on_integer_weights():
+    s, t = [0, 0, 1, 2, 1, 3, 4], [1, 2, 3, 3, 4, 5, 5]
+    w = [2.0, 1.0, 1.0, 2.0, 3.0, 2.0, 1.0]
+    g = OracleGraph(6, s, t, w, False)
+    dial = g.betweenness(weighted=True, algorithm="dial")
+    assert np.array_equal(dial, g.betweenness(weighted=True, algorithm="dijkstra"))
+    assert np.array_equal(dial, g.betweenness(weighted=True, algorithm="auto"))
+
+
+def test_strengths_are_inverted():
+    g = OracleGraph(3, [0, 1, 0], [1, 2, 2], [1.0, 1.0, 4.0], False)
+    assert list(g.betweenness(weighted=True, weights_as="strength")) == [0.0, 0.0, 0.0]
+    assert list(g.betweenness(weighted=True)) == [0.0, 1.0, 0.0]
+
+
+# ---- graph build: graph.rs:354-412 -------------------------------------------
+
+def rows(off, tgt, v):
+    return list(tgt[off[v]:off[v + 1]])
+
+
+def test_directed_rows_are_sorted_and_complete():
+    g = OracleGraph(4, [2, 0, 0, 1, 0], [1, 3, 1, 2, 1], None, True)
+    off, tgt, _ = g.csr()
+    assert rows(off, tgt, 0) == [1, 1, 3]
+    assert rows(off, tgt, 1) == [2]
+    assert rows(off, tgt, 2) == [1]
+    off, tgt, _ = g.csr(incoming=True)
+    assert rows(off, tgt, 1) == [0, 0, 2]
+    assert rows(off, tgt, 3) == [0]
+
+
+def test_undirected_stores_both_directions_and_loops_once():
+    g = OracleGraph(3, [0, 1, 2], [1, 2, 2], [1.0, 2.0, 3.0], False)
+    off, tgt, w = g.csr()
+    assert rows(off, tgt, 1) == [0, 2] and list(w[off[1]:off[2]]) == [1.0, 2.0]
+    assert rows(off, tgt, 2) == [1, 2] and list(w[off[2]:off[3]]) == [2.0, 3.0]
+    assert g.arc_count() == 5
+
+
+def test_parallel_arcs_keep_input_order():
+    g = OracleGraph(3, [0, 0, 0, 0], [2, 1, 2, 1], [5.0, 6.0, 7.0, 8.0], True)
+    off, tgt, w = g.csr()
+    assert rows(off, tgt, 0) == [1, 1, 2, 2]
+    assert list(w[:4]) == [6.0, 8.0, 5.0, 7.0]
+
+
+def test_rejects_bad_input():
+    with pytest.raises(OracleError, match=r"^edge 0 refers to node index 2, but valid "
+                                          r"indices are 0..2$"):
+        OracleGraph(2, [0], [2], None, True)
+    with pytest.raises(OracleError, match=r"^edge 0 has a missing, NaN, or infinite weight$"):
+        OracleGraph(2, [0], [1], [float("nan")], True)
+    with pytest.raises(OracleError, match=r"^negative node or edge count$"):
+        OracleGraph(-1, [], [], None, True)
+
+
+# ---- path lengths: weights.rs:290-321 -----------------------------------------
+
+def chain(w):
+    n = len(w) + 1
+    return OracleGraph(n, list(range(len(w))), list(range(1, n)), w, True)
+
+
+def test_chooses_the_cheapest_exact_algorithm():
+    assert chain([2.5, 2.5]).path_lengths() == ("uniform", 0)
+    assert chain([1.0, 3.0]).path_lengths() == ("integer", 3)
+    assert chain([1.0, 5000.0]).path_lengths()[0] == "real"
+    assert chain([0.5, 2.0]).path_lengths()[0] == "real"
+    assert chain([0.5, 2.0]).path_lengths(weighted=False) == ("uniform", 0)
+    assert chain([1.0, 0.5]).path_lengths(weights_as="strength") == ("integer", 2)
+    with pytest.raises(OracleError):
+        chain([0.5]).path_lengths(algorithm="dial")
+    with pytest.raises(OracleError):
+        chain([0.0]).path_lengths(weights_as="strength")
+    with pytest.raises(OracleError, match=r"^the graph has no edge weights$"):
+        OracleGraph(2, [0], [1], None, True).path_lengths()
+
+
+# ---- parallel.rs:644-665 -------------------------------------------------------
+
+def test_threads_follow_the_work():
+    G = 1 << 17
+    assert threads_for(8, 100, 10) == 1
+    assert threads_for(8, 100, 3 * G) == 3
+    assert threads_for(8, 2, 100 * G) == 2
+    assert threads_for(8, 100, 2**64 - 1) == min(8, 100)
+    assert threads_for(1, 100, 2**64 - 1) == 1
+    assert threads_for(0, 100, 2**64 - 1) == min(available_cores(), 100)
+
+
+def test_sums_in_order_for_every_thread_count():
+    for items in (0, 1, 5, 63, 64, 65, 1000, 4097):
+        one = ordered_sum_bits(items, 1)
+        for t in (2, 3, 8, 100):
+            assert ordered_sum_bits(items, t) == one, (items, t)
+
+
+# ---- exact definition on random graphs ----------------------------------------
+
+@pytest.mark.parametrize("seed", range(40))
+def test_definition_random(seed):
+    rng = np.random.default_rng(1000 + seed)
+    n, s, t, directed = random_case(rng, max_n=14)
+    w = rng.integers(1, 5, size=len(s)).astype(np.float64)
+    g = OracleGraph(n, s, t, w, directed)
+    for normalized in (False, True):
+        want = definition.betweenness(n, s, t, w, directed, normalized=normalized)
+        assert close(g.betweenness(normalized=normalized), want, 1e-12, 1e-12)
+    want = definition.betweenness(n, s, t, w, directed, weighted=True)
+    assert close(g.betweenness(weighted=True), want, 1e-12, 1e-12)
+    want = definition.betweenness(n, s, t, w, directed, weighted=True, strength=True)
+    assert close(g.betweenness(weighted=True, weights_as="strength"), want, 1e-9, 1e-9)
+
+
+# ---- closed forms and the distance-sum identity -------------------------------
+
+def test_closed_forms():
+    for n in (2, 3, 7, 50):
+        r = bc(n, list(range(n - 1)), list(range(1, n)))
+        assert list(r) == [float(i * (n - 1 - i)) for i in range(n)]
+    n = 9
+    r = bc(n, [0] * (n - 1), list(range(1, n)))
+    assert r[0] == (n - 1) * (n - 2) / 2 and all(x == 0 for x in r[1:])
+    s, t = zip(*[(i, j) for i in range(6) for j in range(i + 1, 6)])
+    assert all(x == 0 for x in bc(6, s, t))
+    n = 11
+    r = bc(n, list(range(n)), [(i + 1) % n for i in range(n)], directed=True)
+    assert all(x == (n - 1) * (n - 2) / 2 for x in r)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_distance_sum_identity(seed):
+    # sum_v b(v) = sum over ordered reachable pairs s != t of (d(s,t) - 1),
+    # halved for undirected graphs (each interior node of each shortest
+    # path gets the path's share; shares of one pair sum to d - 1).
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(20, 200))
+    m = int(rng.integers(n, 4 * n))
+    s = rng.integers(0, n, m)
+    t = rng.integers(0, n, m)
+    directed = bool(seed % 2)
+    g = OracleGraph(n, s, t, None, directed)
+    A = csr_matrix((np.ones(m), (s, t)), shape=(n, n))
+    D = shortest_path(A, directed=directed, unweighted=True)
+    fin = np.isfinite(D) & (D > 0)
+    total = float(np.sum(D[fin] - 1))
+    if not directed:
+        total /= 2
+    assert close([g.betweenness().sum()], [total], 1e-12, 1e-9)
+
+
+# ---- randomised invariants: crates/engine/tests/properties.rs ----------------
+
+CASES = 60
+
+
+def test_relabeling_nodes_permutes_results():
+    rng = np.random.default_rng(1)
+    for _ in range(CASES):
+        n, s, t, directed = random_case(rng)
+        perm = rng.permutation(n)
+        a = bc(n, s, t, directed=directed)
+        b = bc(n, perm[s], perm[t], directed=directed)
+        assert close(a, b[perm])
+
+
+def test_unit_weights_match_unweighted():
+    rng = np.random.default_rng(2)
+    for _ in range(CASES):
+        n, s, t, directed = random_case(rng)
+        g = OracleGraph(n, s, t, np.ones(len(s)), directed)
+        assert close(g.betweenness(), g.betweenness(weighted=True))
+
+
+def test_results_do_not_depend_on_thread_count():
+    rng = np.random.default_rng(3)
+    for _ in range(CASES):
+        n, s, t, directed = random_case(rng)
+        g = OracleGraph(n, s, t, None, directed)
+        assert np.array_equal(g.betweenness(threads=1), g.betweenness(threads=4))
+
+
+def test_undirected_betweenness_matches_symmetric_directed():
+    rng = np.random.default_rng(5)
+    for _ in range(CASES):
+        n, s, t, _ = random_case(rng)
+        keep = s != t
+        s2 = np.concatenate([s, t[keep]])
+        t2 = np.concatenate([t, s[keep]])
+        bu = bc(n, s, t)
+        bd = bc(n, s2, t2, directed=True) / 2
+        assert close(bu, bd)
+
+
+def test_every_path_algorithm_agrees():
+    rng = np.random.default_rng(6)
+    for _ in range(CASES):
+        n, s, t, directed = random_case(rng)
+        w = (1 + rng.integers(0, 6, len(s))).astype(np.float64)
+        g = OracleGraph(n, s, t, w, directed)
+        dial = g.betweenness(weighted=True, algorithm="dial")
+        heap = g.betweenness(weighted=True, algorithm="dijkstra")
+        assert close(dial, heap)
+        assert close(dial, g.betweenness(weighted=True))
+        h = OracleGraph(n, s, t, 1.0 / w, directed)
+        assert close(g.betweenness(weighted=True, weights_as="strength"),
+                     h.betweenness(weighted=True))
+
+
+def test_uniform_weights_match_unweighted_under_every_algorithm():
+    rng = np.random.default_rng(7)
+    for _ in range(CASES):
+        n, s, t, directed = random_case(rng)
+        k = float(1 + rng.integers(0, 9))
+        g = OracleGraph(n, s, t, np.full(len(s), k), directed)
+        plain = g.betweenness()
+        for alg in ("auto", "dial", "dijkstra"):
+            assert close(plain, g.betweenness(weighted=True, algorithm=alg))
+
+
+def test_full_source_list_is_ordinary_betweenness():
+    # path_centrality.rs:955-997: the full source list gives betweenness().
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        n, s, t, directed = random_case(rng)
+        g = OracleGraph(n, s, t, None, directed)
+        assert np.array_equal(g.betweenness_sources(np.arange(n)), g.betweenness())
