@@ -1,0 +1,201 @@
+"""Benchmark: exact betweenness GTEPS (sources x edges / s) on one MI355X.
+
+Default workload: directed R-MAT scale 18 (n = 262,144, edge factor 16,
+a,b,c,d = 0.57,0.19,0.19,0.05, seed 1, loops dropped, duplicates removed,
+isolates kept; m = 3,939,372), every node a source, FP64 sigma, dependencies
+and scores. One step = one full all-sources betweenness through the C ABI
+(``netbc_betweenness_sources``), with the graph already resident on the
+device (the warm-up call builds the device CSR).
+
+The printed JSON line carries:
+- ``roofline`` for the dominant kernel (the dependency sweep), from its
+  launch durations measured with HIP events on the stream it runs on and
+  its algorithmic bytes (8 B per reached out-arc: column index and head
+  level; 32 B per reached node: row offsets, sigma read, coefficient and
+  dependency writes; summed over sources);
+- ``cpu_baseline``: the CPU oracle (the reference's algorithm restated in
+  C, all host cores) timed on a strided sample of the same sources, and a
+  bit-exact check of the device result on that sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("GTEPS for exact betweenness (sources×edges/s), FP32-δ vs FP64 at rel err "
+          "≤1e-6")
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(name: str):
+    from knappnet_amd import generate as gen
+    if name == "rmat18":
+        el = gen.rmat(18, 16, directed=True, seed=1)
+        return el, None, dict(workload="rmat18_directed_all_sources", scale=18, edge_factor=16,
+                              abcd=[0.57, 0.19, 0.19, 0.05], seed=1)
+    if name == "er2000":
+        el = gen.er_2000()
+        return el, None, dict(workload="er_gnm_2000_10000_all_sources", seed=0)
+    if name == "ba200k":
+        el = gen.ba_weighted()
+        return el, None, dict(workload="ba_200000_m4_int_weights_1_16_all_sources", seed=3)
+    if name == "rmat22":
+        el = gen.rmat(22, 16, directed=False, seed=1)
+        src = gen.sample_sources(el.n, 8192, 7)
+        return el, src, dict(workload="rmat22_undirected_8192_sampled_sources", scale=22,
+                             edge_factor=16, seed=1, source_seed=7)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rmat18")
+    ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 dependencies")
+    ap.add_argument("--slots", type=int, default=0)
+    ap.add_argument("--direction", type=int, default=0)
+    ap.add_argument("--max-sources", type=int, default=0,
+                    help="time only the first K sources (experiments; not the headline)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target CPU time of the sampled baseline (0 = skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
+                                   "of it (sources would shard across GPUs with one final "
+                                   "all-reduce, see the design notes)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from knappnet_amd.engine import Graph, device_info
+
+    t0 = time.perf_counter()
+    el, sources, config = build_workload(args.workload)
+    gen_s = time.perf_counter() - t0
+    weighted = el.weights is not None
+    if args.max_sources:
+        pool = np.arange(el.n) if sources is None else sources
+        sources = np.asarray(pool[: args.max_sources])
+        config = dict(config, workload=config["workload"] + f"_first{len(sources)}")
+    g = Graph(el.n, el.src, el.tgt, el.weights, el.directed)
+    k = el.n if sources is None else len(sources)
+    print(f"# device: {device_info()}; workload {config['workload']} n={el.n} m={el.m} "
+          f"sources={k} (generated in {gen_s:.1f}s)", flush=True)
+
+    kw = dict(sources=sources, weighted=weighted, precision=args.precision, slots=args.slots,
+              direction=args.direction)
+    for i in range(args.warmup):
+        t = time.perf_counter()
+        res, diag = g.betweenness_sources(**kw)
+        print(f"# warmup {i}: {time.perf_counter() - t:.3f}s build_ms={diag['build_ms']:.1f}",
+              flush=True)
+
+    times, diags = [], []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        res, diag = g.betweenness_sources(**kw)
+        dt = time.perf_counter() - t
+        times.append(dt)
+        diags.append(diag)
+        print(f"# step {i}: {dt:.3f}s kernel_ms={diag['kernel_ms']:.1f} "
+              f"sweep_ms={diag['dominant_ms']:.1f} levels={diag['levels']}", flush=True)
+        if any(diag["work"]):
+            print(f"# work counters: {diag['work']} reached_arcs={diag['reached_arcs']} "
+                  f"reached_nodes={diag['reached_nodes']}", flush=True)
+    total = sum(times)
+    sec = total / len(times)
+    gteps = k * el.m / sec / 1e9
+
+    d = diags[-1]
+    sweep_bytes = 8 * d["reached_arcs"] + 32 * d["reached_nodes"]
+    sweep_s = d["dominant_ms"] / 1e3
+    achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "bfs_backward_kernel<double>" if args.precision == 0 else
+                  "bfs_backward_kernel<float>",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5),
+        "traffic": None,
+        "launches": d["dominant_launches"],
+        "avg_launch_us": round(d["dominant_ms"] * 1e3 / max(d["dominant_launches"], 1), 2),
+        "algorithmic_bytes_per_run": int(sweep_bytes),
+    }
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle.oracle import OracleGraph, available_cores
+        o = OracleGraph(el.n, el.src, el.tgt, el.weights, el.directed)
+        pool = np.arange(el.n) if sources is None else np.asarray(sources)
+        cores = available_cores()
+        take = min(len(pool), 256)
+        while True:
+            stride = max(1, len(pool) // take)
+            sample = pool[::stride][:take]
+            t = time.perf_counter()
+            want = o.betweenness_sources(sample, weighted=weighted, threads=0)
+            ct = time.perf_counter() - t
+            if ct >= args.cpu_seconds / 4 or take >= len(pool):
+                break
+            take = min(len(pool), int(take * max(2.0, args.cpu_seconds / max(ct, 1e-3))))
+        got, _ = g.betweenness_sources(sample, weighted=weighted, precision=args.precision)
+        exact = bool(np.array_equal(got, want))
+        nz = (got != 0) | (want != 0)
+        rel = float(np.max(np.abs(got - want)[nz] / np.maximum(np.abs(want[nz]), 1e-300))) \
+            if nz.any() else 0.0
+        cpu_gteps = len(sample) * el.m / ct / 1e9
+        cpu = {"value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": cores,
+               "kind": "port", "seconds": round(ct, 3),
+               "sample": f"{len(sample)} of {k} sources (every {stride}th), all {cores} host "
+                         f"threads, CPU: {cpu_model()}",
+               "check": "bit-exact" if exact else f"max rel err {rel:.3e}"}
+        print(f"# cpu sample: {len(sample)} sources in {ct:.2f}s -> {cpu_gteps:.3f} GTEPS; "
+              f"device vs oracle on sample: {cpu['check']}", flush=True)
+
+    line = {
+        "metric": METRIC,
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f32-delta/f64-sigma",
+        "data": "synthetic",
+        "config": dict(config, n=el.n, m=el.m, sources=k, directed=el.directed,
+                       weighted=weighted, slots=d["slots"]),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(gteps / cpu["value"], 1) if cpu else None,
+        "sigma_overflow": d["sigma_overflow"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

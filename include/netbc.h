@@ -68,7 +68,16 @@ typedef struct netbc_options {
     int32_t slots;     /* 64-source batches per launch; 0 = automatic */
     int32_t direction; /* 0 automatic, 1 top-down only, 2 bottom-up when
                           the frontier allows */
-    int32_t reserved;
+    int32_t exact;     /* 1: the reference's summation order everywhere, so
+                          FP64 scores match it bit for bit. 0 (default):
+                          rows longer than 1024 arcs are summed in 256-arc
+                          chunks whose partial sums are added in chunk
+                          order, and runs of 4096 or more sources batch
+                          sources with similar neighbourhoods together
+                          (summing them in that order). Both are
+                          deterministic and change only the association of
+                          FP64 sums (scores within 1e-12 relative); levels
+                          and path counts are bit-exact either way. */
     double alpha;      /* direction switch factor; 0 = default */
 } netbc_options;
 
@@ -87,6 +96,8 @@ typedef struct netbc_diagnostics {
     int32_t slots;           /* 64-source batches per launch used */
     int32_t reserved;
     double build_ms;         /* device CSR/CSC build time (first call) */
+    int64_t work[8];         /* engine work counters; zero unless the library
+                                is built with NETBC_COUNTERS */
 } netbc_diagnostics;
 
 /* Betweenness restricted to a source list (NULL with k = n: every node,
@@ -101,6 +112,25 @@ int32_t netbc_betweenness_sources(const GraphHandle *h, const int64_t *sources,
                                   const netbc_options *opt, double *out,
                                   netbc_diagnostics *diag, char *err,
                                   size_t err_len);
+
+/* Betweenness of many independent graphs in one call (null-model and
+ * permutation replicates). Graph i has node_counts[i] nodes and
+ * edge_counts[i] edges, listed one graph after another in src/tgt (node
+ * ids local to the graph) and weights (NULL, or one per edge). `out`
+ * receives the graphs' scores one after another (sum of node_counts
+ * doubles), each exactly what net_betweenness returns for that graph
+ * alone. Unweighted graphs of up to 2048 nodes run together in one
+ * persistent launch; others run one at a time. Errors name the graph
+ * ("graph 3: edge 5 refers to ..."). No counterpart in netcapi.h: it
+ * replaces the one-call-per-replicate loop of r/R/cug.R:42-47. */
+int32_t netbc_betweenness_batch(int64_t graphs, const int64_t *node_counts,
+                                const int64_t *edge_counts, const int64_t *src,
+                                const int64_t *tgt, const double *weights,
+                                int32_t directed, int32_t weighted,
+                                int32_t weights_as, int32_t normalized,
+                                const netbc_options *opt, double *out,
+                                netbc_diagnostics *diag, char *err,
+                                size_t err_len);
 
 /* Debug entry: the per-source state of k sources, each row n long:
  * dist = hop count (breadth-first) or integer distance (Dial),

@@ -11,8 +11,11 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_int32, c_int64,
                     c_size_t, c_uint32, c_uint64, c_void_p)
 from pathlib import Path
 
+import os
+
 PKG = Path(__file__).resolve().parent
-DEVICE_LIB = PKG / "libnetbc.so"
+# NETBC_LIB names another build of the device library (for A/B timing).
+DEVICE_LIB = Path(os.environ.get("NETBC_LIB", PKG / "libnetbc.so"))
 GEN_LIB = PKG / "libnetgen.so"
 
 _device = None
@@ -22,7 +25,7 @@ _gen = None
 class Options(Structure):
     """netbc_options (include/netbc.h)."""
     _fields_ = [("precision", c_int32), ("slots", c_int32), ("direction", c_int32),
-                ("reserved", c_int32), ("alpha", c_double)]
+                ("exact", c_int32), ("alpha", c_double)]
 
 
 class Diagnostics(Structure):
@@ -32,10 +35,13 @@ class Diagnostics(Structure):
                 ("reached_arcs", c_int64), ("levels", c_int64),
                 ("kernel_ms", c_double), ("dominant_ms", c_double),
                 ("dominant_launches", c_int64), ("slots", c_int32),
-                ("reserved", c_int32), ("build_ms", c_double)]
+                ("reserved", c_int32), ("build_ms", c_double), ("work", c_int64 * 8)]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        d = {name: getattr(self, name) for name, _ in self._fields_
+             if name not in ("reserved", "work")}
+        d["work"] = list(self.work)
+        return d
 
 
 def _load(path: Path, what: str) -> ctypes.CDLL:
@@ -67,6 +73,11 @@ def device() -> ctypes.CDLL:
             c_void_p, POINTER(c_int64), c_int64, c_int32, c_int32, c_int32,
             POINTER(Options), POINTER(c_double), POINTER(Diagnostics), c_char_p, c_size_t]
         lib.netbc_betweenness_sources.restype = c_int32
+        lib.netbc_betweenness_batch.argtypes = [
+            c_int64, POINTER(c_int64), POINTER(c_int64), POINTER(c_int64), POINTER(c_int64),
+            POINTER(c_double), c_int32, c_int32, c_int32, c_int32, POINTER(Options),
+            POINTER(c_double), POINTER(Diagnostics), c_char_p, c_size_t]
+        lib.netbc_betweenness_batch.restype = c_int32
         lib.netbc_state_dump.argtypes = [c_void_p, c_int32, c_int32, c_int32, POINTER(c_int64),
                                          c_int64, POINTER(c_uint64), POINTER(c_double),
                                          POINTER(c_double), c_char_p, c_size_t]

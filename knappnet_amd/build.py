@@ -17,7 +17,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
-HIP_SOURCES = ["graph.hip", "batch.hip", "capi.hip"]
+HIP_SOURCES = ["graph.hip", "batch.hip", "small.hip", "capi.hip"]
 ARCH = os.environ.get("NETBC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -49,11 +49,12 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build_device_library() -> Path:
-    out = PKG / "libnetbc.so"
+def build_device_library(out: Path | None = None, defines: tuple = ()) -> Path:
+    out = out or PKG / "libnetbc.so"
     tmp = out.with_suffix(".so.tmp")
     srcs = [str(CSRC / s) for s in HIP_SOURCES]
-    _run([hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), *srcs, "-o", str(tmp)])
+    _run([hipcc(), *HIPCC_FLAGS, *[f"-D{d}" for d in defines], "-I", str(ROOT / "include"),
+          *srcs, "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

@@ -22,6 +22,8 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 
 #include "netbc_internal.h"
 
@@ -41,16 +43,37 @@ struct Ctrl {
     uint32_t tlen[2];
     uint32_t qoverflow;
     uint32_t pending;
-    uint32_t pad;
+    uint32_t hlen;  // heavy-row entries listed (positions in the level queue)
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
     unsigned long long reached_nodes;
     unsigned long long reached_arcs;
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
+    // Work counters (filled only in builds with NETBC_COUNTERS):
+    // 0 expand entries, 1 expand arcs, 2 expand pushed arcs, 3 pushed lanes,
+    // 4 sweep entries, 5 sweep arcs, 6 sweep contributing arcs, 7 their lanes.
+    unsigned long long work[8];
 };
 
+#ifdef NETBC_COUNTERS
+#define NETBC_COUNT(var, x) (var) += (x)
+#define NETBC_FLUSH(ctrl, i, var)                                             \
+    do {                                                                      \
+        unsigned long long t_ = wave_sum_u64(var);                            \
+        if (lane_id() == 0 && t_) atomicAdd(&(ctrl)->work[i], t_);            \
+    } while (0)
+#else
+#define NETBC_COUNT(var, x) (void)0
+#define NETBC_FLUSH(ctrl, i, var) (void)0
+#endif
+
 constexpr double kTwo53 = 9007199254740992.0;
+// Rows loaded ahead of the ordered sum in the dependency sweeps.
+#ifndef NETBC_AHEAD
+#define NETBC_AHEAD 8
+#endif
+constexpr int kAhead = NETBC_AHEAD;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ size_t global_wave() {
@@ -81,8 +104,10 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x)
     return x;
 }
 
-__device__ __forceinline__ void note_sigma(Ctrl* ctrl, double s, bool active) {
-    double mx = wave_max(active ? s : 0.0);
+// Publishes a wave's largest path count (one atomic per wave, at the end of
+// a kernel) and flags counts beyond 2^53.
+__device__ __forceinline__ void note_sigma(Ctrl* ctrl, double s) {
+    double mx = wave_max(s);
     if (lane_id() == 0 && mx > 0.0) {
         unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
         atomicMax(&ctrl->max_sigma_bits, bits);
@@ -96,7 +121,9 @@ __device__ __forceinline__ void note_sigma(Ctrl* ctrl, double s, bool active) {
 __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t item0, int64_t k,
                                     int slots, size_t n, uint64_t* __restrict__ vis,
                                     uint64_t* __restrict__ g0, double* __restrict__ sig,
-                                    QEnt* __restrict__ q, Ctrl* ctrl, uint32_t* __restrict__ dist) {
+                                    QEnt* __restrict__ q, Ctrl* ctrl, uint32_t* __restrict__ dist,
+                                    const uint8_t* __restrict__ heavy_flag,
+                                    uint32_t* __restrict__ hq) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
     int b = t / kLanes, l = t % kLanes;
@@ -111,6 +138,7 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
     if (dist) dist[mi * kLanes + l] = 0;
     uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
     q[pos] = QEnt{s, (uint32_t)b, bit};
+    if (hq && heavy_flag[s]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t count, uint32_t v) {
@@ -127,17 +155,25 @@ __global__ void fill_u64_kernel(uint64_t* __restrict__ p, size_t count, uint64_t
 
 // ---- breadth-first search: forward -----------------------------------------
 
-// Top-down step of level d: every frontier entry (v, b, M) offers its lanes
-// to each out-neighbour x that has not yet been reached in them. The first
-// offer to x in this level records x in the touched list. Threads also
-// clear the stale level d-1 entries of the next-level bitmap.
+// Top-down step of level d. Every frontier entry (v, b, M) offers its
+// lanes to each out-neighbour x not yet reached in them (nv = M & ~vis[x]),
+// records x in the touched list on the first offer of the level, and adds
+// its path counts into x's row: sigma[x][l] += sigma[v][l] for l in nv.
+// Path counts are integers below 2^53 (checked), so these FP64 atomic adds
+// are exact in any order and give the reference's sums bit for bit. Threads
+// also clear the stale level d-1 entries of the next-level bitmap.
 __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
-                                  uint32_t zbeg, uint32_t zend, const uint32_t* __restrict__ off,
+                                  uint32_t zbeg, uint32_t zend, const uint32_t* __restrict__ hq,
+                                  uint32_t hbeg, uint32_t hend, uint32_t max_chunks,
+                                  const uint32_t* __restrict__ off,
                                   const uint32_t* __restrict__ idx,
                                   const uint64_t* __restrict__ vis, uint64_t* __restrict__ cand,
                                   uint64_t* __restrict__ nxt, uint2* __restrict__ touched,
-                                  uint32_t* __restrict__ tlen, size_t n) {
+                                  uint32_t* __restrict__ tlen, double* __restrict__ sig,
+                                  const uint8_t* __restrict__ hub_flag,
+                                  const uint8_t* __restrict__ heavy_flag, Ctrl* ctrl, size_t n) {
     const int lane = lane_id();
+    [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nth = (size_t)gridDim.x * blockDim.x;
     for (size_t i = zbeg + tid; i < zend; i += nth) {
@@ -145,17 +181,40 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
         nxt[(size_t)e.b * n + e.v] = 0;
     }
     const uint64_t lt = lanemask_lt(lane);
-    for (size_t e = qbeg + global_wave(); e < qend; e += wave_count()) {
-        QEnt en = q[e];
+    const size_t nq = qend - qbeg;
+    const size_t tasks = nq + (size_t)(hend - hbeg) * max_chunks;
+    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+        // A task is a whole light row, or one chunk of a heavy row.
+        QEnt en;
+        uint32_t a0, a1;
+        if (w < nq) {
+            en = q[qbeg + w];
+            if (heavy_flag[en.v]) continue;
+            a0 = off[en.v];
+            a1 = off[en.v + 1];
+        } else {
+            size_t t = w - nq;
+            en = q[hq[hbeg + t / max_chunks]];
+            a0 = off[en.v] + (uint32_t)(t % max_chunks) * kChunkArcs;
+            a1 = min(a0 + kChunkArcs, off[en.v + 1]);
+            if (a0 >= a1) continue;
+        }
         size_t base = (size_t)en.b * n;
-        uint32_t a0 = off[en.v], a1 = off[en.v + 1];
+        const bool mine = (en.m >> lane) & 1;
+        const double sv = mine ? sig[(base + en.v) * kLanes + lane] : 0.0;
+        if (lane == 0) {
+            NETBC_COUNT(c0, 1);
+            NETBC_COUNT(c1, a1 - a0);
+        }
         for (uint32_t c = a0; c < a1; c += 64) {
             uint32_t a = c + lane;
             bool ok = a < a1;
             uint32_t x = ok ? idx[a] : 0;
             uint64_t nv = ok ? (en.m & ~vis[base + x]) : 0;
+            NETBC_COUNT(c2, nv != 0);
+            NETBC_COUNT(c3, __popcll(nv));
             bool first = false;
-            if (nv) {
+            if (nv && (cand[base + x] & nv) != nv) {
                 unsigned long long old =
                     atomicOr((unsigned long long*)&cand[base + x], (unsigned long long)nv);
                 first = old == 0;
@@ -168,58 +227,124 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
                 pos = rl32(pos, leader);
                 if (first) touched[pos + __popcll(bal & lt)] = make_uint2(x, en.b);
             }
+            // Hubs pull their counts (bfs_hub_pull_kernel) instead.
+            uint64_t push = __ballot(nv != 0 && !hub_flag[x]);
+            while (push) {
+                int j = __ffsll((unsigned long long)push) - 1;
+                push &= push - 1;
+                uint32_t xx = rl32(x, j);
+                uint64_t m2 = rl64(nv, j);
+                if ((m2 >> lane) & 1) unsafeAtomicAdd(&sig[(base + xx) * kLanes + lane], sv);
+            }
         }
+    }
+    NETBC_FLUSH(ctrl, 0, c0);
+    NETBC_FLUSH(ctrl, 1, c1);
+    NETBC_FLUSH(ctrl, 2, c2);
+    NETBC_FLUSH(ctrl, 3, c3);
+}
+
+// Path counts of hubs reached at level d+1, pulled instead of pushed: task
+// (hub h, slot b, chunk c) sums sigma[u][l] over in-arcs u->x of chunk c
+// with u at level d in a new lane l of x, and adds the partial sum into
+// sigma[x][l]. Partial sums of integers below 2^53 are exact, so the order
+// of the adds does not matter.
+__global__ void bfs_hub_pull_kernel(const uint32_t* __restrict__ hub_node,
+                                    const uint32_t* __restrict__ hub_chunk0, uint32_t hubs,
+                                    uint32_t chunks, int slots, const uint32_t* __restrict__ in_off,
+                                    const uint32_t* __restrict__ in_idx,
+                                    const uint64_t* __restrict__ cur,
+                                    const uint64_t* __restrict__ cand, double* __restrict__ sig,
+                                    size_t n) {
+    const int lane = lane_id();
+    const size_t tasks = (size_t)chunks * slots;
+    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+        const int b = (int)(w / chunks);
+        const uint32_t c = (uint32_t)(w % chunks);
+        // hub of chunk c: largest h with hub_chunk0[h] <= c
+        uint32_t lo = 0, hi = hubs;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) / 2;
+            if (hub_chunk0[mid] <= c) lo = mid; else hi = mid;
+        }
+        const uint32_t x = hub_node[lo];
+        const size_t base = (size_t)b * n;
+        const uint64_t N = cand[base + x];
+        if (!N) continue;
+        const uint32_t a_lo = in_off[x] + (c - hub_chunk0[lo]) * kHubChunk;
+        const uint32_t a_hi = min(a_lo + kHubChunk, in_off[x + 1]);
+        double acc = 0.0;
+        bool any = false;
+        for (uint32_t a0 = a_lo; a0 < a_hi; a0 += 64) {
+            const uint32_t a = a0 + lane;
+            const bool ok = a < a_hi;
+            const uint32_t u = ok ? in_idx[a] : 0;
+            const uint64_t mm = ok ? (cur[base + u] & N) : 0;
+            uint64_t bal = __ballot(mm != 0);
+            any |= bal != 0;
+            while (bal) {
+                double val[kAhead];
+#pragma unroll
+                for (int k = 0; k < kAhead; k++) {
+                    val[k] = 0.0;
+                    if (bal) {
+                        int j = __ffsll((unsigned long long)bal) - 1;
+                        bal &= bal - 1;
+                        uint32_t uu = rl32(u, j);
+                        uint64_t m2 = rl64(mm, j);
+                        if ((m2 >> lane) & 1) val[k] = sig[(base + uu) * kLanes + lane];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kAhead; k++) acc += val[k];
+            }
+        }
+        if (any && ((N >> lane) & 1) && acc != 0.0)
+            unsafeAtomicAdd(&sig[(base + x) * kLanes + lane], acc);
     }
 }
 
-// Path counts of the nodes first reached at level d+1: for each touched
-// node x with new lanes N, sigma[x][l] = sum over in-arcs u->x with u at
-// level d in lane l of sigma[u][l]. Appends (x, b, N) to the level queue.
-__global__ void bfs_sigma_kernel(const uint2* __restrict__ touched,
-                                 const uint32_t* __restrict__ tlen_cur,
-                                 uint32_t* __restrict__ tlen_next,
-                                 const uint32_t* __restrict__ in_off,
-                                 const uint32_t* __restrict__ in_idx,
-                                 const uint64_t* __restrict__ cur, uint64_t* __restrict__ cand,
-                                 uint64_t* __restrict__ vis, uint64_t* __restrict__ nxt,
-                                 double* __restrict__ sig, QEnt* __restrict__ q, uint32_t qcap,
-                                 Ctrl* ctrl, size_t n) {
+// Closes level d: every touched node x takes its new lanes N (the OR of
+// all offers), which become visited, form the level d+1 bitmap and join
+// the level queue as (x, b, N).
+__global__ void bfs_finalize_kernel(const uint2* __restrict__ touched,
+                                    const uint32_t* __restrict__ tlen_cur,
+                                    uint32_t* __restrict__ tlen_next,
+                                    uint64_t* __restrict__ cand, uint64_t* __restrict__ vis,
+                                    uint64_t* __restrict__ nxt, QEnt* __restrict__ q,
+                                    uint32_t qcap, const uint8_t* __restrict__ heavy_flag,
+                                    uint32_t* __restrict__ hq, Ctrl* ctrl, size_t n) {
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) *tlen_next = 0;
     const uint32_t T = *tlen_cur;
-    for (size_t t = global_wave(); t < T; t += wave_count()) {
-        uint2 te = touched[t];
-        uint32_t x = te.x;
-        size_t base = (size_t)te.y * n;
-        uint64_t N = cand[base + x];
-        double acc = 0.0;
-        uint32_t a0 = in_off[x], a1 = in_off[x + 1];
-        for (uint32_t c = a0; c < a1; c += 64) {
-            uint32_t a = c + lane;
-            bool ok = a < a1;
-            uint32_t u = ok ? in_idx[a] : 0;
-            uint64_t mm = ok ? (cur[base + u] & N) : 0;
-            uint64_t bal = __ballot(mm != 0);
-            while (bal) {
-                int j = __ffsll((unsigned long long)bal) - 1;
-                bal &= bal - 1;
-                uint32_t uu = rl32(u, j);
-                uint64_t m2 = rl64(mm, j);
-                if ((m2 >> lane) & 1) acc += sig[(base + uu) * kLanes + lane];
-            }
+    const uint64_t lt = lanemask_lt(lane);
+    size_t nth = (size_t)gridDim.x * blockDim.x;
+    for (size_t t0 = (size_t)blockIdx.x * blockDim.x; t0 < T; t0 += nth) {
+        size_t t = t0 + threadIdx.x;
+        bool ok = t < T;
+        uint2 te = ok ? touched[t] : make_uint2(0, 0);
+        size_t mi = (size_t)te.y * n + te.x;
+        uint64_t N = 0;
+        if (ok) {
+            N = cand[mi];
+            cand[mi] = 0;
+            vis[mi] |= N;
+            nxt[mi] = N;
         }
-        bool mine = (N >> lane) & 1;
-        if (mine) sig[(base + x) * kLanes + lane] = acc;
-        note_sigma(ctrl, acc, mine);
-        if (lane == 0) {
-            vis[base + x] |= N;
-            nxt[base + x] = N;
-            cand[base + x] = 0;
-            uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
-            if (pos < qcap)
-                q[pos] = QEnt{x, te.y, (unsigned long long)N};
-            else
-                ctrl->qoverflow = 1;
+        uint64_t bal = __ballot(ok);
+        if (bal) {
+            int leader = __ffsll((unsigned long long)bal) - 1;
+            uint32_t pos = 0;
+            if (lane == leader) pos = atomicAdd(&ctrl->qlen, (uint32_t)__popcll(bal));
+            pos = rl32(pos, leader) + (uint32_t)__popcll(bal & lt);
+            if (ok) {
+                if (pos < qcap) {
+                    q[pos] = QEnt{te.x, te.y, (unsigned long long)N};
+                    if (heavy_flag[te.x]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
+                } else {
+                    ctrl->qoverflow = 1;
+                }
+            }
         }
     }
 }
@@ -258,14 +383,69 @@ struct Dep<float> {
 // Dependencies of the level-d entries. R is the level d+1 bitmap, W the
 // bitmap being turned from level d+2 into level d (lanes are disjoint, so
 // the OR and AND-NOT updates commute).
+// Adds the coefficients of the successors of `en` among arcs [a0, a1) to
+// dv, per lane in CSR order; up to kAhead rows are loaded before they are
+// added.
+template <class C>
+__device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t a1,
+                                           const uint32_t* __restrict__ idx,
+                                           const uint64_t* __restrict__ R,
+                                           const C* __restrict__ coef, size_t n, Dep<C>& dv,
+                                           unsigned long long& c6, unsigned long long& c7) {
+    const int lane = lane_id();
+    const size_t base = (size_t)en.b * n;
+    for (uint32_t c = a0; c < a1; c += 64) {
+        uint32_t a = c + lane;
+        bool ok = a < a1;
+        uint32_t x = ok ? idx[a] : 0;
+        uint64_t mm = ok ? (R[base + x] & en.m) : 0;
+        NETBC_COUNT(c6, mm != 0);
+        NETBC_COUNT(c7, __popcll(mm));
+        uint64_t bal = __ballot(mm != 0);
+        while (bal) {
+            C val[kAhead];
+            bool use[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) {
+                use[u] = false;
+                val[u] = 0;
+                if (bal) {
+                    int j = __ffsll((unsigned long long)bal) - 1;
+                    bal &= bal - 1;
+                    uint32_t xx = rl32(x, j);
+                    uint64_t m2 = rl64(mm, j);
+                    use[u] = (m2 >> lane) & 1;
+                    if (use[u]) val[u] = coef[(base + xx) * kLanes + lane];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kAhead; u++)
+                if (use[u]) dv.add(val[u]);
+        }
+    }
+}
+
+// Dependencies of the level-d entries. R is the level d+1 bitmap, W the
+// bitmap being turned from level d+2 into level d (lanes are disjoint, so
+// the OR and AND-NOT updates commute). Light rows are summed by one wave in
+// CSR order. Heavy rows (listed in hq) are split into chunk tasks that each
+// write a per-lane partial sum (part), added in chunk order by
+// bfs_combine_kernel; with split == 0 every row is summed whole.
 template <class C>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level,
+                                    const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
+                                    uint32_t max_chunks, int split,
+                                    const uint8_t* __restrict__ heavy_flag,
+                                    double* __restrict__ part,
                                     const uint32_t* __restrict__ off,
                                     const uint32_t* __restrict__ idx,
                                     const uint64_t* __restrict__ R, uint64_t* __restrict__ W,
-                                    double* __restrict__ sig, C* __restrict__ coef, size_t n) {
+                                    double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
+                                    size_t n) {
     const int lane = lane_id();
+    double smax = 0.0;
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
     if (!last_level) {
         size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
         size_t nth = (size_t)gridDim.x * blockDim.x;
@@ -278,27 +458,32 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             atomicAnd((unsigned long long*)&W[(size_t)e.b * n + e.v], ~e.m);
         }
     }
-    for (size_t e = qbeg + global_wave(); e < qend; e += wave_count()) {
-        QEnt en = q[e];
+    split = split && !last_level;
+    const size_t nq = qend - qbeg;
+    const size_t tasks = nq + (split ? (size_t)(hend - hbeg) * max_chunks : 0);
+    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+        if (w >= nq) {
+            // One chunk of a heavy row: its partial sum.
+            size_t t = w - nq;
+            QEnt en = q[hq[hbeg + t / max_chunks]];
+            uint32_t a0 = off[en.v] + (uint32_t)(t % max_chunks) * kChunkArcs;
+            uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
+            if (a0 >= a1) continue;
+            Dep<C> dv;
+            sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
+            part[t * kLanes + lane] = dv.value();
+            continue;
+        }
+        QEnt en = q[qbeg + w];
+        if (split && heavy_flag[en.v]) continue;
         size_t base = (size_t)en.b * n;
         uint64_t M = en.m;
         Dep<C> dv;
+        if (lane == 0) NETBC_COUNT(c4, 1);
         if (!last_level) {
             uint32_t a0 = off[en.v], a1 = off[en.v + 1];
-            for (uint32_t c = a0; c < a1; c += 64) {
-                uint32_t a = c + lane;
-                bool ok = a < a1;
-                uint32_t x = ok ? idx[a] : 0;
-                uint64_t mm = ok ? (R[base + x] & M) : 0;
-                uint64_t bal = __ballot(mm != 0);
-                while (bal) {
-                    int j = __ffsll((unsigned long long)bal) - 1;
-                    bal &= bal - 1;
-                    uint32_t xx = rl32(x, j);
-                    uint64_t m2 = rl64(mm, j);
-                    if ((m2 >> lane) & 1) dv.add(coef[(base + xx) * kLanes + lane]);
-                }
-            }
+            if (lane == 0) NETBC_COUNT(c5, a1 - a0);
+            sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
         }
         if ((M >> lane) & 1) {
             size_t r = (base + en.v) * kLanes + lane;
@@ -306,8 +491,42 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             double delta = s * dv.value();
             coef[r] = Dep<C>::coef(delta, s);
             sig[r] = delta;
+            smax = fmax(smax, s);
         }
     }
+    note_sigma(ctrl, smax);
+    NETBC_FLUSH(ctrl, 4, c4);
+    NETBC_FLUSH(ctrl, 5, c5);
+    NETBC_FLUSH(ctrl, 6, c6);
+    NETBC_FLUSH(ctrl, 7, c7);
+}
+
+// Finishes the heavy rows of a level: dv = the chunk partial sums added in
+// chunk order, then the row's dependencies and coefficients.
+template <class C>
+__global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* __restrict__ hq,
+                                   uint32_t hbeg, uint32_t hend, uint32_t max_chunks,
+                                   const double* __restrict__ part,
+                                   const uint32_t* __restrict__ off, double* __restrict__ sig,
+                                   C* __restrict__ coef, Ctrl* ctrl, size_t n) {
+    const int lane = lane_id();
+    double smax = 0.0;
+    for (size_t h = global_wave(); h < hend - hbeg; h += wave_count()) {
+        QEnt en = q[hq[hbeg + h]];
+        uint32_t deg = off[en.v + 1] - off[en.v];
+        uint32_t chunks = (deg + kChunkArcs - 1) / kChunkArcs;
+        double dv = 0.0;
+        for (uint32_t c = 0; c < chunks; c++) dv += part[(h * max_chunks + c) * kLanes + lane];
+        if ((en.m >> lane) & 1) {
+            size_t r = ((size_t)en.b * n + en.v) * kLanes + lane;
+            double s = sig[r];
+            double delta = s * dv;
+            coef[r] = Dep<C>::coef(delta, s);
+            sig[r] = delta;
+            smax = fmax(smax, s);
+        }
+    }
+    note_sigma(ctrl, smax);
 }
 
 // ---- integer lengths (Dial / delta-stepping with unit buckets) -------------
@@ -325,6 +544,7 @@ __global__ void dial_settle_kernel(const uint2* __restrict__ list, const uint32_
                                    double* __restrict__ sig, QEnt* __restrict__ q, uint32_t qcap,
                                    Ctrl* ctrl, size_t n) {
     const int lane = lane_id();
+    double smax = 0.0;
     const uint32_t T = min(blen[k], (uint32_t)list_cap);
     for (size_t t = global_wave(); t < T; t += wave_count()) {
         uint2 te = list[t];
@@ -357,9 +577,10 @@ __global__ void dial_settle_kernel(const uint2* __restrict__ list, const uint32_
                 }
             }
         }
-        bool mine = (M >> lane) & 1;
-        if (mine) sig[(base + x) * kLanes + lane] = acc;
-        note_sigma(ctrl, acc, mine);
+        if ((M >> lane) & 1) {
+            sig[(base + x) * kLanes + lane] = acc;
+            smax = fmax(smax, acc);
+        }
         if (lane == 0) {
             vis[base + x] |= M;
             uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
@@ -369,6 +590,7 @@ __global__ void dial_settle_kernel(const uint2* __restrict__ list, const uint32_
                 ctrl->qoverflow = 1;
         }
     }
+    note_sigma(ctrl, smax);
 }
 
 // Relaxes the out-arcs of the entries settled at distance D. A lane whose
@@ -628,6 +850,42 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
     if (mx > kTwo53) ctrl->sig_overflow = 1;
 }
 
+// ---- batch composition ----------------------------------------------------
+
+// Runs with at least this many sources batch them by neighbourhood; smaller
+// runs keep list order (and so the reference's summation order).
+constexpr int64_t kReorderMin = 4096;
+
+// Sort key of item i: sources with the same out-degree and the same first
+// two out-neighbours have (nearly) the same shortest-path DAG from the
+// second level on, so placing them in one 64-source batch lets one row
+// access serve many lanes. Key = degree (clamped) | first | second
+// neighbour; the stable sort keeps list order among equal keys.
+__global__ void source_key_kernel(const int64_t* __restrict__ list, int64_t k,
+                                  const uint32_t* __restrict__ off,
+                                  const uint32_t* __restrict__ idx, uint32_t n, int b,
+                                  int deg_bits, uint64_t* __restrict__ key,
+                                  uint32_t* __restrict__ val) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    uint32_t s = (uint32_t)(list ? list[i] : i);
+    uint32_t a0 = off[s], a1 = off[s + 1];
+    uint64_t f1 = a1 > a0 ? idx[a0] : n;
+    uint64_t f2 = a1 > a0 + 1 ? idx[a0 + 1] : n;
+    uint64_t deg = a1 - a0;
+    uint64_t dmax = deg_bits > 0 ? ((1ull << deg_bits) - 1) : 0;
+    if (deg > dmax) deg = dmax;
+    key[i] = (deg_bits > 0 ? deg << (2 * b) : 0) | (f1 << b) | f2;
+    val[i] = (uint32_t)i;
+}
+
+__global__ void gather_sources_kernel(const int64_t* __restrict__ list, int64_t k,
+                                      const uint32_t* __restrict__ perm,
+                                      int64_t* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) out[i] = list ? list[perm[i]] : (int64_t)perm[i];
+}
+
 // ---- cross-source reduction -----------------------------------------------
 
 // Adds this super-batch's dependencies into the per-node group
@@ -635,7 +893,7 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
 // the last one added, the accumulator is first merged into bc and reset
 // (groups without contributions would add +0.0, which changes nothing).
 // This is sum_in_order's sequence of operations on every node.
-__global__ void accumulate_kernel(const uint64_t* __restrict__ vis, const double* __restrict__ sig,
+__global__ void accumulate_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                   int slots, size_t n, int64_t item0, int64_t k, int64_t gsize,
                                   const int64_t* __restrict__ list, double* __restrict__ acc,
                                   double* __restrict__ bc, int64_t* __restrict__ gcur,
@@ -650,11 +908,14 @@ __global__ void accumulate_kernel(const uint64_t* __restrict__ vis, const double
             if (ib >= k) break;
             uint64_t M = vis[(size_t)b * n + v];
             rn += (unsigned long long)__popcll(M);
+            double* row = sig + ((size_t)b * n + v) * kLanes;
             while (M) {
                 int l = __ffsll((unsigned long long)M) - 1;
                 M &= M - 1;
                 int64_t i = ib + l;
                 int64_t s = list ? list[i] : i;
+                double x = row[l];
+                row[l] = 0.0;  // rows start at zero for the next super-batch
                 if ((size_t)s == v) continue;
                 int64_t gi = i / gsize;
                 if (gi != g) {
@@ -662,7 +923,7 @@ __global__ void accumulate_kernel(const uint64_t* __restrict__ vis, const double
                     a = 0.0;
                     g = gi;
                 }
-                a += sig[((size_t)b * n + v) * kLanes + l];
+                a += x;
             }
         }
         acc[v] = a;
@@ -675,6 +936,20 @@ __global__ void accumulate_kernel(const uint64_t* __restrict__ vis, const double
     if (lane_id() == 0 && (rn || ra)) {
         atomicAdd(&ctrl->reached_nodes, rn);
         atomicAdd(&ctrl->reached_arcs, ra);
+    }
+}
+
+// Zeroes the per-lane rows of every reached (node, lane) of the slots, so
+// the path-count rows start at zero for the next super-batch.
+__global__ void clear_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
+                                  size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint64_t M = vis[i];
+    while (M) {
+        int l = __ffsll((unsigned long long)M) - 1;
+        M &= M - 1;
+        sig[i * kLanes + l] = 0.0;
     }
 }
 
@@ -695,6 +970,63 @@ inline unsigned blocks_for_threads(size_t threads) {
 }
 inline unsigned blocks_for_waves(size_t waves) { return blocks_for_threads(waves * 64); }
 
+// Blocks of a grid-stride kernel: enough for `waves` waves, but no more
+// than the device holds at once, so waves stay resident and loop over
+// their work instead of being launched one per item.
+std::mutex g_resident_mu;
+std::unordered_map<const void*, unsigned> g_resident;
+
+template <class K>
+unsigned resident_blocks(K kernel, size_t waves) {
+    unsigned cap;
+    {
+        std::lock_guard<std::mutex> lock(g_resident_mu);
+        auto it = g_resident.find((const void*)kernel);
+        if (it == g_resident.end()) {
+            int dev = 0, cus = 0, per = 0;
+            NETBC_HIP(hipGetDevice(&dev));
+            NETBC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0));
+            cap = (unsigned)std::max(1, cus * std::max(per, 1));
+            g_resident.emplace((const void*)kernel, cap);
+        } else {
+            cap = it->second;
+        }
+    }
+    return std::min(blocks_for_waves(waves), cap);
+}
+
+// Durations of one kernel's launches from pairs of events recorded around
+// each launch on the stream it runs on; read back only once the stream has
+// been synchronised, so timing adds no host waits.
+struct KernelTimer {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    double ms = 0.0;
+    int64_t launches = 0;
+    void mark(hipStream_t st) {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            NETBC_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        NETBC_HIP(hipEventRecord(pool[used++], st));
+    }
+    // Call only after the stream has been synchronised.
+    void collect() {
+        for (size_t i = 0; i + 1 < used; i += 2) {
+            float m = 0.0f;
+            NETBC_HIP(hipEventElapsedTime(&m, pool[i], pool[i + 1]));
+            ms += m;
+            launches++;
+        }
+        used = 0;
+    }
+    ~KernelTimer() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 // Device scratch of the batched engine, cached on the graph handle.
 struct Workspace {
     int slots = 0;
@@ -703,17 +1035,28 @@ struct Workspace {
     DevBuf<double> sig, coef64, acc;
     DevBuf<float> coef32;
     DevBuf<uint32_t> dist, blen;
-    DevBuf<int64_t> gcur, list;
+    DevBuf<int64_t> gcur, list, list2;
     DevBuf<QEnt> q;
+    DevBuf<uint32_t> hq;  // positions of heavy-row entries in q
+    DevBuf<double> part;  // per-chunk partial sums of heavy rows
     size_t qcap = 0;
     DevBuf<uint2> touched, blists;
     size_t list_cap = 0;
     DevBuf<Ctrl> ctrl;
     Ctrl* host_ctrl = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, evd0 = nullptr, evd1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    KernelTimer sweep;  // the dependency sweep (dominant kernel)
+    // The path-count rows are zero between runs; a run that stops early
+    // (an error) leaves them dirty, and the next run clears them first.
+    bool rows_dirty = false;
+    void clean_rows(hipStream_t st) {
+        if (!rows_dirty) return;
+        NETBC_HIP(hipMemsetAsync(sig.p, 0, sig.n * sizeof(double), st));
+        rows_dirty = false;
+    }
     ~Workspace() {
         if (host_ctrl) (void)hipHostFree(host_ctrl);
-        for (hipEvent_t e : {ev0, ev1, evd0, evd1})
+        for (hipEvent_t e : {ev0, ev1})
             if (e) (void)hipEventDestroy(e);
     }
 };
@@ -728,8 +1071,6 @@ Workspace& get_workspace(Graph& g) {
         NETBC_HIP(hipHostMalloc((void**)&ws->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
         NETBC_HIP(hipEventCreate(&ws->ev0));
         NETBC_HIP(hipEventCreate(&ws->ev1));
-        NETBC_HIP(hipEventCreate(&ws->evd0));
-        NETBC_HIP(hipEventCreate(&ws->evd1));
     }
     return *static_cast<Workspace*>(g.workspace);
 }
@@ -782,11 +1123,16 @@ struct Engine {
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
+    std::vector<uint32_t> hv;       // heavy-row list boundaries per level
     std::vector<uint32_t> lv_dist;  // Dial: distance of each level
 
-    float dom_ms = 0.0f;
-
     void launch_check() { NETBC_HIP(hipGetLastError()); }
+
+    void clear_rows() {
+        clear_rows_kernel<<<blocks_for_threads((size_t)slots * n), kBlock, 0, st>>>(
+            ws.vis.p, ws.sig.p, (size_t)slots * n);
+        launch_check();
+    }
 
     void reset_slots() {
         size_t nm = (size_t)slots * n;
@@ -817,31 +1163,47 @@ struct Engine {
         const DeviceGraph& d = g.dev;
         reset_slots();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
-            d_list, item0, k, slots, n, ws.vis.p, ws.g0.p, ws.sig.p, ws.q.p, ws.ctrl.p, nullptr);
+            d_list, item0, k, slots, n, ws.vis.p, ws.g0.p, ws.sig.p, ws.q.p, ws.ctrl.p, nullptr,
+            d.heavy_flag.p, ws.hq.p);
         launch_check();
         read_ctrl(ws, st);
+        ws.sweep.collect();
         lv.assign(1, 0);
         lv.push_back(ws.host_ctrl->qlen);
+        hv.assign(1, 0);
+        hv.push_back(ws.host_ctrl->hlen);
         uint64_t* G[2] = {ws.g0.p, ws.g1.p};
-        unsigned sigma_blocks = blocks_for_waves(std::min<size_t>((size_t)slots * n, 1u << 16));
+        unsigned finalize_blocks =
+            blocks_for_threads(std::min<size_t>((size_t)slots * n, (size_t)1 << 20));
         for (int dl = 0;; dl++) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
+            uint32_t hb = hv[dl], he = hv[dl + 1];
             uint32_t zb = dl > 0 ? lv[dl - 1] : 0, ze = dl > 0 ? lv[dl] : 0;
-            size_t waves = std::max<size_t>(qe - qb, (ze - zb + 63) / 64);
-            bfs_expand_kernel<<<blocks_for_waves(waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, zb, ze, d.out_off.p, d.out_idx.p, ws.vis.p, ws.cand.p,
-                G[(dl + 1) & 1], ws.touched.p, &ws.ctrl.p->tlen[dl & 1], n);
+            size_t tasks = (size_t)(qe - qb) + (size_t)(he - hb) * d.max_chunks;
+            size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
+            bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
+                ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, d.max_chunks, d.out_off.p, d.out_idx.p,
+                ws.vis.p, ws.cand.p, G[(dl + 1) & 1], ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
+                ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.ctrl.p, n);
             launch_check();
-            bfs_sigma_kernel<<<sigma_blocks, kBlock, 0, st>>>(
+            if (d.hubs) {
+                bfs_hub_pull_kernel<<<resident_blocks(bfs_hub_pull_kernel, (size_t)d.hub_chunks * slots),
+                                      kBlock, 0, st>>>(d.hub_node.p, d.hub_chunk0.p, d.hubs, d.hub_chunks,
+                                            slots, d.in_off_p(), d.in_idx_p(), G[dl & 1],
+                                            ws.cand.p, ws.sig.p, n);
+                launch_check();
+            }
+            bfs_finalize_kernel<<<finalize_blocks, kBlock, 0, st>>>(
                 ws.touched.p, &ws.ctrl.p->tlen[dl & 1], &ws.ctrl.p->tlen[(dl + 1) & 1],
-                d.in_off_p(), d.in_idx_p(), G[dl & 1], ws.cand.p, ws.vis.p, G[(dl + 1) & 1],
-                ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p, n);
+                ws.cand.p, ws.vis.p, G[(dl + 1) & 1], ws.q.p, (uint32_t)ws.qcap, d.heavy_flag.p,
+                ws.hq.p, ws.ctrl.p, n);
             launch_check();
             read_ctrl(ws, st);
             if (ws.host_ctrl->qoverflow) return false;
             uint32_t ql = ws.host_ctrl->qlen;
             if (ql == qe) break;
             lv.push_back(ql);
+            hv.push_back(ws.host_ctrl->hlen);
             diag.levels++;
         }
         return true;
@@ -852,21 +1214,27 @@ struct Engine {
         const DeviceGraph& d = g.dev;
         uint64_t* G[2] = {ws.g0.p, ws.g1.p};
         int D = (int)lv.size() - 2;  // last level with entries
+        const int split = opt.exact ? 0 : 1;
         for (int dl = D; dl >= 1; dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
+            uint32_t hb = hv[dl], he = hv[dl + 1];
             uint32_t ab = dl + 2 <= D ? lv[dl + 2] : 0, ae = dl + 2 <= D ? lv[dl + 3] : 0;
-            size_t waves = std::max<size_t>(qe - qb, (ae - ab + 63) / 64);
-            NETBC_HIP(hipEventRecord(ws.evd0, st));
-            bfs_backward_kernel<C><<<blocks_for_waves(waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, ab, ae, dl == D, d.out_off.p, d.out_idx.p, G[(dl + 1) & 1],
-                G[dl & 1], ws.sig.p, coef, n);
+            bool heavy = split && dl < D && he > hb;
+            size_t tasks = (size_t)(qe - qb) + (heavy ? (size_t)(he - hb) * d.max_chunks : 0);
+            size_t waves = std::max<size_t>(tasks, (ae - ab + 63) / 64);
+            ws.sweep.mark(st);
+            bfs_backward_kernel<C><<<resident_blocks(bfs_backward_kernel<C>, waves), kBlock, 0, st>>>(
+                ws.q.p, qb, qe, ab, ae, dl == D, ws.hq.p, hb, he, d.max_chunks, split,
+                d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p, G[(dl + 1) & 1], G[dl & 1],
+                ws.sig.p, coef, ws.ctrl.p, n);
             launch_check();
-            NETBC_HIP(hipEventRecord(ws.evd1, st));
-            NETBC_HIP(hipEventSynchronize(ws.evd1));
-            float ms = 0.0f;
-            NETBC_HIP(hipEventElapsedTime(&ms, ws.evd0, ws.evd1));
-            dom_ms += ms;
-            diag.dominant_launches++;
+            if (heavy) {
+                bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
+                    ws.q.p, ws.hq.p, hb, he, d.max_chunks, ws.part.p, d.out_off.p, ws.sig.p, coef,
+                    ws.ctrl.p, n);
+                launch_check();
+            }
+            ws.sweep.mark(st);
         }
     }
 
@@ -875,22 +1243,24 @@ struct Engine {
         reset_slots();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, ws.q.p, ws.ctrl.p,
-            ws.dist.p);
+            ws.dist.p, d.heavy_flag.p, nullptr);
         launch_check();
         read_ctrl(ws, st);
+        ws.sweep.collect();
         lv.assign(1, 0);
         lv.push_back(ws.host_ctrl->qlen);
         lv_dist.assign(1, 0);
         int W = (int)L.max + 1;
         int words = (W + 63) / 64;
-        unsigned settle_blocks = blocks_for_waves(std::min<size_t>((size_t)slots * n, 1u << 16));
+        unsigned settle_blocks =
+            resident_blocks(dial_settle_kernel, std::min<size_t>((size_t)slots * n, 1u << 16));
         const uint32_t* in_len = d.directed ? L.in_len.p : L.out_len.p;
         // Distance 0: relax the sources' arcs.
         uint32_t D = 0;
         for (;;) {
             uint32_t qb = lv[lv.size() - 2], qe = lv.back();
             bool have = lv_dist.back() == D && qe > qb;
-            dial_relax_kernel<<<blocks_for_waves(have ? qe - qb : 1), kBlock, 0, st>>>(
+            dial_relax_kernel<<<resident_blocks(dial_relax_kernel, have ? qe - qb : 1), kBlock, 0, st>>>(
                 ws.q.p, have ? qb : 0, have ? qe : 0, D, W, words, (int)(D % (uint32_t)W),
                 ws.blen.p, ws.blists.p, ws.list_cap, ws.inb.p, d.out_off.p, d.out_idx.p,
                 L.out_len.p, ws.dist.p, ws.vis.p, ws.ctrl.p, n);
@@ -924,34 +1294,24 @@ struct Engine {
         int D = (int)lv.size() - 2;
         for (int dl = D; dl >= 1; dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
-            NETBC_HIP(hipEventRecord(ws.evd0, st));
-            dial_backward_kernel<C><<<blocks_for_waves(qe - qb), kBlock, 0, st>>>(
+            ws.sweep.mark(st);
+            dial_backward_kernel<C><<<resident_blocks(dial_backward_kernel<C>, qe - qb), kBlock, 0, st>>>(
                 ws.q.p, qb, qe, lv_dist[dl], d.out_off.p, d.out_idx.p, L.out_len.p, ws.dist.p,
                 ws.vis.p, ws.sig.p, coef, n);
             launch_check();
-            NETBC_HIP(hipEventRecord(ws.evd1, st));
-            NETBC_HIP(hipEventSynchronize(ws.evd1));
-            float ms = 0.0f;
-            NETBC_HIP(hipEventElapsedTime(&ms, ws.evd0, ws.evd1));
-            dom_ms += ms;
-            diag.dominant_launches++;
+            ws.sweep.mark(st);
         }
     }
 
     void run_dijkstra(int64_t item0, DjkState& dst) {
         const DeviceGraph& d = g.dev;
         reset_slots();
-        NETBC_HIP(hipEventRecord(ws.evd0, st));
+        ws.sweep.mark(st);
         dijkstra_kernel<<<slots, kLanes, 0, st>>>(
             d_list, item0, k, slots, n, d.out_off.p, d.out_idx.p, L.out_real.p, dst, ws.vis.p,
             ws.sig.p, ws.ctrl.p);
         launch_check();
-        NETBC_HIP(hipEventRecord(ws.evd1, st));
-        NETBC_HIP(hipEventSynchronize(ws.evd1));
-        float ms = 0.0f;
-        NETBC_HIP(hipEventElapsedTime(&ms, ws.evd0, ws.evd1));
-        dom_ms += ms;
-        diag.dominant_launches++;
+        ws.sweep.mark(st);
     }
 };
 
@@ -966,6 +1326,8 @@ void grow_queue(Workspace& ws) {
     size_t nq = std::min<size_t>(ws.qcap * 2, 0xFFFFFFFFull);
     ws.q.release();
     ws.q.alloc(nq);
+    ws.hq.release();
+    ws.hq.alloc(nq);
     ws.qcap = nq;
 }
 
@@ -976,7 +1338,9 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     ws.slots = slots;
     ws.n = g.n;
     ws.vis.alloc(nm);
+    double* before = ws.sig.p;
     ws.sig.alloc(nm * kLanes);
+    if (ws.sig.p != before) ws.rows_dirty = true;  // rows must start at zero
     if (L.kind == kUniform) {
         ws.cand.alloc(nm);
         ws.g0.alloc(nm);
@@ -989,6 +1353,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
         if (qcap > ws.qcap || !ws.q.p) {
             ws.q.release();
             ws.q.alloc(qcap);
+            ws.hq.release();
+            ws.hq.alloc(qcap);
             ws.qcap = qcap;
         }
     }
@@ -1004,6 +1370,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
             ws.list_cap = nm;
         }
     }
+    if (L.kind == kUniform && g.dev.heavy)  // one level holds each (heavy row, slot) once
+        ws.part.alloc((size_t)slots * g.dev.heavy * g.dev.max_chunks * kLanes);
     ws.acc.alloc(n);
     ws.gcur.alloc(n);
     ws.ctrl.alloc(1);
@@ -1033,7 +1401,37 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                                  st));
         d_list = ws.list.p;
     }
+    if (!opt.exact && k >= kReorderMin && L.kind != kReal) {
+        // Batch sources by neighbourhood (see source_key_kernel). Items are
+        // summed in the new order, which changes only the association of
+        // the cross-source sum.
+        const DeviceGraph& dg = g.dev;
+        int b = node_bits(n + 1);
+        int deg_bits = std::max(0, std::min(64 - 2 * b, 24));
+        DevBuf<uint64_t> k0, k1;
+        DevBuf<uint32_t> v0, v1;
+        k0.alloc(k);
+        k1.alloc(k);
+        v0.alloc(k);
+        v1.alloc(k);
+        source_key_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(
+            d_list, k, dg.out_off.p, dg.out_idx.p, (uint32_t)n, b, deg_bits, k0.p, v0.p);
+        NETBC_HIP(hipGetLastError());
+        int which = sort_pairs_u64(k0.p, v0.p, k1.p, v1.p, k, 2 * b + deg_bits, st);
+        ws.list2.alloc(k);
+        gather_sources_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(
+            d_list, k, which ? v1.p : v0.p, ws.list2.p);
+        NETBC_HIP(hipGetLastError());
+        NETBC_HIP(hipStreamSynchronize(st));
+        d_list = ws.list2.p;
+    }
     NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, sizeof(Ctrl), st));
+    ws.clean_rows(st);
+    ws.rows_dirty = true;
+    NETBC_HIP(hipStreamSynchronize(st));
+    ws.sweep.collect();
+    ws.sweep.ms = 0.0;
+    ws.sweep.launches = 0;
     DevBuf<double> djk_buf;
     DevBuf<uint32_t> djk_u32;
     DjkState djk{};
@@ -1054,8 +1452,10 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             for (;;) {
                 bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_dial(item0);
                 if (ok) break;
-                // Level queue overflow: grow and repeat the super-batch.
+                // Level queue overflow: grow and repeat the super-batch from
+                // clean rows.
                 if (ws.host_ctrl->qoverflow == 2) throw Error("bucket list overflow");
+                eng.clear_rows();
                 grow_queue(ws);
             }
             if (opt.precision) {
@@ -1075,10 +1475,13 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     NETBC_HIP(hipGetLastError());
     NETBC_HIP(hipEventRecord(ws.ev1, st));
     read_ctrl(ws, st);
+    ws.rows_dirty = false;
+    ws.sweep.collect();
     float ms = 0.0f;
     NETBC_HIP(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
     diag.kernel_ms += ms;
-    diag.dominant_ms += eng.dom_ms;
+    diag.dominant_ms += ws.sweep.ms;
+    diag.dominant_launches += ws.sweep.launches;
     Ctrl& c = *ws.host_ctrl;
     double mx;
     unsigned long long bits = c.max_sigma_bits;
@@ -1087,6 +1490,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     diag.sigma_overflow |= c.sig_overflow ? 1 : 0;
     diag.reached_nodes += (int64_t)c.reached_nodes;
     diag.reached_arcs += (int64_t)c.reached_arcs;
+    for (int i = 0; i < 8; i++) diag.work[i] += (int64_t)c.work[i];
 }
 
 void finish_scores(Graph& g, bool normalized, double* d_bc, double* out) {
@@ -1124,6 +1528,8 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     ws.list.alloc(k);
     NETBC_HIP(hipMemcpyAsync(ws.list.p, sources, k * sizeof(int64_t), hipMemcpyHostToDevice, st));
     NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, sizeof(Ctrl), st));
+    ws.clean_rows(st);
+    ws.rows_dirty = true;
     RunDiag diag;
     Engine eng{g, L, ws, o, diag, st, n, slots, ws.list.p, k};
     std::vector<double> rows((size_t)slots * n * kLanes);
@@ -1134,6 +1540,7 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_dial(item0);
             if (ok) break;
             if (ws.host_ctrl->qoverflow == 2) throw Error("bucket list overflow");
+            eng.clear_rows();
             grow_queue(ws);
         }
         int64_t cnt = std::min<int64_t>((int64_t)slots * kLanes, k - item0);
@@ -1181,6 +1588,7 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             else eng.backward_dial<double>(ws.coef64.p);
         }
         NETBC_HIP(hipMemcpyAsync(rows.data(), ws.sig.p, rows.size() * 8, hipMemcpyDeviceToHost, st));
+        eng.clear_rows();
         NETBC_HIP(hipStreamSynchronize(st));
         for (int64_t t = 0; t < cnt; t++) {
             int b = (int)(t / kLanes), l = (int)(t % kLanes);
@@ -1192,6 +1600,7 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             }
         }
     }
+    ws.rows_dirty = false;
 }
 
 }  // namespace netbc

@@ -6,12 +6,15 @@
 // validation order and texts as Graph::from_edges (graph.rs:142-234) and
 // error.rs:62-99.
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <new>
 
 #include "../../include/netbc.h"
 #include "netbc_internal.h"
+#include "small.h"
 
 #define NETBC_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -88,29 +91,109 @@ void fill_diag(netbc_diagnostics* out, const netbc::RunDiag& d, double build_ms)
     out->slots = d.slots;
     out->reserved = 0;
     out->build_ms = build_ms;
+    for (int i = 0; i < 8; i++) out->work[i] = d.work[i];
 }
 
-void run_sources(netbc::Graph& g, const int64_t* sources, int64_t k, int32_t weighted,
-                 int32_t weights_as, int32_t normalized, const netbc_options* opt, double* out,
-                 netbc_diagnostics* diag) {
-    std::lock_guard<std::mutex> lock(g.mu);
-    if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
-    double build_ms = prepare(g);
-    netbc::Lengths L;
-    netbc::classify_lengths(g, weighted != 0, weights_as, L);
+netbc::RunOptions options(const netbc_options* opt) {
     netbc::RunOptions o;
     if (opt) {
         o.precision = opt->precision;
         o.slots = opt->slots;
         o.direction = opt->direction;
         o.alpha = opt->alpha;
+        o.exact = opt->exact;
     }
+    return o;
+}
+
+void merge_diag(netbc::RunDiag& into, const netbc::RunDiag& d) {
+    into.max_sigma = std::max(into.max_sigma, d.max_sigma);
+    into.sigma_overflow |= d.sigma_overflow;
+    into.path_kind = std::max(into.path_kind, d.path_kind);
+    into.reached_nodes += d.reached_nodes;
+    into.reached_arcs += d.reached_arcs;
+    into.levels += d.levels;
+    into.kernel_ms += d.kernel_ms;
+    into.dominant_ms += d.dominant_ms;
+    into.dominant_launches += d.dominant_launches;
+    into.slots = std::max(into.slots, d.slots);
+    for (int i = 0; i < 8; i++) into.work[i] += d.work[i];
+}
+
+// Betweenness of one handle over a source list (sources == nullptr: item i
+// is node i), through the batched engine; returns the run's diagnostics.
+netbc::RunDiag run_graph(netbc::Graph& g, const int64_t* sources, int64_t k, int32_t weighted,
+                         int32_t weights_as, int32_t normalized, const netbc_options* opt,
+                         double* out, double* build_ms) {
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
+    double b = prepare(g);
+    if (build_ms) *build_ms = b;
+    netbc::Lengths L;
+    netbc::classify_lengths(g, weighted != 0, weights_as, L);
     netbc::RunDiag d;
     netbc::DevBuf<double> bc;
     bc.alloc(g.n);
-    netbc::run_batched(g, L, sources, k, o, bc.p, d);
+    netbc::run_batched(g, L, sources, k, options(opt), bc.p, d);
     netbc::finish_scores(g, normalized != 0, bc.p, out);
+    return d;
+}
+
+void run_sources(netbc::Graph& g, const int64_t* sources, int64_t k, int32_t weighted,
+                 int32_t weights_as, int32_t normalized, const netbc_options* opt, double* out,
+                 netbc_diagnostics* diag) {
+    double build_ms = 0.0;
+    netbc::RunDiag d = run_graph(g, sources, k, weighted, weights_as, normalized, opt, out,
+                                 &build_ms);
     fill_diag(diag, d, build_ms);
+}
+
+// The reference's scale for one graph (betweenness.rs:100-113).
+double graph_scale(uint64_t n, bool directed, bool normalized) {
+    double scale = directed ? 1.0 : 0.5;
+    if (normalized && n > 2) {
+        double pairs = (double)((n - 1) * (n - 2));
+        scale *= directed ? 1.0 / pairs : 2.0 / pairs;
+    }
+    return scale;
+}
+
+// Fills a host graph from edges, with net_graph_new's checks and texts
+// (graph.rs:142-234); `prefix` names the graph within a batch.
+void fill_graph(netbc::Graph& g, int64_t n, const int64_t* src, const int64_t* tgt,
+                const double* weights, int64_t m, bool directed, const std::string& prefix) {
+    if (n < 0 || m < 0) throw netbc::Error(prefix + "negative node or edge count");
+    if ((uint64_t)n > 0xFFFFFFFFull)
+        throw netbc::Error(prefix + "node count (" + std::to_string(n) +
+                           ") exceeds the supported maximum");
+    if (weights) {
+        for (int64_t e = 0; e < m; e++)
+            if (!std::isfinite(weights[e]))
+                throw netbc::Error(prefix + "edge " + std::to_string(e) +
+                                   " has a missing, NaN, or infinite weight");
+    }
+    if ((uint64_t)m > 0xFFFFFFFFull / 2)
+        throw netbc::Error(prefix + "edge count (" + std::to_string(m) +
+                           ") exceeds the supported maximum");
+    g.n = (uint64_t)n;
+    g.m = (uint64_t)m;
+    g.directed = directed;
+    g.src.resize((size_t)m);
+    g.tgt.resize((size_t)m);
+    for (int64_t e = 0; e < m; e++) {
+        int64_t ends[2] = {src[e], tgt[e]};
+        for (int64_t v : ends)
+            if (v < 0 || (uint64_t)v >= (uint64_t)n)
+                throw netbc::Error(prefix + "edge " + std::to_string(e) +
+                                   " refers to node index " + std::to_string(v) +
+                                   ", but valid indices are 0.." + std::to_string(n));
+        g.src[e] = (uint32_t)src[e];
+        g.tgt[e] = (uint32_t)tgt[e];
+    }
+    if (weights) {
+        g.has_weights = true;
+        g.w.assign(weights, weights + m);
+    }
 }
 
 }  // namespace
@@ -119,41 +202,9 @@ NETBC_EXPORT int32_t net_graph_new(int64_t n, const int64_t* src, const int64_t*
                                    const double* weights, int64_t m, int32_t directed,
                                    GraphHandle** out, char* err, size_t err_len) {
     return guard(err, err_len, [&] {
-        if (n < 0 || m < 0) throw netbc::Error("negative node or edge count");
-        if ((uint64_t)n > 0xFFFFFFFFull)
-            throw netbc::Error("node count (" + std::to_string(n) +
-                               ") exceeds the supported maximum");
-        if (weights) {
-            for (int64_t e = 0; e < m; e++)
-                if (!std::isfinite(weights[e]))
-                    throw netbc::Error("edge " + std::to_string(e) +
-                                       " has a missing, NaN, or infinite weight");
-        }
-        if ((uint64_t)m > 0xFFFFFFFFull / 2)
-            throw netbc::Error("edge count (" + std::to_string(m) +
-                               ") exceeds the supported maximum");
         auto* h = new GraphHandle();
-        netbc::Graph& g = h->g;
-        g.n = (uint64_t)n;
-        g.m = (uint64_t)m;
-        g.directed = directed != 0;
         try {
-            g.src.resize((size_t)m);
-            g.tgt.resize((size_t)m);
-            for (int64_t e = 0; e < m; e++) {
-                int64_t ends[2] = {src[e], tgt[e]};
-                for (int64_t v : ends)
-                    if (v < 0 || (uint64_t)v >= (uint64_t)n)
-                        throw netbc::Error("edge " + std::to_string(e) +
-                                           " refers to node index " + std::to_string(v) +
-                                           ", but valid indices are 0.." + std::to_string(n));
-                g.src[e] = (uint32_t)src[e];
-                g.tgt[e] = (uint32_t)tgt[e];
-            }
-            if (weights) {
-                g.has_weights = true;
-                g.w.assign(weights, weights + m);
-            }
+            fill_graph(h->g, n, src, tgt, weights, m, directed != 0, "");
         } catch (...) {
             delete h;
             throw;
@@ -267,5 +318,103 @@ NETBC_EXPORT int32_t netbc_device_info(char* buf, size_t len) {
         std::string s = std::string(p.name) + ", " + p.gcnArchName + ", " +
                         std::to_string(p.multiProcessorCount) + " CUs";
         write_err(buf, len, s);
+    });
+}
+
+NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node_counts,
+                                             const int64_t* edge_counts, const int64_t* src,
+                                             const int64_t* tgt, const double* weights,
+                                             int32_t directed, int32_t weighted,
+                                             int32_t weights_as, int32_t normalized,
+                                             const netbc_options* opt, double* out,
+                                             netbc_diagnostics* diag, char* err,
+                                             size_t err_len) {
+    return guard(err, err_len, [&] {
+        if (graphs < 0) throw netbc::Error("negative graph count");
+        check_kind(weights_as);
+        if (opt && (opt->precision < 0 || opt->precision > 1))
+            throw netbc::Error("unknown precision code " + std::to_string(opt->precision));
+        if (weighted && !weights) throw netbc::Error("the graph has no edge weights");
+        // Validate every graph as net_graph_new would, and lay them out as
+        // one union graph with consecutive node ranges.
+        std::vector<uint64_t> node0((size_t)graphs + 1, 0), edge0((size_t)graphs + 1, 0);
+        for (int64_t i = 0; i < graphs; i++) {
+            if (node_counts[i] < 0 || edge_counts[i] < 0)
+                throw netbc::Error("graph " + std::to_string(i) +
+                                   ": negative node or edge count");
+            node0[i + 1] = node0[i] + (uint64_t)node_counts[i];
+            edge0[i + 1] = edge0[i] + (uint64_t)edge_counts[i];
+        }
+        if (node0[graphs] > 0xFFFFFFFFull || edge0[graphs] > 0xFFFFFFFFull / 2)
+            throw netbc::Error("the batch exceeds 2^32 nodes or 2^31 edges in total");
+        netbc::Graph U;
+        U.n = node0[graphs];
+        U.m = edge0[graphs];
+        U.directed = directed != 0;
+        U.src.resize(U.m);
+        U.tgt.resize(U.m);
+        if (weights) {
+            U.has_weights = true;
+            U.w.assign(weights, weights + U.m);
+        }
+        for (int64_t i = 0; i < graphs; i++) {
+            netbc::Graph one;
+            uint64_t e0 = edge0[i];
+            fill_graph(one, node_counts[i], src + e0, tgt + e0, weights ? weights + e0 : nullptr,
+                       edge_counts[i], directed != 0, "graph " + std::to_string(i) + ": ");
+            for (uint64_t e = 0; e < one.m; e++) {
+                U.src[e0 + e] = one.src[e] + (uint32_t)node0[i];
+                U.tgt[e0 + e] = one.tgt[e] + (uint32_t)node0[i];
+            }
+        }
+        netbc::RunDiag total;
+        netbc::RunOptions o = options(opt);
+        std::vector<netbc::SmallGraph> descs((size_t)graphs);
+        std::vector<uint32_t> node_graph(U.n);
+        uint64_t gsum_off = 0;
+        bool any_device = false;
+        for (int64_t i = 0; i < graphs; i++) {
+            netbc::SmallGraph& G = descs[i];
+            uint64_t n = (uint64_t)node_counts[i];
+            G.n = (uint32_t)n;
+            G.node0 = (uint32_t)node0[i];
+            G.gs = n ? (uint32_t)((n + 63) / 64) : 1;
+            G.groups = n ? (uint32_t)((n + G.gs - 1) / G.gs) : 0;
+            G.scale = graph_scale(n, directed != 0, normalized != 0);
+            G.device = (!weighted && n > 0 && n <= netbc::kSmallMaxNodes) ? 1 : 0;
+            G.gsum_off = gsum_off;
+            G.pad = 0;
+            if (G.device) {
+                gsum_off += (uint64_t)G.groups * n;
+                any_device = true;
+            } else {
+                G.groups = 0;
+            }
+            for (uint64_t v = 0; v < n; v++) node_graph[node0[i] + v] = (uint32_t)i;
+        }
+        std::vector<uint32_t> too_deep((size_t)graphs, 0);
+        if (any_device) {
+            prepare(U);
+            netbc::run_small_batch(U, descs, node_graph, o, out, too_deep, total);
+        }
+        // Graphs the one-workgroup engine does not take: the batched engine,
+        // one graph at a time.
+        for (int64_t i = 0; i < graphs; i++) {
+            if (descs[i].device && !too_deep[i]) continue;
+            if (node_counts[i] == 0) continue;
+            netbc::Graph one;
+            uint64_t e0 = edge0[i];
+            fill_graph(one, node_counts[i], src + e0, tgt + e0, weights ? weights + e0 : nullptr,
+                       edge_counts[i], directed != 0, "");
+            netbc::RunDiag d;
+            try {
+                d = run_graph(one, nullptr, node_counts[i], weighted, weights_as, normalized, opt,
+                              out + node0[i], nullptr);
+            } catch (const netbc::Error& e) {
+                throw netbc::Error("graph " + std::to_string(i) + ": " + e.what());
+            }
+            merge_diag(total, d);
+        }
+        fill_diag(diag, total, 0.0);
     });
 }

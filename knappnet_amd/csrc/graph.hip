@@ -360,6 +360,13 @@ std::string rust_f64(double x) {
 
 }  // namespace
 
+int sort_pairs_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t len, int bits,
+                   hipStream_t st) {
+    return radix_sort_pairs(k0, v0, k1, v1, len, bits, st);
+}
+
+int node_bits(uint64_t n) { return bits_for(n); }
+
 Graph::~Graph() {
     if (workspace && workspace_free) workspace_free(workspace);
     if (stream) (void)hipStreamDestroy(stream);
@@ -420,6 +427,45 @@ void build_device_graph(Graph& g) {
         build_one(g.n, g.m, true, true, ds.p, dt.p, g.has_weights ? dw.p : nullptr, nullptr,
                   arcs, d.in_off, d.in_idx, d.in_w, st);
     NETBC_HIP(hipStreamSynchronize(st));
+    // Hub tables from the in-degrees.
+    std::vector<uint32_t> ioff(g.n + 1);
+    NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> flag(std::max<uint64_t>(g.n, 1), 0);
+    std::vector<uint32_t> hub, chunk0(1, 0);
+    for (uint64_t v = 0; v < g.n; v++) {
+        uint32_t deg = ioff[v + 1] - ioff[v];
+        if (deg > kHubIn) {
+            flag[v] = 1;
+            hub.push_back((uint32_t)v);
+            chunk0.push_back(chunk0.back() + (deg + kHubChunk - 1) / kHubChunk);
+        }
+    }
+    d.hubs = (uint32_t)hub.size();
+    d.hub_chunks = chunk0.back();
+    d.hub_flag.alloc(flag.size());
+    NETBC_HIP(hipMemcpy(d.hub_flag.p, flag.data(), flag.size(), hipMemcpyHostToDevice));
+    d.hub_node.alloc(hub.size());
+    d.hub_chunk0.alloc(chunk0.size());
+    if (!hub.empty())
+        NETBC_HIP(hipMemcpy(d.hub_node.p, hub.data(), hub.size() * 4, hipMemcpyHostToDevice));
+    NETBC_HIP(hipMemcpy(d.hub_chunk0.p, chunk0.data(), chunk0.size() * 4, hipMemcpyHostToDevice));
+    // Heavy rows from the out-degrees.
+    std::vector<uint32_t> ooff(g.n + 1);
+    NETBC_HIP(hipMemcpy(ooff.data(), d.out_off.p, (g.n + 1) * 4, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> hflag(std::max<uint64_t>(g.n, 1), 0);
+    uint32_t maxdeg = 0;
+    d.heavy = 0;
+    for (uint64_t v = 0; v < g.n; v++) {
+        uint32_t deg = ooff[v + 1] - ooff[v];
+        maxdeg = std::max(maxdeg, deg);
+        if (deg > kHeavyOut) {
+            hflag[v] = 1;
+            d.heavy++;
+        }
+    }
+    d.max_chunks = std::max<uint32_t>(1, (maxdeg + kChunkArcs - 1) / kChunkArcs);
+    d.heavy_flag.alloc(hflag.size());
+    NETBC_HIP(hipMemcpy(d.heavy_flag.p, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
     g.dev_ready = true;
 }
 

@@ -16,6 +16,10 @@ namespace netbc {
 constexpr int kLanes = 64;        // sources per batch = wavefront width on gfx950
 constexpr int kBlock = 256;       // threads per block for the batched kernels
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kHubIn = 512;     // in-degree above which a node is a hub
+constexpr uint32_t kHubChunk = 512;  // in-arcs per hub pull task
+constexpr uint32_t kHeavyOut = 1024; // out-degree above which a row is split into tasks
+constexpr uint32_t kChunkArcs = 256; // out-arcs per heavy-row task
 
 // An error that reaches the C caller as status 1 and its message.
 struct Error : std::runtime_error {
@@ -70,6 +74,16 @@ struct DeviceGraph {
     bool weighted = false;
     DevBuf<uint32_t> out_off, out_idx, in_off, in_idx;
     DevBuf<double> out_w, in_w;  // arc weights in CSR / CSC order
+    // Hubs: nodes with more than kHubIn in-arcs. Their path counts are
+    // pulled in chunks of kHubChunk in-arcs instead of pushed.
+    DevBuf<uint8_t> hub_flag;          // per node
+    DevBuf<uint32_t> hub_node;         // hub ids
+    DevBuf<uint32_t> hub_chunk0;       // first chunk of each hub (prefix), size hubs + 1
+    uint32_t hubs = 0, hub_chunks = 0;
+    // Heavy rows: out-degree above kHeavyOut, expanded and swept in tasks
+    // of kChunkArcs arcs.
+    DevBuf<uint8_t> heavy_flag;        // per node
+    uint32_t heavy = 0, max_chunks = 1;
     const uint32_t* in_off_p() const { return directed ? in_off.p : out_off.p; }
     const uint32_t* in_idx_p() const { return directed ? in_idx.p : out_idx.p; }
     const double* in_w_p() const { return directed ? in_w.p : out_w.p; }
@@ -109,6 +123,7 @@ struct RunOptions {
     int direction = 0;      // 0 auto, 1 top-down only, 2 bottom-up whenever possible
     double alpha = 0.0;     // bottom-up switch: frontier arcs > unvisited arcs / alpha (0 = default)
     int keep_sigma = 0;     // debug: keep per-source sigma and levels
+    int exact = 0;          // 1: sum every row's successors in CSR order (no heavy-row split)
 };
 
 struct RunDiag {
@@ -124,6 +139,7 @@ struct RunDiag {
     int64_t dominant_bytes = 0;  // algorithmic bytes of the dominant kernel, summed
     int32_t slots = 0;
     int64_t bottom_up_levels = 0;
+    int64_t work[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // engine work counters (diagnostic builds)
 };
 
 // Builds the device CSR/CSC of g (graph.hip). Caller holds g.mu.
@@ -149,5 +165,13 @@ void finish_scores(Graph& g, bool normalized, double* d_bc, double* out);
 
 // Ensures g has a stream on the current device.
 void ensure_stream(Graph& g);
+
+// Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
+// keys (graph.hip). Returns 0 if the result is in (k0, v0), 1 if in (k1, v1).
+int sort_pairs_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t len, int bits,
+                   hipStream_t st);
+
+// Bits needed for node ids 0..n-1 (at least 1).
+int node_bits(uint64_t n);
 
 }  // namespace netbc

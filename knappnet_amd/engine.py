@@ -82,11 +82,11 @@ class Graph:
     def betweenness_sources(self, sources=None, weighted: bool = False,
                             normalized: bool = False, weights_as: str = "distance",
                             precision: int = 0, slots: int = 0, direction: int = 0,
-                            alpha: float = 0.0):
+                            alpha: float = 0.0, exact: bool = False):
         """``netbc_betweenness_sources``: returns (scores, diagnostics dict)."""
         out = np.zeros(max(self.n, 1), dtype=np.float64)
         err = ctypes.create_string_buffer(ERR_LEN)
-        opt = _native.Options(precision, slots, direction, 0, alpha)
+        opt = _native.Options(precision, slots, direction, int(bool(exact)), alpha)
         diag = _native.Diagnostics()
         kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
         if sources is None:
@@ -129,6 +129,38 @@ class Graph:
                                     None if w is None else _ptr(w, c_double), err, ERR_LEN):
             raise EngineError(err.value.decode())
         return off, tgt[:arcs], None if w is None else w[:arcs]
+
+
+def betweenness_batch(graphs, directed: bool = False, weighted: bool = False,
+                      normalized: bool = False, weights_as: str = "distance",
+                      precision: int = 0, slots: int = 0):
+    """``netbc_betweenness_batch`` over a list of (n, src, tgt[, weights])
+    graphs; returns (list of per-graph score arrays, diagnostics dict)."""
+    lib = _native.device()
+    ns = np.array([int(g[0]) for g in graphs], dtype=np.int64)
+    ms = np.array([len(g[1]) for g in graphs], dtype=np.int64)
+    src = _i64(np.concatenate([np.asarray(g[1], dtype=np.int64) for g in graphs])
+               if graphs else np.zeros(0, np.int64))
+    tgt = _i64(np.concatenate([np.asarray(g[2], dtype=np.int64) for g in graphs])
+               if graphs else np.zeros(0, np.int64))
+    w = None
+    if weighted:
+        w = np.ascontiguousarray(np.concatenate([np.asarray(g[3], dtype=np.float64)
+                                                 for g in graphs]))
+    out = np.zeros(max(int(ns.sum()), 1), dtype=np.float64)
+    err = ctypes.create_string_buffer(ERR_LEN)
+    opt = _native.Options(precision, slots, 0, 0, 0.0)
+    diag = _native.Diagnostics()
+    kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+    if lib.netbc_betweenness_batch(len(graphs), _ptr(ns, c_int64), _ptr(ms, c_int64),
+                                   _ptr(src, c_int64), _ptr(tgt, c_int64),
+                                   None if w is None else _ptr(w, c_double), int(bool(directed)),
+                                   int(bool(weighted)), kind, int(bool(normalized)),
+                                   ctypes.byref(opt), _ptr(out, c_double), ctypes.byref(diag),
+                                   err, ERR_LEN):
+        raise EngineError(err.value.decode())
+    parts = np.split(out[: int(ns.sum())], np.cumsum(ns)[:-1]) if len(graphs) else []
+    return parts, diag.as_dict()
 
 
 def device_info() -> str:

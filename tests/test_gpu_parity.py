@@ -30,6 +30,16 @@ def pair(n, s, t, w=None, directed=False):
     return Graph(n, s, t, w, directed), OracleGraph(n, s, t, w, directed)
 
 
+def assert_rel(a, b, what, tol=1e-12):
+    """FP64 scores with heavy rows summed in chunks: within 1e-12 relative."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    err = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+    err[(a == 0) & (b == 0)] = 0.0
+    assert float(err.max(initial=0.0)) <= tol, f"{what}: max rel err {err.max():.3e}"
+
+
 def assert_same(a, b, what):
     a = np.asarray(a)
     b = np.asarray(b)
@@ -164,10 +174,29 @@ def test_er_2000_bitexact():
 
 @pytest.mark.parametrize("directed", [True, False])
 def test_rmat_bitexact(directed):
-    el = gen.rmat(12, 16, directed=directed, seed=5)
-    say("rmat12", directed, el.m)
+    # R-MAT scale 12 has rows longer than 1024 arcs: the exact mode sums them
+    # in CSR order (bit-exact); the default splits them into chunks
+    # (deterministic, within 1e-12).
+    el = gen.rmat(12, 32, directed=directed, seed=5)
+    deg = np.bincount(el.src, minlength=el.n) + (0 if directed else
+                                                 np.bincount(el.tgt, minlength=el.n))
+    say("rmat12", directed, el.m, "max degree", int(deg.max()))
+    assert deg.max() > 1024
     g, o = pair(el.n, el.src, el.tgt, None, directed)
-    assert_same(g.betweenness(), o.betweenness(threads=0), "rmat12")
+    want = o.betweenness(threads=0)
+    exact, _ = g.betweenness_sources(exact=True)
+    assert_same(exact, want, "rmat12 exact")
+    fast = g.betweenness()
+    assert_rel(fast, want, "rmat12 split")
+    again, _ = g.betweenness_sources()
+    assert_same(again, fast, "split mode is deterministic")
+    # levels and path counts stay bit-exact in either mode
+    srcs = np.array([0, 1, 2, 17, 4095])
+    dist, sig, _ = g.state_dump(srcs)
+    for i, src in enumerate(srcs):
+        _, du, _, osig, _, _ = o.single_source(src)
+        assert_same(dist[i], du, f"dist {src}")
+        assert_same(sig[i], osig, f"sigma {src}")
 
 
 def test_dial_ba_bitexact():
@@ -213,10 +242,30 @@ def test_source_list_matches_oracle():
     srcs = gen.sample_sources(el.n, 300, 7)
     g, o = pair(el.n, el.src, el.tgt, None, True)
     say("source list")
+    want = o.betweenness_sources(srcs, threads=0)
+    got, _ = g.betweenness_sources(srcs, exact=True)
+    assert_same(got, want, "sources")
     got, _ = g.betweenness_sources(srcs)
-    assert_same(got, o.betweenness_sources(srcs, threads=0), "sources")
+    assert_rel(got, want, "sources split")
     full, _ = g.betweenness_sources(np.arange(el.n))
     assert_same(full, g.betweenness(), "full list")
+
+
+def test_default_mode_large_source_sets():
+    # 4096 or more sources: batches by neighbourhood (and chunked heavy rows)
+    # in the default mode, reference order in the exact mode.
+    for directed in (True, False):
+        el = gen.rmat(13, 8, directed=directed, seed=21)
+        g, o = pair(el.n, el.src, el.tgt, None, directed)
+        say("default mode", directed, el.n, el.m)
+        want = o.betweenness(threads=0)
+        exact, _ = g.betweenness_sources(exact=True)
+        assert_same(exact, want, "exact")
+        fast, _ = g.betweenness_sources()
+        assert_rel(fast, want, "default")
+        assert_same(g.betweenness_sources()[0], fast, "deterministic")
+        assert_rel(g.betweenness(normalized=True), o.betweenness(normalized=True, threads=0),
+                   "normalized")
 
 
 def test_fp32_dependencies_within_1e6():
@@ -251,3 +300,48 @@ def test_sigma_overflow_is_flagged():
     g = Graph(n, s[: 4 * 20], t[: 4 * 20], None, True)
     _, diag = g.betweenness_sources()
     assert diag["sigma_overflow"] == 0 and diag["max_sigma"] == 2.0**20
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_batch_of_graphs_bitexact(directed):
+    from knappnet_amd.engine import betweenness_batch
+    rng = np.random.default_rng(31)
+    graphs = []
+    for i in range(40):
+        n = int(rng.integers(1, 300)) if i % 7 else int(rng.integers(1500, 2100))
+        m = int(rng.integers(0, 4 * n))
+        graphs.append((n, rng.integers(0, n, m), rng.integers(0, n, m)))
+    graphs.append((0, [], []))
+    graphs.append((2500, rng.integers(0, 2500, 5000), rng.integers(0, 2500, 5000)))
+    say("batch", directed, len(graphs))
+    for normalized in (False, True):
+        parts, diag = betweenness_batch(graphs, directed=directed, normalized=normalized)
+        for i, (n, s, t) in enumerate(graphs):
+            want = OracleGraph(n, s, t, None, directed).betweenness(normalized=normalized,
+                                                                    threads=0)
+            assert_same(parts[i], want, f"graph {i} (n={n})")
+
+
+def test_batch_watts_strogatz_and_weighted():
+    from knappnet_amd.engine import betweenness_batch
+    sizes = gen.ws_sizes(24, 11)
+    els = [gen.small_world(int(sizes[i]), 6, 0.1, 11 + i, 0) for i in range(24)]
+    graphs = [(e.n, e.src, e.tgt) for e in els]
+    say("ws batch")
+    parts, diag = betweenness_batch(graphs)
+    for i, e in enumerate(els):
+        assert_same(parts[i], OracleGraph(e.n, e.src, e.tgt, None, False).betweenness(threads=0),
+                    f"ws {i}")
+    # a deep graph (path of 600 nodes: 599 levels) goes to the level-synchronous engine
+    deep = (600, np.arange(599), np.arange(1, 600))
+    parts, _ = betweenness_batch([deep, graphs[0]])
+    assert list(parts[0]) == [float(i * (599 - i)) for i in range(600)]
+    rng = np.random.default_rng(4)
+    wg = [(e.n, e.src, e.tgt, 1.0 + rng.integers(0, 16, e.m)) for e in els[:4]]
+    parts, _ = betweenness_batch(wg, weighted=True)
+    for i, (n, s, t, w) in enumerate(wg):
+        assert_same(parts[i], OracleGraph(n, s, t, w, False).betweenness(weighted=True,
+                                                                          threads=0), f"w {i}")
+    with pytest.raises(EngineError, match=r"^graph 1: edge 0 refers to node index 5, but "
+                                          r"valid indices are 0..3$"):
+        betweenness_batch([(2, [0], [1]), (3, [5], [0])])
