@@ -56,6 +56,65 @@ def build_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def run_ws_batch(args) -> None:
+    """4,096 Watts-Strogatz graphs (n uniform in [50, 2000], k = 6, p = 0.1)
+    in one netbc_betweenness_batch call, against one net_betweenness call
+    per graph (timed on a prefix of the graphs)."""
+    from knappnet_amd import generate as gen
+    from knappnet_amd.engine import Graph, betweenness_batch, device_info
+    t0 = time.perf_counter()
+    els = gen.ws_batch(4096, 11)
+    gen_s = time.perf_counter() - t0
+    graphs = [(e.n, e.src, e.tgt) for e in els]
+    work = sum(e.n * e.m for e in els)  # sources x edges
+    print(f"# device: {device_info()}; 4096 Watts-Strogatz graphs, sum n = "
+          f"{sum(e.n for e in els)}, sum m = {sum(e.m for e in els)} "
+          f"(generated in {gen_s:.1f}s)", flush=True)
+    for i in range(args.warmup):
+        parts, diag = betweenness_batch(graphs, precision=args.precision)
+    times = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        parts, diag = betweenness_batch(graphs, precision=args.precision)
+        times.append(time.perf_counter() - t)
+        print(f"# step {i}: {times[-1]:.3f}s kernel_ms={diag['kernel_ms']:.1f}", flush=True)
+    sec = sum(times) / len(times)
+    # one launch per graph, the reference's caller pattern (r/R/cug.R:42-47)
+    k1 = min(len(els), 256)
+    t = time.perf_counter()
+    one = [Graph(e.n, e.src, e.tgt, None, False).betweenness() for e in els[:k1]]
+    t1 = time.perf_counter() - t
+    per_graph_s = t1 / sum(e.n * e.m for e in els[:k1]) * work
+    same = all(np.array_equal(a, b) for a, b in zip(one, parts[:k1]))
+    cpu = None
+    if args.cpu_seconds > 0:
+        from oracle.oracle import OracleGraph, available_cores
+        sel = els[:: max(1, len(els) // 64)]
+        t = time.perf_counter()
+        ok = True
+        for e in sel:
+            r = OracleGraph(e.n, e.src, e.tgt, None, False).betweenness(threads=0)
+        ct = time.perf_counter() - t
+        cw = sum(e.n * e.m for e in sel)
+        cpu = {"value": round(cw / ct / 1e9, 4), "unit": "GTEPS", "cores": available_cores(),
+               "kind": "port", "seconds": round(ct, 3),
+               "sample": f"{len(sel)} of 4096 graphs (every 64th), one at a time, all host "
+                         f"threads, CPU: {cpu_model()}"}
+    line = {
+        "metric": METRIC, "value": round(work / sec / 1e9, 3), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == 0 else "f32-delta/f64-sigma", "data": "synthetic",
+        "config": {"workload": "watts_strogatz_4096_graphs_all_sources", "k": 6, "p": 0.1,
+                   "n_range": [50, 2000], "base_seed": 11, "graphs_per_s": round(4096 / sec, 1)},
+        "one_graph_per_launch": {"seconds_extrapolated": round(per_graph_s, 3),
+                                 "timed_graphs": k1, "speedup": round(per_graph_s / sec, 2),
+                                 "bit_identical": same},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -75,6 +134,8 @@ def main() -> None:
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 dependencies")
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--direction", type=int, default=0)
+    ap.add_argument("--compare-precision", action="store_true",
+                    help="also run the other precision and report the relative error")
     ap.add_argument("--max-sources", type=int, default=0,
                     help="time only the first K sources (experiments; not the headline)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -88,6 +149,10 @@ def main() -> None:
         sys.exit(2)
 
     from knappnet_amd.engine import Graph, device_info
+
+    if args.workload == "ws4096":
+        run_ws_batch(args)
+        return
 
     t0 = time.perf_counter()
     el, sources, config = build_workload(args.workload)
@@ -159,7 +224,11 @@ def main() -> None:
             ct = time.perf_counter() - t
             if ct >= args.cpu_seconds / 4 or take >= len(pool):
                 break
-            take = min(len(pool), int(take * max(2.0, args.cpu_seconds / max(ct, 1e-3))))
+            take = min(len(pool), int(take * max(2.0, args.cpu_seconds / 2 / max(ct, 1e-3))))
+        # best of two timed runs of the final sample (the host may be shared)
+        t = time.perf_counter()
+        o.betweenness_sources(sample, weighted=weighted, threads=0)
+        ct = min(ct, time.perf_counter() - t)
         got, _ = g.betweenness_sources(sample, weighted=weighted, precision=args.precision)
         exact = bool(np.array_equal(got, want))
         nz = (got != 0) | (want != 0)
@@ -173,6 +242,24 @@ def main() -> None:
                "check": "bit-exact" if exact else f"max rel err {rel:.3e}"}
         print(f"# cpu sample: {len(sample)} sources in {ct:.2f}s -> {cpu_gteps:.3f} GTEPS; "
               f"device vs oracle on sample: {cpu['check']}", flush=True)
+
+    precision_check = None
+    if args.compare_precision:
+        other = 1 - args.precision
+        t = time.perf_counter()
+        res2, d2 = g.betweenness_sources(sources=sources, weighted=weighted, precision=other,
+                                         slots=args.slots, direction=args.direction)
+        t2 = time.perf_counter() - t
+        f64, f32 = (res, res2) if args.precision == 0 else (res2, res)
+        nz = f64 != 0
+        rel = np.abs(f32[nz] - f64[nz]) / np.abs(f64[nz])
+        precision_check = {
+            "fp32_delta_vs_fp64_max_rel_err": float(rel.max()) if nz.any() else 0.0,
+            "fp32_delta_vs_fp64_mean_rel_err": float(rel.mean()) if nz.any() else 0.0,
+            "other_precision_seconds": round(t2, 3),
+            "fp32_speedup_over_fp64": round((sec / t2) if args.precision == 0 else (t2 / sec), 3),
+        }
+        print(f"# precision check: {precision_check}", flush=True)
 
     line = {
         "metric": METRIC,
@@ -192,6 +279,7 @@ def main() -> None:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(gteps / cpu["value"], 1) if cpu else None,
+        "precision_check": precision_check,
         "sigma_overflow": d["sigma_overflow"],
     }
     print(json.dumps(line), flush=True)

@@ -42,8 +42,9 @@ struct Ctrl {
     uint32_t qlen;
     uint32_t tlen[2];
     uint32_t qoverflow;
-    uint32_t pending;
-    uint32_t hlen;  // heavy-row entries listed (positions in the level queue)
+    uint32_t hlen;       // heavy-row entries listed (positions in the level queue)
+    uint32_t left[2];    // integer lengths: tentative distances still pending
+    uint32_t hchunks;    // chunk tasks of the heavy rows listed last
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
@@ -74,6 +75,8 @@ constexpr double kTwo53 = 9007199254740992.0;
 #define NETBC_AHEAD 8
 #endif
 constexpr int kAhead = NETBC_AHEAD;
+// Touched nodes staged per wave before one append to the shared list.
+constexpr uint32_t kTouchBuf = 256;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ size_t global_wave() {
@@ -123,7 +126,8 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
                                     uint64_t* __restrict__ g0, double* __restrict__ sig,
                                     QEnt* __restrict__ q, Ctrl* ctrl, uint32_t* __restrict__ dist,
                                     const uint8_t* __restrict__ heavy_flag,
-                                    uint32_t* __restrict__ hq) {
+                                    uint32_t* __restrict__ hq, uint64_t* __restrict__ pend,
+                                    int words) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
     int b = t / kLanes, l = t % kLanes;
@@ -136,6 +140,7 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
     if (g0) atomicOr((unsigned long long*)&g0[mi], bit);
     sig[mi * kLanes + l] = 1.0;
     if (dist) dist[mi * kLanes + l] = 0;
+    if (pend) atomicOr((unsigned long long*)&pend[mi * words], 1ull);  // bucket of distance 0
     uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
     q[pos] = QEnt{s, (uint32_t)b, bit};
     if (hq && heavy_flag[s]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
@@ -155,6 +160,61 @@ __global__ void fill_u64_kernel(uint64_t* __restrict__ p, size_t count, uint64_t
 
 // ---- breadth-first search: forward -----------------------------------------
 
+// Chunk counts of the heavy rows listed in hq[hbeg, *hend): hpre[i] = first
+// chunk task of entry i (relative to hbeg); *total = all of them. (hpre at
+// *hend is overwritten by the next level's scan.)
+__global__ void heavy_scan_kernel(const uint32_t* __restrict__ hq, uint32_t hbeg,
+                                  const uint32_t* __restrict__ hend,
+                                  const QEnt* __restrict__ q, const uint32_t* __restrict__ off,
+                                  uint32_t* __restrict__ hpre, uint32_t* __restrict__ total) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    const uint32_t he = *hend;
+    const int lane = lane_id(), wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t i0 = hbeg; i0 < he; i0 += blockDim.x) {
+        uint32_t i = i0 + threadIdx.x;
+        uint32_t c = 0;
+        if (i < he) {
+            uint32_t v = q[hq[i]].v;
+            c = (off[v + 1] - off[v] + kChunkArcs - 1) / kChunkArcs;
+        }
+        uint32_t x = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        uint32_t before = carry, tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            if (w < wid) before += wsum[w];
+            tot += wsum[w];
+        }
+        if (i < he) hpre[i] = before + x - c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        hpre[he] = carry;
+        *total = carry;
+    }
+}
+
+// Heavy entry of chunk task t: the last i in [hbeg, hend) with hpre[i] <= t.
+__device__ __forceinline__ uint32_t heavy_of_task(const uint32_t* __restrict__ hpre,
+                                                  uint32_t hbeg, uint32_t hend, uint32_t t) {
+    uint32_t lo = hbeg, hi = hend;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (hpre[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 // Top-down step of level d. Every frontier entry (v, b, M) offers its
 // lanes to each out-neighbour x not yet reached in them (nv = M & ~vis[x]),
 // records x in the touched list on the first offer of the level, and adds
@@ -164,7 +224,8 @@ __global__ void fill_u64_kernel(uint64_t* __restrict__ p, size_t count, uint64_t
 // also clear the stale level d-1 entries of the next-level bitmap.
 __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                   uint32_t zbeg, uint32_t zend, const uint32_t* __restrict__ hq,
-                                  uint32_t hbeg, uint32_t hend, uint32_t max_chunks,
+                                  uint32_t hbeg, uint32_t hend, const uint32_t* __restrict__ hpre,
+                                  uint32_t hchunks,
                                   const uint32_t* __restrict__ off,
                                   const uint32_t* __restrict__ idx,
                                   const uint64_t* __restrict__ vis, uint64_t* __restrict__ cand,
@@ -180,9 +241,21 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
         QEnt e = q[i];
         nxt[(size_t)e.b * n + e.v] = 0;
     }
+    // Touched nodes are staged per wave in LDS and appended kTouchBuf at a
+    // time, so the shared list counter sees one atomic per batch of nodes.
+    __shared__ uint2 tbuf[kBlock / 64][kTouchBuf];
+    uint2* mybuf = tbuf[threadIdx.x >> 6];
+    uint32_t tcnt = 0;  // wave-uniform
+    auto flush = [&]() {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(tlen, tcnt);
+        at = rl32(at, 0);
+        for (uint32_t i = lane; i < tcnt; i += 64) touched[at + i] = mybuf[i];
+        tcnt = 0;
+    };
     const uint64_t lt = lanemask_lt(lane);
     const size_t nq = qend - qbeg;
-    const size_t tasks = nq + (size_t)(hend - hbeg) * max_chunks;
+    const size_t tasks = nq + hchunks;
     for (size_t w = global_wave(); w < tasks; w += wave_count()) {
         // A task is a whole light row, or one chunk of a heavy row.
         QEnt en;
@@ -193,11 +266,11 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
             a0 = off[en.v];
             a1 = off[en.v + 1];
         } else {
-            size_t t = w - nq;
-            en = q[hq[hbeg + t / max_chunks]];
-            a0 = off[en.v] + (uint32_t)(t % max_chunks) * kChunkArcs;
+            uint32_t t = (uint32_t)(w - nq);
+            uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
+            en = q[hq[h]];
+            a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
             a1 = min(a0 + kChunkArcs, off[en.v + 1]);
-            if (a0 >= a1) continue;
         }
         size_t base = (size_t)en.b * n;
         const bool mine = (en.m >> lane) & 1;
@@ -221,11 +294,10 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
             }
             uint64_t bal = __ballot(first);
             if (bal) {
-                int leader = __ffsll((unsigned long long)bal) - 1;
-                uint32_t pos = 0;
-                if (lane == leader) pos = atomicAdd(tlen, (uint32_t)__popcll(bal));
-                pos = rl32(pos, leader);
-                if (first) touched[pos + __popcll(bal & lt)] = make_uint2(x, en.b);
+                uint32_t k = (uint32_t)__popcll(bal);
+                if (tcnt + k > kTouchBuf) flush();
+                if (first) mybuf[tcnt + __popcll(bal & lt)] = make_uint2(x, en.b);
+                tcnt += k;
             }
             // Hubs pull their counts (bfs_hub_pull_kernel) instead.
             uint64_t push = __ballot(nv != 0 && !hub_flag[x]);
@@ -238,6 +310,7 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
             }
         }
     }
+    if (tcnt) flush();
     NETBC_FLUSH(ctrl, 0, c0);
     NETBC_FLUSH(ctrl, 1, c1);
     NETBC_FLUSH(ctrl, 2, c2);
@@ -435,7 +508,7 @@ template <class C>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level,
                                     const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
-                                    uint32_t max_chunks, int split,
+                                    const uint32_t* __restrict__ hpre, uint32_t hchunks, int split,
                                     const uint8_t* __restrict__ heavy_flag,
                                     double* __restrict__ part,
                                     const uint32_t* __restrict__ off,
@@ -460,15 +533,15 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
     }
     split = split && !last_level;
     const size_t nq = qend - qbeg;
-    const size_t tasks = nq + (split ? (size_t)(hend - hbeg) * max_chunks : 0);
+    const size_t tasks = nq + (split ? hchunks : 0);
     for (size_t w = global_wave(); w < tasks; w += wave_count()) {
         if (w >= nq) {
             // One chunk of a heavy row: its partial sum.
-            size_t t = w - nq;
-            QEnt en = q[hq[hbeg + t / max_chunks]];
-            uint32_t a0 = off[en.v] + (uint32_t)(t % max_chunks) * kChunkArcs;
+            uint32_t t = (uint32_t)(w - nq);
+            uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
+            QEnt en = q[hq[h]];
+            uint32_t a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
             uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
-            if (a0 >= a1) continue;
             Dep<C> dv;
             sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
             part[t * kLanes + lane] = dv.value();
@@ -505,18 +578,20 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
 // chunk order, then the row's dependencies and coefficients.
 template <class C>
 __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* __restrict__ hq,
-                                   uint32_t hbeg, uint32_t hend, uint32_t max_chunks,
-                                   const double* __restrict__ part,
+                                   uint32_t hbeg, uint32_t hend, const uint32_t* __restrict__ hpre,
+                                   uint32_t hchunks, const double* __restrict__ part,
                                    const uint32_t* __restrict__ off, double* __restrict__ sig,
                                    C* __restrict__ coef, Ctrl* ctrl, size_t n) {
     const int lane = lane_id();
     double smax = 0.0;
     for (size_t h = global_wave(); h < hend - hbeg; h += wave_count()) {
         QEnt en = q[hq[hbeg + h]];
-        uint32_t deg = off[en.v + 1] - off[en.v];
-        uint32_t chunks = (deg + kChunkArcs - 1) / kChunkArcs;
+        // The next level's prefix starts at hpre[hend], so the last row of
+        // this level ends at the level's chunk total.
+        const uint32_t c0 = hpre[hbeg + h];
+        const uint32_t c1 = hbeg + h + 1 < hend ? hpre[hbeg + h + 1] : hchunks;
         double dv = 0.0;
-        for (uint32_t c = 0; c < chunks; c++) dv += part[(h * max_chunks + c) * kLanes + lane];
+        for (uint32_t c = c0; c < c1; c++) dv += part[(size_t)c * kLanes + lane];
         if ((en.m >> lane) & 1) {
             size_t r = ((size_t)en.b * n + en.v) * kLanes + lane;
             double s = sig[r];
@@ -531,124 +606,165 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
 
 // ---- integer lengths (Dial / delta-stepping with unit buckets) -------------
 
-// Drains bucket k at distance D: the lanes of x whose tentative distance is
-// D are final. Their path counts are pulled over in-arcs u->x of length
-// len with dist[u] + len == D, and (x, b, M) joins the level queue.
-__global__ void dial_settle_kernel(const uint2* __restrict__ list, const uint32_t* __restrict__ blen,
-                                   size_t list_cap, uint32_t D, int k, int words,
-                                   uint64_t* __restrict__ inb,
-                                   const uint32_t* __restrict__ in_off,
-                                   const uint32_t* __restrict__ in_idx,
-                                   const uint32_t* __restrict__ in_len,
-                                   const uint32_t* __restrict__ dist, uint64_t* __restrict__ vis,
-                                   double* __restrict__ sig, QEnt* __restrict__ q, uint32_t qcap,
-                                   Ctrl* ctrl, size_t n) {
-    const int lane = lane_id();
-    double smax = 0.0;
-    const uint32_t T = min(blen[k], (uint32_t)list_cap);
-    for (size_t t = global_wave(); t < T; t += wave_count()) {
-        uint2 te = list[t];
-        uint32_t x = te.x;
-        size_t base = (size_t)te.y * n;
-        if (lane == 0)
-            atomicAnd((unsigned long long*)&inb[(base + x) * words + (k >> 6)],
-                      ~(1ull << (k & 63)));
-        uint64_t M = __ballot(dist[(base + x) * kLanes + lane] == D);
-        if (M == 0) continue;
-        double acc = 0.0;
-        uint32_t a0 = in_off[x], a1 = in_off[x + 1];
-        for (uint32_t c = a0; c < a1; c += 64) {
-            uint32_t a = c + lane;
-            bool ok = a < a1;
-            uint32_t u = ok ? in_idx[a] : 0;
-            uint32_t len = ok ? in_len[a] : 0;
-            uint64_t mm = ok ? (vis[base + u] & M) : 0;
-            uint64_t bal = __ballot(mm != 0);
-            while (bal) {
-                int j = __ffsll((unsigned long long)bal) - 1;
-                bal &= bal - 1;
-                uint32_t uu = rl32(u, j);
-                uint32_t ll = rl32(len, j);
-                uint64_t m2 = rl64(mm, j);
-                if ((m2 >> lane) & 1) {
-                    size_t r = (base + uu) * kLanes + lane;
-                    uint32_t du = dist[r];
-                    if ((uint64_t)du + ll == D) acc += sig[r];
-                }
-            }
-        }
-        if ((M >> lane) & 1) {
-            sig[(base + x) * kLanes + lane] = acc;
-            smax = fmax(smax, acc);
-        }
-        if (lane == 0) {
-            vis[base + x] |= M;
-            uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
-            if (pos < qcap)
-                q[pos] = QEnt{x, te.y, (unsigned long long)M};
-            else
-                ctrl->qoverflow = 1;
-        }
-    }
-    note_sigma(ctrl, smax);
-}
-
-// Relaxes the out-arcs of the entries settled at distance D. A lane whose
-// tentative distance drops puts its node into bucket (D + len) mod W once.
-__global__ void dial_relax_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
-                                  uint32_t D, int W, int words, int k_drained,
-                                  uint32_t* __restrict__ blen, uint2* __restrict__ lists,
-                                  size_t list_cap, uint64_t* __restrict__ inb,
-                                  const uint32_t* __restrict__ off,
-                                  const uint32_t* __restrict__ idx,
-                                  const uint32_t* __restrict__ len_arr,
-                                  uint32_t* __restrict__ dist, const uint64_t* __restrict__ vis,
-                                  Ctrl* ctrl, size_t n) {
+// Integer lengths, forward step D (Dial's algorithm with exact integer
+// distances, betweenness.rs:214-295, run for 64 sources per slot). Each
+// (slot, node) keeps a bitmask of the buckets (distance mod W, W = max
+// length + 1) in which some lane holds a tentative distance; all tentative
+// distances lie in [D, D + W), so bucket D mod W holds exactly distance D.
+//
+// dial_scan_kernel lists the (slot, node) pairs with bucket D mod W set and
+// clears it, and records whether any bucket is still set anywhere.
+__global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int kb, size_t count,
+                                 uint2* __restrict__ list, uint32_t* __restrict__ llen,
+                                 uint32_t* __restrict__ llen_next, uint32_t* __restrict__ left,
+                                 uint32_t* __restrict__ left_next, size_t n) {
+    __shared__ uint2 sbuf[kBlock / 64][kTouchBuf];
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicSub(&ctrl->pending, blen[k_drained]);
-        blen[k_drained] = 0;
+        *llen_next = 0;
+        *left_next = 0;
     }
-    for (size_t e = qbeg + global_wave(); e < qend; e += wave_count()) {
-        QEnt en = q[e];
-        size_t base = (size_t)en.b * n;
-        uint64_t M = en.m;
-        uint32_t a0 = off[en.v], a1 = off[en.v + 1];
-        for (uint32_t c = a0; c < a1; c += 64) {
-            uint32_t a = c + lane;
-            bool ok = a < a1;
-            uint32_t y = ok ? idx[a] : 0;
-            uint32_t len = ok ? len_arr[a] : 0;
-            uint64_t mm = ok ? (M & ~vis[base + y]) : 0;
-            uint64_t bal = __ballot(mm != 0);
-            while (bal) {
-                int j = __ffsll((unsigned long long)bal) - 1;
-                bal &= bal - 1;
-                uint32_t yy = rl32(y, j);
-                uint32_t nd = D + rl32(len, j);
-                uint64_t m2 = rl64(mm, j);
-                bool improved = false;
-                if ((m2 >> lane) & 1) {
-                    uint32_t* p = &dist[(base + yy) * kLanes + lane];
-                    if (nd < *p) improved = atomicMin(p, nd) > nd;
-                }
-                if (__ballot(improved) && lane == 0) {
-                    int kk = (int)(nd % (uint32_t)W);
-                    unsigned long long bit = 1ull << (kk & 63);
-                    unsigned long long old = atomicOr(
-                        (unsigned long long*)&inb[(base + yy) * words + (kk >> 6)], bit);
-                    if (!(old & bit)) {
-                        uint32_t pos = atomicAdd(&blen[kk], 1u);
-                        atomicAdd(&ctrl->pending, 1u);
-                        if (pos < list_cap)
-                            lists[(size_t)kk * list_cap + pos] = make_uint2(yy, en.b);
-                        else
-                            ctrl->qoverflow = 2;
+    uint2* mybuf = sbuf[threadIdx.x >> 6];
+    uint32_t cnt = 0;
+    auto flush = [&]() {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(llen, cnt);
+        at = rl32(at, 0);
+        for (uint32_t i = lane; i < cnt; i += 64) list[at + i] = mybuf[i];
+        cnt = 0;
+    };
+    const uint64_t lt = lanemask_lt(lane);
+    const unsigned long long bit = 1ull << (kb & 63);
+    int any = 0;
+    for (size_t w0 = global_wave() * 64; w0 < count; w0 += wave_count() * 64) {
+        size_t i = w0 + lane;
+        bool hit = false;
+        if (i < count) {
+            uint64_t* p = &pend[i * words];
+            uint64_t word = p[kb >> 6];
+            hit = (word & bit) != 0;
+            if (hit) p[kb >> 6] = word & ~bit;
+            uint64_t rest = hit ? (word & ~bit) : word;
+            for (int j = 0; j < words && !rest; j++)
+                if (j != (kb >> 6)) rest |= p[j];
+            any |= rest != 0;
+        }
+        uint64_t bal = __ballot(hit);
+        if (bal) {
+            uint32_t k = (uint32_t)__popcll(bal);
+            if (cnt + k > kTouchBuf) flush();
+            if (hit) mybuf[cnt + __popcll(bal & lt)] = make_uint2((uint32_t)(i % n), (uint32_t)(i / n));
+            cnt += k;
+        }
+    }
+    if (cnt) flush();
+    if (__syncthreads_or(any) && threadIdx.x == 0) atomicOr(left, 1u);
+}
+
+// Settles and relaxes the listed pairs at distance D. The lanes of x whose
+// tentative distance is D are final (M). Their path counts are pulled over
+// in-arcs u->x of length len with dist[u] + len == D, and (x, b, M) joins
+// the level queue (both skipped for the sources at D = 0). Then every
+// out-arc x->y of length len offers D + len to the lanes of y not yet
+// settled, by atomic minimum, and marks y's bucket (D + len) mod W.
+// Settled distances (<= D) never change here, since offers are > D.
+__global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t* __restrict__ llen,
+                                 uint32_t D, int W, int words, int first_step,
+                                 uint64_t* __restrict__ pend,
+                                 const uint32_t* __restrict__ in_off,
+                                 const uint32_t* __restrict__ in_idx,
+                                 const uint32_t* __restrict__ in_len,
+                                 const uint32_t* __restrict__ off,
+                                 const uint32_t* __restrict__ idx,
+                                 const uint32_t* __restrict__ len_arr,
+                                 uint32_t* __restrict__ dist, uint64_t* __restrict__ vis,
+                                 double* __restrict__ sig, QEnt* __restrict__ q, uint32_t qcap,
+                                 Ctrl* ctrl, size_t n) {
+    __shared__ QEnt qbuf[kBlock / 64][kTouchBuf / 2];
+    const int lane = lane_id();
+    QEnt* mybuf = qbuf[threadIdx.x >> 6];
+    uint32_t cnt = 0;
+    auto flush = [&]() {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(&ctrl->qlen, cnt);
+        at = rl32(at, 0);
+        for (uint32_t i = lane; i < cnt; i += 64) {
+            if (at + i < qcap) q[at + i] = mybuf[i];
+            else ctrl->qoverflow = 1;
+        }
+        cnt = 0;
+    };
+    double smax = 0.0;
+    const uint32_t T = *llen;
+    for (size_t t = global_wave(); t < T; t += wave_count()) {
+        const uint2 te = list[t];
+        const uint32_t x = te.x;
+        const size_t base = (size_t)te.y * n;
+        const uint64_t M = __ballot(dist[(base + x) * kLanes + lane] == D);
+        if (M == 0) continue;
+        if (!first_step) {
+            double acc = 0.0;
+            const uint32_t a0 = in_off[x], a1 = in_off[x + 1];
+            for (uint32_t c = a0; c < a1; c += 64) {
+                const uint32_t a = c + lane;
+                const bool ok = a < a1;
+                const uint32_t u = ok ? in_idx[a] : 0;
+                const uint32_t ln = ok ? in_len[a] : 0;
+                const uint64_t mm = ok ? (vis[base + u] & M) : 0;
+                uint64_t bal = __ballot(mm != 0);
+                while (bal) {
+                    const int j = __ffsll((unsigned long long)bal) - 1;
+                    bal &= bal - 1;
+                    const uint32_t uu = rl32(u, j);
+                    const uint32_t ll = rl32(ln, j);
+                    const uint64_t m2 = rl64(mm, j);
+                    if ((m2 >> lane) & 1) {
+                        const size_t r = (base + uu) * kLanes + lane;
+                        if ((uint64_t)dist[r] + ll == D) acc += sig[r];
                     }
                 }
             }
+            if ((M >> lane) & 1) {
+                sig[(base + x) * kLanes + lane] = acc;
+                smax = fmax(smax, acc);
+            }
+            if (lane == 0) vis[base + x] |= M;
+            if (cnt + 1 > kTouchBuf / 2) flush();
+            if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
+            cnt += 1;
+        }
+        const uint32_t a0 = off[x], a1 = off[x + 1];
+        for (uint32_t c = a0; c < a1; c += 64) {
+            const uint32_t a = c + lane;
+            const bool ok = a < a1;
+            const uint32_t y = ok ? idx[a] : 0;
+            const uint32_t ln = ok ? len_arr[a] : 0;
+            const uint64_t mm = ok ? (M & ~vis[base + y]) : 0;
+            uint64_t bal = __ballot(mm != 0);
+            while (bal) {
+                const int j = __ffsll((unsigned long long)bal) - 1;
+                bal &= bal - 1;
+                const uint32_t yy = rl32(y, j);
+                const uint32_t nd = D + rl32(ln, j);
+                const uint64_t m2 = rl64(mm, j);
+                bool offer = false;
+                if ((m2 >> lane) & 1) {
+                    uint32_t* p = &dist[(base + yy) * kLanes + lane];
+                    if (nd < *p) {
+                        atomicMin(p, nd);
+                        offer = true;
+                    }
+                }
+                if (__ballot(offer) && lane == 0) {
+                    const uint32_t kk = nd % (uint32_t)W;
+                    atomicOr((unsigned long long*)&pend[(base + yy) * words + (kk >> 6)],
+                             1ull << (kk & 63));
+                }
+            }
         }
     }
+    if (cnt) flush();
+    note_sigma(ctrl, smax);
 }
 
 template <class C>
@@ -953,6 +1069,53 @@ __global__ void clear_rows_kernel(const uint64_t* __restrict__ vis, double* __re
     }
 }
 
+// The same sum for runs whose source order is free (neighbourhood
+// batching): a wave per node reads each slot's row coalesced, masks the
+// unreached lanes and the node's own source lane, reduces the row with a
+// fixed butterfly and adds it to bc[v] in slot order (deterministic). The
+// rows are zeroed for the next super-batch.
+__global__ void accumulate_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
+                                       int slots, size_t n, int64_t item0, int64_t k,
+                                       const int64_t* __restrict__ list,
+                                       double* __restrict__ bc,
+                                       const uint32_t* __restrict__ off, Ctrl* ctrl) {
+    const int lane = lane_id();
+    unsigned long long rn = 0, ra = 0;
+    for (size_t v = global_wave(); v < n; v += wave_count()) {
+        double B = 0.0;
+        unsigned long long cnt = 0;
+        for (int b = 0; b < slots; b++) {
+            int64_t ib = item0 + (int64_t)b * kLanes;
+            if (ib >= k) break;
+            uint64_t M = vis[(size_t)b * n + v];
+            if (!M) continue;
+            cnt += (unsigned long long)__popcll(M);
+            double* row = sig + ((size_t)b * n + v) * kLanes;
+            double x = 0.0;
+            if ((M >> lane) & 1) {
+                int64_t i = ib + lane;
+                int64_t s = list ? list[i] : i;
+                x = (size_t)s == v ? 0.0 : row[lane];
+                row[lane] = 0.0;
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+            B += x;
+        }
+        if (lane == 0) {
+            if (B != 0.0) bc[v] += B;
+            rn += cnt;
+            ra += cnt * (unsigned long long)(off[v + 1] - off[v]);
+        }
+    }
+    rn = wave_sum_u64(rn);
+    ra = wave_sum_u64(ra);
+    if (lane_id() == 0 && (rn || ra)) {
+        atomicAdd(&ctrl->reached_nodes, rn);
+        atomicAdd(&ctrl->reached_arcs, ra);
+    }
+}
+
 // Final merge of the last group.
 __global__ void finish_kernel(double* __restrict__ bc, const double* __restrict__ acc, size_t n) {
     size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1031,17 +1194,18 @@ struct KernelTimer {
 struct Workspace {
     int slots = 0;
     uint64_t n = 0;
-    DevBuf<uint64_t> vis, cand, g0, g1, inb;
+    DevBuf<uint64_t> vis, cand, g0, g1;
+    DevBuf<uint64_t> inb;  // integer lengths: pending-bucket bitmasks per (slot, node)
     DevBuf<double> sig, coef64, acc;
     DevBuf<float> coef32;
-    DevBuf<uint32_t> dist, blen;
+    DevBuf<uint32_t> dist;
     DevBuf<int64_t> gcur, list, list2;
     DevBuf<QEnt> q;
-    DevBuf<uint32_t> hq;  // positions of heavy-row entries in q
+    DevBuf<uint32_t> hq;    // positions of heavy-row entries in q
+    DevBuf<uint32_t> hpre;  // chunk-task prefix over hq (per level)
     DevBuf<double> part;  // per-chunk partial sums of heavy rows
     size_t qcap = 0;
-    DevBuf<uint2> touched, blists;
-    size_t list_cap = 0;
+    DevBuf<uint2> touched;
     DevBuf<Ctrl> ctrl;
     Ctrl* host_ctrl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1083,7 +1247,7 @@ void read_ctrl(Workspace& ws, hipStream_t st) {
 // Sizes (in bytes per node and slot) of the per-slot arrays.
 size_t slot_bytes_per_node(const Lengths& L, int precision) {
     size_t b = 3 * 8 + kLanes * 8 + kLanes * (precision ? 4 : 8);
-    if (L.kind == kInteger) b += kLanes * 4 + 8 * ((L.max + 1 + 63) / 64) + 8 * (L.max + 1);
+    if (L.kind == kInteger) b += kLanes * 4 + 8 * ((L.max + 1 + 63) / 64) + 8;
     if (L.kind == kReal) b += 0;
     return b;
 }
@@ -1124,6 +1288,14 @@ struct Engine {
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
     std::vector<uint32_t> hv;       // heavy-row list boundaries per level
+    std::vector<uint32_t> hc;       // chunk tasks of each level's heavy rows
+
+    // Prefix of chunk counts for the heavy rows listed from hbeg on.
+    void scan_heavy(uint32_t hbeg) {
+        heavy_scan_kernel<<<1, 1024, 0, st>>>(ws.hq.p, hbeg, &ws.ctrl.p->hlen, ws.q.p,
+                                              g.dev.out_off.p, ws.hpre.p, &ws.ctrl.p->hchunks);
+        launch_check();
+    }
     std::vector<uint32_t> lv_dist;  // Dial: distance of each level
 
     void launch_check() { NETBC_HIP(hipGetLastError()); }
@@ -1150,7 +1322,6 @@ struct Engine {
                               kBlock, 0, st>>>(ws.dist.p, nm * kLanes, kNone);
             int words = (int)((L.max + 1 + 63) / 64);
             fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.inb.p, nm * words, 0);
-            fill_u32_kernel<<<1, kBlock, 0, st>>>(ws.blen.p, L.max + 1, 0);
             launch_check();
         }
         NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, offsetof(Ctrl, sig_overflow), st));
@@ -1164,14 +1335,18 @@ struct Engine {
         reset_slots();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, ws.g0.p, ws.sig.p, ws.q.p, ws.ctrl.p, nullptr,
-            d.heavy_flag.p, ws.hq.p);
+            d.heavy_flag.p, ws.hq.p, nullptr, 0);
         launch_check();
         read_ctrl(ws, st);
         ws.sweep.collect();
         lv.assign(1, 0);
         lv.push_back(ws.host_ctrl->qlen);
         hv.assign(1, 0);
+        hc.clear();
+        scan_heavy(0);
+        read_ctrl(ws, st);
         hv.push_back(ws.host_ctrl->hlen);
+        hc.push_back(ws.host_ctrl->hchunks);
         uint64_t* G[2] = {ws.g0.p, ws.g1.p};
         unsigned finalize_blocks =
             blocks_for_threads(std::min<size_t>((size_t)slots * n, (size_t)1 << 20));
@@ -1179,10 +1354,11 @@ struct Engine {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             uint32_t hb = hv[dl], he = hv[dl + 1];
             uint32_t zb = dl > 0 ? lv[dl - 1] : 0, ze = dl > 0 ? lv[dl] : 0;
-            size_t tasks = (size_t)(qe - qb) + (size_t)(he - hb) * d.max_chunks;
+            size_t tasks = (size_t)(qe - qb) + hc[dl];
             size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
             bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, d.max_chunks, d.out_off.p, d.out_idx.p,
+                ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
+                d.out_idx.p,
                 ws.vis.p, ws.cand.p, G[(dl + 1) & 1], ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
                 ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.ctrl.p, n);
             launch_check();
@@ -1198,12 +1374,14 @@ struct Engine {
                 ws.cand.p, ws.vis.p, G[(dl + 1) & 1], ws.q.p, (uint32_t)ws.qcap, d.heavy_flag.p,
                 ws.hq.p, ws.ctrl.p, n);
             launch_check();
+            scan_heavy(hv.back());
             read_ctrl(ws, st);
             if (ws.host_ctrl->qoverflow) return false;
             uint32_t ql = ws.host_ctrl->qlen;
             if (ql == qe) break;
             lv.push_back(ql);
             hv.push_back(ws.host_ctrl->hlen);
+            hc.push_back(ws.host_ctrl->hchunks);
             diag.levels++;
         }
         return true;
@@ -1220,17 +1398,22 @@ struct Engine {
             uint32_t hb = hv[dl], he = hv[dl + 1];
             uint32_t ab = dl + 2 <= D ? lv[dl + 2] : 0, ae = dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
-            size_t tasks = (size_t)(qe - qb) + (heavy ? (size_t)(he - hb) * d.max_chunks : 0);
+            size_t tasks = (size_t)(qe - qb) + (heavy ? hc[dl] : 0);
             size_t waves = std::max<size_t>(tasks, (ae - ab + 63) / 64);
+            if (heavy && ws.part.n < (size_t)hc[dl] * kLanes) {
+                NETBC_HIP(hipStreamSynchronize(st));  // the old buffer may be in use
+                ws.part.alloc((size_t)hc[dl] * kLanes * 2);
+            }
             ws.sweep.mark(st);
             bfs_backward_kernel<C><<<resident_blocks(bfs_backward_kernel<C>, waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, ab, ae, dl == D, ws.hq.p, hb, he, d.max_chunks, split,
+                ws.q.p, qb, qe, ab, ae, dl == D, ws.hq.p, hb, he, ws.hpre.p, hc[dl], split,
                 d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p, G[(dl + 1) & 1], G[dl & 1],
                 ws.sig.p, coef, ws.ctrl.p, n);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
-                    ws.q.p, ws.hq.p, hb, he, d.max_chunks, ws.part.p, d.out_off.p, ws.sig.p, coef,
+                    ws.q.p, ws.hq.p, hb, he, ws.hpre.p, hc[dl], ws.part.p, d.out_off.p, ws.sig.p,
+                    coef,
                     ws.ctrl.p, n);
                 launch_check();
             }
@@ -1241,49 +1424,43 @@ struct Engine {
     bool run_dial(int64_t item0) {
         const DeviceGraph& d = g.dev;
         reset_slots();
+        const int W = (int)L.max + 1;
+        const int words = (W + 63) / 64;
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, ws.q.p, ws.ctrl.p,
-            ws.dist.p, d.heavy_flag.p, nullptr);
+            ws.dist.p, d.heavy_flag.p, nullptr, ws.inb.p, words);
         launch_check();
         read_ctrl(ws, st);
         ws.sweep.collect();
         lv.assign(1, 0);
         lv.push_back(ws.host_ctrl->qlen);
         lv_dist.assign(1, 0);
-        int W = (int)L.max + 1;
-        int words = (W + 63) / 64;
-        unsigned settle_blocks =
-            resident_blocks(dial_settle_kernel, std::min<size_t>((size_t)slots * n, 1u << 16));
+        const size_t pairs = (size_t)slots * n;
+        const unsigned scan_blocks = resident_blocks(dial_scan_kernel, (pairs + 63) / 64);
+        const unsigned step_blocks = resident_blocks(dial_step_kernel, pairs);
         const uint32_t* in_len = d.directed ? L.in_len.p : L.out_len.p;
-        // Distance 0: relax the sources' arcs.
-        uint32_t D = 0;
-        for (;;) {
-            uint32_t qb = lv[lv.size() - 2], qe = lv.back();
-            bool have = lv_dist.back() == D && qe > qb;
-            dial_relax_kernel<<<resident_blocks(dial_relax_kernel, have ? qe - qb : 1), kBlock, 0, st>>>(
-                ws.q.p, have ? qb : 0, have ? qe : 0, D, W, words, (int)(D % (uint32_t)W),
-                ws.blen.p, ws.blists.p, ws.list_cap, ws.inb.p, d.out_off.p, d.out_idx.p,
-                L.out_len.p, ws.dist.p, ws.vis.p, ws.ctrl.p, n);
+        for (uint32_t D = 0;; D++) {
+            const int p = D & 1;
+            dial_scan_kernel<<<scan_blocks, kBlock, 0, st>>>(
+                ws.inb.p, words, (int)(D % (uint32_t)W), pairs, ws.touched.p,
+                &ws.ctrl.p->tlen[p], &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p],
+                &ws.ctrl.p->left[p ^ 1], n);
+            launch_check();
+            dial_step_kernel<<<step_blocks, kBlock, 0, st>>>(
+                ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, ws.inb.p, d.in_off_p(),
+                d.in_idx_p(), in_len, d.out_off.p, d.out_idx.p, L.out_len.p, ws.dist.p,
+                ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p, n);
             launch_check();
             read_ctrl(ws, st);
             if (ws.host_ctrl->qoverflow) return false;
-            if (ws.host_ctrl->pending == 0) break;
-            D++;
-            int kb = (int)(D % (uint32_t)W);
-            dial_settle_kernel<<<settle_blocks, kBlock, 0, st>>>(
-                ws.blists.p + (size_t)kb * ws.list_cap, ws.blen.p, ws.list_cap, D, kb, words,
-                ws.inb.p,
-                d.in_off_p(), d.in_idx_p(), in_len, ws.dist.p, ws.vis.p, ws.sig.p, ws.q.p,
-                (uint32_t)ws.qcap, ws.ctrl.p, n);
-            launch_check();
-            read_ctrl(ws, st);
-            if (ws.host_ctrl->qoverflow) return false;
-            uint32_t ql = ws.host_ctrl->qlen;
+            const uint32_t ql = ws.host_ctrl->qlen;
             if (ql > lv.back()) {
                 lv.push_back(ql);
                 lv_dist.push_back(D);
                 diag.levels++;
             }
+            // Done when this step listed nothing and no bucket is pending.
+            if (ws.host_ctrl->tlen[p] == 0 && ws.host_ctrl->left[p] == 0) break;
         }
         return true;
     }
@@ -1328,6 +1505,8 @@ void grow_queue(Workspace& ws) {
     ws.q.alloc(nq);
     ws.hq.release();
     ws.hq.alloc(nq);
+    ws.hpre.release();
+    ws.hpre.alloc(nq + 1);
     ws.qcap = nq;
 }
 
@@ -1355,6 +1534,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
             ws.q.alloc(qcap);
             ws.hq.release();
             ws.hq.alloc(qcap);
+            ws.hpre.release();
+            ws.hpre.alloc(qcap + 1);
             ws.qcap = qcap;
         }
     }
@@ -1363,15 +1544,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
         int words = (W + 63) / 64;
         ws.dist.alloc(nm * kLanes);
         ws.inb.alloc(nm * words);
-        ws.blen.alloc(W);
-        if (ws.list_cap < nm || !ws.blists.p) {
-            ws.blists.release();
-            ws.blists.alloc((size_t)W * nm);
-            ws.list_cap = nm;
-        }
+        ws.touched.alloc(nm);
     }
-    if (L.kind == kUniform && g.dev.heavy)  // one level holds each (heavy row, slot) once
-        ws.part.alloc((size_t)slots * g.dev.heavy * g.dev.max_chunks * kLanes);
     ws.acc.alloc(n);
     ws.gcur.alloc(n);
     ws.ctrl.alloc(1);
@@ -1401,7 +1575,8 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                                  st));
         d_list = ws.list.p;
     }
-    if (!opt.exact && k >= kReorderMin && L.kind != kReal) {
+    const bool free_order = !opt.exact && k >= kReorderMin && L.kind != kReal;
+    if (free_order) {
         // Batch sources by neighbourhood (see source_key_kernel). Items are
         // summed in the new order, which changes only the association of
         // the cross-source sum.
@@ -1454,7 +1629,6 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                 if (ok) break;
                 // Level queue overflow: grow and repeat the super-batch from
                 // clean rows.
-                if (ws.host_ctrl->qoverflow == 2) throw Error("bucket list overflow");
                 eng.clear_rows();
                 grow_queue(ws);
             }
@@ -1466,9 +1640,14 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                 else eng.backward_dial<double>(ws.coef64.p);
             }
         }
-        accumulate_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
-            ws.vis.p, ws.sig.p, slots, n, item0, k, gsize, d_list, ws.acc.p, d_bc, ws.gcur.p,
-            g.dev.out_off.p, ws.ctrl.p);
+        if (free_order)
+            accumulate_rows_kernel<<<resident_blocks(accumulate_rows_kernel, n), kBlock, 0,
+                                     st>>>(ws.vis.p, ws.sig.p, slots, n, item0, k, d_list, d_bc,
+                                           g.dev.out_off.p, ws.ctrl.p);
+        else
+            accumulate_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
+                ws.vis.p, ws.sig.p, slots, n, item0, k, gsize, d_list, ws.acc.p, d_bc, ws.gcur.p,
+                g.dev.out_off.p, ws.ctrl.p);
         NETBC_HIP(hipGetLastError());
     }
     finish_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(d_bc, ws.acc.p, n);
@@ -1539,7 +1718,6 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         for (;;) {
             bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_dial(item0);
             if (ok) break;
-            if (ws.host_ctrl->qoverflow == 2) throw Error("bucket list overflow");
             eng.clear_rows();
             grow_queue(ws);
         }
