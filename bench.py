@@ -86,6 +86,11 @@ def run_ws_batch(args) -> None:
     t1 = time.perf_counter() - t
     per_graph_s = t1 / sum(e.n * e.m for e in els[:k1]) * work
     same = all(np.array_equal(a, b) for a, b in zip(one, parts[:k1]))
+    tr = load_traffic(config["workload"], args.precision)
+    if tr:
+        roofline["traffic"] = tr.get("bytes_per_launch")
+        roofline["traffic_source"] = tr.get("source")
+
     cpu = None
     if args.cpu_seconds > 0:
         from oracle.oracle import OracleGraph, available_cores
@@ -115,6 +120,36 @@ def run_ws_batch(args) -> None:
     print(json.dumps(line), flush=True)
 
 
+def calibration_copy(nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of nbytes through the HIP runtime: in a
+    rocprofv3 --pmc run its FETCH_SIZE / WRITE_SIZE readings scale the
+    counters of the other kernels to bytes."""
+    import ctypes
+    from knappnet_amd import _native
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    err = ctypes.create_string_buffer(256)
+    if _native.device().netbc_calibration_copy(b, a, nbytes, err, 256):
+        raise RuntimeError(err.value.decode())
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"# calibration copy: {nbytes} bytes by netbc_copy_kernel", flush=True)
+
+
+def load_traffic(workload: str, precision: int):
+    """Measured HBM bytes per launch of the dependency sweep, from the
+    committed rocprofv3 --pmc summary for this workload (or None)."""
+    path = ROOT / "profiles" / f"traffic_{workload}_p{precision}.json"
+    if not path.exists():
+        return None
+    try:
+        return json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -134,6 +169,8 @@ def main() -> None:
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 dependencies")
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--direction", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy first (scales rocprofv3 --pmc counters)")
     ap.add_argument("--compare-precision", action="store_true",
                     help="also run the other precision and report the relative error")
     ap.add_argument("--max-sources", type=int, default=0,
@@ -150,6 +187,8 @@ def main() -> None:
 
     from knappnet_amd.engine import Graph, device_info
 
+    if args.calibrate_copy:
+        calibration_copy()
     if args.workload == "ws4096":
         run_ws_batch(args)
         return
@@ -204,10 +243,16 @@ def main() -> None:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": None,
+        "traffic_source": None,
         "launches": d["dominant_launches"],
         "avg_launch_us": round(d["dominant_ms"] * 1e3 / max(d["dominant_launches"], 1), 2),
         "algorithmic_bytes_per_run": int(sweep_bytes),
     }
+
+    tr = load_traffic(config["workload"], args.precision)
+    if tr:
+        roofline["traffic"] = tr.get("bytes_per_launch")
+        roofline["traffic_source"] = tr.get("source")
 
     cpu = None
     if args.cpu_seconds > 0:

@@ -155,6 +155,13 @@ int32_t netbc_graph_csr(const GraphHandle *h, int32_t incoming,
                         uint32_t *offsets, uint32_t *targets, double *weights,
                         char *err, size_t err_len);
 
+/* Diagnostics: copies `bytes` (a multiple of 16) from one device buffer to
+ * another with a plain streaming kernel (netbc_copy_kernel), so hardware
+ * traffic counters read in the same profiling run can be scaled to bytes.
+ * Both pointers must be device allocations. No counterpart in netcapi.h. */
+int32_t netbc_calibration_copy(void *dst, const void *src, size_t bytes,
+                               char *err, size_t err_len);
+
 /* Name of the device the library runs on (e.g. "AMD Instinct MI355X,
  * gfx950, 256 CUs"). */
 int32_t netbc_device_info(char *buf, size_t len);

@@ -87,6 +87,8 @@ def device() -> ctypes.CDLL:
         lib.netbc_graph_csr.argtypes = [c_void_p, c_int32, POINTER(c_uint32), POINTER(c_uint32),
                                         POINTER(c_double), c_char_p, c_size_t]
         lib.netbc_graph_csr.restype = c_int32
+        lib.netbc_calibration_copy.argtypes = [c_void_p, c_void_p, c_size_t, c_char_p, c_size_t]
+        lib.netbc_calibration_copy.restype = c_int32
         lib.netbc_device_info.argtypes = [c_char_p, c_size_t]
         lib.netbc_device_info.restype = c_int32
         _device = lib

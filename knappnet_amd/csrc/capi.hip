@@ -418,3 +418,25 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
         fill_diag(diag, total, 0.0);
     });
 }
+
+namespace {
+// A plain streaming copy (16 bytes per thread per iteration).
+__global__ void netbc_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                  size_t count) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < count; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+NETBC_EXPORT int32_t netbc_calibration_copy(void* dst, const void* src, size_t bytes, char* err,
+                                            size_t err_len) {
+    return guard(err, err_len, [&] {
+        ensure_device();
+        if (bytes % 16) throw netbc::Error("bytes must be a multiple of 16");
+        size_t count = bytes / 16;
+        netbc_copy_kernel<<<2048, 256>>>((uint4*)dst, (const uint4*)src, count);
+        NETBC_HIP(hipGetLastError());
+        NETBC_HIP(hipDeviceSynchronize());
+    });
+}

@@ -26,7 +26,10 @@
 namespace netbc {
 namespace {
 
-constexpr int kSmallBlock = 256;
+#ifndef NETBC_SMALL_BLOCK
+#define NETBC_SMALL_BLOCK 512
+#endif
+constexpr int kSmallBlock = NETBC_SMALL_BLOCK;
 constexpr int kSmallWaves = kSmallBlock / 64;
 
 __device__ __forceinline__ uint32_t srl32(uint32_t x, int j) {
