@@ -168,7 +168,10 @@ def main() -> None:
     ap.add_argument("--workload", default="rmat18")
     ap.add_argument("--precision", type=int, default=0, help="0 FP64, 1 FP32 dependencies")
     ap.add_argument("--slots", type=int, default=0)
-    ap.add_argument("--direction", type=int, default=0)
+    ap.add_argument("--direction", type=int, default=0,
+                    help="0 automatic, 1 top-down only, 2 bottom-up at every level")
+    ap.add_argument("--alpha", type=float, default=0.0,
+                    help="bottom-up when frontier out-arcs x alpha > unreached in-arcs (0: default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scales rocprofv3 --pmc counters)")
     ap.add_argument("--compare-precision", action="store_true",
@@ -207,7 +210,7 @@ def main() -> None:
           f"sources={k} (generated in {gen_s:.1f}s)", flush=True)
 
     kw = dict(sources=sources, weighted=weighted, precision=args.precision, slots=args.slots,
-              direction=args.direction)
+              direction=args.direction, alpha=args.alpha)
     for i in range(args.warmup):
         t = time.perf_counter()
         res, diag = g.betweenness_sources(**kw)
@@ -222,7 +225,8 @@ def main() -> None:
         times.append(dt)
         diags.append(diag)
         print(f"# step {i}: {dt:.3f}s kernel_ms={diag['kernel_ms']:.1f} "
-              f"sweep_ms={diag['dominant_ms']:.1f} levels={diag['levels']}", flush=True)
+              f"sweep_ms={diag['dominant_ms']:.1f} levels={diag['levels']} "
+              f"bottom_up_steps={diag['bottom_up_levels']}", flush=True)
         if any(diag["work"]):
             print(f"# work counters: {diag['work']} reached_arcs={diag['reached_arcs']} "
                   f"reached_nodes={diag['reached_nodes']}", flush=True)
@@ -326,6 +330,7 @@ def main() -> None:
         "speedup_vs_cpu": round(gteps / cpu["value"], 1) if cpu else None,
         "precision_check": precision_check,
         "sigma_overflow": d["sigma_overflow"],
+        "search": {"levels": d["levels"], "bottom_up_steps": d["bottom_up_levels"]},
     }
     print(json.dumps(line), flush=True)
 

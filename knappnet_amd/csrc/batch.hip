@@ -45,6 +45,11 @@ struct Ctrl {
     uint32_t hlen;       // heavy-row entries listed (positions in the level queue)
     uint32_t left[2];    // integer lengths: tentative distances still pending
     uint32_t hchunks;    // chunk tasks of the heavy rows listed last
+    // Direction choice: out-arcs of the entries of a level (alternating by
+    // level parity) and in-arcs of the (slot, node) pairs whose active lanes
+    // are all reached.
+    unsigned long long front_arcs[2];
+    unsigned long long closed_arcs;
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
@@ -77,6 +82,12 @@ constexpr double kTwo53 = 9007199254740992.0;
 constexpr int kAhead = NETBC_AHEAD;
 // Touched nodes staged per wave before one append to the shared list.
 constexpr uint32_t kTouchBuf = 256;
+// A level is searched bottom-up when the out-arcs of its entries times
+// this factor exceed the in-arcs of the pairs with unreached lanes.
+#ifndef NETBC_ALPHA
+#define NETBC_ALPHA 1.0
+#endif
+constexpr double kBottomUpAlpha = NETBC_ALPHA;
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ size_t global_wave() {
@@ -127,7 +138,7 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
                                     QEnt* __restrict__ q, Ctrl* ctrl, uint32_t* __restrict__ dist,
                                     const uint8_t* __restrict__ heavy_flag,
                                     uint32_t* __restrict__ hq, uint64_t* __restrict__ pend,
-                                    int words) {
+                                    int words, const uint32_t* __restrict__ out_off) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
     int b = t / kLanes, l = t % kLanes;
@@ -144,6 +155,7 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
     uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
     q[pos] = QEnt{s, (uint32_t)b, bit};
     if (hq && heavy_flag[s]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
+    if (out_off) atomicAdd(&ctrl->front_arcs[0], (unsigned long long)(out_off[s + 1] - out_off[s]));
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t count, uint32_t v) {
@@ -215,28 +227,183 @@ __device__ __forceinline__ uint32_t heavy_of_task(const uint32_t* __restrict__ h
     return lo;
 }
 
-// Top-down step of level d. Every frontier entry (v, b, M) offers its
-// lanes to each out-neighbour x not yet reached in them (nv = M & ~vis[x]),
-// records x in the touched list on the first offer of the level, and adds
-// its path counts into x's row: sigma[x][l] += sigma[v][l] for l in nv.
-// Path counts are integers below 2^53 (checked), so these FP64 atomic adds
-// are exact in any order and give the reference's sums bit for bit. Threads
-// also clear the stale level d-1 entries of the next-level bitmap.
+__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Active lanes of slot b when `left` sources remain from the slot-0 lane 0
+// of the super-batch on.
+__device__ __forceinline__ uint64_t active_lanes(int64_t left, uint32_t b) {
+    int64_t r = left - (int64_t)b * kLanes;
+    if (r <= 0) return 0;
+    return r >= kLanes ? ~0ull : (~0ull >> (kLanes - r));
+}
+
+// Items per wavefront task when `count` items are spread over `waves`
+// wavefronts: about four tasks each, 1 to `most` items per task. Few items
+// per task keep small levels spread over the GPU; many keep the memory
+// system busy on large ones.
+__host__ __device__ __forceinline__ uint32_t group_size(size_t count, size_t waves,
+                                                        uint32_t most = 64) {
+    size_t g = (count + 4 * waves - 1) / (4 * waves);
+    return (uint32_t)(g < 1 ? 1 : (g > most ? most : g));
+}
+
+// Entries per dependency-sweep task at most (measured on R-MAT-18: 4 beats
+// 16 by 11% and 64 by 21%; the sweep's rows are long enough that a few
+// entries keep a wavefront's loads in flight).
+#ifndef NETBC_SWEEP_GROUP
+#define NETBC_SWEEP_GROUP 4
+#endif
+constexpr uint32_t kSweepGroup = NETBC_SWEEP_GROUP;
+
+// Exclusive prefix of per-lane counts over the wavefront; total = their sum.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
+    const int lane = lane_id();
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    total = rl32(incl, 63);
+    return incl - v;
+}
+
+// The lane whose items hold position t of the concatenation of the lanes'
+// items (pre: exclusive prefix of the lanes' item counts): the last lane c
+// with pre[c] <= t. Lanes without items share their prefix with the next
+// lane that has some, so the last such lane is the owner; for t past the
+// end the result is lane 63.
+__device__ __forceinline__ int owner_lane(uint32_t pre, uint32_t t) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        uint32_t pm = (uint32_t)__shfl((int)pre, lo + step, 64);
+        if (pm <= t) lo += step;
+    }
+    return lo;
+}
+
+// Path counts of the wavefront's 64 pairs; lane c holds pair c: slot b,
+// node x, the lanes U sought and the in-row [i0, i0 + deg) (deg = 0: none).
+// The in-rows are walked as one concatenation, 64 arcs at a time; an arc
+// u->x hits in the lanes m = cur[u] & U in which u is at level d. Hits are
+// then taken in order, so each pair's hits are contiguous and its sums stay
+// in registers until its last one: sigma[x][l] = sum of sigma[u][l] over
+// its hits with l in m (integers below 2^53: exact in any order, so equal
+// to the reference's pushes). With succ, every hit also records that u has
+// a successor in lanes m. Returns in lane c the lanes N_c = OR of its hits.
+// With kAdd the sums are added atomically (exact: integer partial sums)
+// instead of stored, for pairs whose in-row is split over several tasks.
+template <bool kAdd>
+__device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64_t U, uint32_t i0,
+                                                  uint32_t deg, const uint32_t* __restrict__ in_idx,
+                                                  const uint64_t* __restrict__ cur,
+                                                  uint64_t* __restrict__ succ,
+                                                  double* __restrict__ sig, size_t n,
+                                                  unsigned long long& arcs_seen,
+                                                  unsigned long long& hits_seen) {
+    const int lane = lane_id();
+    uint32_t T;
+    const uint32_t pre = wave_prefix(deg, T);
+    uint64_t mine = 0;
+    int curc = -1;  // pair whose hits are being summed (wave-uniform)
+    uint64_t ncur = 0;
+    double acc = 0.0;
+    auto close_pair = [&]() {
+        if (curc < 0) return;
+        const uint32_t xs = rl32(x, curc), bs = rl32(b, curc);
+        if ((ncur >> lane) & 1) {
+            double* dst = &sig[((size_t)bs * n + xs) * kLanes + lane];
+            if (kAdd) unsafeAtomicAdd(dst, acc);
+            else *dst = acc;
+        }
+        if (lane == curc) mine = ncur;
+        acc = 0.0;
+        ncur = 0;
+    };
+    if (lane == 0) NETBC_COUNT(arcs_seen, T);
+    for (uint32_t t0 = 0; t0 < T; t0 += 64) {
+        const uint32_t t = t0 + lane;
+        const int c = owner_lane(pre, t);
+        const uint32_t bc = (uint32_t)__shfl((int)b, c, 64);
+        const uint32_t ic = (uint32_t)__shfl((int)i0, c, 64);
+        const uint32_t pc = (uint32_t)__shfl((int)pre, c, 64);
+        const uint64_t Uc = __shfl(U, c, 64);
+        uint32_t u = 0;
+        uint64_t m = 0;
+        if (t < T) {
+            u = in_idx[ic + (t - pc)];
+            m = cur[(size_t)bc * n + u] & Uc;
+            if (succ && m)
+                atomicOr((unsigned long long*)&succ[(size_t)bc * n + u], (unsigned long long)m);
+        }
+        NETBC_COUNT(hits_seen, m != 0);
+        uint64_t hits = __ballot(m != 0);
+        while (hits) {
+            double val[kAhead];
+            int hc[kAhead];
+            uint64_t hm[kAhead];
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                val[k] = 0.0;
+                hc[k] = -1;
+                hm[k] = 0;
+                if (hits) {
+                    const int j = __ffsll((unsigned long long)hits) - 1;
+                    hits &= hits - 1;
+                    const uint32_t uu = rl32(u, j);
+                    const uint32_t bb = rl32(bc, j);
+                    hm[k] = rl64(m, j);
+                    hc[k] = (int)rl32((uint32_t)c, j);
+                    if ((hm[k] >> lane) & 1) val[k] = sig[((size_t)bb * n + uu) * kLanes + lane];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                if (hc[k] < 0) break;
+                if (hc[k] != curc) {
+                    close_pair();
+                    curc = hc[k];
+                }
+                ncur |= hm[k];
+                acc += val[k];
+            }
+        }
+    }
+    close_pair();
+    return mine;
+}
+
+// Top-down step of level d: discovery. Every frontier entry (v, b, M)
+// offers its lanes to each out-neighbour x not yet reached in them
+// (nv = M & ~vis[x]); offers are ORed into cand[x], and x enters the
+// touched list on the first offer of the level. The lanes in which v
+// offered anything are those in which it has a successor at level d+1
+// (succ). Path counts are gathered afterwards (bfs_gather_kernel,
+// bfs_hub_kernel). Light rows are taken 64 entries per wavefront and walked
+// as one concatenation 64 arcs at a time; heavy rows (listed in hq) as
+// tasks of kChunkArcs arcs. Threads also clear the stale level d-1 entries
+// of the next-level bitmap.
 __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                   uint32_t zbeg, uint32_t zend, const uint32_t* __restrict__ hq,
                                   uint32_t hbeg, uint32_t hend, const uint32_t* __restrict__ hpre,
-                                  uint32_t hchunks,
-                                  const uint32_t* __restrict__ off,
+                                  uint32_t hchunks, const uint32_t* __restrict__ off,
                                   const uint32_t* __restrict__ idx,
                                   const uint64_t* __restrict__ vis, uint64_t* __restrict__ cand,
-                                  uint64_t* __restrict__ nxt, uint2* __restrict__ touched,
-                                  uint32_t* __restrict__ tlen, double* __restrict__ sig,
-                                  const uint8_t* __restrict__ hub_flag,
-                                  const uint8_t* __restrict__ heavy_flag, Ctrl* ctrl, size_t n) {
+                                  uint64_t* __restrict__ nxt, uint64_t* __restrict__ succ,
+                                  uint2* __restrict__ touched, uint32_t* __restrict__ tlen,
+                                  const uint8_t* __restrict__ heavy_flag, Ctrl* ctrl,
+                                  unsigned long long* __restrict__ front_done, uint32_t gsz,
+                                  size_t n) {
     const int lane = lane_id();
     [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nth = (size_t)gridDim.x * blockDim.x;
+    if (tid == 0) *front_done = 0;  // read by the host before this launch
     for (size_t i = zbeg + tid; i < zend; i += nth) {
         QEnt e = q[i];
         nxt[(size_t)e.b * n + e.v] = 0;
@@ -254,60 +421,95 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
         tcnt = 0;
     };
     const uint64_t lt = lanemask_lt(lane);
+    // Offers lanes nv of slot bb to x (ok: this lane holds an arc).
+    auto offer = [&](uint32_t x, uint32_t bb, uint64_t nv) {
+        bool first = false;
+        if (nv) {
+            unsigned long long* cp = (unsigned long long*)&cand[(size_t)bb * n + x];
+            if ((*cp & nv) != nv) first = atomicOr(cp, (unsigned long long)nv) == 0;
+        }
+        NETBC_COUNT(c2, nv != 0);
+        NETBC_COUNT(c3, __popcll(nv));
+        uint64_t bal = __ballot(first);
+        if (bal) {
+            uint32_t k = (uint32_t)__popcll(bal);
+            if (tcnt + k > kTouchBuf) flush();
+            if (first) mybuf[tcnt + __popcll(bal & lt)] = make_uint2(x, bb);
+            tcnt += k;
+        }
+    };
     const size_t nq = qend - qbeg;
-    const size_t tasks = nq + hchunks;
+    const size_t groups = (nq + gsz - 1) / gsz;
+    const size_t tasks = groups + hchunks;
     for (size_t w = global_wave(); w < tasks; w += wave_count()) {
-        // A task is a whole light row, or one chunk of a heavy row.
-        QEnt en;
-        uint32_t a0, a1;
-        if (w < nq) {
-            en = q[qbeg + w];
-            if (heavy_flag[en.v]) continue;
-            a0 = off[en.v];
-            a1 = off[en.v + 1];
+        if (w < groups) {
+            // gsz light entries, their rows as one concatenation.
+            const size_t i = qbeg + w * gsz + lane;
+            QEnt en{0, 0, 0};
+            uint32_t a0 = 0, deg = 0;
+            if ((uint32_t)lane < gsz && i < qend) {
+                en = q[i];
+                if (!heavy_flag[en.v]) {
+                    a0 = off[en.v];
+                    deg = off[en.v + 1] - a0;
+                }
+                NETBC_COUNT(c0, 1);
+            }
+            uint32_t T;
+            const uint32_t pre = wave_prefix(deg, T);
+            if (lane == 0) NETBC_COUNT(c1, T);
+            for (uint32_t t0 = 0; t0 < T; t0 += 64) {
+                const uint32_t t = t0 + lane;
+                const int c = owner_lane(pre, t);
+                const uint32_t bc = (uint32_t)__shfl((int)en.b, c, 64);
+                const uint32_t vc = (uint32_t)__shfl((int)en.v, c, 64);
+                const uint32_t ac = (uint32_t)__shfl((int)a0, c, 64);
+                const uint32_t pc = (uint32_t)__shfl((int)pre, c, 64);
+                const uint64_t mc = __shfl(en.m, c, 64);
+                uint32_t x = 0;
+                uint64_t nv = 0;
+                if (t < T) {
+                    x = idx[ac + (t - pc)];
+                    nv = mc & ~vis[(size_t)bc * n + x];
+                }
+                offer(x, bc, nv);
+                // successor lanes of each entry: OR over each run of lanes
+                // with the same owner (owners ascend with the lane)
+                uint64_t run = nv;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    uint64_t y = __shfl_up(run, o, 64);
+                    int cy = __shfl_up(c, o, 64);
+                    if (lane >= o && cy == c) run |= y;
+                }
+                const int cn = __shfl_down(c, 1, 64);
+                if ((lane == 63 || cn != c) && run)
+                    atomicOr((unsigned long long*)&succ[(size_t)bc * n + vc], (unsigned long long)run);
+            }
         } else {
-            uint32_t t = (uint32_t)(w - nq);
-            uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
-            en = q[hq[h]];
-            a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
-            a1 = min(a0 + kChunkArcs, off[en.v + 1]);
-        }
-        size_t base = (size_t)en.b * n;
-        const bool mine = (en.m >> lane) & 1;
-        const double sv = mine ? sig[(base + en.v) * kLanes + lane] : 0.0;
-        if (lane == 0) {
-            NETBC_COUNT(c0, 1);
-            NETBC_COUNT(c1, a1 - a0);
-        }
-        for (uint32_t c = a0; c < a1; c += 64) {
-            uint32_t a = c + lane;
-            bool ok = a < a1;
-            uint32_t x = ok ? idx[a] : 0;
-            uint64_t nv = ok ? (en.m & ~vis[base + x]) : 0;
-            NETBC_COUNT(c2, nv != 0);
-            NETBC_COUNT(c3, __popcll(nv));
-            bool first = false;
-            if (nv && (cand[base + x] & nv) != nv) {
-                unsigned long long old =
-                    atomicOr((unsigned long long*)&cand[base + x], (unsigned long long)nv);
-                first = old == 0;
+            // One chunk of a heavy row.
+            const uint32_t t = (uint32_t)(w - groups);
+            const uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
+            const QEnt en = q[hq[h]];
+            const uint32_t a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
+            const uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
+            const size_t base = (size_t)en.b * n;
+            uint64_t offered = 0;
+            if (lane == 0) NETBC_COUNT(c1, a1 - a0);
+            for (uint32_t c = a0; c < a1; c += 64) {
+                const uint32_t a = c + lane;
+                uint32_t x = 0;
+                uint64_t nv = 0;
+                if (a < a1) {
+                    x = idx[a];
+                    nv = en.m & ~vis[base + x];
+                }
+                offered |= nv;
+                offer(x, en.b, nv);
             }
-            uint64_t bal = __ballot(first);
-            if (bal) {
-                uint32_t k = (uint32_t)__popcll(bal);
-                if (tcnt + k > kTouchBuf) flush();
-                if (first) mybuf[tcnt + __popcll(bal & lt)] = make_uint2(x, en.b);
-                tcnt += k;
-            }
-            // Hubs pull their counts (bfs_hub_pull_kernel) instead.
-            uint64_t push = __ballot(nv != 0 && !hub_flag[x]);
-            while (push) {
-                int j = __ffsll((unsigned long long)push) - 1;
-                push &= push - 1;
-                uint32_t xx = rl32(x, j);
-                uint64_t m2 = rl64(nv, j);
-                if ((m2 >> lane) & 1) unsafeAtomicAdd(&sig[(base + xx) * kLanes + lane], sv);
-            }
+            offered = wave_or(offered);
+            if (lane == 0 && offered)
+                atomicOr((unsigned long long*)&succ[base + en.v], (unsigned long long)offered);
         }
     }
     if (tcnt) flush();
@@ -317,99 +519,112 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
     NETBC_FLUSH(ctrl, 3, c3);
 }
 
-// Path counts of hubs reached at level d+1, pulled instead of pushed: task
-// (hub h, slot b, chunk c) sums sigma[u][l] over in-arcs u->x of chunk c
-// with u at level d in a new lane l of x, and adds the partial sum into
-// sigma[x][l]. Partial sums of integers below 2^53 are exact, so the order
-// of the adds does not matter.
-__global__ void bfs_hub_pull_kernel(const uint32_t* __restrict__ hub_node,
-                                    const uint32_t* __restrict__ hub_chunk0, uint32_t hubs,
-                                    uint32_t chunks, int slots, const uint32_t* __restrict__ in_off,
-                                    const uint32_t* __restrict__ in_idx,
-                                    const uint64_t* __restrict__ cur,
-                                    const uint64_t* __restrict__ cand, double* __restrict__ sig,
-                                    size_t n) {
+// Hub tasks of level d: task (slot b, chunk of hub x's in-arcs) gathers,
+// over the chunk's arcs u->x, sigma[u][l] for the lanes l of N held by u at
+// level d, and adds the partial sum into sigma[x][l] (integers below 2^53:
+// the partial sums are exact and their order does not matter). After a
+// top-down expansion N = cand[x], the lanes in which x was reached. In a
+// bottom-up step (bottom_up != 0) N = active & ~vis[x]: the lanes found are
+// ORed into cand[x], x enters the touched list with its first lanes, and
+// the in-neighbours record their successor lanes (succ). A wavefront takes
+// gsz tasks and walks the arcs of those with N != 0 as one concatenation
+// (gather_counts).
+__global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chunks, int slots,
+                               const uint32_t* __restrict__ in_idx,
+                               const uint64_t* __restrict__ cur, uint64_t* __restrict__ cand,
+                               double* __restrict__ sig, int bottom_up,
+                               const uint64_t* __restrict__ vis, uint64_t* __restrict__ succ,
+                               uint2* __restrict__ touched, uint32_t* __restrict__ tlen,
+                               int64_t left, uint32_t gsz, size_t n) {
     const int lane = lane_id();
+    const uint64_t lt = lanemask_lt(lane);
     const size_t tasks = (size_t)chunks * slots;
-    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
-        const int b = (int)(w / chunks);
-        const uint32_t c = (uint32_t)(w % chunks);
-        // hub of chunk c: largest h with hub_chunk0[h] <= c
-        uint32_t lo = 0, hi = hubs;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) / 2;
-            if (hub_chunk0[mid] <= c) lo = mid; else hi = mid;
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
+    for (size_t w0 = global_wave() * gsz; w0 < tasks; w0 += wave_count() * gsz) {
+        const size_t t = w0 + lane;
+        uint32_t b = 0;
+        uint4 ht = make_uint4(0, 0, 0, 0);
+        uint64_t N = 0;
+        if ((uint32_t)lane < gsz && t < tasks) {
+            b = (uint32_t)(t / chunks);
+            ht = hub_task[t - (size_t)b * chunks];
+            const size_t p = (size_t)b * n + ht.x;
+            N = bottom_up ? (active_lanes(left, b) & ~vis[p]) : cand[p];
         }
-        const uint32_t x = hub_node[lo];
-        const size_t base = (size_t)b * n;
-        const uint64_t N = cand[base + x];
-        if (!N) continue;
-        const uint32_t a_lo = in_off[x] + (c - hub_chunk0[lo]) * kHubChunk;
-        const uint32_t a_hi = min(a_lo + kHubChunk, in_off[x + 1]);
-        double acc = 0.0;
-        bool any = false;
-        for (uint32_t a0 = a_lo; a0 < a_hi; a0 += 64) {
-            const uint32_t a = a0 + lane;
-            const bool ok = a < a_hi;
-            const uint32_t u = ok ? in_idx[a] : 0;
-            const uint64_t mm = ok ? (cur[base + u] & N) : 0;
-            uint64_t bal = __ballot(mm != 0);
-            any |= bal != 0;
-            while (bal) {
-                double val[kAhead];
-#pragma unroll
-                for (int k = 0; k < kAhead; k++) {
-                    val[k] = 0.0;
-                    if (bal) {
-                        int j = __ffsll((unsigned long long)bal) - 1;
-                        bal &= bal - 1;
-                        uint32_t uu = rl32(u, j);
-                        uint64_t m2 = rl64(mm, j);
-                        if ((m2 >> lane) & 1) val[k] = sig[(base + uu) * kLanes + lane];
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kAhead; k++) acc += val[k];
+        const uint32_t deg = N ? ht.z - ht.y : 0;
+        const uint64_t got = gather_counts<true>(ht.x, b, N, ht.y, deg, in_idx, cur,
+                                                 bottom_up ? succ : nullptr, sig, n, c4, c5);
+        if (bottom_up) {
+            bool first = false;
+            if (got)
+                first = atomicOr((unsigned long long*)&cand[(size_t)b * n + ht.x],
+                                 (unsigned long long)got) == 0;
+            const uint64_t bal = __ballot(first);
+            if (bal) {
+                uint32_t pos = 0;
+                if (lane == 0) pos = atomicAdd(tlen, (uint32_t)__popcll(bal));
+                pos = rl32(pos, 0) + (uint32_t)__popcll(bal & lt);
+                if (first) touched[pos] = make_uint2(ht.x, b);
             }
         }
-        if (any && ((N >> lane) & 1) && acc != 0.0)
-            unsafeAtomicAdd(&sig[(base + x) * kLanes + lane], acc);
     }
 }
 
-// Closes level d: every touched node x takes its new lanes N (the OR of
-// all offers), which become visited, form the level d+1 bitmap and join
-// the level queue as (x, b, N).
-__global__ void bfs_finalize_kernel(const uint2* __restrict__ touched,
-                                    const uint32_t* __restrict__ tlen_cur,
-                                    uint32_t* __restrict__ tlen_next,
-                                    uint64_t* __restrict__ cand, uint64_t* __restrict__ vis,
-                                    uint64_t* __restrict__ nxt, QEnt* __restrict__ q,
-                                    uint32_t qcap, const uint8_t* __restrict__ heavy_flag,
-                                    uint32_t* __restrict__ hq, Ctrl* ctrl, size_t n) {
+// Closes level d: every touched pair (x, b) takes its new lanes N = cand[x]
+// (the OR of all offers), which become visited, form the level d+1 bitmap
+// and join the level queue as (x, b, N). The path counts of non-hub pairs
+// are gathered here from their in-arcs (gather_counts with U = N; after a
+// bottom-up step only hubs are listed, whose counts come from
+// bfs_hub_kernel). Also counts the out-arcs of the new entries and the
+// in-arcs of pairs whose active lanes are now all reached (the direction
+// choice of the next step).
+__global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
+                                  const uint32_t* __restrict__ tlen_cur,
+                                  uint32_t* __restrict__ tlen_next,
+                                  const uint32_t* __restrict__ in_off,
+                                  const uint32_t* __restrict__ in_idx,
+                                  const uint32_t* __restrict__ out_off,
+                                  const uint64_t* __restrict__ cur, uint64_t* __restrict__ cand,
+                                  uint64_t* __restrict__ vis, uint64_t* __restrict__ nxt,
+                                  double* __restrict__ sig, const uint8_t* __restrict__ hub_flag,
+                                  const uint8_t* __restrict__ heavy_flag, QEnt* __restrict__ q,
+                                  uint32_t qcap, uint32_t* __restrict__ hq, Ctrl* ctrl,
+                                  int64_t left, unsigned long long* __restrict__ front_next,
+                                  size_t n) {
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) *tlen_next = 0;
     const uint32_t T = *tlen_cur;
     const uint64_t lt = lanemask_lt(lane);
-    size_t nth = (size_t)gridDim.x * blockDim.x;
-    for (size_t t0 = (size_t)blockIdx.x * blockDim.x; t0 < T; t0 += nth) {
-        size_t t = t0 + threadIdx.x;
-        bool ok = t < T;
-        uint2 te = ok ? touched[t] : make_uint2(0, 0);
-        size_t mi = (size_t)te.y * n + te.x;
+    unsigned long long fa = 0, closed = 0;
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
+    const uint32_t gsz = group_size(T, wave_count());
+    for (size_t t0 = global_wave() * gsz; t0 < T; t0 += wave_count() * gsz) {
+        const size_t t = t0 + lane;
+        const bool ok = (uint32_t)lane < gsz && t < T;
+        const uint2 te = ok ? touched[t] : make_uint2(0, 0);
+        const size_t p = (size_t)te.y * n + te.x;
         uint64_t N = 0;
+        uint32_t i0 = 0, i1 = 0, deg = 0;
         if (ok) {
-            N = cand[mi];
-            cand[mi] = 0;
-            vis[mi] |= N;
-            nxt[mi] = N;
+            N = cand[p];
+            i0 = in_off[te.x];
+            i1 = in_off[te.x + 1];
+            if (!hub_flag[te.x]) deg = i1 - i0;
         }
-        uint64_t bal = __ballot(ok);
+        (void)gather_counts<false>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig, n, c4, c5);
+        if (ok) {
+            cand[p] = 0;
+            const uint64_t vn = vis[p] | N;
+            vis[p] = vn;
+            nxt[p] = N;
+            fa += out_off[te.x + 1] - out_off[te.x];
+            if (vn == active_lanes(left, te.y)) closed += i1 - i0;
+        }
+        const uint64_t bal = __ballot(ok);
         if (bal) {
-            int leader = __ffsll((unsigned long long)bal) - 1;
             uint32_t pos = 0;
-            if (lane == leader) pos = atomicAdd(&ctrl->qlen, (uint32_t)__popcll(bal));
-            pos = rl32(pos, leader) + (uint32_t)__popcll(bal & lt);
+            if (lane == 0) pos = atomicAdd(&ctrl->qlen, (uint32_t)__popcll(bal));
+            pos = rl32(pos, 0) + (uint32_t)__popcll(bal & lt);
             if (ok) {
                 if (pos < qcap) {
                     q[pos] = QEnt{te.x, te.y, (unsigned long long)N};
@@ -419,6 +634,95 @@ __global__ void bfs_finalize_kernel(const uint2* __restrict__ touched,
                 }
             }
         }
+    }
+    fa = wave_sum_u64(fa);
+    closed = wave_sum_u64(closed);
+    if (lane == 0) {
+        if (fa) atomicAdd(front_next, fa);
+        if (closed) atomicAdd(&ctrl->closed_arcs, closed);
+    }
+}
+
+// Bottom-up step of level d (direction-optimising search). Every (slot b,
+// node x) pair that is not a hub, has in-arcs and has active lanes not yet
+// reached, U = active & ~vis[x], gathers over its in-arcs the lanes held by
+// level-d in-neighbours and their path counts (gather_counts, recording
+// successor lanes in succ); the pairs reached join level d+1 here. A
+// wavefront takes 64 consecutive pairs. Hubs are left to bfs_hub_kernel and
+// bfs_gather_kernel. Threads also clear the level d-1 entries of the next
+// bitmap (AND-NOT, commuting with the OR of the new lanes: a node's lanes
+// at levels d-1 and d+1 are disjoint).
+__global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
+                                const uint32_t* __restrict__ in_off,
+                                const uint32_t* __restrict__ in_idx,
+                                const uint32_t* __restrict__ out_off, uint64_t* __restrict__ vis,
+                                const uint64_t* __restrict__ cur, uint64_t* __restrict__ nxt,
+                                uint64_t* __restrict__ succ, double* __restrict__ sig,
+                                const uint8_t* __restrict__ hub_flag,
+                                const uint8_t* __restrict__ heavy_flag, QEnt* q, uint32_t qcap,
+                                uint32_t* __restrict__ hq, Ctrl* ctrl,
+                                unsigned long long* __restrict__ front_next,
+                                unsigned long long* __restrict__ front_done, int slots,
+                                int64_t left, size_t n) {
+    const int lane = lane_id();
+    size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nth = (size_t)gridDim.x * blockDim.x;
+    if (tid == 0) *front_done = 0;  // read by the host before this launch
+    for (size_t i = zbeg + tid; i < zend; i += nth) {
+        QEnt e = qz[i];
+        atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
+    }
+    const uint64_t lt = lanemask_lt(lane);
+    const size_t pairs = (size_t)slots * n;
+    unsigned long long fa = 0, closed = 0;
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
+    for (size_t p0 = global_wave() * 64; p0 < pairs; p0 += wave_count() * 64) {
+        const size_t p = p0 + lane;
+        const bool ok = p < pairs;
+        const uint32_t b = ok ? (uint32_t)(p / n) : 0;
+        const uint32_t x = ok ? (uint32_t)(p - (size_t)b * n) : 0;
+        uint64_t U = 0, V = 0;
+        uint32_t i0 = 0, deg = 0;
+        if (ok) {
+            const uint64_t A = active_lanes(left, b);
+            if (A) {
+                V = vis[p];
+                U = A & ~V;
+                if (U && !hub_flag[x]) {
+                    i0 = in_off[x];
+                    deg = in_off[x + 1] - i0;
+                }
+            }
+        }
+        const uint64_t mine =
+            gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5);
+        if (mine) {
+            const uint64_t vn = V | mine;
+            vis[p] = vn;
+            atomicOr((unsigned long long*)&nxt[p], (unsigned long long)mine);
+            fa += out_off[x + 1] - out_off[x];
+            if (vn == active_lanes(left, b)) closed += deg;
+        }
+        const uint64_t bal = __ballot(mine != 0);
+        if (bal) {
+            uint32_t pos = 0;
+            if (lane == 0) pos = atomicAdd(&ctrl->qlen, (uint32_t)__popcll(bal));
+            pos = rl32(pos, 0) + (uint32_t)__popcll(bal & lt);
+            if (mine) {
+                if (pos < qcap) {
+                    q[pos] = QEnt{x, b, (unsigned long long)mine};
+                    if (heavy_flag[x]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
+                } else {
+                    ctrl->qoverflow = 1;
+                }
+            }
+        }
+    }
+    fa = wave_sum_u64(fa);
+    closed = wave_sum_u64(closed);
+    if (lane == 0) {
+        if (fa) atomicAdd(front_next, fa);
+        if (closed) atomicAdd(&ctrl->closed_arcs, closed);
     }
 }
 
@@ -453,12 +757,9 @@ struct Dep<float> {
     }
 };
 
-// Dependencies of the level-d entries. R is the level d+1 bitmap, W the
-// bitmap being turned from level d+2 into level d (lanes are disjoint, so
-// the OR and AND-NOT updates commute).
-// Adds the coefficients of the successors of `en` among arcs [a0, a1) to
-// dv, per lane in CSR order; up to kAhead rows are loaded before they are
-// added.
+// Adds the coefficients of the successors of `en` (R: the level d+1
+// bitmap) among arcs [a0, a1) to dv, per lane in CSR order; up to kAhead
+// rows are loaded before they are added.
 template <class C>
 __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t a1,
                                            const uint32_t* __restrict__ idx,
@@ -500,10 +801,14 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
 
 // Dependencies of the level-d entries. R is the level d+1 bitmap, W the
 // bitmap being turned from level d+2 into level d (lanes are disjoint, so
-// the OR and AND-NOT updates commute). Light rows are summed by one wave in
-// CSR order. Heavy rows (listed in hq) are split into chunk tasks that each
-// write a per-lane partial sum (part), added in chunk order by
-// bfs_combine_kernel; with split == 0 every row is summed whole.
+// the OR and AND-NOT updates commute). A wavefront takes 64 light entries:
+// the rows of those with a successor in their lanes (succ) are walked as
+// one concatenation, 64 arcs at a time, and the successors' coefficients
+// are added per lane in CSR order (hits are taken in order, so one row's
+// sum is complete before the next row's starts); the other entries have no
+// dependency (delta = 0). Heavy rows (listed in hq) are split into chunk
+// tasks that each write a per-lane partial sum (part), added in chunk order
+// by bfs_combine_kernel; with split == 0 every row is summed whole.
 template <class C>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level,
@@ -514,8 +819,9 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     const uint32_t* __restrict__ off,
                                     const uint32_t* __restrict__ idx,
                                     const uint64_t* __restrict__ R, uint64_t* __restrict__ W,
+                                    const uint64_t* __restrict__ succ,
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
-                                    size_t n) {
+                                    uint32_t gsz, size_t n) {
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
@@ -532,39 +838,138 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
         }
     }
     split = split && !last_level;
+    // delta, coefficient and path-count bookkeeping of lane l of entry (v, b, M)
+    auto finish = [&](uint32_t v, uint32_t b, uint64_t M, double s, double dvv) {
+        if ((M >> lane) & 1) {
+            const size_t r = ((size_t)b * n + v) * kLanes + lane;
+            const double delta = s * dvv;
+            coef[r] = Dep<C>::coef(delta, s);
+            sig[r] = delta;
+            smax = fmax(smax, s);
+        }
+    };
     const size_t nq = qend - qbeg;
-    const size_t tasks = nq + (split ? hchunks : 0);
+    const size_t groups = (nq + gsz - 1) / gsz;
+    const size_t tasks = groups + (split ? hchunks : 0);
     for (size_t w = global_wave(); w < tasks; w += wave_count()) {
-        if (w >= nq) {
+        if (w >= groups) {
             // One chunk of a heavy row: its partial sum.
-            uint32_t t = (uint32_t)(w - nq);
+            uint32_t t = (uint32_t)(w - groups);
             uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
             QEnt en = q[hq[h]];
             uint32_t a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
             uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
             Dep<C> dv;
-            sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
+            if (succ[(size_t)en.b * n + en.v] & en.m)
+                sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
             part[t * kLanes + lane] = dv.value();
             continue;
         }
-        QEnt en = q[qbeg + w];
-        if (split && heavy_flag[en.v]) continue;
-        size_t base = (size_t)en.b * n;
-        uint64_t M = en.m;
-        Dep<C> dv;
-        if (lane == 0) NETBC_COUNT(c4, 1);
-        if (!last_level) {
-            uint32_t a0 = off[en.v], a1 = off[en.v + 1];
-            if (lane == 0) NETBC_COUNT(c5, a1 - a0);
-            sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
+        const size_t i = qbeg + w * gsz + lane;
+        QEnt en{0, 0, 0};
+        bool live = false;  // this lane's entry is finished by this wave
+        uint32_t a0 = 0, deg = 0;
+        if ((uint32_t)lane < gsz && i < qend) {
+            en = q[i];
+            live = !(split && heavy_flag[en.v]);
+            // rows without a successor in any of their lanes add nothing
+            if (live && !last_level && (succ[(size_t)en.b * n + en.v] & en.m)) {
+                a0 = off[en.v];
+                deg = off[en.v + 1] - a0;
+            }
         }
-        if ((M >> lane) & 1) {
-            size_t r = (base + en.v) * kLanes + lane;
-            double s = sig[r];
-            double delta = s * dv.value();
-            coef[r] = Dep<C>::coef(delta, s);
-            sig[r] = delta;
-            smax = fmax(smax, s);
+        NETBC_COUNT(c4, live);
+        uint32_t T;
+        const uint32_t pre = wave_prefix(deg, T);
+        if (lane == 0) NETBC_COUNT(c5, T);
+        bool done = !live;
+        int cur = -1;       // entry whose successors are being summed (wave-uniform)
+        Dep<C> dv;
+        double sv = 0.0;    // its path counts, loaded when it starts
+        auto close_entry = [&]() {
+            if (cur < 0) return;
+            finish(rl32(en.v, cur), rl32(en.b, cur), rl64(en.m, cur), sv, dv.value());
+            if (lane == cur) done = true;
+            dv = Dep<C>();
+        };
+        auto open_entry = [&](int c) {
+            cur = c;
+            const uint32_t vv = rl32(en.v, c), bb = rl32(en.b, c);
+            const uint64_t mm = rl64(en.m, c);
+            sv = ((mm >> lane) & 1) ? sig[((size_t)bb * n + vv) * kLanes + lane] : 0.0;
+        };
+        for (uint32_t t0 = 0; t0 < T; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            const int c = owner_lane(pre, t);
+            const uint32_t bc = (uint32_t)__shfl((int)en.b, c, 64);
+            const uint32_t ac = (uint32_t)__shfl((int)a0, c, 64);
+            const uint32_t pc = (uint32_t)__shfl((int)pre, c, 64);
+            const uint64_t mc = __shfl(en.m, c, 64);
+            uint32_t x = 0;
+            uint64_t mm = 0;
+            if (t < T) {
+                x = idx[ac + (t - pc)];
+                mm = R[(size_t)bc * n + x] & mc;
+            }
+            NETBC_COUNT(c6, mm != 0);
+            NETBC_COUNT(c7, __popcll(mm));
+            uint64_t bal = __ballot(mm != 0);
+            while (bal) {
+                C val[kAhead];
+                bool use[kAhead];
+                int hc[kAhead];
+#pragma unroll
+                for (int k = 0; k < kAhead; k++) {
+                    use[k] = false;
+                    val[k] = 0;
+                    hc[k] = -1;
+                    if (bal) {
+                        const int j = __ffsll((unsigned long long)bal) - 1;
+                        bal &= bal - 1;
+                        const uint32_t xx = rl32(x, j), bb = rl32(bc, j);
+                        const uint64_t m2 = rl64(mm, j);
+                        hc[k] = (int)rl32((uint32_t)c, j);
+                        use[k] = (m2 >> lane) & 1;
+                        if (use[k]) val[k] = coef[((size_t)bb * n + xx) * kLanes + lane];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kAhead; k++) {
+                    if (hc[k] < 0) break;
+                    if (hc[k] != cur) {
+                        close_entry();
+                        open_entry(hc[k]);
+                    }
+                    if (use[k]) dv.add(val[k]);
+                }
+            }
+        }
+        close_entry();
+        // The rest have no dependency; their path counts are loaded kAhead
+        // entries at a time.
+        uint64_t rest = __ballot(!done);
+        while (rest) {
+            double s[kAhead];
+            int ec[kAhead];
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                s[k] = 0.0;
+                ec[k] = -1;
+                if (rest) {
+                    const int j = __ffsll((unsigned long long)rest) - 1;
+                    rest &= rest - 1;
+                    ec[k] = j;
+                    const uint32_t vv = rl32(en.v, j), bb = rl32(en.b, j);
+                    const uint64_t mm = rl64(en.m, j);
+                    if ((mm >> lane) & 1) s[k] = sig[((size_t)bb * n + vv) * kLanes + lane];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kAhead; k++) {
+                if (ec[k] < 0) break;
+                finish(rl32(en.v, ec[k]), rl32(en.b, ec[k]), rl64(en.m, ec[k]), s[k],
+                       Dep<C>().value());
+            }
         }
     }
     note_sigma(ctrl, smax);
@@ -1159,6 +1564,12 @@ unsigned resident_blocks(K kernel, size_t waves) {
     return std::min(blocks_for_waves(waves), cap);
 }
 
+// Wavefronts of `kernel` that fit on the device at once.
+template <class K>
+size_t resident_waves(K kernel) {
+    return (size_t)resident_blocks(kernel, ~(size_t)0 >> 8) * (kBlock / 64);
+}
+
 // Durations of one kernel's launches from pairs of events recorded around
 // each launch on the stream it runs on; read back only once the stream has
 // been synchronised, so timing adds no host waits.
@@ -1195,6 +1606,7 @@ struct Workspace {
     int slots = 0;
     uint64_t n = 0;
     DevBuf<uint64_t> vis, cand, g0, g1;
+    DevBuf<uint64_t> succ;  // lanes in which a node has a successor on the next level
     DevBuf<uint64_t> inb;  // integer lengths: pending-bucket bitmasks per (slot, node)
     DevBuf<double> sig, coef64, acc;
     DevBuf<float> coef32;
@@ -1313,6 +1725,7 @@ struct Engine {
         launch_check();
         if (L.kind == kUniform) {
             fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.cand.p, nm, 0);
+            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.succ.p, nm, 0);
             fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.g0.p, nm, 0);
             fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.g1.p, nm, 0);
             launch_check();
@@ -1335,7 +1748,7 @@ struct Engine {
         reset_slots();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, ws.g0.p, ws.sig.p, ws.q.p, ws.ctrl.p, nullptr,
-            d.heavy_flag.p, ws.hq.p, nullptr, 0);
+            d.heavy_flag.p, ws.hq.p, nullptr, 0, d.out_off.p);
         launch_check();
         read_ctrl(ws, st);
         ws.sweep.collect();
@@ -1348,31 +1761,61 @@ struct Engine {
         hv.push_back(ws.host_ctrl->hlen);
         hc.push_back(ws.host_ctrl->hchunks);
         uint64_t* G[2] = {ws.g0.p, ws.g1.p};
-        unsigned finalize_blocks =
-            blocks_for_threads(std::min<size_t>((size_t)slots * n, (size_t)1 << 20));
+        const int64_t left = k - item0;  // sources from lane 0 of slot 0 on
+        const uint64_t live_slots =
+            (uint64_t)std::min<int64_t>(slots, (left + kLanes - 1) / kLanes);
+        const size_t pairs = (size_t)slots * n;
+        const double alpha = opt.alpha > 0.0 ? opt.alpha : kBottomUpAlpha;
+        unsigned long long* front = ws.ctrl.p->front_arcs;
         for (int dl = 0;; dl++) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             uint32_t hb = hv[dl], he = hv[dl + 1];
             uint32_t zb = dl > 0 ? lv[dl - 1] : 0, ze = dl > 0 ? lv[dl] : 0;
-            size_t tasks = (size_t)(qe - qb) + hc[dl];
-            size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
-            bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
-                d.out_idx.p,
-                ws.vis.p, ws.cand.p, G[(dl + 1) & 1], ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
-                ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.ctrl.p, n);
-            launch_check();
-            if (d.hubs) {
-                bfs_hub_pull_kernel<<<resident_blocks(bfs_hub_pull_kernel, (size_t)d.hub_chunks * slots),
-                                      kBlock, 0, st>>>(d.hub_node.p, d.hub_chunk0.p, d.hubs, d.hub_chunks,
-                                            slots, d.in_off_p(), d.in_idx_p(), G[dl & 1],
-                                            ws.cand.p, ws.sig.p, n);
+            // Direction of this step: top-down costs the out-arcs of the
+            // level's entries, bottom-up the in-arcs of the pairs with
+            // unreached active lanes.
+            const double f_arcs = (double)ws.host_ctrl->front_arcs[dl & 1];
+            const double open_arcs =
+                (double)live_slots * (double)d.arcs - (double)ws.host_ctrl->closed_arcs;
+            const bool bottom_up = opt.direction == 2 ||
+                                   (opt.direction == 0 && f_arcs * alpha > open_arcs);
+            if (bottom_up) {
+                diag.bottom_up_levels++;
+                bfs_pull_kernel<<<resident_blocks(bfs_pull_kernel, (pairs + 63) / 64), kBlock, 0,
+                                  st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
+                                        ws.vis.p, G[dl & 1], G[(dl + 1) & 1], ws.succ.p, ws.sig.p,
+                                        d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
+                                        ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
+                                        slots, left, n);
+                launch_check();
+            } else {
+                const uint32_t gsz =
+                    group_size(qe - qb, resident_waves(bfs_expand_kernel));
+                size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + hc[dl];
+                size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
+                bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
+                    ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
+                    d.out_idx.p, ws.vis.p, ws.cand.p, G[(dl + 1) & 1], ws.succ.p, ws.touched.p,
+                    &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, &front[dl & 1], gsz, n);
                 launch_check();
             }
-            bfs_finalize_kernel<<<finalize_blocks, kBlock, 0, st>>>(
-                ws.touched.p, &ws.ctrl.p->tlen[dl & 1], &ws.ctrl.p->tlen[(dl + 1) & 1],
-                ws.cand.p, ws.vis.p, G[(dl + 1) & 1], ws.q.p, (uint32_t)ws.qcap, d.heavy_flag.p,
-                ws.hq.p, ws.ctrl.p, n);
+            if (d.hubs) {
+                const size_t hub_tasks = (size_t)d.hub_chunks * slots;
+                const uint32_t gsz = group_size(hub_tasks, resident_waves(bfs_hub_kernel));
+                bfs_hub_kernel<<<resident_blocks(bfs_hub_kernel, (hub_tasks + gsz - 1) / gsz),
+                                 kBlock, 0, st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
+                                                  G[dl & 1], ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
+                                                  ws.vis.p, ws.succ.p, ws.touched.p,
+                                                  &ws.ctrl.p->tlen[dl & 1], left, gsz, n);
+                launch_check();
+            }
+            bfs_gather_kernel<<<resident_blocks(bfs_gather_kernel, (pairs + 63) / 64), kBlock, 0,
+                                st>>>(ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
+                                      &ws.ctrl.p->tlen[(dl + 1) & 1], d.in_off_p(), d.in_idx_p(),
+                                      d.out_off.p, G[dl & 1], ws.cand.p, ws.vis.p, G[(dl + 1) & 1],
+                                      ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.q.p,
+                                      (uint32_t)ws.qcap, ws.hq.p, ws.ctrl.p, left,
+                                      &front[(dl + 1) & 1], n);
             launch_check();
             scan_heavy(hv.back());
             read_ctrl(ws, st);
@@ -1398,8 +1841,10 @@ struct Engine {
             uint32_t hb = hv[dl], he = hv[dl + 1];
             uint32_t ab = dl + 2 <= D ? lv[dl + 2] : 0, ae = dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
-            size_t tasks = (size_t)(qe - qb) + (heavy ? hc[dl] : 0);
-            size_t waves = std::max<size_t>(tasks, (ae - ab + 63) / 64);
+            const uint32_t gsz =
+                group_size(qe - qb, resident_waves(bfs_backward_kernel<C>), kSweepGroup);
+            size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
+            size_t waves = std::max<size_t>(tasks, (qe - qb + ae - ab + 63) / 64);
             if (heavy && ws.part.n < (size_t)hc[dl] * kLanes) {
                 NETBC_HIP(hipStreamSynchronize(st));  // the old buffer may be in use
                 ws.part.alloc((size_t)hc[dl] * kLanes * 2);
@@ -1408,7 +1853,7 @@ struct Engine {
             bfs_backward_kernel<C><<<resident_blocks(bfs_backward_kernel<C>, waves), kBlock, 0, st>>>(
                 ws.q.p, qb, qe, ab, ae, dl == D, ws.hq.p, hb, he, ws.hpre.p, hc[dl], split,
                 d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p, G[(dl + 1) & 1], G[dl & 1],
-                ws.sig.p, coef, ws.ctrl.p, n);
+                ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz, n);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
@@ -1428,7 +1873,7 @@ struct Engine {
         const int words = (W + 63) / 64;
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, ws.q.p, ws.ctrl.p,
-            ws.dist.p, d.heavy_flag.p, nullptr, ws.inb.p, words);
+            ws.dist.p, d.heavy_flag.p, nullptr, ws.inb.p, words, nullptr);
         launch_check();
         read_ctrl(ws, st);
         ws.sweep.collect();
@@ -1522,6 +1967,7 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     if (ws.sig.p != before) ws.rows_dirty = true;  // rows must start at zero
     if (L.kind == kUniform) {
         ws.cand.alloc(nm);
+        ws.succ.alloc(nm);
         ws.g0.alloc(nm);
         ws.g1.alloc(nm);
         ws.touched.alloc(nm);

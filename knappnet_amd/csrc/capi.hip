@@ -89,7 +89,7 @@ void fill_diag(netbc_diagnostics* out, const netbc::RunDiag& d, double build_ms)
     out->dominant_ms = d.dominant_ms;
     out->dominant_launches = d.dominant_launches;
     out->slots = d.slots;
-    out->reserved = 0;
+    out->bottom_up_levels = (int32_t)d.bottom_up_levels;
     out->build_ms = build_ms;
     for (int i = 0; i < 8; i++) out->work[i] = d.work[i];
 }
@@ -113,6 +113,7 @@ void merge_diag(netbc::RunDiag& into, const netbc::RunDiag& d) {
     into.reached_nodes += d.reached_nodes;
     into.reached_arcs += d.reached_arcs;
     into.levels += d.levels;
+    into.bottom_up_levels += d.bottom_up_levels;
     into.kernel_ms += d.kernel_ms;
     into.dominant_ms += d.dominant_ms;
     into.dominant_launches += d.dominant_launches;

@@ -9,6 +9,7 @@
 // (row << b | column) carrying the edge index, so the device arrays equal
 // the reference's host arrays element for element.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -431,24 +432,24 @@ void build_device_graph(Graph& g) {
     std::vector<uint32_t> ioff(g.n + 1);
     NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));
     std::vector<uint8_t> flag(std::max<uint64_t>(g.n, 1), 0);
-    std::vector<uint32_t> hub, chunk0(1, 0);
+    std::vector<uint4> task;
+    d.hubs = 0;
     for (uint64_t v = 0; v < g.n; v++) {
         uint32_t deg = ioff[v + 1] - ioff[v];
         if (deg > kHubIn) {
             flag[v] = 1;
-            hub.push_back((uint32_t)v);
-            chunk0.push_back(chunk0.back() + (deg + kHubChunk - 1) / kHubChunk);
+            d.hubs++;
+            for (uint32_t a = ioff[v]; a < ioff[v + 1]; a += kHubChunk)
+                task.push_back(make_uint4((uint32_t)v, a, std::min(a + kHubChunk, ioff[v + 1]), 0));
         }
     }
-    d.hubs = (uint32_t)hub.size();
-    d.hub_chunks = chunk0.back();
+    d.hub_chunks = (uint32_t)task.size();
     d.hub_flag.alloc(flag.size());
     NETBC_HIP(hipMemcpy(d.hub_flag.p, flag.data(), flag.size(), hipMemcpyHostToDevice));
-    d.hub_node.alloc(hub.size());
-    d.hub_chunk0.alloc(chunk0.size());
-    if (!hub.empty())
-        NETBC_HIP(hipMemcpy(d.hub_node.p, hub.data(), hub.size() * 4, hipMemcpyHostToDevice));
-    NETBC_HIP(hipMemcpy(d.hub_chunk0.p, chunk0.data(), chunk0.size() * 4, hipMemcpyHostToDevice));
+    d.hub_task.alloc(task.size());
+    if (!task.empty())
+        NETBC_HIP(hipMemcpy(d.hub_task.p, task.data(), task.size() * sizeof(uint4),
+                            hipMemcpyHostToDevice));
     // Heavy rows from the out-degrees.
     std::vector<uint32_t> ooff(g.n + 1);
     NETBC_HIP(hipMemcpy(ooff.data(), d.out_off.p, (g.n + 1) * 4, hipMemcpyDeviceToHost));

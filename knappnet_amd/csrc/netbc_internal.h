@@ -16,8 +16,14 @@ namespace netbc {
 constexpr int kLanes = 64;        // sources per batch = wavefront width on gfx950
 constexpr int kBlock = 256;       // threads per block for the batched kernels
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint32_t kHubIn = 512;     // in-degree above which a node is a hub
-constexpr uint32_t kHubChunk = 512;  // in-arcs per hub pull task
+#ifndef NETBC_HUB_IN
+#define NETBC_HUB_IN 64
+#endif
+#ifndef NETBC_HUB_CHUNK
+#define NETBC_HUB_CHUNK 512
+#endif
+constexpr uint32_t kHubIn = NETBC_HUB_IN;        // in-degree above which a node is a hub
+constexpr uint32_t kHubChunk = NETBC_HUB_CHUNK;  // in-arcs per hub pull task
 constexpr uint32_t kHeavyOut = 1024; // out-degree above which a row is split into tasks
 constexpr uint32_t kChunkArcs = 256; // out-arcs per heavy-row task
 
@@ -74,11 +80,11 @@ struct DeviceGraph {
     bool weighted = false;
     DevBuf<uint32_t> out_off, out_idx, in_off, in_idx;
     DevBuf<double> out_w, in_w;  // arc weights in CSR / CSC order
-    // Hubs: nodes with more than kHubIn in-arcs. Their path counts are
-    // pulled in chunks of kHubChunk in-arcs instead of pushed.
+    // Hubs: nodes with more than kHubIn in-arcs. Their path counts (and,
+    // bottom-up, their new lanes) are gathered in chunks of kHubChunk
+    // in-arcs, one task per chunk and slot.
     DevBuf<uint8_t> hub_flag;          // per node
-    DevBuf<uint32_t> hub_node;         // hub ids
-    DevBuf<uint32_t> hub_chunk0;       // first chunk of each hub (prefix), size hubs + 1
+    DevBuf<uint4> hub_task;            // per chunk: (hub, first in-arc, end in-arc, 0)
     uint32_t hubs = 0, hub_chunks = 0;
     // Heavy rows: out-degree above kHeavyOut, expanded and swept in tasks
     // of kChunkArcs arcs.

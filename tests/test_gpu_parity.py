@@ -199,6 +199,41 @@ def test_rmat_bitexact(directed):
         assert_same(sig[i], osig, f"sigma {src}")
 
 
+@pytest.mark.parametrize("directed", [True, False])
+def test_search_directions_bitexact(directed):
+    # Top-down only (1), bottom-up at every level (2) and the automatic
+    # choice (0) give the oracle's scores bit for bit. R-MAT scale 12 has
+    # hubs (in-degree > 512) and heavy rows (out-degree > 1024).
+    el = gen.rmat(12, 32, directed=directed, seed=9)
+    g, o = pair(el.n, el.src, el.tgt, None, directed)
+    want = o.betweenness(threads=0)
+    for direction in (1, 2, 0):
+        say("directions", directed, direction)
+        got, diag = g.betweenness_sources(exact=True, direction=direction)
+        assert_same(got, want, f"direction {direction}")
+        if direction == 1:
+            assert diag["bottom_up_levels"] == 0
+        else:
+            assert diag["bottom_up_levels"] > 0
+    # partial last batch and repeated sources, bottom-up throughout
+    srcs = np.concatenate([gen.sample_sources(el.n, 150, 3), [7, 7, 7, 4095]])
+    want = o.betweenness_sources(srcs, threads=0)
+    for direction in (2, 0):
+        got, _ = g.betweenness_sources(srcs, exact=True, direction=direction)
+        assert_same(got, want, f"source list, direction {direction}")
+
+
+def test_search_directions_random_multigraphs():
+    rng = np.random.default_rng(4242)
+    for case in range(12):
+        n, s, t, directed = random_case(rng, max_n=300)
+        g, o = pair(n, s, t, None, directed)
+        want = o.betweenness(threads=0)
+        for direction in (1, 2):
+            got, _ = g.betweenness_sources(exact=True, direction=direction)
+            assert_same(got, want, f"case {case} direction {direction}")
+
+
 def test_dial_ba_bitexact():
     el = gen.ba_weighted(n=3000, m=4, seed=5)
     say("ba3000", el.m)
