@@ -259,6 +259,27 @@ __host__ __device__ __forceinline__ uint32_t group_size(size_t count, size_t wav
 #endif
 constexpr uint32_t kSweepGroup = NETBC_SWEEP_GROUP;
 
+// XCD-local work. Workgroups are dispatched to the device's 8 XCDs round-
+// robin, so workgroup i runs on XCD i % 8, and each XCD has its own L2.
+// Kernels whose work is ordered by slot give XCD k the slots b with
+// b % 8 == k, so an XCD's L2 holds the bitmaps of its own slots only.
+// Their grids are multiples of 8 (xcd_grid), so every XCD class has waves.
+constexpr uint32_t kXcds = 8;
+struct XcdWave {
+    uint32_t xcd;   // XCD class of this wavefront's workgroup
+    size_t w;       // wavefront index within the class
+    size_t stride;  // wavefronts in the class
+};
+__device__ __forceinline__ XcdWave xcd_wave() {
+    const size_t wpb = blockDim.x >> 6;
+    return XcdWave{blockIdx.x % kXcds, (size_t)(blockIdx.x / kXcds) * wpb + (threadIdx.x >> 6),
+                   (size_t)(gridDim.x / kXcds) * wpb};
+}
+// Slots b < slots with b % 8 == k.
+__device__ __forceinline__ uint32_t xcd_slots(uint32_t k, int slots) {
+    return (uint32_t)slots > k ? ((uint32_t)slots - k + kXcds - 1) / kXcds : 0;
+}
+
 // Exclusive prefix of per-lane counts over the wavefront; total = their sum.
 __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
     const int lane = lane_id();
@@ -527,8 +548,8 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
 // bottom-up step (bottom_up != 0) N = active & ~vis[x]: the lanes found are
 // ORed into cand[x], x enters the touched list with its first lanes, and
 // the in-neighbours record their successor lanes (succ). A wavefront takes
-// gsz tasks and walks the arcs of those with N != 0 as one concatenation
-// (gather_counts).
+// gsz tasks of one slot and walks the arcs of those with N != 0 as one
+// concatenation (gather_counts); slots are spread over the XCDs (xcd_wave).
 __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chunks, int slots,
                                const uint32_t* __restrict__ in_idx,
                                const uint64_t* __restrict__ cur, uint64_t* __restrict__ cand,
@@ -538,16 +559,19 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
                                int64_t left, uint32_t gsz, size_t n) {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt(lane);
-    const size_t tasks = (size_t)chunks * slots;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
-    for (size_t w0 = global_wave() * gsz; w0 < tasks; w0 += wave_count() * gsz) {
-        const size_t t = w0 + lane;
+    const XcdWave xw = xcd_wave();
+    const size_t gps = (chunks + gsz - 1) / gsz;  // task groups per slot
+    const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
+    for (size_t g = xw.w; g < groups; g += xw.stride) {
+        const uint32_t bs = xw.xcd + kXcds * (uint32_t)(g / gps);
+        const uint32_t c = (uint32_t)(g % gps) * gsz + lane;
         uint32_t b = 0;
         uint4 ht = make_uint4(0, 0, 0, 0);
         uint64_t N = 0;
-        if ((uint32_t)lane < gsz && t < tasks) {
-            b = (uint32_t)(t / chunks);
-            ht = hub_task[t - (size_t)b * chunks];
+        if ((uint32_t)lane < gsz && c < chunks) {
+            b = bs;
+            ht = hub_task[c];
             const size_t p = (size_t)b * n + ht.x;
             N = bottom_up ? (active_lanes(left, b) & ~vis[p]) : cand[p];
         }
@@ -648,7 +672,8 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
 // reached, U = active & ~vis[x], gathers over its in-arcs the lanes held by
 // level-d in-neighbours and their path counts (gather_counts, recording
 // successor lanes in succ); the pairs reached join level d+1 here. A
-// wavefront takes 64 consecutive pairs. Hubs are left to bfs_hub_kernel and
+// wavefront takes 64 consecutive pairs of one slot; slots are spread over
+// the XCDs (xcd_wave). Hubs are left to bfs_hub_kernel and
 // bfs_gather_kernel. Threads also clear the level d-1 entries of the next
 // bitmap (AND-NOT, commuting with the OR of the new lanes: a node's lanes
 // at levels d-1 and d+1 are disjoint).
@@ -673,14 +698,17 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
     }
     const uint64_t lt = lanemask_lt(lane);
-    const size_t pairs = (size_t)slots * n;
     unsigned long long fa = 0, closed = 0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
-    for (size_t p0 = global_wave() * 64; p0 < pairs; p0 += wave_count() * 64) {
-        const size_t p = p0 + lane;
-        const bool ok = p < pairs;
-        const uint32_t b = ok ? (uint32_t)(p / n) : 0;
-        const uint32_t x = ok ? (uint32_t)(p - (size_t)b * n) : 0;
+    const XcdWave xw = xcd_wave();
+    const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
+    const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
+    for (size_t g = xw.w; g < groups; g += xw.stride) {
+        const uint32_t b = xw.xcd + kXcds * (uint32_t)(g / gps);
+        const uint32_t x0 = (uint32_t)(g % gps) * 64;
+        const bool ok = x0 + (uint32_t)lane < n;
+        const uint32_t x = ok ? x0 + lane : 0;
+        const size_t p = (size_t)b * n + x;
         uint64_t U = 0, V = 0;
         uint32_t i0 = 0, deg = 0;
         if (ok) {
@@ -1564,6 +1592,11 @@ unsigned resident_blocks(K kernel, size_t waves) {
     return std::min(blocks_for_waves(waves), cap);
 }
 
+// Grid of a kernel that splits its work by XCD (xcd_wave): a multiple of 8.
+inline unsigned xcd_grid(unsigned blocks) {
+    return std::max<unsigned>(kXcds, (blocks + kXcds - 1) / kXcds * kXcds);
+}
+
 // Wavefronts of `kernel` that fit on the device at once.
 template <class K>
 size_t resident_waves(K kernel) {
@@ -1657,19 +1690,24 @@ void read_ctrl(Workspace& ws, hipStream_t st) {
 }
 
 // Sizes (in bytes per node and slot) of the per-slot arrays.
+// bitmaps (visited, offers, two levels, successors), the touched list, about
+// four queue entries with their heavy-row slots, and the per-lane rows.
 size_t slot_bytes_per_node(const Lengths& L, int precision) {
-    size_t b = 3 * 8 + kLanes * 8 + kLanes * (precision ? 4 : 8);
+    size_t b = 5 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
     if (L.kind == kInteger) b += kLanes * 4 + 8 * ((L.max + 1 + 63) / 64) + 8;
-    if (L.kind == kReal) b += 0;
     return b;
 }
 
+// Slots per super-batch: breadth-first runs take 64 (R-MAT-18, all sources:
+// 8 slots 1.12 s, 16 0.99 s, 32 0.93 s, 64 0.90 s per 65,536 sources: wider
+// launches spread each level's fixed costs), integer lengths 16, real
+// lengths 4; all within a 96 GiB share of the device's memory.
 int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& opt) {
     int64_t batches = (k + kLanes - 1) / kLanes;
-    int want = opt.slots > 0 ? std::min(opt.slots, 1024) : 16;
-    if (L.kind == kReal) want = opt.slots > 0 ? opt.slots : 4;
+    int want = L.kind == kUniform ? 64 : (L.kind == kInteger ? 16 : 4);
+    if (opt.slots > 0) want = std::min(opt.slots, 1024);
     size_t per = slot_bytes_per_node(L, opt.precision) * std::max<uint64_t>(g.n, 1);
-    size_t budget = (size_t)48 << 30;
+    size_t budget = (size_t)96 << 30;
     int by_mem = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     int s = std::min<int64_t>({(int64_t)want, batches, (int64_t)by_mem});
     return std::max(1, s);
@@ -1781,8 +1819,8 @@ struct Engine {
                                    (opt.direction == 0 && f_arcs * alpha > open_arcs);
             if (bottom_up) {
                 diag.bottom_up_levels++;
-                bfs_pull_kernel<<<resident_blocks(bfs_pull_kernel, (pairs + 63) / 64), kBlock, 0,
-                                  st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
+                bfs_pull_kernel<<<xcd_grid(resident_blocks(bfs_pull_kernel, (pairs + 63) / 64)),
+                                  kBlock, 0, st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, G[dl & 1], G[(dl + 1) & 1], ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
@@ -1802,7 +1840,8 @@ struct Engine {
             if (d.hubs) {
                 const size_t hub_tasks = (size_t)d.hub_chunks * slots;
                 const uint32_t gsz = group_size(hub_tasks, resident_waves(bfs_hub_kernel));
-                bfs_hub_kernel<<<resident_blocks(bfs_hub_kernel, (hub_tasks + gsz - 1) / gsz),
+                bfs_hub_kernel<<<xcd_grid(resident_blocks(bfs_hub_kernel,
+                                                          (hub_tasks + gsz - 1) / gsz)),
                                  kBlock, 0, st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
                                                   G[dl & 1], ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
                                                   ws.vis.p, ws.succ.p, ws.touched.p,
