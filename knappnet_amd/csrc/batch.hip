@@ -1502,48 +1502,75 @@ __global__ void clear_rows_kernel(const uint64_t* __restrict__ vis, double* __re
     }
 }
 
+// Zeroes each source's own entry (sigma = 1 at level 0, which the sweep
+// leaves in place) so that the row sums below need no per-lane source test.
+__global__ void zero_sources_kernel(const QEnt* __restrict__ q, uint32_t count,
+                                    double* __restrict__ sig, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const QEnt e = q[i];
+    uint64_t M = e.m;
+    while (M) {
+        int l = __ffsll((unsigned long long)M) - 1;
+        M &= M - 1;
+        sig[((size_t)e.b * n + e.v) * kLanes + l] = 0.0;
+    }
+}
+
 // The same sum for runs whose source order is free (neighbourhood
-// batching): a wave per node reads each slot's row coalesced, masks the
-// unreached lanes and the node's own source lane, reduces the row with a
-// fixed butterfly and adds it to bc[v] in slot order (deterministic). The
-// rows are zeroed for the next super-batch.
+// batching). A wave per node v: lane b reads the reached lanes of v in
+// slot b (64 slots at a time), then the rows of the slots that reached v
+// are read kAhead at a time and added per lane in slot order; one fixed
+// butterfly adds the lanes, and bc[v] += the result (deterministic for a
+// given slot count). Sources' own entries are zero (zero_sources_kernel).
+// The rows are zeroed for the next super-batch.
 __global__ void accumulate_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                        int slots, size_t n, int64_t item0, int64_t k,
-                                       const int64_t* __restrict__ list,
                                        double* __restrict__ bc,
                                        const uint32_t* __restrict__ off, Ctrl* ctrl) {
     const int lane = lane_id();
     unsigned long long rn = 0, ra = 0;
     for (size_t v = global_wave(); v < n; v += wave_count()) {
-        double B = 0.0;
+        double x = 0.0;
         unsigned long long cnt = 0;
-        for (int b = 0; b < slots; b++) {
-            int64_t ib = item0 + (int64_t)b * kLanes;
-            if (ib >= k) break;
-            uint64_t M = vis[(size_t)b * n + v];
-            if (!M) continue;
+        for (int b0 = 0; b0 < slots; b0 += 64) {
+            const int b = b0 + lane;
+            uint64_t M = 0;
+            if (b < slots && item0 + (int64_t)b * kLanes < k) M = vis[(size_t)b * n + v];
             cnt += (unsigned long long)__popcll(M);
-            double* row = sig + ((size_t)b * n + v) * kLanes;
-            double x = 0.0;
-            if ((M >> lane) & 1) {
-                int64_t i = ib + lane;
-                int64_t s = list ? list[i] : i;
-                x = (size_t)s == v ? 0.0 : row[lane];
-                row[lane] = 0.0;
-            }
+            uint64_t act = __ballot(M != 0);
+            while (act) {
+                double val[kAhead];
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-            B += x;
+                for (int j = 0; j < kAhead; j++) {
+                    val[j] = 0.0;
+                    if (act) {
+                        const int s = __ffsll((unsigned long long)act) - 1;
+                        act &= act - 1;
+                        const uint64_t Ms = rl64(M, s);
+                        if ((Ms >> lane) & 1) {
+                            double* r = &sig[((size_t)(b0 + s) * n + v) * kLanes + lane];
+                            val[j] = *r;
+                            *r = 0.0;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kAhead; j++) x += val[j];
+            }
         }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        cnt = wave_sum_u64(cnt);
         if (lane == 0) {
-            if (B != 0.0) bc[v] += B;
+            if (x != 0.0) bc[v] += x;
             rn += cnt;
             ra += cnt * (unsigned long long)(off[v + 1] - off[v]);
         }
     }
     rn = wave_sum_u64(rn);
     ra = wave_sum_u64(ra);
-    if (lane_id() == 0 && (rn || ra)) {
+    if (lane == 0 && (rn || ra)) {
         atomicAdd(&ctrl->reached_nodes, rn);
         atomicAdd(&ctrl->reached_arcs, ra);
     }
@@ -2125,11 +2152,17 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                 else eng.backward_dial<double>(ws.coef64.p);
             }
         }
-        if (free_order)
+        if (free_order) {
+            const uint32_t nsrc = eng.lv.size() > 1 ? eng.lv[1] : 0;
+            if (nsrc) {
+                zero_sources_kernel<<<blocks_for_threads(nsrc), kBlock, 0, st>>>(ws.q.p, nsrc,
+                                                                                ws.sig.p, n);
+                NETBC_HIP(hipGetLastError());
+            }
             accumulate_rows_kernel<<<resident_blocks(accumulate_rows_kernel, n), kBlock, 0,
-                                     st>>>(ws.vis.p, ws.sig.p, slots, n, item0, k, d_list, d_bc,
+                                     st>>>(ws.vis.p, ws.sig.p, slots, n, item0, k, d_bc,
                                            g.dev.out_off.p, ws.ctrl.p);
-        else
+        } else
             accumulate_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
                 ws.vis.p, ws.sig.p, slots, n, item0, k, gsize, d_list, ws.acc.p, d_bc, ws.gcur.p,
                 g.dev.out_off.p, ws.ctrl.p);
