@@ -80,6 +80,11 @@ constexpr double kTwo53 = 9007199254740992.0;
 #define NETBC_AHEAD 8
 #endif
 constexpr int kAhead = NETBC_AHEAD;
+// Level bitmap buffers per slot: levels below kKeptLevels keep their own
+// buffer from the forward search to the sweep, deeper ones alternate
+// between the last two (then the sweep rebuilds them from the queue).
+constexpr int kLevelBufs = 16;
+constexpr int kKeptLevels = kLevelBufs - 2;
 // Touched nodes staged per wave before one append to the shared list.
 constexpr uint32_t kTouchBuf = 256;
 // A level is searched bottom-up when the out-arcs of its entries times
@@ -827,9 +832,10 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
     }
 }
 
-// Dependencies of the level-d entries. R is the level d+1 bitmap, W the
-// bitmap being turned from level d+2 into level d (lanes are disjoint, so
-// the OR and AND-NOT updates commute). A wavefront takes 64 light entries:
+// Dependencies of the level-d entries. R is the level d+1 bitmap. With
+// rebuild, W is turned from the level d+2 bitmap into level d's (lanes are
+// disjoint, so the OR and AND-NOT updates commute); levels kept from the
+// forward search need no rebuild. A wavefront takes up to 64 light entries:
 // the rows of those with a successor in their lanes (succ) are walked as
 // one concatenation, 64 arcs at a time, and the successors' coefficients
 // are added per lane in CSR order (hits are taken in order, so one row's
@@ -839,7 +845,7 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
 // by bfs_combine_kernel; with split == 0 every row is summed whole.
 template <class C>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
-                                    uint32_t abeg, uint32_t aend, int last_level,
+                                    uint32_t abeg, uint32_t aend, int last_level, int rebuild,
                                     const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
                                     const uint32_t* __restrict__ hpre, uint32_t hchunks, int split,
                                     const uint8_t* __restrict__ heavy_flag,
@@ -853,7 +859,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
-    if (!last_level) {
+    if (rebuild) {
         size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
         size_t nth = (size_t)gridDim.x * blockDim.x;
         for (size_t i = qbeg + tid; i < qend; i += nth) {
@@ -1665,7 +1671,13 @@ struct KernelTimer {
 struct Workspace {
     int slots = 0;
     uint64_t n = 0;
-    DevBuf<uint64_t> vis, cand, g0, g1;
+    DevBuf<uint64_t> vis, cand;
+    // Level bitmaps: kLevelBufs buffers of slots * n words. Levels below
+    // kKeptLevels keep their own buffer from the forward search to the
+    // sweep; deeper levels alternate between the last two buffers and are
+    // rebuilt from the level queue during the sweep.
+    DevBuf<uint64_t> levels;
+    size_t zero_words = 0;  // leading words of `levels` that may hold bits
     DevBuf<uint64_t> succ;  // lanes in which a node has a successor on the next level
     DevBuf<uint64_t> inb;  // integer lengths: pending-bucket bitmasks per (slot, node)
     DevBuf<double> sig, coef64, acc;
@@ -1717,10 +1729,12 @@ void read_ctrl(Workspace& ws, hipStream_t st) {
 }
 
 // Sizes (in bytes per node and slot) of the per-slot arrays.
-// bitmaps (visited, offers, two levels, successors), the touched list, about
-// four queue entries with their heavy-row slots, and the per-lane rows.
+// bitmaps (visited, offers, successors, the level buffers), the touched
+// list, about four queue entries with their heavy-row slots, and the
+// per-lane rows.
 size_t slot_bytes_per_node(const Lengths& L, int precision) {
-    size_t b = 5 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
+    size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
+    if (L.kind == kUniform) b += kLevelBufs * 8;
     if (L.kind == kInteger) b += kLanes * 4 + 8 * ((L.max + 1 + 63) / 64) + 8;
     return b;
 }
@@ -1791,8 +1805,11 @@ struct Engine {
         if (L.kind == kUniform) {
             fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.cand.p, nm, 0);
             fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.succ.p, nm, 0);
-            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.g0.p, nm, 0);
-            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.g1.p, nm, 0);
+            if (ws.zero_words > 0)
+                fill_u64_kernel<<<blocks_for_threads(std::min<size_t>(ws.zero_words,
+                                                                      (size_t)1 << 22)),
+                                  kBlock, 0, st>>>(ws.levels.p, ws.zero_words, 0);
+            ws.zero_words = 0;
             launch_check();
         }
         if (L.kind == kInteger) {
@@ -1812,7 +1829,7 @@ struct Engine {
         const DeviceGraph& d = g.dev;
         reset_slots();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
-            d_list, item0, k, slots, n, ws.vis.p, ws.g0.p, ws.sig.p, ws.q.p, ws.ctrl.p, nullptr,
+            d_list, item0, k, slots, n, ws.vis.p, level_bits(0), ws.sig.p, ws.q.p, ws.ctrl.p, nullptr,
             d.heavy_flag.p, ws.hq.p, nullptr, 0, d.out_off.p);
         launch_check();
         read_ctrl(ws, st);
@@ -1825,7 +1842,6 @@ struct Engine {
         read_ctrl(ws, st);
         hv.push_back(ws.host_ctrl->hlen);
         hc.push_back(ws.host_ctrl->hchunks);
-        uint64_t* G[2] = {ws.g0.p, ws.g1.p};
         const int64_t left = k - item0;  // sources from lane 0 of slot 0 on
         const uint64_t live_slots =
             (uint64_t)std::min<int64_t>(slots, (left + kLanes - 1) / kLanes);
@@ -1835,7 +1851,12 @@ struct Engine {
         for (int dl = 0;; dl++) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             uint32_t hb = hv[dl], he = hv[dl + 1];
-            uint32_t zb = dl > 0 ? lv[dl - 1] : 0, ze = dl > 0 ? lv[dl] : 0;
+            // The next level's buffer holds level dl-1 only where buffers
+            // alternate; those stale entries are cleared during the step.
+            const bool stale = dl - 1 >= kKeptLevels;
+            uint32_t zb = stale ? lv[dl - 1] : 0, ze = stale ? lv[dl] : 0;
+            uint64_t* cur = level_bits(dl);
+            uint64_t* nxt = level_bits(dl + 1);
             // Direction of this step: top-down costs the out-arcs of the
             // level's entries, bottom-up the in-arcs of the pairs with
             // unreached active lanes.
@@ -1848,7 +1869,7 @@ struct Engine {
                 diag.bottom_up_levels++;
                 bfs_pull_kernel<<<xcd_grid(resident_blocks(bfs_pull_kernel, (pairs + 63) / 64)),
                                   kBlock, 0, st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
-                                        ws.vis.p, G[dl & 1], G[(dl + 1) & 1], ws.succ.p, ws.sig.p,
+                                        ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
                                         slots, left, n);
@@ -1860,7 +1881,7 @@ struct Engine {
                 size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
                 bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
-                    d.out_idx.p, ws.vis.p, ws.cand.p, G[(dl + 1) & 1], ws.succ.p, ws.touched.p,
+                    d.out_idx.p, ws.vis.p, ws.cand.p, nxt, ws.succ.p, ws.touched.p,
                     &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, &front[dl & 1], gsz, n);
                 launch_check();
             }
@@ -1870,7 +1891,7 @@ struct Engine {
                 bfs_hub_kernel<<<xcd_grid(resident_blocks(bfs_hub_kernel,
                                                           (hub_tasks + gsz - 1) / gsz)),
                                  kBlock, 0, st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
-                                                  G[dl & 1], ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
+                                                  cur, ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
                                                   ws.vis.p, ws.succ.p, ws.touched.p,
                                                   &ws.ctrl.p->tlen[dl & 1], left, gsz, n);
                 launch_check();
@@ -1878,14 +1899,17 @@ struct Engine {
             bfs_gather_kernel<<<resident_blocks(bfs_gather_kernel, (pairs + 63) / 64), kBlock, 0,
                                 st>>>(ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
                                       &ws.ctrl.p->tlen[(dl + 1) & 1], d.in_off_p(), d.in_idx_p(),
-                                      d.out_off.p, G[dl & 1], ws.cand.p, ws.vis.p, G[(dl + 1) & 1],
+                                      d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
                                       ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.q.p,
                                       (uint32_t)ws.qcap, ws.hq.p, ws.ctrl.p, left,
                                       &front[(dl + 1) & 1], n);
             launch_check();
             scan_heavy(hv.back());
             read_ctrl(ws, st);
-            if (ws.host_ctrl->qoverflow) return false;
+            if (ws.host_ctrl->qoverflow) {
+                ws.zero_words = ws.levels.n;
+                return false;
+            }
             uint32_t ql = ws.host_ctrl->qlen;
             if (ql == qe) break;
             lv.push_back(ql);
@@ -1893,32 +1917,45 @@ struct Engine {
             hc.push_back(ws.host_ctrl->hchunks);
             diag.levels++;
         }
+        // buffers written: those of levels 0 .. D+1 (D: last level with entries)
+        const int D = (int)lv.size() - 2;
+        ws.zero_words = (size_t)std::min(D + 2, kLevelBufs) * slots * n;
         return true;
+    }
+
+    // Bitmap buffer of level d (see Workspace::levels).
+    uint64_t* level_bits(int d) {
+        const int b = d < kKeptLevels ? d : kKeptLevels + ((d - kKeptLevels) & 1);
+        return ws.levels.p + (size_t)b * slots * n;
     }
 
     template <class C>
     void backward_bfs(C* coef) {
         const DeviceGraph& d = g.dev;
-        uint64_t* G[2] = {ws.g0.p, ws.g1.p};
         int D = (int)lv.size() - 2;  // last level with entries
         const int split = opt.exact ? 0 : 1;
         for (int dl = D; dl >= 1; dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             uint32_t hb = hv[dl], he = hv[dl + 1];
-            uint32_t ab = dl + 2 <= D ? lv[dl + 2] : 0, ae = dl + 2 <= D ? lv[dl + 3] : 0;
+            // Levels in alternating buffers are rebuilt for the next step:
+            // W = level dl, from the buffer holding level dl+2.
+            const bool rebuild = dl >= kKeptLevels && dl < D;
+            uint32_t ab = rebuild && dl + 2 <= D ? lv[dl + 2] : 0;
+            uint32_t ae = rebuild && dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
             const uint32_t gsz =
                 group_size(qe - qb, resident_waves(bfs_backward_kernel<C>), kSweepGroup);
             size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
-            size_t waves = std::max<size_t>(tasks, (qe - qb + ae - ab + 63) / 64);
+            size_t waves = std::max<size_t>(tasks, rebuild ? (qe - qb + ae - ab + 63) / 64 : 0);
             if (heavy && ws.part.n < (size_t)hc[dl] * kLanes) {
                 NETBC_HIP(hipStreamSynchronize(st));  // the old buffer may be in use
                 ws.part.alloc((size_t)hc[dl] * kLanes * 2);
             }
             ws.sweep.mark(st);
             bfs_backward_kernel<C><<<resident_blocks(bfs_backward_kernel<C>, waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, ab, ae, dl == D, ws.hq.p, hb, he, ws.hpre.p, hc[dl], split,
-                d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p, G[(dl + 1) & 1], G[dl & 1],
+                ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl], split,
+                d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p, level_bits(dl + 1),
+                level_bits(dl),
                 ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz, n);
             launch_check();
             if (heavy) {
@@ -2034,8 +2071,9 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     if (L.kind == kUniform) {
         ws.cand.alloc(nm);
         ws.succ.alloc(nm);
-        ws.g0.alloc(nm);
-        ws.g1.alloc(nm);
+        uint64_t* lv_before = ws.levels.p;
+        ws.levels.alloc(nm * kLevelBufs);
+        if (ws.levels.p != lv_before) ws.zero_words = ws.levels.n;
         ws.touched.alloc(nm);
     }
     if (L.kind != kReal) {

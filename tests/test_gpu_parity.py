@@ -223,6 +223,29 @@ def test_search_directions_bitexact(directed):
         assert_same(got, want, f"source list, direction {direction}")
 
 
+@pytest.mark.parametrize("directed", [True, False])
+def test_deep_graphs_bitexact(directed):
+    # A path of 400 nodes with short random chords (and 20 random edges):
+    # depth 37 undirected, 104 directed, beyond the 14 level bitmaps kept
+    # from the forward search, so deeper ones alternate and are rebuilt
+    # during the sweep; many tied shortest paths.
+    rng = np.random.default_rng(17 + directed)
+    n = 400
+    a = rng.integers(0, n - 4, 300)
+    src = np.concatenate([np.arange(n - 1), a, rng.integers(0, n, 20)])
+    tgt = np.concatenate([np.arange(1, n), a + rng.integers(2, 4, 300), rng.integers(0, n, 20)])
+    g, o = pair(n, src, tgt, None, directed)
+    want = o.betweenness(threads=0)
+    for direction in (1, 2, 0):
+        say("deep", directed, direction)
+        got, diag = g.betweenness_sources(exact=True, direction=direction)
+        assert diag["levels"] > 32
+        assert_same(got, want, f"deep direction {direction}")
+    got, _ = g.betweenness_sources(np.arange(n).repeat(12)[: 4096 + 100])
+    assert_rel(got, o.betweenness_sources(np.arange(n).repeat(12)[: 4096 + 100], threads=0),
+               "deep, neighbourhood batches")
+
+
 def test_search_directions_random_multigraphs():
     rng = np.random.default_rng(4242)
     for case in range(12):
