@@ -50,6 +50,9 @@ struct Ctrl {
     // are all reached.
     unsigned long long front_arcs[2];
     unsigned long long closed_arcs;
+    // Work tickets of dynamically scheduled kernels; the last wavefront to
+    // finish resets both for the next launch.
+    uint32_t ticket, exits;
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
@@ -256,11 +259,11 @@ __host__ __device__ __forceinline__ uint32_t group_size(size_t count, size_t wav
     return (uint32_t)(g < 1 ? 1 : (g > most ? most : g));
 }
 
-// Entries per dependency-sweep task at most (measured on R-MAT-18: 4 beats
-// 16 by 11% and 64 by 21%; the sweep's rows are long enough that a few
-// entries keep a wavefront's loads in flight).
+// Entries per dependency-sweep task at most. Tasks are handed out by
+// ticket; on R-MAT-18 groups of 32 beat 16 and 64 by 5-7%, and 4 (too many
+// tickets) by 2x.
 #ifndef NETBC_SWEEP_GROUP
-#define NETBC_SWEEP_GROUP 4
+#define NETBC_SWEEP_GROUP 32
 #endif
 constexpr uint32_t kSweepGroup = NETBC_SWEEP_GROUP;
 
@@ -283,6 +286,21 @@ __device__ __forceinline__ XcdWave xcd_wave() {
 // Slots b < slots with b % 8 == k.
 __device__ __forceinline__ uint32_t xcd_slots(uint32_t k, int slots) {
     return (uint32_t)slots > k ? ((uint32_t)slots - k + kXcds - 1) / kXcds : 0;
+}
+
+// Next task of a dynamically scheduled kernel (wave-uniform).
+__device__ __forceinline__ uint32_t next_ticket(uint32_t* ticket) {
+    uint32_t t = 0;
+    if (lane_id() == 0) t = atomicAdd(ticket, 1u);
+    return rl32(t, 0);
+}
+// Called once per wavefront after its last ticket: the last one out resets
+// the counters for the next launch on the stream.
+__device__ __forceinline__ void retire_tickets(uint32_t* ticket, uint32_t* exits) {
+    if (lane_id() == 0 && atomicAdd(exits, 1u) == (uint32_t)wave_count() - 1) {
+        *ticket = 0;
+        *exits = 0;
+    }
 }
 
 // Exclusive prefix of per-lane counts over the wavefront; total = their sum.
@@ -885,7 +903,12 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
     const size_t nq = qend - qbeg;
     const size_t groups = (nq + gsz - 1) / gsz;
     const size_t tasks = groups + (split ? hchunks : 0);
-    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+    // Tasks are handed out by ticket, the next one fetched while the
+    // current one runs.
+    uint32_t next = next_ticket(&ctrl->ticket);
+    while (next < tasks) {
+        const size_t w = next;
+        next = next_ticket(&ctrl->ticket);
         if (w >= groups) {
             // One chunk of a heavy row: its partial sum.
             uint32_t t = (uint32_t)(w - groups);
@@ -1006,6 +1029,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             }
         }
     }
+    retire_tickets(&ctrl->ticket, &ctrl->exits);
     note_sigma(ctrl, smax);
     NETBC_FLUSH(ctrl, 4, c4);
     NETBC_FLUSH(ctrl, 5, c5);
