@@ -568,7 +568,8 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
 // level d, and adds the partial sum into sigma[x][l] (integers below 2^53:
 // the partial sums are exact and their order does not matter). After a
 // top-down expansion N = cand[x], the lanes in which x was reached. In a
-// bottom-up step (bottom_up != 0) N = active & ~vis[x]: the lanes found are
+// bottom-up step (bottom_up != 0) N = F_b & ~vis[x] (F_b: the lanes still
+// searching, flanes): the lanes found are
 // ORed into cand[x], x enters the touched list with its first lanes, and
 // the in-neighbours record their successor lanes (succ). A wavefront takes
 // gsz tasks of one slot and walks the arcs of those with N != 0 as one
@@ -579,7 +580,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
                                double* __restrict__ sig, int bottom_up,
                                const uint64_t* __restrict__ vis, uint64_t* __restrict__ succ,
                                uint2* __restrict__ touched, uint32_t* __restrict__ tlen,
-                               int64_t left, uint32_t gsz, size_t n) {
+                               const uint64_t* __restrict__ flanes, uint32_t gsz, size_t n) {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt(lane);
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
@@ -596,7 +597,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
             b = bs;
             ht = hub_task[c];
             const size_t p = (size_t)b * n + ht.x;
-            N = bottom_up ? (active_lanes(left, b) & ~vis[p]) : cand[p];
+            N = bottom_up ? (flanes[b] & ~vis[p]) : cand[p];
         }
         const uint32_t deg = N ? ht.z - ht.y : 0;
         const uint64_t got = gather_counts<true>(ht.x, b, N, ht.y, deg, in_idx, cur,
@@ -615,6 +616,24 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
             }
         }
     }
+}
+
+// Lanes of each slot that hold entries of a level (the lanes whose search
+// is still running): fl[b] |= the masks of the level's entries in slot b,
+// ORed per block in LDS and added with one atomic per block and slot.
+__global__ void frontier_lanes_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
+                                      uint64_t* __restrict__ fl, int slots) {
+    extern __shared__ unsigned long long fl_lds[];
+    for (int b = threadIdx.x; b < slots; b += blockDim.x) fl_lds[b] = 0;
+    __syncthreads();
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = qbeg + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < qend; i += nth) {
+        const QEnt e = q[i];
+        atomicOr(&fl_lds[e.b], (unsigned long long)e.m);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < slots; b += blockDim.x)
+        if (fl_lds[b]) atomicOr((unsigned long long*)&fl[b], fl_lds[b]);
 }
 
 // Closes level d: every touched pair (x, b) takes its new lanes N = cand[x]
@@ -691,8 +710,9 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
 }
 
 // Bottom-up step of level d (direction-optimising search). Every (slot b,
-// node x) pair that is not a hub, has in-arcs and has active lanes not yet
-// reached, U = active & ~vis[x], gathers over its in-arcs the lanes held by
+// node x) pair that is not a hub, has in-arcs and has lanes not yet reached
+// among those still searching, U = F_b & ~vis[x] (F_b: the lanes with
+// level-d entries, flanes), gathers over its in-arcs the lanes held by
 // level-d in-neighbours and their path counts (gather_counts, recording
 // successor lanes in succ); the pairs reached join level d+1 here. A
 // wavefront takes 64 consecutive pairs of one slot; slots are spread over
@@ -710,8 +730,9 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
                                 const uint8_t* __restrict__ heavy_flag, QEnt* q, uint32_t qcap,
                                 uint32_t* __restrict__ hq, Ctrl* ctrl,
                                 unsigned long long* __restrict__ front_next,
-                                unsigned long long* __restrict__ front_done, int slots,
-                                int64_t left, size_t n) {
+                                unsigned long long* __restrict__ front_done,
+                                const uint64_t* __restrict__ flanes, int slots, int64_t left,
+                                size_t n) {
     const int lane = lane_id();
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nth = (size_t)gridDim.x * blockDim.x;
@@ -728,6 +749,8 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
     for (size_t g = xw.w; g < groups; g += xw.stride) {
         const uint32_t b = xw.xcd + kXcds * (uint32_t)(g / gps);
+        const uint64_t F = flanes[b];  // lanes still searching in this slot
+        if (!F) continue;
         const uint32_t x0 = (uint32_t)(g % gps) * 64;
         const bool ok = x0 + (uint32_t)lane < n;
         const uint32_t x = ok ? x0 + lane : 0;
@@ -735,10 +758,9 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         uint64_t U = 0, V = 0;
         uint32_t i0 = 0, deg = 0;
         if (ok) {
-            const uint64_t A = active_lanes(left, b);
-            if (A) {
-                V = vis[p];
-                U = A & ~V;
+            V = vis[p];
+            U = F & ~V;
+            {
                 if (U && !hub_flag[x]) {
                     i0 = in_off[x];
                     deg = in_off[x + 1] - i0;
@@ -1703,6 +1725,7 @@ struct Workspace {
     DevBuf<uint64_t> levels;
     size_t zero_words = 0;  // leading words of `levels` that may hold bits
     DevBuf<uint64_t> succ;  // lanes in which a node has a successor on the next level
+    DevBuf<uint64_t> flanes;  // per level parity and slot: lanes with entries on the level
     DevBuf<uint64_t> inb;  // integer lengths: pending-bucket bitmasks per (slot, node)
     DevBuf<double> sig, coef64, acc;
     DevBuf<float> coef32;
@@ -1863,6 +1886,7 @@ struct Engine {
         hv.assign(1, 0);
         hc.clear();
         scan_heavy(0);
+        frontier(0, 0, lv[1]);
         read_ctrl(ws, st);
         hv.push_back(ws.host_ctrl->hlen);
         hc.push_back(ws.host_ctrl->hchunks);
@@ -1896,7 +1920,7 @@ struct Engine {
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
-                                        slots, left, n);
+                                        frontier_lanes(dl), slots, left, n);
                 launch_check();
             } else {
                 const uint32_t gsz =
@@ -1917,7 +1941,8 @@ struct Engine {
                                  kBlock, 0, st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
                                                   cur, ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
                                                   ws.vis.p, ws.succ.p, ws.touched.p,
-                                                  &ws.ctrl.p->tlen[dl & 1], left, gsz, n);
+                                                  &ws.ctrl.p->tlen[dl & 1],
+                                                  frontier_lanes(dl), gsz, n);
                 launch_check();
             }
             bfs_gather_kernel<<<resident_blocks(bfs_gather_kernel, (pairs + 63) / 64), kBlock, 0,
@@ -1939,12 +1964,27 @@ struct Engine {
             lv.push_back(ql);
             hv.push_back(ws.host_ctrl->hlen);
             hc.push_back(ws.host_ctrl->hchunks);
+            frontier(dl + 1, qe, ql);
             diag.levels++;
         }
         // buffers written: those of levels 0 .. D+1 (D: last level with entries)
         const int D = (int)lv.size() - 2;
         ws.zero_words = (size_t)std::min(D + 2, kLevelBufs) * slots * n;
         return true;
+    }
+
+    // Lanes with entries on level d, per slot (flanes, by level parity).
+    uint64_t* frontier_lanes(int d) { return ws.flanes.p + (size_t)(d & 1) * slots; }
+    void frontier(int d, uint32_t qb, uint32_t qe) {
+        if (opt.direction == 1) return;  // top-down only: not needed
+        uint64_t* fl = frontier_lanes(d);
+        NETBC_HIP(hipMemsetAsync(fl, 0, (size_t)slots * sizeof(uint64_t), st));
+        if (qe > qb) {
+            unsigned blocks = std::min<unsigned>(256, blocks_for_threads(qe - qb));
+            frontier_lanes_kernel<<<blocks, kBlock, (size_t)slots * sizeof(uint64_t), st>>>(
+                ws.q.p, qb, qe, fl, slots);
+            launch_check();
+        }
     }
 
     // Bitmap buffer of level d (see Workspace::levels).
@@ -2095,6 +2135,7 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     if (L.kind == kUniform) {
         ws.cand.alloc(nm);
         ws.succ.alloc(nm);
+        ws.flanes.alloc((size_t)2 * slots);
         uint64_t* lv_before = ws.levels.p;
         ws.levels.alloc(nm * kLevelBufs);
         if (ws.levels.p != lv_before) ws.zero_words = ws.levels.n;
