@@ -96,7 +96,7 @@ typedef struct netbc_diagnostics {
     int32_t slots;           /* 64-source batches per launch used */
     int32_t bottom_up_levels; /* search levels run bottom-up */
     double build_ms;         /* device CSR/CSC build time (first call) */
-    int64_t work[8];         /* engine work counters; zero unless the library
+    int64_t work[16];        /* engine work counters; zero unless the library
                                 is built with NETBC_COUNTERS */
 } netbc_diagnostics;
 

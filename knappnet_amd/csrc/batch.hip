@@ -60,9 +60,12 @@ struct Ctrl {
     unsigned long long reached_arcs;
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
     // Work counters (filled only in builds with NETBC_COUNTERS):
-    // 0 expand entries, 1 expand arcs, 2 expand pushed arcs, 3 pushed lanes,
-    // 4 sweep entries, 5 sweep arcs, 6 sweep contributing arcs, 7 their lanes.
-    unsigned long long work[8];
+    // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes,
+    // 4 sweep entries, 5 sweep arcs, 6 sweep contributing arcs, 7 their lanes,
+    // 8 bottom-up arcs, 9 their hits, 10 gathered arcs, 11 their hits,
+    // 12 hub-task arcs, 13 their hits, 14 hub tasks with lanes sought,
+    // 15 bottom-up pairs with lanes sought.
+    unsigned long long work[16];
 };
 
 #ifdef NETBC_COUNTERS
@@ -580,10 +583,11 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
                                double* __restrict__ sig, int bottom_up,
                                const uint64_t* __restrict__ vis, uint64_t* __restrict__ succ,
                                uint2* __restrict__ touched, uint32_t* __restrict__ tlen,
-                               const uint64_t* __restrict__ flanes, uint32_t gsz, size_t n) {
+                               const uint64_t* __restrict__ flanes, Ctrl* ctrl, uint32_t gsz,
+                               size_t n) {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt(lane);
-    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
     const XcdWave xw = xcd_wave();
     const size_t gps = (chunks + gsz - 1) / gsz;  // task groups per slot
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
@@ -600,6 +604,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
             N = bottom_up ? (flanes[b] & ~vis[p]) : cand[p];
         }
         const uint32_t deg = N ? ht.z - ht.y : 0;
+        NETBC_COUNT(c6, N != 0);
         const uint64_t got = gather_counts<true>(ht.x, b, N, ht.y, deg, in_idx, cur,
                                                  bottom_up ? succ : nullptr, sig, n, c4, c5);
         if (bottom_up) {
@@ -616,6 +621,9 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
             }
         }
     }
+    NETBC_FLUSH(ctrl, 12, c4);
+    NETBC_FLUSH(ctrl, 13, c5);
+    NETBC_FLUSH(ctrl, 14, c6);
 }
 
 // Lanes of each slot that hold entries of a level (the lanes whose search
@@ -701,6 +709,8 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
             }
         }
     }
+    NETBC_FLUSH(ctrl, 10, c4);
+    NETBC_FLUSH(ctrl, 11, c5);
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
     if (lane == 0) {
@@ -743,7 +753,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     }
     const uint64_t lt = lanemask_lt(lane);
     unsigned long long fa = 0, closed = 0;
-    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
     const XcdWave xw = xcd_wave();
     const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
@@ -767,6 +777,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
                 }
             }
         }
+        NETBC_COUNT(c6, deg != 0);
         const uint64_t mine =
             gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5);
         if (mine) {
@@ -791,6 +802,9 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
             }
         }
     }
+    NETBC_FLUSH(ctrl, 8, c4);
+    NETBC_FLUSH(ctrl, 9, c5);
+    NETBC_FLUSH(ctrl, 15, c6);
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
     if (lane == 0) {
@@ -1942,7 +1956,7 @@ struct Engine {
                                                   cur, ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
                                                   ws.vis.p, ws.succ.p, ws.touched.p,
                                                   &ws.ctrl.p->tlen[dl & 1],
-                                                  frontier_lanes(dl), gsz, n);
+                                                  frontier_lanes(dl), ws.ctrl.p, gsz, n);
                 launch_check();
             }
             bfs_gather_kernel<<<resident_blocks(bfs_gather_kernel, (pairs + 63) / 64), kBlock, 0,
@@ -2290,7 +2304,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     diag.sigma_overflow |= c.sig_overflow ? 1 : 0;
     diag.reached_nodes += (int64_t)c.reached_nodes;
     diag.reached_arcs += (int64_t)c.reached_arcs;
-    for (int i = 0; i < 8; i++) diag.work[i] += (int64_t)c.work[i];
+    for (int i = 0; i < 16; i++) diag.work[i] += (int64_t)c.work[i];
 }
 
 void finish_scores(Graph& g, bool normalized, double* d_bc, double* out) {

@@ -91,7 +91,7 @@ void fill_diag(netbc_diagnostics* out, const netbc::RunDiag& d, double build_ms)
     out->slots = d.slots;
     out->bottom_up_levels = (int32_t)d.bottom_up_levels;
     out->build_ms = build_ms;
-    for (int i = 0; i < 8; i++) out->work[i] = d.work[i];
+    for (int i = 0; i < 16; i++) out->work[i] = d.work[i];
 }
 
 netbc::RunOptions options(const netbc_options* opt) {
@@ -118,7 +118,7 @@ void merge_diag(netbc::RunDiag& into, const netbc::RunDiag& d) {
     into.dominant_ms += d.dominant_ms;
     into.dominant_launches += d.dominant_launches;
     into.slots = std::max(into.slots, d.slots);
-    for (int i = 0; i < 8; i++) into.work[i] += d.work[i];
+    for (int i = 0; i < 16; i++) into.work[i] += d.work[i];
 }
 
 // Betweenness of one handle over a source list (sources == nullptr: item i

@@ -145,7 +145,7 @@ struct RunDiag {
     int64_t dominant_bytes = 0;  // algorithmic bytes of the dominant kernel, summed
     int32_t slots = 0;
     int64_t bottom_up_levels = 0;
-    int64_t work[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // engine work counters (diagnostic builds)
+    int64_t work[16] = {};  // engine work counters (diagnostic builds)
 };
 
 // Builds the device CSR/CSC of g (graph.hip). Caller holds g.mu.
