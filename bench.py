@@ -174,8 +174,8 @@ def main() -> None:
                     help="bottom-up when frontier out-arcs x alpha > unreached in-arcs (0: default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scales rocprofv3 --pmc counters)")
-    ap.add_argument("--compare-precision", action="store_true",
-                    help="also run the other precision and report the relative error")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP32-dependency run and the plain FP64 run")
     ap.add_argument("--max-sources", type=int, default=0,
                     help="time only the first K sources (experiments; not the headline)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
@@ -293,20 +293,31 @@ def main() -> None:
               f"device vs oracle on sample: {cpu['check']}", flush=True)
 
     precision_check = None
-    if args.compare_precision:
-        other = 1 - args.precision
+    if not args.no_compare and args.precision == 0:
+        # The batched FP32-dependency form against FP64 (error) and against
+        # the plain FP64 form: sources in the reference's order and sums in
+        # its order, bit-identical to the CPU engine.
         t = time.perf_counter()
-        res2, d2 = g.betweenness_sources(sources=sources, weighted=weighted, precision=other,
-                                         slots=args.slots, direction=args.direction)
-        t2 = time.perf_counter() - t
-        f64, f32 = (res, res2) if args.precision == 0 else (res2, res)
-        nz = f64 != 0
-        rel = np.abs(f32[nz] - f64[nz]) / np.abs(f64[nz])
+        r32, _ = g.betweenness_sources(sources=sources, weighted=weighted, precision=1,
+                                       slots=args.slots, direction=args.direction)
+        t32 = time.perf_counter() - t
+        t = time.perf_counter()
+        rplain, _ = g.betweenness_sources(sources=sources, weighted=weighted, precision=0,
+                                          exact=True, slots=args.slots, direction=args.direction)
+        tplain = time.perf_counter() - t
+        nz = rplain != 0
+        rel = np.abs(r32[nz] - rplain[nz]) / np.abs(rplain[nz])
+        relb = np.abs(res[nz] - rplain[nz]) / np.abs(rplain[nz])
         precision_check = {
-            "fp32_delta_vs_fp64_max_rel_err": float(rel.max()) if nz.any() else 0.0,
-            "fp32_delta_vs_fp64_mean_rel_err": float(rel.mean()) if nz.any() else 0.0,
-            "other_precision_seconds": round(t2, 3),
-            "fp32_speedup_over_fp64": round((sec / t2) if args.precision == 0 else (t2 / sec), 3),
+            "fp32_delta": {"seconds": round(t32, 3), "gteps": round(k * el.m / t32 / 1e9, 3),
+                           "max_rel_err_vs_fp64": float(rel.max()) if nz.any() else 0.0,
+                           "mean_rel_err_vs_fp64": float(rel.mean()) if nz.any() else 0.0},
+            "plain_fp64": {"seconds": round(tplain, 3),
+                           "gteps": round(k * el.m / tplain / 1e9, 3),
+                           "form": "sources in the reference's order, sums in its order"},
+            "batched_fp64_vs_plain_max_rel_err": float(relb.max()) if nz.any() else 0.0,
+            "fp32_speedup_vs_plain_fp64": round(tplain / t32, 3),
+            "fp32_speedup_vs_batched_fp64": round(sec / t32, 3),
         }
         print(f"# precision check: {precision_check}", flush=True)
 

@@ -1471,27 +1471,87 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
 // runs keep list order (and so the reference's summation order).
 constexpr int64_t kReorderMin = 4096;
 
-// Sort key of item i: sources with the same out-degree and the same first
-// two out-neighbours have (nearly) the same shortest-path DAG from the
-// second level on, so placing them in one 64-source batch lets one row
-// access serve many lanes. Key = degree (clamped) | first | second
-// neighbour; the stable sort keeps list order among equal keys.
+// Landmark distances for batching sources: search along reversed arcs from
+// each landmark (lane k of a 32-bit mask = landmark k), level by level up
+// to level 15. landmark_offer_kernel: every frontier node v (cur[v] != 0)
+// offers its lanes to its in-neighbours u not yet reached in them (a
+// wavefront per frontier node, 64 in-arcs at a time);
+// landmark_settle_kernel: new lanes become reached and form the next
+// frontier, and dist[u][k] = level for them.
+__global__ void landmark_offer_kernel(const uint32_t* __restrict__ in_off,
+                                      const uint32_t* __restrict__ in_idx,
+                                      const uint32_t* __restrict__ cur,
+                                      const uint32_t* __restrict__ vis,
+                                      uint32_t* __restrict__ nxt, size_t n) {
+    const int lane = lane_id();
+    for (size_t v0 = global_wave() * 64; v0 < n; v0 += wave_count() * 64) {
+        const size_t v = v0 + lane;
+        const uint32_t M = v < n ? cur[v] : 0u;
+        uint64_t act = __ballot(M != 0);
+        while (act) {
+            const int j = __ffsll((unsigned long long)act) - 1;
+            act &= act - 1;
+            const uint32_t vj = (uint32_t)(v0 + j);
+            const uint32_t Mj = rl32(M, j);
+            const uint32_t a1 = in_off[vj + 1];
+            for (uint32_t a = in_off[vj] + lane; a < a1; a += 64) {
+                const uint32_t u = in_idx[a];
+                const uint32_t nw = Mj & ~vis[u];
+                if (nw) atomicOr(&nxt[u], nw);
+            }
+        }
+    }
+}
+
+__global__ void landmark_settle_kernel(uint32_t* __restrict__ cur, uint32_t* __restrict__ vis,
+                                       uint32_t* __restrict__ nxt, uint8_t* __restrict__ dist,
+                                       int level, int lanes, size_t n) {
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    for (size_t u = (size_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += nth) {
+        const uint32_t N = nxt[u] & ~vis[u];
+        nxt[u] = 0;
+        cur[u] = N;
+        if (!N) continue;
+        vis[u] |= N;
+        for (uint32_t M = N; M; M &= M - 1) {
+            const int k = __ffs((int)M) - 1;
+            if (k < lanes) dist[u * kLandmarks + k] = (uint8_t)level;
+        }
+    }
+}
+
+// Sort key of the source at position i of the current order perm (which
+// 0: second out-neighbour, 1: first out-neighbour, 2: landmark distances,
+// landmark 0 in the top four bits). Sorting stably by the three in turn
+// batches sources whose searches look alike: same distances to the
+// landmarks, then same first neighbours.
 __global__ void source_key_kernel(const int64_t* __restrict__ list, int64_t k,
+                                  const uint32_t* __restrict__ perm,
                                   const uint32_t* __restrict__ off,
-                                  const uint32_t* __restrict__ idx, uint32_t n, int b,
-                                  int deg_bits, uint64_t* __restrict__ key,
-                                  uint32_t* __restrict__ val) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                  const uint32_t* __restrict__ idx,
+                                  const uint8_t* __restrict__ dist, uint32_t n, int which,
+                                  uint64_t* __restrict__ key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k) return;
-    uint32_t s = (uint32_t)(list ? list[i] : i);
-    uint32_t a0 = off[s], a1 = off[s + 1];
-    uint64_t f1 = a1 > a0 ? idx[a0] : n;
-    uint64_t f2 = a1 > a0 + 1 ? idx[a0 + 1] : n;
-    uint64_t deg = a1 - a0;
-    uint64_t dmax = deg_bits > 0 ? ((1ull << deg_bits) - 1) : 0;
-    if (deg > dmax) deg = dmax;
-    key[i] = (deg_bits > 0 ? deg << (2 * b) : 0) | (f1 << b) | f2;
-    val[i] = (uint32_t)i;
+    const uint32_t p = perm[i];
+    const uint32_t s = (uint32_t)(list ? list[p] : p);
+    const uint32_t a0 = off[s], a1 = off[s + 1];
+    uint64_t kv;
+    if (which == 0) {
+        kv = a1 > a0 + 1 ? idx[a0 + 1] : n;
+    } else if (which == 1) {
+        kv = a1 > a0 ? idx[a0] : n;
+    } else {
+        kv = 0;
+        const uint8_t* d = dist + (size_t)s * kLandmarks;
+        for (int j = 0; j < kLandmarks; j++) kv = (kv << 4) | (uint64_t)(d[j] & 15u);
+    }
+    key[i] = kv;
+}
+
+__global__ void iota_kernel(uint32_t* __restrict__ v, int64_t k) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) v[i] = (uint32_t)i;
 }
 
 __global__ void gather_sources_kernel(const int64_t* __restrict__ list, int64_t k,
@@ -2180,6 +2240,74 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     ws.ctrl.alloc(1);
 }
 
+// Orders the k sources of a free-order run (list d_list, nullptr: 0..k-1)
+// so that sources whose searches look alike share batches: three stable
+// sorts, by second out-neighbour, by first out-neighbour, and by the
+// distances to the graph's landmarks (capped at 15), the last one deciding
+// first. Sources in one batch then reach most nodes on the same level, so
+// the 64-lane rows carry more lanes per entry. Writes the list to ws.list2.
+void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hipStream_t st) {
+    const DeviceGraph& dg = g.dev;
+    const size_t n = g.n;
+    const int lanes = (int)dg.landmarks.size();
+    DevBuf<uint8_t> dist;
+    dist.alloc(n * kLandmarks);
+    NETBC_HIP(hipMemsetAsync(dist.p, 15, n * kLandmarks, st));
+    DevBuf<uint32_t> cur, vis, nxt;
+    cur.alloc(n);
+    vis.alloc(n);
+    nxt.alloc(n);
+    NETBC_HIP(hipMemsetAsync(cur.p, 0, n * 4, st));
+    NETBC_HIP(hipMemsetAsync(nxt.p, 0, n * 4, st));
+    if (lanes > 0) {
+        // level 0: the landmarks themselves
+        std::vector<uint32_t> seed(n, 0);
+        for (int j = 0; j < lanes; j++) seed[dg.landmarks[j]] |= 1u << j;
+        NETBC_HIP(hipMemcpyAsync(cur.p, seed.data(), n * 4, hipMemcpyHostToDevice, st));
+        NETBC_HIP(hipMemcpyAsync(vis.p, seed.data(), n * 4, hipMemcpyHostToDevice, st));
+        for (int j = 0; j < lanes; j++)
+            NETBC_HIP(hipMemsetAsync(dist.p + (size_t)dg.landmarks[j] * kLandmarks + j, 0, 1, st));
+        const unsigned offer_blocks = resident_blocks(landmark_offer_kernel, (n + 63) / 64);
+        const unsigned settle_blocks = blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22));
+        for (int level = 1; level < 15; level++) {
+            landmark_offer_kernel<<<offer_blocks, kBlock, 0, st>>>(dg.in_off_p(), dg.in_idx_p(),
+                                                                   cur.p, vis.p, nxt.p, n);
+            NETBC_HIP(hipGetLastError());
+            landmark_settle_kernel<<<settle_blocks, kBlock, 0, st>>>(cur.p, vis.p, nxt.p, dist.p,
+                                                                     level, lanes, n);
+            NETBC_HIP(hipGetLastError());
+        }
+        // the seed buffer must outlive the copies
+        NETBC_HIP(hipStreamSynchronize(st));
+    }
+    DevBuf<uint64_t> k0, k1;
+    DevBuf<uint32_t> v0, v1;
+    k0.alloc(k);
+    k1.alloc(k);
+    v0.alloc(k);
+    v1.alloc(k);
+    iota_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(v0.p, k);
+    NETBC_HIP(hipGetLastError());
+    uint64_t* key = k0.p;
+    uint64_t* key_tmp = k1.p;
+    uint32_t* perm = v0.p;
+    uint32_t* perm_tmp = v1.p;
+    const int b = node_bits(n + 1);
+    for (int which = 0; which < 3; which++) {
+        source_key_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(
+            d_list, k, perm, dg.out_off.p, dg.out_idx.p, dist.p, (uint32_t)n, which, key);
+        NETBC_HIP(hipGetLastError());
+        if (sort_pairs_u64(key, perm, key_tmp, perm_tmp, k, which < 2 ? b : 4 * kLandmarks, st)) {
+            std::swap(key, key_tmp);
+            std::swap(perm, perm_tmp);
+        }
+    }
+    ws.list2.alloc(k);
+    gather_sources_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(d_list, k, perm, ws.list2.p);
+    NETBC_HIP(hipGetLastError());
+    NETBC_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
@@ -2206,27 +2334,10 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     }
     const bool free_order = !opt.exact && k >= kReorderMin && L.kind != kReal;
     if (free_order) {
-        // Batch sources by neighbourhood (see source_key_kernel). Items are
+        // Batch sources whose searches look alike (order_sources). Items are
         // summed in the new order, which changes only the association of
         // the cross-source sum.
-        const DeviceGraph& dg = g.dev;
-        int b = node_bits(n + 1);
-        int deg_bits = std::max(0, std::min(64 - 2 * b, 24));
-        DevBuf<uint64_t> k0, k1;
-        DevBuf<uint32_t> v0, v1;
-        k0.alloc(k);
-        k1.alloc(k);
-        v0.alloc(k);
-        v1.alloc(k);
-        source_key_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(
-            d_list, k, dg.out_off.p, dg.out_idx.p, (uint32_t)n, b, deg_bits, k0.p, v0.p);
-        NETBC_HIP(hipGetLastError());
-        int which = sort_pairs_u64(k0.p, v0.p, k1.p, v1.p, k, 2 * b + deg_bits, st);
-        ws.list2.alloc(k);
-        gather_sources_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(
-            d_list, k, which ? v1.p : v0.p, ws.list2.p);
-        NETBC_HIP(hipGetLastError());
-        NETBC_HIP(hipStreamSynchronize(st));
+        order_sources(g, ws, d_list, k, st);
         d_list = ws.list2.p;
     }
     NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, sizeof(Ctrl), st));

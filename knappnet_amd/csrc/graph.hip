@@ -465,6 +465,18 @@ void build_device_graph(Graph& g) {
         }
     }
     d.max_chunks = std::max<uint32_t>(1, (maxdeg + kChunkArcs - 1) / kChunkArcs);
+    // Landmarks: largest in-degree first, ties by id.
+    {
+        std::vector<uint32_t> cand(g.n);
+        for (uint64_t v = 0; v < g.n; v++) cand[v] = (uint32_t)v;
+        const size_t want = std::min<size_t>(kLandmarks, g.n);
+        auto by_in = [&](uint32_t a, uint32_t b) {
+            uint32_t da = ioff[a + 1] - ioff[a], db = ioff[b + 1] - ioff[b];
+            return da != db ? da > db : a < b;
+        };
+        std::partial_sort(cand.begin(), cand.begin() + want, cand.end(), by_in);
+        d.landmarks.assign(cand.begin(), cand.begin() + want);
+    }
     d.heavy_flag.alloc(hflag.size());
     NETBC_HIP(hipMemcpy(d.heavy_flag.p, hflag.data(), hflag.size(), hipMemcpyHostToDevice));
     g.dev_ready = true;

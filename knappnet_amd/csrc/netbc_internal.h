@@ -26,6 +26,7 @@ constexpr uint32_t kHubIn = NETBC_HUB_IN;        // in-degree above which a node
 constexpr uint32_t kHubChunk = NETBC_HUB_CHUNK;  // in-arcs per hub pull task
 constexpr uint32_t kHeavyOut = 1024; // out-degree above which a row is split into tasks
 constexpr uint32_t kChunkArcs = 256; // out-arcs per heavy-row task
+constexpr int kLandmarks = 16;       // landmark searches for batching sources
 
 // An error that reaches the C caller as status 1 and its message.
 struct Error : std::runtime_error {
@@ -90,6 +91,9 @@ struct DeviceGraph {
     // of kChunkArcs arcs.
     DevBuf<uint8_t> heavy_flag;        // per node
     uint32_t heavy = 0, max_chunks = 1;
+    // Landmarks for batching sources: the nodes of largest in-degree
+    // (ties by id), at most kLandmarks of them.
+    std::vector<uint32_t> landmarks;
     const uint32_t* in_off_p() const { return directed ? in_off.p : out_off.p; }
     const uint32_t* in_idx_p() const { return directed ? in_idx.p : out_idx.p; }
     const double* in_w_p() const { return directed ? in_w.p : out_w.p; }
