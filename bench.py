@@ -86,10 +86,6 @@ def run_ws_batch(args) -> None:
     t1 = time.perf_counter() - t
     per_graph_s = t1 / sum(e.n * e.m for e in els[:k1]) * work
     same = all(np.array_equal(a, b) for a, b in zip(one, parts[:k1]))
-    tr = load_traffic(config["workload"], args.precision)
-    if tr:
-        roofline["traffic"] = tr.get("bytes_per_launch")
-        roofline["traffic_source"] = tr.get("source")
 
     cpu = None
     if args.cpu_seconds > 0:

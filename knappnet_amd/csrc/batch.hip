@@ -60,7 +60,8 @@ struct Ctrl {
     unsigned long long reached_arcs;
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
     // Work counters (filled only in builds with NETBC_COUNTERS):
-    // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes,
+    // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes
+    // (integer lengths: 0 settled pairs, 1 their in-arcs, 2 out-arcs, 3 offers),
     // 4 sweep entries, 5 sweep arcs, 6 sweep contributing arcs, 7 their lanes,
     // 8 bottom-up arcs, 9 their hits, 10 gathered arcs, 11 their hits,
     // 12 hub-task arcs, 13 their hits, 14 hub tasks with lanes sought,
@@ -1194,6 +1195,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         cnt = 0;
     };
     double smax = 0.0;
+    [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     const uint32_t T = *llen;
     for (size_t t = global_wave(); t < T; t += wave_count()) {
         const uint2 te = list[t];
@@ -1201,6 +1203,11 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         const size_t base = (size_t)te.y * n;
         const uint64_t M = __ballot(dist[(base + x) * kLanes + lane] == D);
         if (M == 0) continue;
+        if (lane == 0) {
+            NETBC_COUNT(c0, 1);
+            NETBC_COUNT(c1, in_off[x + 1] - in_off[x]);
+            NETBC_COUNT(c2, off[x + 1] - off[x]);
+        }
         if (!first_step) {
             double acc = 0.0;
             const uint32_t a0 = in_off[x], a1 = in_off[x + 1];
@@ -1239,6 +1246,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             const uint32_t y = ok ? idx[a] : 0;
             const uint32_t ln = ok ? len_arr[a] : 0;
             const uint64_t mm = ok ? (M & ~vis[base + y]) : 0;
+            NETBC_COUNT(c3, mm != 0);
             uint64_t bal = __ballot(mm != 0);
             while (bal) {
                 const int j = __ffsll((unsigned long long)bal) - 1;
@@ -1264,6 +1272,10 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
     }
     if (cnt) flush();
     note_sigma(ctrl, smax);
+    NETBC_FLUSH(ctrl, 0, c0);
+    NETBC_FLUSH(ctrl, 1, c1);
+    NETBC_FLUSH(ctrl, 2, c2);
+    NETBC_FLUSH(ctrl, 3, c3);
 }
 
 template <class C>
