@@ -65,7 +65,7 @@ typedef struct netbc_options {
     int32_t precision; /* 0: FP64 dependencies; 1: FP32 dependencies with
                           Kahan-compensated sums (sigma and the cross-source
                           sum stay FP64) */
-    int32_t slots;     /* 64-source batches per launch; 0 = automatic */
+    int32_t slots;     /* 64-source batches per launch; 0 = tuned default */
     int32_t direction; /* 0 automatic, 1 top-down only, 2 bottom-up when
                           the frontier allows */
     int32_t exact;     /* 1: the reference's summation order everywhere, so
@@ -73,12 +73,15 @@ typedef struct netbc_options {
                           rows longer than 1024 arcs are summed in 256-arc
                           chunks whose partial sums are added in chunk
                           order, and runs of 4096 or more sources batch
-                          sources with similar neighbourhoods together
+                          sources whose searches look alike together
                           (summing them in that order). Both are
                           deterministic and change only the association of
                           FP64 sums (scores within 1e-12 relative); levels
                           and path counts are bit-exact either way. */
-    double alpha;      /* direction switch factor; 0 = default */
+    double alpha;      /* direction switch factor; 0 = tuned default */
+    int32_t sweep_group; /* entries per dependency-sweep task at most (1 to
+                            64); 0 = tuned default */
+    int32_t reserved;  /* must be 0 */
 } netbc_options;
 
 /* What a run saw. */

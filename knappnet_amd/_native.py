@@ -25,7 +25,8 @@ _gen = None
 class Options(Structure):
     """netbc_options (include/netbc.h)."""
     _fields_ = [("precision", c_int32), ("slots", c_int32), ("direction", c_int32),
-                ("exact", c_int32), ("alpha", c_double)]
+                ("exact", c_int32), ("alpha", c_double), ("sweep_group", c_int32),
+                ("reserved", c_int32)]
 
 
 class Diagnostics(Structure):

@@ -26,6 +26,7 @@
 #include <unordered_map>
 
 #include "netbc_internal.h"
+#include "tuning.h"
 
 namespace netbc {
 namespace {
@@ -1872,14 +1873,39 @@ size_t slot_bytes_per_node(const Lengths& L, int precision) {
     return b;
 }
 
-// Slots per super-batch: breadth-first runs take 64 (R-MAT-18, all sources:
-// 8 slots 1.12 s, 16 0.99 s, 32 0.93 s, 64 0.90 s per 65,536 sources: wider
-// launches spread each level's fixed costs), integer lengths 16, real
-// lengths 4; all within a 96 GiB share of the device's memory.
+// The tuning table's entry for this graph and length class (tuning.h), or
+// nullptr (real lengths, or no class matches).
+const TuneEntry* tuning_for(const Graph& g, const Lengths& L) {
+    if (L.kind == kReal) return nullptr;
+    const int kind = L.kind == kUniform ? 0 : 1;
+    int lg = 0;
+    while (lg < 63 && (2ull << lg) <= g.n) lg++;
+    const double deg = g.n ? (double)g.dev.arcs / (double)g.n : 0.0;
+    for (const TuneEntry& e : kTuning) {
+        if (e.kind != kind) continue;
+        if (e.directed >= 0 && e.directed != (g.directed ? 1 : 0)) continue;
+        if (lg < e.log2n_min || lg > e.log2n_max) continue;
+        if (deg < e.deg_min || deg >= e.deg_max) continue;
+        return &e;
+    }
+    return nullptr;
+}
+
+// Options with the tuned defaults filled in where the caller left 0.
+RunOptions tuned_options(const Graph& g, const Lengths& L, const RunOptions& opt) {
+    RunOptions o = opt;
+    const TuneEntry* t = tuning_for(g, L);
+    if (o.slots <= 0) o.slots = t ? t->slots : (L.kind == kReal ? 4 : 16);
+    if (!(o.alpha > 0.0)) o.alpha = t ? t->alpha : kBottomUpAlpha;
+    if (o.sweep_group <= 0) o.sweep_group = t ? t->sweep_group : (int)kSweepGroup;
+    return o;
+}
+
+// Slots per super-batch: the tuned (or requested) count, capped by the
+// number of batches and by a 96 GiB share of the device's memory.
 int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& opt) {
     int64_t batches = (k + kLanes - 1) / kLanes;
-    int want = L.kind == kUniform ? 64 : (L.kind == kInteger ? 16 : 4);
-    if (opt.slots > 0) want = std::min(opt.slots, 1024);
+    int want = std::min(std::max(opt.slots, 1), 1024);
     size_t per = slot_bytes_per_node(L, opt.precision) * std::max<uint64_t>(g.n, 1);
     size_t budget = (size_t)96 << 30;
     int by_mem = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
@@ -2094,7 +2120,8 @@ struct Engine {
             uint32_t ae = rebuild && dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
             const uint32_t gsz =
-                group_size(qe - qb, resident_waves(bfs_backward_kernel<C>), kSweepGroup);
+                group_size(qe - qb, resident_waves(bfs_backward_kernel<C>),
+                           (uint32_t)std::min(std::max(opt.sweep_group, 1), 64));
             size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
             size_t waves = std::max<size_t>(tasks, rebuild ? (qe - qb + ae - ab + 63) / 64 : 0);
             if (heavy && ws.part.n < (size_t)hc[dl] * kLanes) {
@@ -2323,7 +2350,8 @@ void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hi
 }  // namespace
 
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt, double* d_bc, RunDiag& diag) {
+                 const RunOptions& opt_in, double* d_bc, RunDiag& diag) {
+    const RunOptions opt = tuned_options(g, L, opt_in);
     hipStream_t st = g.stream;
     size_t n = g.n;
     diag.path_kind = L.kind;
@@ -2458,7 +2486,7 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     if (n == 0 || k == 0) return;
     if (L.kind == kReal) throw Error("the state dump covers breadth-first and Dial searches");
     Workspace& ws = get_workspace(g);
-    RunOptions o = opt;
+    RunOptions o = tuned_options(g, L, opt);
     int slots = choose_slots(g, L, k, o);
     size_t qcap = std::max<size_t>(queue_capacity(slots, n), ws.qcap);
     ensure_workspace(ws, g, L, slots, o.precision, qcap);

@@ -94,6 +94,20 @@ void fill_diag(netbc_diagnostics* out, const netbc::RunDiag& d, double build_ms)
     for (int i = 0; i < 16; i++) out->work[i] = d.work[i];
 }
 
+// Rejects option values the engine does not define.
+void check_options(const netbc_options* opt) {
+    if (!opt) return;
+    if (opt->precision < 0 || opt->precision > 1)
+        throw netbc::Error("unknown precision code " + std::to_string(opt->precision));
+    if (opt->slots < 0) throw netbc::Error("negative slot count");
+    if (opt->direction < 0 || opt->direction > 2)
+        throw netbc::Error("unknown direction code " + std::to_string(opt->direction));
+    if (opt->sweep_group < 0 || opt->sweep_group > 64)
+        throw netbc::Error("sweep group " + std::to_string(opt->sweep_group) +
+                           " is outside 0..64");
+    if (!(opt->alpha >= 0.0)) throw netbc::Error("negative or NaN direction factor");
+}
+
 netbc::RunOptions options(const netbc_options* opt) {
     netbc::RunOptions o;
     if (opt) {
@@ -102,6 +116,7 @@ netbc::RunOptions options(const netbc_options* opt) {
         o.direction = opt->direction;
         o.alpha = opt->alpha;
         o.exact = opt->exact;
+        o.sweep_group = opt->sweep_group;
     }
     return o;
 }
@@ -254,8 +269,7 @@ NETBC_EXPORT int32_t netbc_betweenness_sources(const GraphHandle* h, const int64
         } else {
             k = (int64_t)g.n;
         }
-        if (opt && (opt->precision < 0 || opt->precision > 1))
-            throw netbc::Error("unknown precision code " + std::to_string(opt->precision));
+        check_options(opt);
         run_sources(g, sources, k, weighted, weights_as, normalized, opt, out, diag);
     });
 }
@@ -333,8 +347,7 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
     return guard(err, err_len, [&] {
         if (graphs < 0) throw netbc::Error("negative graph count");
         check_kind(weights_as);
-        if (opt && (opt->precision < 0 || opt->precision > 1))
-            throw netbc::Error("unknown precision code " + std::to_string(opt->precision));
+        check_options(opt);
         if (weighted && !weights) throw netbc::Error("the graph has no edge weights");
         // Validate every graph as net_graph_new would, and lay them out as
         // one union graph with consecutive node ranges.

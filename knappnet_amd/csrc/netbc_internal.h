@@ -134,6 +134,7 @@ struct RunOptions {
     double alpha = 0.0;     // bottom-up switch: frontier arcs > unvisited arcs / alpha (0 = default)
     int keep_sigma = 0;     // debug: keep per-source sigma and levels
     int exact = 0;          // 1: sum every row's successors in CSR order (no heavy-row split)
+    int sweep_group = 0;    // entries per sweep task at most (0 = tuned default)
 };
 
 struct RunDiag {

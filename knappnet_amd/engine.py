@@ -82,11 +82,12 @@ class Graph:
     def betweenness_sources(self, sources=None, weighted: bool = False,
                             normalized: bool = False, weights_as: str = "distance",
                             precision: int = 0, slots: int = 0, direction: int = 0,
-                            alpha: float = 0.0, exact: bool = False):
+                            alpha: float = 0.0, exact: bool = False, sweep_group: int = 0):
         """``netbc_betweenness_sources``: returns (scores, diagnostics dict)."""
         out = np.zeros(max(self.n, 1), dtype=np.float64)
         err = ctypes.create_string_buffer(ERR_LEN)
-        opt = _native.Options(precision, slots, direction, int(bool(exact)), alpha)
+        opt = _native.Options(precision, slots, direction, int(bool(exact)), alpha,
+                              sweep_group, 0)
         diag = _native.Diagnostics()
         kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
         if sources is None:

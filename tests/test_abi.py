@@ -104,3 +104,19 @@ def test_no_device_is_an_error_not_a_fallback():
     g = Graph(4, [0, 1, 2], [1, 2, 3], None, False)
     with pytest.raises(EngineError, match="no HIP device"):
         g.betweenness()
+
+
+def test_tuning_table_writer_compiles(tmp_path):
+    """The autotuner's table is valid C++ with the layout the library reads."""
+    import subprocess
+    from knappnet_amd.autotune import write_table
+    out = tmp_path / "tuning.h"
+    write_table(out, [((0, 1), "R-MAT", {"slots": 64, "alpha": 1.0, "sweep_group": 32}),
+                      ((1, -1), "BA", {"slots": 16, "alpha": 0.5, "sweep_group": 16})], "gfx950")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "tuning.h"\n'
+                   'static_assert(sizeof(netbc::kTuning) / sizeof(netbc::kTuning[0]) == 2, "");\n'
+                   'int main() { return netbc::kTuning[1].slots == 16 ? 0 : 1; }\n')
+    exe = tmp_path / "use"
+    subprocess.run(["g++", "-std=c++17", "-I", str(tmp_path), str(src), "-o", str(exe)], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
