@@ -440,6 +440,11 @@ typedef struct {
     Heap heap;
     /* optional per-source delta capture (debug entry) */
     double *delta_out;
+    /* edge betweenness (reach.rs:401-448): coef holds delta, eb is the
+     * lent per-edge accumulator, arc_edge maps CSR arcs to input edges */
+    int edge;
+    double *eb;
+    const uint32_t *arc_edge;
 } Workspace;
 
 static void ws_init(Workspace *ws, uint64_t n) {
@@ -479,6 +484,46 @@ static inline void credit(Workspace *ws, uint32_t v, uint64_t s, double delta) {
     if (v != s && ws->bc) ws->bc[v] += delta;
 }
 
+/* approx_eq with EPSILON = 1e-10, betweenness.rs:44, 117-120 (and
+ * reach.rs:85, 255-257). */
+static inline int approx_eq(double a, double b) {
+    double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+    return fabs(a - b) <= 1e-10 * m;
+}
+
+/* EdgeWorkspace::accumulate, reach.rs:401-448: farthest nodes first, every
+ * arc (v, w) on a shortest path receives c = sigma_v / sigma_w * (1 +
+ * delta_w), credited to its input edge; delta_v is the sum of its arcs' c
+ * in CSR order. kind: 0 hops, 1 integer distances, 2 real distances. */
+static void backward_edges(Workspace *ws, const OGraph *g, int kind, const uint32_t *ilens,
+                           const double *rlens) {
+    const OCsr *adj = &g->out;
+    for (uint64_t i = ws->norder; i-- > 0;) {
+        uint32_t v = ws->order[i];
+        uint32_t rank_v = ws->rank[v];
+        double dv = 0.0;
+        for (uint64_t k = adj->offsets[v]; k < adj->offsets[v + 1]; k++) {
+            uint32_t w = adj->targets[k];
+            int on_path;
+            if (kind == 0)
+                on_path = ws->hops[w] != U32_MAX_ && ws->hops[w] == ws->hops[v] + 1;
+            else if (kind == 1)
+                on_path = ws->rank[w] != U32_MAX_ && ws->rank[w] > rank_v &&
+                          ws->idist[v] + (uint64_t)ilens[k] == ws->idist[w];
+            else
+                on_path = ws->rank[w] != U32_MAX_ && ws->rank[w] > rank_v &&
+                          approx_eq(ws->dist[v] + rlens[k], ws->dist[w]);
+            if (on_path) {
+                double c = ws->sigma[v] / ws->sigma[w] * (1.0 + ws->coef[w]);
+                if (ws->eb) ws->eb[ws->arc_edge[k]] += c;
+                dv += c;
+            }
+        }
+        ws->coef[v] = dv;
+        if (ws->delta_out) ws->delta_out[v] = dv;
+    }
+}
+
 /* Workspace::single_source_bfs, betweenness.rs:161-210. */
 static void single_source_bfs(Workspace *ws, const OGraph *g, uint64_t s) {
     const OCsr *adj = &g->out;
@@ -498,6 +543,10 @@ static void single_source_bfs(Workspace *ws, const OGraph *g, uint64_t s) {
             }
             if (ws->hops[w] == next) ws->sigma[w] += ws->sigma[v];
         }
+    }
+    if (ws->edge) {
+        backward_edges(ws, g, 0, NULL, NULL);
+        return;
     }
     for (uint64_t i = ws->norder; i-- > 0;) {
         uint32_t v = ws->order[i];
@@ -566,6 +615,10 @@ static void single_source_dial(Workspace *ws, const OGraph *g, const uint32_t *l
             }
         }
     }
+    if (ws->edge) {
+        backward_edges(ws, g, 1, lens, NULL);
+        return;
+    }
     for (uint64_t i = ws->norder; i-- > 0;) {
         uint32_t v = ws->order[i];
         uint32_t rank_v = ws->rank[v];
@@ -591,12 +644,6 @@ static void reset_touched_int(Workspace *ws) {
         ws->coef[v] = 0.0;
         ws->rank[v] = U32_MAX_;
     }
-}
-
-/* approx_eq with EPSILON = 1e-10, betweenness.rs:44, 117-120. */
-static inline int approx_eq(double a, double b) {
-    double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
-    return fabs(a - b) <= 1e-10 * m;
 }
 
 /* Workspace::single_source_dijkstra, betweenness.rs:297-368. */
@@ -633,6 +680,12 @@ static void single_source_dijkstra(Workspace *ws, const OGraph *g, const double 
                 h_push_or_decrease(&ws->heap, w, alt);
             }
         }
+    }
+    if (ws->edge) {
+        /* reach.rs:323-370 pops a lazy binary heap ordered by (distance,
+         * node); the indexed heap settles in the same order */
+        backward_edges(ws, g, 2, NULL, lens);
+        return;
     }
     for (uint64_t i = ws->norder; i-- > 0;) {
         uint32_t v = ws->order[i];
@@ -735,11 +788,17 @@ typedef struct {
     double **pool;
     uint64_t npool;
     double *bc;
+    /* edge betweenness: accumulators of m doubles, arc -> edge map */
+    int edge;
+    const uint32_t *arc_edge;
 } SumCtx;
 
+static uint64_t acc_len(const SumCtx *c) { return c->edge ? c->g->m : c->g->n; }
+
 static void merge_group(SumCtx *c, double *acc) {
-    /* merge closure, betweenness.rs:91-96 */
-    for (uint64_t v = 0; v < c->g->n; v++) {
+    /* merge closures, betweenness.rs:91-96 and reach.rs:201-206 */
+    uint64_t len = acc_len(c);
+    for (uint64_t v = 0; v < len; v++) {
         c->bc[v] += acc[v];
         acc[v] = 0.0;
     }
@@ -748,12 +807,16 @@ static void merge_group(SumCtx *c, double *acc) {
 static void work_group(SumCtx *c, Workspace *ws, double *acc, uint64_t grp) {
     uint64_t lo = grp * c->size, hi = (grp + 1) * c->size;
     if (hi > c->items) hi = c->items;
-    ws->bc = acc;
+    ws->edge = c->edge;
+    ws->arc_edge = c->arc_edge;
+    if (c->edge) ws->eb = acc;
+    else ws->bc = acc;
     for (uint64_t i = lo; i < hi; i++) {
         uint64_t s = c->sources ? (uint64_t)c->sources[i] : i;
         run_source(ws, c->g, c->L, s);
     }
     ws->bc = NULL;
+    ws->eb = NULL;
 }
 
 static void *sum_worker(void *arg) {
@@ -767,7 +830,7 @@ static void *sum_worker(void *arg) {
         while (grp >= c->next + c->window) pthread_cond_wait(&c->turn, &c->mu);
         double *acc = c->npool ? c->pool[--c->npool] : NULL;
         pthread_mutex_unlock(&c->mu);
-        if (!acc) acc = calloc(c->g->n ? c->g->n : 1, sizeof(double));
+        if (!acc) acc = calloc(acc_len(c) ? acc_len(c) : 1, sizeof(double));
         work_group(c, &ws, acc, grp);
         pthread_mutex_lock(&c->mu);
         c->done[grp] = acc;
@@ -795,7 +858,7 @@ static void sum_in_order(SumCtx *c, uint64_t threads) {
     if (threads == 1) {
         Workspace ws;
         ws_init(&ws, c->g->n);
-        double *acc = calloc(c->g->n ? c->g->n : 1, sizeof(double));
+        double *acc = calloc(acc_len(c) ? acc_len(c) : 1, sizeof(double));
         for (uint64_t grp = 0; grp < c->groups; grp++) {
             work_group(c, &ws, acc, grp);
             merge_group(c, acc);
@@ -904,6 +967,77 @@ int32_t orc_betweenness_sources(const OGraph *g, const int64_t *sources, int64_t
                              search_work((uint64_t)k, g->n, g->out.arcs));
     sum_in_order(&c, t);
     scale_scores(g, normalized, out);
+    lengths_free(&L);
+    return ORC_OK;
+}
+
+/* arc_edges, reach.rs:234-253: the input edge behind every out-arc in CSR
+ * order. Arcs are generated edge by edge (for undirected non-loop edges the
+ * reverse right after) and stably sorted by (source, target), which is the
+ * order Csr::build leaves them in. */
+static int arc_cmp(const void *pa, const void *pb) {
+    const uint64_t *a = pa, *b = pb;
+    if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+    if (a[1] != b[1]) return a[1] < b[1] ? -1 : 1;
+    return a[2] < b[2] ? -1 : (a[2] > b[2]);
+}
+
+static uint32_t *arc_edges(const OGraph *g) {
+    uint64_t arcs = g->out.arcs, na = 0;
+    uint64_t *t = malloc((arcs ? arcs : 1) * 4 * sizeof *t); /* src, tgt, gen, edge */
+    for (uint64_t e = 0; e < g->m; e++) {
+        uint64_t *r = t + 4 * na;
+        r[0] = g->src[e]; r[1] = g->tgt[e]; r[2] = na; r[3] = e;
+        na++;
+        if (!g->directed && g->src[e] != g->tgt[e]) {
+            r = t + 4 * na;
+            r[0] = g->tgt[e]; r[1] = g->src[e]; r[2] = na; r[3] = e;
+            na++;
+        }
+    }
+    qsort(t, na, 4 * sizeof *t, arc_cmp);
+    uint32_t *map = malloc((na ? na : 1) * sizeof *map);
+    for (uint64_t i = 0; i < na; i++) map[i] = (uint32_t)t[4 * i + 3];
+    free(t);
+    return map;
+}
+
+/* net_edge_betweenness, crates/capi/src/roles.rs:77-100 -> edge_betweenness,
+ * reach.rs:174-232: per-edge sums over all sources in 64 ordered groups,
+ * then scale = (directed ? 1 : 0.5), divided by n(n - 1) as a double when
+ * normalized (times 2 undirected; NaN below 2 nodes). out: m doubles in
+ * input edge order. */
+int32_t orc_edge_betweenness(const OGraph *g, int32_t weighted, int32_t weights_as,
+                             int32_t normalized, int32_t threads, double *out, char *err,
+                             size_t err_len) {
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, 0, &L, err, err_len)) return ORC_FAILED;
+    uint64_t n = g->n, m = g->m;
+    memset(out, 0, m * sizeof *out);
+    uint32_t *map = arc_edges(g);
+    SumCtx c;
+    memset(&c, 0, sizeof c);
+    c.g = g;
+    c.L = &L;
+    c.items = n;
+    c.bc = out;
+    c.edge = 1;
+    c.arc_edge = map;
+    uint64_t t = threads_for(threads > 0 ? (uint64_t)threads : 0, n,
+                             search_work(n, n, g->out.arcs));
+    sum_in_order(&c, t);
+    double scale = g->directed ? 1.0 : 0.5;
+    if (normalized) {
+        double pairs = (double)n * ((double)n - 1.0);
+        if (n < 2) scale = NAN;
+        else if (g->directed) scale = scale / pairs;
+        else scale = scale * 2.0 / pairs;
+    }
+    if (scale != 1.0)
+        for (uint64_t e = 0; e < m; e++) out[e] *= scale;
+    free(map);
     lengths_free(&L);
     return ORC_OK;
 }

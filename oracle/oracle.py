@@ -46,6 +46,9 @@ def lib() -> ctypes.CDLL:
         L.orc_betweenness_alg.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
                                           POINTER(c_double), c_char_p, c_size_t]
         L.orc_betweenness_alg.restype = c_int32
+        L.orc_edge_betweenness.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32,
+                                           POINTER(c_double), ctypes.c_char_p, ctypes.c_size_t]
+        L.orc_edge_betweenness.restype = c_int32
         L.orc_betweenness_sources.argtypes = [c_void_p, POINTER(c_int64), c_int64, c_int32,
                                               c_int32, c_int32, c_int32, POINTER(c_double),
                                               c_char_p, c_size_t]
@@ -89,6 +92,7 @@ class OracleGraph:
             raise OracleError(err.value.decode())
         self.directed = bool(directed)
         self.weighted = w is not None
+        self.m = len(s)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -126,6 +130,16 @@ class OracleGraph:
                                       ERR_LEN):
             raise OracleError(err.value.decode())
         return out[: self.n]
+
+    def edge_betweenness(self, weighted=False, normalized=False, threads=1,
+                         weights_as="distance") -> np.ndarray:
+        """reach.rs:174-232: one value per input edge."""
+        out = np.zeros(max(self.m, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        if self.L.orc_edge_betweenness(self.h, int(weighted), KIND[weights_as], int(normalized),
+                                       int(threads), _p(out, c_double), err, ERR_LEN):
+            raise OracleError(err.value.decode())
+        return out[: self.m]
 
     def betweenness_sources(self, sources, weighted=False, normalized=False, threads=1,
                             weights_as="distance") -> np.ndarray:

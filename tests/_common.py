@@ -3,10 +3,15 @@ import numpy as np
 
 # validation/compare.py:313-314: |a - b| <= abs + rel * max(|a|, |b|).
 def close(a, b, rel=1e-9, abs_=1e-9):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    """NaN matches NaN (a normalized score of a one-node graph is NaN)."""
+    a = np.asarray([np.nan if x is None else x for x in np.ravel(a)], dtype=np.float64)
+    b = np.asarray([np.nan if x is None else x for x in np.ravel(b)], dtype=np.float64)
     if a.shape != b.shape:
         return False
+    nan = np.isnan(a) | np.isnan(b)
+    if not np.array_equal(np.isnan(a), np.isnan(b)):
+        return False
+    a, b = a[~nan], b[~nan]
     return bool(np.all(np.abs(a - b) <= abs_ + rel * np.maximum(np.abs(a), np.abs(b))))
 
 
@@ -33,17 +38,34 @@ VARIANTS = {
 }
 
 
+EDGE_VARIANTS = {
+    "edge_betweenness": dict(weighted=False, normalized=False),
+    "edge_betweenness_normalized": dict(weighted=False, normalized=True),
+    "edge_betweenness_weighted": dict(weighted=True, normalized=False),
+    "edge_betweenness_weighted_strength": dict(weighted=True, normalized=False,
+                                               weights_as="strength"),
+}
+
+
 def tolerance(variant):
-    # validation/manifest.d/core.yaml:86, 107, 127, 140
-    return (1e-9, 1e-12) if variant == "betweenness_normalized" else (1e-9, 1e-9)
+    # validation/manifest.d/core.yaml:86, 107, 127, 140 and
+    # validation/manifest.d/roles.yaml:131, 151, 166, 179
+    normalized = variant in ("betweenness_normalized", "edge_betweenness_normalized")
+    return (1e-9, 1e-12) if normalized else (1e-9, 1e-9)
 
 
-# NetworkX counts parallel edges as one path and compares reciprocal
-# lengths exactly (validation/manifest.d/core.yaml:90-96, 134-140).
+# NetworkX counts parallel edges as one path (and credits each parallel edge
+# with the whole amount), compares reciprocal lengths exactly and leaves a
+# one-node graph unscaled (validation/manifest.d/core.yaml:90-96, 134-140;
+# roles.yaml:139-145, 156-159, 181-189).
 def networkx_documented(fx, variant):
     tags = set(fx["tags"])
-    if "multi_edges" in tags and variant in ("betweenness", "betweenness_normalized"):
+    if "multi_edges" in tags and variant in ("betweenness", "betweenness_normalized",
+                                             "edge_betweenness",
+                                             "edge_betweenness_normalized"):
         return True
-    if variant == "betweenness_weighted_strength" and "integer_weights" in tags:
+    if variant.endswith("weighted_strength") and "integer_weights" in tags:
+        return True
+    if variant == "edge_betweenness_normalized" and fx["n"] < 2:
         return True
     return False

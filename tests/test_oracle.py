@@ -4,7 +4,8 @@ Checks, in order of strength:
 - the reference's committed goldens for all 28 fixtures (definition oracle
   and NetworkX, with the reference's documented NetworkX differences);
 - the reference's unit known answers (betweenness.rs:387-517,
-  graph.rs:354-412, weights.rs:290-321, parallel.rs:644-665);
+  reach.rs:731-826, graph.rs:354-412, weights.rs:290-321,
+  parallel.rs:644-665);
 - an exact-rational evaluation of the definition (tests/definition.py) on
   random multigraphs, weighted and unweighted;
 - closed forms and the distance-sum identity;
@@ -16,7 +17,8 @@ from scipy.sparse import csr_matrix
 from scipy.sparse.csgraph import shortest_path
 
 import definition
-from _common import VARIANTS, close, networkx_documented, random_case, tolerance
+from _common import (EDGE_VARIANTS, VARIANTS, close, networkx_documented, random_case,
+                     tolerance)
 from oracle.oracle import (OracleError, OracleGraph, available_cores, ordered_sum_bits,
                            threads_for)
 
@@ -49,6 +51,34 @@ def test_goldens_networkx(fixtures, variant):
         if variant not in fx["golden"] or "networkx" not in fx["golden"][variant]:
             continue
         got = og(fx).betweenness(**VARIANTS[variant])
+        ok = close(got, fx["golden"][variant]["networkx"], rel, ab)
+        if networkx_documented(fx, variant):
+            continue  # documented difference; the definition golden rules
+        assert ok, (name, variant)
+
+
+@pytest.mark.parametrize("variant", list(EDGE_VARIANTS))
+def test_edge_goldens_definition(fixtures, variant):
+    rel, ab = tolerance(variant)
+    checked = 0
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"]:
+            continue
+        g = og(fx)
+        for threads in (1, 4):
+            got = g.edge_betweenness(threads=threads, **EDGE_VARIANTS[variant])
+            assert close(got, fx["golden"][variant]["definition"], rel, ab), (name, variant)
+        checked += 1
+    assert checked >= 7
+
+
+@pytest.mark.parametrize("variant", list(EDGE_VARIANTS))
+def test_edge_goldens_networkx(fixtures, variant):
+    rel, ab = tolerance(variant)
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"] or "networkx" not in fx["golden"][variant]:
+            continue
+        got = og(fx).edge_betweenness(**EDGE_VARIANTS[variant])
         ok = close(got, fx["golden"][variant]["networkx"], rel, ab)
         if networkx_documented(fx, variant):
             continue  # documented difference; the definition golden rules
@@ -111,6 +141,50 @@ def test_strengths_are_inverted():
     g = OracleGraph(3, [0, 1, 0], [1, 2, 2], [1.0, 1.0, 4.0], False)
     assert list(g.betweenness(weighted=True, weights_as="strength")) == [0.0, 0.0, 0.0]
     assert list(g.betweenness(weighted=True)) == [0.0, 1.0, 0.0]
+
+
+# ---- edge betweenness known answers: reach.rs:731-826 ---------------------------
+
+def ebc(n, s, t, w=None, directed=False, **kw):
+    return OracleGraph(n, s, t, w, directed).edge_betweenness(**kw)
+
+
+def test_edge_betweenness_by_hand():
+    assert list(ebc(3, [0, 1], [1, 2])) == [2.0, 2.0]
+    assert list(ebc(3, [0, 1], [1, 2], directed=True)) == [2.0, 2.0]
+    # parallel edges split the paths; a loop carries nothing
+    assert list(ebc(3, [0, 0, 1, 2], [1, 1, 2, 2])) == [1.0, 1.0, 2.0, 0.0]
+    sq = ([0, 1, 2, 3], [1, 2, 3, 0])
+    assert list(ebc(4, *sq)) == [2.0] * 4
+    assert close(ebc(4, *sq, normalized=True), [2.0 / 6.0] * 4, 1e-15, 1e-15)
+    # normalisation is undefined below two nodes
+    assert np.isnan(ebc(1, [0], [0], normalized=True)).all()
+    assert ebc(0, [], []).size == 0
+
+
+def test_edge_betweenness_sums_to_total_path_length():
+    e = [(0, 1), (0, 2), (1, 2), (2, 3), (3, 4), (4, 5), (3, 5), (1, 5)]
+    s, t = zip(*e)
+    eb = ebc(6, s, t)
+    D = shortest_path(csr_matrix((np.ones(len(e)), (s, t)), shape=(6, 6)), directed=False,
+                      unweighted=True)
+    assert abs(eb.sum() - D[np.triu_indices(6, 1)].sum()) < 1e-12
+
+
+def test_weighted_edge_flow_conservation():
+    # Into every node v flows b(v) + (nodes reaching v): each path through
+    # v enters it once, and each path ending at v enters it once.
+    e = [(0, 1), (0, 2), (1, 3), (2, 3), (1, 4), (3, 5), (4, 5)]
+    w = [2.0, 1.0, 1.0, 2.0, 3.0, 2.0, 1.0]
+    s, t = zip(*e)
+    g = OracleGraph(6, s, t, w, True)
+    eb = g.edge_betweenness(weighted=True)
+    nb = g.betweenness(weighted=True)
+    D = shortest_path(csr_matrix((w, (s, t)), shape=(6, 6)), directed=True)
+    reached = (np.isfinite(D) & (D > 0)).sum(axis=0)
+    inflow = np.zeros(6)
+    np.add.at(inflow, np.asarray(t), eb)
+    assert close(inflow, nb + reached, 1e-12, 1e-12)
 
 
 # ---- graph build: graph.rs:354-412 -------------------------------------------
@@ -211,6 +285,23 @@ def test_definition_random(seed):
     assert close(g.betweenness(weighted=True), want, 1e-12, 1e-12)
     want = definition.betweenness(n, s, t, w, directed, weighted=True, strength=True)
     assert close(g.betweenness(weighted=True, weights_as="strength"), want, 1e-9, 1e-9)
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_edge_definition_random(seed):
+    rng = np.random.default_rng(2000 + seed)
+    n, s, t, directed = random_case(rng, max_n=12)
+    w = rng.integers(1, 5, size=len(s)).astype(np.float64)
+    g = OracleGraph(n, s, t, w, directed)
+    for normalized in (False, True):
+        want = definition.edge_betweenness(n, s, t, w, directed, normalized=normalized)
+        assert close(g.edge_betweenness(normalized=normalized), want, 1e-12, 1e-12)
+    want = definition.edge_betweenness(n, s, t, w, directed, weighted=True)
+    assert close(g.edge_betweenness(weighted=True), want, 1e-12, 1e-12)
+    want = definition.edge_betweenness(n, s, t, w, directed, weighted=True, strength=True)
+    assert close(g.edge_betweenness(weighted=True, weights_as="strength"), want, 1e-9, 1e-9)
+    # results never depend on the thread count
+    assert np.array_equal(g.edge_betweenness(threads=1), g.edge_betweenness(threads=3))
 
 
 # ---- closed forms and the distance-sum identity -------------------------------
