@@ -57,6 +57,19 @@ int32_t net_betweenness(const GraphHandle *h, int32_t weighted,
                         int32_t threads, double *out, char *err,
                         size_t err_len);
 
+/* Replaces net_edge_betweenness, netcapi.h:393-399 (crates/capi/src/
+ * roles.rs:77-100, crates/engine/src/metrics/reach.rs:174-232).
+ * Shortest-path betweenness of every edge, in input order: an arc u -> v
+ * on a shortest s-t path carries sigma_su * sigma_vt / sigma_st of the
+ * pair, parallel edges share it, loops carry nothing, undirected edges
+ * count each pair once; normalized != 0 divides by n(n-1) (directed) or
+ * n(n-1)/2 (NaN below two nodes). `out` holds m doubles. `threads` is
+ * accepted and ignored. */
+int32_t net_edge_betweenness(const GraphHandle *h, int32_t weighted,
+                             int32_t weights_as, int32_t normalized,
+                             int32_t threads, double *out, char *err,
+                             size_t err_len);
+
 /* ---- device-path entry points --------------------------------------- */
 
 /* Run options. Zero-initialised means: FP64 dependencies, automatic batch
@@ -115,6 +128,20 @@ int32_t netbc_betweenness_sources(const GraphHandle *h, const int64_t *sources,
                                   const netbc_options *opt, double *out,
                                   netbc_diagnostics *diag, char *err,
                                   size_t err_len);
+
+/* Edge betweenness restricted to a source list (NULL: every node,
+ * identical to net_edge_betweenness), scaled as reach.rs:207-224; `out`
+ * holds m doubles. Options and diagnostics as netbc_betweenness_sources
+ * (`exact` orders each node's dependency sum as the reference; the edge
+ * sums are always within 1e-12 of the reference's, not bit-identical). No
+ * counterpart in netcapi.h. */
+int32_t netbc_edge_betweenness_sources(const GraphHandle *h,
+                                       const int64_t *sources, int64_t k,
+                                       int32_t weighted, int32_t weights_as,
+                                       int32_t normalized,
+                                       const netbc_options *opt, double *out,
+                                       netbc_diagnostics *diag, char *err,
+                                       size_t err_len);
 
 /* Betweenness of many independent graphs in one call (null-model and
  * permutation replicates). Graph i has node_counts[i] nodes and

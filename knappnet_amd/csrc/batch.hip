@@ -148,10 +148,8 @@ __device__ __forceinline__ void note_sigma(Ctrl* ctrl, double s) {
 __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t item0, int64_t k,
                                     int slots, size_t n, uint64_t* __restrict__ vis,
                                     uint64_t* __restrict__ g0, double* __restrict__ sig,
-                                    QEnt* __restrict__ q, Ctrl* ctrl, uint32_t* __restrict__ dist,
-                                    const uint8_t* __restrict__ heavy_flag,
-                                    uint32_t* __restrict__ hq, uint64_t* __restrict__ pend,
-                                    int words, const uint32_t* __restrict__ out_off) {
+                                    uint32_t* __restrict__ dist, uint64_t* __restrict__ pend,
+                                    int words) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
     int b = t / kLanes, l = t % kLanes;
@@ -165,8 +163,28 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
     sig[mi * kLanes + l] = 1.0;
     if (dist) dist[mi * kLanes + l] = 0;
     if (pend) atomicOr((unsigned long long*)&pend[mi * words], 1ull);  // bucket of distance 0
+}
+
+// The level-0 queue entries: one per (slot, source node), holding every
+// lane of the slot that starts at that node (a source listed twice in one
+// slot gets one entry with two lanes, as every later level has), written
+// by the lowest such lane.
+__global__ void source_entries_kernel(const int64_t* __restrict__ list, int64_t item0, int64_t k,
+                                      int slots, size_t n, const uint64_t* __restrict__ vis,
+                                      QEnt* __restrict__ q, Ctrl* ctrl,
+                                      const uint8_t* __restrict__ heavy_flag,
+                                      uint32_t* __restrict__ hq,
+                                      const uint32_t* __restrict__ out_off) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots * kLanes) return;
+    int b = t / kLanes, l = t % kLanes;
+    int64_t i = item0 + t;
+    if (i >= k) return;
+    uint32_t s = (uint32_t)(list ? list[i] : i);
+    const uint64_t M = vis[(size_t)b * n + s];
+    if (__ffsll((unsigned long long)M) - 1 != l) return;
     uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
-    q[pos] = QEnt{s, (uint32_t)b, bit};
+    q[pos] = QEnt{s, (uint32_t)b, (unsigned long long)M};
     if (hq && heavy_flag[s]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
     if (out_off) atomicAdd(&ctrl->front_arcs[0], (unsigned long long)(out_off[s + 1] - out_off[s]));
 }
@@ -846,14 +864,58 @@ struct Dep<float> {
     }
 };
 
+// Edge credit of up to K arcs taken by one wave (reach.rs:287-309: arc
+// v -> w on a shortest path carries sigma_v / sigma_w * (1 + delta_w) per
+// source). c[h] holds lane l's credit of hit h (0 where the lane's source
+// does not use the arc), ak[h] the arc (kNone: no hit) and bk[h] its slot.
+// The 64 lanes of every hit are added by a fixed reduce-scatter: log2(K)
+// halving exchanges leave lane L with hit L >> (6 - log2 K) summed over the
+// lanes that differ from L in the top log2(K) bits, and a butterfly over
+// the low bits completes it (10 exchanges for 8 hits instead of 48). The
+// totals are added to eacc[slot * arcs + arc]: one sweep launch meets a
+// (slot, arc) pair at most once, so the update needs no atomic.
+template <int K>
+__device__ __forceinline__ void edge_flush(double (&c)[K], const uint32_t (&ak)[K],
+                                           const uint32_t (&bk)[K], double* __restrict__ eacc,
+                                           size_t arcs) {
+    static_assert(K >= 1 && K <= 32 && (K & (K - 1)) == 0, "hits per flush: a power of two");
+    constexpr int kLog = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
+    const int lane = lane_id();
+#pragma unroll
+    for (int s = 0; s < kLog; s++) {
+        const int d = 32 >> s;
+        const int half = K >> (s + 1);
+        const bool hi = (lane & d) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const double keep = hi ? c[i + half] : c[i];
+            const double send = hi ? c[i] : c[i + half];
+            c[i] = keep + __shfl_xor(send, d, 64);
+        }
+    }
+#pragma unroll
+    for (int o = 32 >> kLog; o > 0; o >>= 1) c[0] += __shfl_xor(c[0], o, 64);
+    const int h = lane >> (6 - kLog);
+    uint32_t a = kNone, b = 0;
+#pragma unroll
+    for (int i = 0; i < K; i++)
+        if (h == i) {
+            a = ak[i];
+            b = bk[i];
+        }
+    if ((lane & ((64 >> kLog) - 1)) == 0 && a != kNone) eacc[(size_t)b * arcs + a] += c[0];
+}
+
 // Adds the coefficients of the successors of `en` (R: the level d+1
 // bitmap) among arcs [a0, a1) to dv, per lane in CSR order; up to kAhead
-// rows are loaded before they are added.
-template <class C>
+// rows are loaded before they are added. With kEdge, each arc's credit
+// sv * coef (sv: the lane's path count of en) goes to eacc.
+template <class C, bool kEdge>
 __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t a1,
                                            const uint32_t* __restrict__ idx,
                                            const uint64_t* __restrict__ R,
                                            const C* __restrict__ coef, size_t n, Dep<C>& dv,
+                                           double sv, double* __restrict__ eacc, size_t arcs,
                                            unsigned long long& c6, unsigned long long& c7) {
     const int lane = lane_id();
     const size_t base = (size_t)en.b * n;
@@ -868,15 +930,19 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
         while (bal) {
             C val[kAhead];
             bool use[kAhead];
+            uint32_t ak[kAhead], bk[kAhead];
 #pragma unroll
             for (int u = 0; u < kAhead; u++) {
                 use[u] = false;
                 val[u] = 0;
+                ak[u] = kNone;
+                bk[u] = en.b;
                 if (bal) {
                     int j = __ffsll((unsigned long long)bal) - 1;
                     bal &= bal - 1;
                     uint32_t xx = rl32(x, j);
                     uint64_t m2 = rl64(mm, j);
+                    ak[u] = c + (uint32_t)j;
                     use[u] = (m2 >> lane) & 1;
                     if (use[u]) val[u] = coef[(base + xx) * kLanes + lane];
                 }
@@ -884,6 +950,12 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
 #pragma unroll
             for (int u = 0; u < kAhead; u++)
                 if (use[u]) dv.add(val[u]);
+            if constexpr (kEdge) {
+                double cv[kAhead];
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) cv[u] = use[u] ? sv * (double)val[u] : 0.0;
+                edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
+            }
         }
     }
 }
@@ -898,8 +970,9 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
 // sum is complete before the next row's starts); the other entries have no
 // dependency (delta = 0). Heavy rows (listed in hq) are split into chunk
 // tasks that each write a per-lane partial sum (part), added in chunk order
-// by bfs_combine_kernel; with split == 0 every row is summed whole.
-template <class C>
+// by bfs_combine_kernel; with split == 0 every row is summed whole. With
+// kEdge every arc's credit is also added to eacc (edge_flush).
+template <class C, bool kEdge>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level, int rebuild,
                                     const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
@@ -911,7 +984,8 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     const uint64_t* __restrict__ R, uint64_t* __restrict__ W,
                                     const uint64_t* __restrict__ succ,
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
-                                    uint32_t gsz, size_t n) {
+                                    uint32_t gsz, size_t n, double* __restrict__ eacc,
+                                    size_t arcs) {
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
@@ -955,8 +1029,13 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             uint32_t a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
             uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
             Dep<C> dv;
-            if (succ[(size_t)en.b * n + en.v] & en.m)
-                sweep_arcs<C>(en, a0, a1, idx, R, coef, n, dv, c6, c7);
+            if (succ[(size_t)en.b * n + en.v] & en.m) {
+                // the row's path counts stay in place until bfs_combine_kernel
+                double sv = 0.0;
+                if (kEdge && ((en.m >> lane) & 1))
+                    sv = sig[((size_t)en.b * n + en.v) * kLanes + lane];
+                sweep_arcs<C, kEdge>(en, a0, a1, idx, R, coef, n, dv, sv, eacc, arcs, c6, c7);
+            }
             part[t * kLanes + lane] = dv.value();
             continue;
         }
@@ -1000,10 +1079,11 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             const uint32_t ac = (uint32_t)__shfl((int)a0, c, 64);
             const uint32_t pc = (uint32_t)__shfl((int)pre, c, 64);
             const uint64_t mc = __shfl(en.m, c, 64);
-            uint32_t x = 0;
+            uint32_t x = 0, arc = 0;
             uint64_t mm = 0;
             if (t < T) {
-                x = idx[ac + (t - pc)];
+                arc = ac + (t - pc);
+                x = idx[arc];
                 mm = R[(size_t)bc * n + x] & mc;
             }
             NETBC_COUNT(c6, mm != 0);
@@ -1013,30 +1093,41 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                 C val[kAhead];
                 bool use[kAhead];
                 int hc[kAhead];
+                uint32_t ak[kAhead], bk[kAhead];
 #pragma unroll
                 for (int k = 0; k < kAhead; k++) {
                     use[k] = false;
                     val[k] = 0;
                     hc[k] = -1;
+                    ak[k] = kNone;
+                    bk[k] = 0;
                     if (bal) {
                         const int j = __ffsll((unsigned long long)bal) - 1;
                         bal &= bal - 1;
                         const uint32_t xx = rl32(x, j), bb = rl32(bc, j);
                         const uint64_t m2 = rl64(mm, j);
                         hc[k] = (int)rl32((uint32_t)c, j);
+                        ak[k] = rl32(arc, j);
+                        bk[k] = bb;
                         use[k] = (m2 >> lane) & 1;
                         if (use[k]) val[k] = coef[((size_t)bb * n + xx) * kLanes + lane];
                     }
                 }
+                [[maybe_unused]] double cv[kAhead];
 #pragma unroll
                 for (int k = 0; k < kAhead; k++) {
-                    if (hc[k] < 0) break;
+                    cv[k] = 0.0;
+                    if (hc[k] < 0) continue;
                     if (hc[k] != cur) {
                         close_entry();
                         open_entry(hc[k]);
                     }
-                    if (use[k]) dv.add(val[k]);
+                    if (use[k]) {
+                        dv.add(val[k]);
+                        if constexpr (kEdge) cv[k] = sv * (double)val[k];
+                    }
                 }
+                if constexpr (kEdge) edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
             }
         }
         close_entry();
@@ -1279,19 +1370,27 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
     NETBC_FLUSH(ctrl, 3, c3);
 }
 
-template <class C>
+// Dependencies of the entries settled at distance D: a wavefront per entry
+// walks its row 64 arcs at a time; for every arc whose head some lane of
+// the entry reached, the lanes that reached it at exactly D + length add
+// its coefficient, in CSR order (up to kAhead rows loaded before they are
+// added). With kEdge every arc's credit also goes to eacc (edge_flush).
+template <class C, bool kEdge>
 __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                      uint32_t D, const uint32_t* __restrict__ off,
                                      const uint32_t* __restrict__ idx,
                                      const uint32_t* __restrict__ len_arr,
                                      const uint32_t* __restrict__ dist,
                                      const uint64_t* __restrict__ vis, double* __restrict__ sig,
-                                     C* __restrict__ coef, size_t n) {
+                                     C* __restrict__ coef, size_t n, double* __restrict__ eacc,
+                                     size_t arcs) {
     const int lane = lane_id();
     for (size_t e = qbeg + global_wave(); e < qend; e += wave_count()) {
         QEnt en = q[e];
         size_t base = (size_t)en.b * n;
         uint64_t M = en.m;
+        const bool mine = (M >> lane) & 1;
+        const double s = mine ? sig[(base + en.v) * kLanes + lane] : 0.0;
         Dep<C> dv;
         uint32_t a0 = off[en.v], a1 = off[en.v + 1];
         for (uint32_t c = a0; c < a1; c += 64) {
@@ -1302,20 +1401,44 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
             uint64_t mm = ok ? (vis[base + x] & M) : 0;
             uint64_t bal = __ballot(mm != 0);
             while (bal) {
-                int j = __ffsll((unsigned long long)bal) - 1;
-                bal &= bal - 1;
-                uint32_t xx = rl32(x, j);
-                uint64_t want = (uint64_t)D + rl32(len, j);
-                uint64_t m2 = rl64(mm, j);
-                if ((m2 >> lane) & 1) {
-                    size_t r = (base + xx) * kLanes + lane;
-                    if ((uint64_t)dist[r] == want) dv.add(coef[r]);
+                C val[kAhead];
+                bool use[kAhead];
+                uint32_t ak[kAhead], bk[kAhead];
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) {
+                    use[u] = false;
+                    val[u] = 0;
+                    ak[u] = kNone;
+                    bk[u] = en.b;
+                    if (bal) {
+                        const int j = __ffsll((unsigned long long)bal) - 1;
+                        bal &= bal - 1;
+                        const uint32_t xx = rl32(x, j);
+                        const uint64_t want = (uint64_t)D + rl32(len, j);
+                        const uint64_t m2 = rl64(mm, j);
+                        ak[u] = c + (uint32_t)j;
+                        if ((m2 >> lane) & 1) {
+                            const size_t r = (base + xx) * kLanes + lane;
+                            if ((uint64_t)dist[r] == want) {
+                                use[u] = true;
+                                val[u] = coef[r];
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kAhead; u++)
+                    if (use[u]) dv.add(val[u]);
+                if constexpr (kEdge) {
+                    double cv[kAhead];
+#pragma unroll
+                    for (int u = 0; u < kAhead; u++) cv[u] = use[u] ? s * (double)val[u] : 0.0;
+                    edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
                 }
             }
         }
-        if ((M >> lane) & 1) {
+        if (mine) {
             size_t r = (base + en.v) * kLanes + lane;
-            double s = sig[r];
             double delta = s * dv.value();
             coef[r] = Dep<C>::coef(delta, s);
             sig[r] = delta;
@@ -1342,12 +1465,13 @@ __device__ __forceinline__ bool h_less(double da, uint32_t va, double db, uint32
 
 // Dijkstra with the reference's 4-ary indexed heap ordered by (distance,
 // node) (weights.rs:155-277) and ties within a relative 1e-10
-// (betweenness.rs:117-120, 297-368). Each thread owns private arrays.
+// (betweenness.rs:117-120, 297-368). Each thread owns private arrays. With
+// eacc, thread t adds every arc's credit to its own row eacc[t * arcs ..].
 __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0, int64_t k,
                                 int slots, size_t n, const uint32_t* __restrict__ off,
                                 const uint32_t* __restrict__ idx, const double* __restrict__ lens,
                                 DjkState st, uint64_t* __restrict__ vis, double* __restrict__ sig,
-                                Ctrl* ctrl) {
+                                Ctrl* ctrl, double* __restrict__ eacc, size_t arcs) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
     int64_t item = item0 + t;
@@ -1462,10 +1586,14 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
         uint32_t rv = rank[v];
         double dvv = dist[v];
         double dv = 0.0;
+        double* erow = eacc ? eacc + (size_t)t * arcs : nullptr;
         for (uint32_t a = off[v]; a < off[v + 1]; a++) {
             uint32_t w = idx[a];
-            if (rank[w] > rv && rank[w] != kNone && approx_eq(dvv + lens[a], dist[w]))
-                dv += coef[w];
+            if (rank[w] > rv && rank[w] != kNone && approx_eq(dvv + lens[a], dist[w])) {
+                const double cw = coef[w];
+                dv += cw;
+                if (erow) erow[a] += sigma[v] * cw;
+            }
         }
         double delta = sigma[v] * dv;
         coef[v] = (1.0 + delta) / sigma[v];
@@ -1717,13 +1845,29 @@ __global__ void accumulate_rows_kernel(const uint64_t* __restrict__ vis, double*
 
 // Final merge of the last group.
 __global__ void finish_kernel(double* __restrict__ bc, const double* __restrict__ acc, size_t n) {
-    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) bc[v] += acc[v];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        bc[v] += acc[v];
 }
 
 __global__ void scale_kernel(double* __restrict__ bc, size_t n, double scale) {
-    size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) bc[v] *= scale;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        bc[v] *= scale;
+}
+
+// Edge sums: every arc's credit summed over the accumulator rows in row
+// order, then added to its input edge. An edge has at most two arcs and
+// starts at zero, and two-term sums do not depend on their order, so the
+// atomic adds leave a deterministic result.
+__global__ void fold_arcs_kernel(const double* __restrict__ eacc, int rows, size_t arcs,
+                                 const uint32_t* __restrict__ arc_edge, double* __restrict__ eb) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < arcs; k += stride) {
+        double x = 0.0;
+        for (int r = 0; r < rows; r++) x += eacc[(size_t)r * arcs + k];
+        if (x != 0.0) atomicAdd(&eb[arc_edge[k]], x);
+    }
 }
 
 inline unsigned blocks_for_threads(size_t threads) {
@@ -1822,6 +1966,9 @@ struct Workspace {
     DevBuf<uint32_t> hq;    // positions of heavy-row entries in q
     DevBuf<uint32_t> hpre;  // chunk-task prefix over hq (per level)
     DevBuf<double> part;  // per-chunk partial sums of heavy rows
+    // Edge runs: per-arc credit, one row of arcs per slot (per thread for
+    // real lengths), summed over the rows at the end (fold_arcs_kernel).
+    DevBuf<double> eacc;
     size_t qcap = 0;
     DevBuf<uint2> touched;
     DevBuf<Ctrl> ctrl;
@@ -1870,6 +2017,7 @@ size_t slot_bytes_per_node(const Lengths& L, int precision) {
     size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
     if (L.kind == kUniform) b += kLevelBufs * 8;
     if (L.kind == kInteger) b += kLanes * 4 + 8 * ((L.max + 1 + 63) / 64) + 8;
+    if (L.kind == kReal) b += kLanes * (4 * 8 + 4 * 4);  // per-thread Dijkstra arrays
     return b;
 }
 
@@ -1902,11 +2050,14 @@ RunOptions tuned_options(const Graph& g, const Lengths& L, const RunOptions& opt
 }
 
 // Slots per super-batch: the tuned (or requested) count, capped by the
-// number of batches and by a 96 GiB share of the device's memory.
-int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& opt) {
+// number of batches and by a 96 GiB share of the device's memory (edge
+// runs add a row of per-arc credit per slot, per thread for real lengths).
+int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& opt,
+                 bool edge = false) {
     int64_t batches = (k + kLanes - 1) / kLanes;
     int want = std::min(std::max(opt.slots, 1), 1024);
     size_t per = slot_bytes_per_node(L, opt.precision) * std::max<uint64_t>(g.n, 1);
+    if (edge) per += (size_t)g.dev.arcs * sizeof(double) * (L.kind == kReal ? kLanes : 1);
     size_t budget = (size_t)96 << 30;
     int by_mem = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     int s = std::min<int64_t>({(int64_t)want, batches, (int64_t)by_mem});
@@ -1916,7 +2067,7 @@ int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& 
 }  // namespace
 
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt, double* d_bc, RunDiag& diag);
+                 const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb);
 
 namespace {
 
@@ -1934,6 +2085,7 @@ struct Engine {
     int slots;
     const int64_t* d_list;
     int64_t k;
+    bool edge = false;  // add every arc's credit to ws.eacc
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -1988,8 +2140,11 @@ struct Engine {
         const DeviceGraph& d = g.dev;
         reset_slots();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
-            d_list, item0, k, slots, n, ws.vis.p, level_bits(0), ws.sig.p, ws.q.p, ws.ctrl.p, nullptr,
-            d.heavy_flag.p, ws.hq.p, nullptr, 0, d.out_off.p);
+            d_list, item0, k, slots, n, ws.vis.p, level_bits(0), ws.sig.p, nullptr, nullptr, 0);
+        launch_check();
+        source_entries_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+            d_list, item0, k, slots, n, ws.vis.p, ws.q.p, ws.ctrl.p, d.heavy_flag.p, ws.hq.p,
+            d.out_off.p);
         launch_check();
         read_ctrl(ws, st);
         ws.sweep.collect();
@@ -2105,12 +2260,25 @@ struct Engine {
         return ws.levels.p + (size_t)b * slots * n;
     }
 
+    // The backward sweep of the super-batch just searched.
     template <class C>
+    void backward(C* coef) {
+        if (L.kind == kUniform) {
+            if (edge) backward_bfs<C, true>(coef);
+            else backward_bfs<C, false>(coef);
+        } else {
+            if (edge) backward_dial<C, true>(coef);
+            else backward_dial<C, false>(coef);
+        }
+    }
+
+    template <class C, bool kEdge>
     void backward_bfs(C* coef) {
         const DeviceGraph& d = g.dev;
         int D = (int)lv.size() - 2;  // last level with entries
         const int split = opt.exact ? 0 : 1;
-        for (int dl = D; dl >= 1; dl--) {
+        // edge runs also sweep the sources: their out-arcs carry credit
+        for (int dl = D; dl >= (kEdge ? 0 : 1); dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             uint32_t hb = hv[dl], he = hv[dl + 1];
             // Levels in alternating buffers are rebuilt for the next step:
@@ -2120,7 +2288,7 @@ struct Engine {
             uint32_t ae = rebuild && dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
             const uint32_t gsz =
-                group_size(qe - qb, resident_waves(bfs_backward_kernel<C>),
+                group_size(qe - qb, resident_waves(bfs_backward_kernel<C, kEdge>),
                            (uint32_t)std::min(std::max(opt.sweep_group, 1), 64));
             size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
             size_t waves = std::max<size_t>(tasks, rebuild ? (qe - qb + ae - ab + 63) / 64 : 0);
@@ -2129,11 +2297,12 @@ struct Engine {
                 ws.part.alloc((size_t)hc[dl] * kLanes * 2);
             }
             ws.sweep.mark(st);
-            bfs_backward_kernel<C><<<resident_blocks(bfs_backward_kernel<C>, waves), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl], split,
-                d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p, level_bits(dl + 1),
-                level_bits(dl),
-                ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz, n);
+            bfs_backward_kernel<C, kEdge>
+                <<<resident_blocks(bfs_backward_kernel<C, kEdge>, waves), kBlock, 0, st>>>(
+                    ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
+                    split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
+                    level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
+                    n, ws.eacc.p, d.arcs);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
@@ -2152,8 +2321,11 @@ struct Engine {
         const int W = (int)L.max + 1;
         const int words = (W + 63) / 64;
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
-            d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, ws.q.p, ws.ctrl.p,
-            ws.dist.p, d.heavy_flag.p, nullptr, ws.inb.p, words, nullptr);
+            d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, ws.dist.p, ws.inb.p, words);
+        launch_check();
+        source_entries_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+            d_list, item0, k, slots, n, ws.vis.p, ws.q.p, ws.ctrl.p, d.heavy_flag.p, nullptr,
+            nullptr);
         launch_check();
         read_ctrl(ws, st);
         ws.sweep.collect();
@@ -2190,16 +2362,17 @@ struct Engine {
         return true;
     }
 
-    template <class C>
+    template <class C, bool kEdge>
     void backward_dial(C* coef) {
         const DeviceGraph& d = g.dev;
         int D = (int)lv.size() - 2;
-        for (int dl = D; dl >= 1; dl--) {
+        for (int dl = D; dl >= (kEdge ? 0 : 1); dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             ws.sweep.mark(st);
-            dial_backward_kernel<C><<<resident_blocks(dial_backward_kernel<C>, qe - qb), kBlock, 0, st>>>(
-                ws.q.p, qb, qe, lv_dist[dl], d.out_off.p, d.out_idx.p, L.out_len.p, ws.dist.p,
-                ws.vis.p, ws.sig.p, coef, n);
+            dial_backward_kernel<C, kEdge>
+                <<<resident_blocks(dial_backward_kernel<C, kEdge>, qe - qb), kBlock, 0, st>>>(
+                    ws.q.p, qb, qe, lv_dist[dl], d.out_off.p, d.out_idx.p, L.out_len.p, ws.dist.p,
+                    ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs);
             launch_check();
             ws.sweep.mark(st);
         }
@@ -2211,7 +2384,7 @@ struct Engine {
         ws.sweep.mark(st);
         dijkstra_kernel<<<slots, kLanes, 0, st>>>(
             d_list, item0, k, slots, n, d.out_off.p, d.out_idx.p, L.out_real.p, dst, ws.vis.p,
-            ws.sig.p, ws.ctrl.p);
+            ws.sig.p, ws.ctrl.p, edge ? ws.eacc.p : nullptr, d.arcs);
         launch_check();
         ws.sweep.mark(st);
     }
@@ -2350,16 +2523,19 @@ void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hi
 }  // namespace
 
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt_in, double* d_bc, RunDiag& diag) {
+                 const RunOptions& opt_in, double* d_bc, RunDiag& diag, double* d_eb) {
     const RunOptions opt = tuned_options(g, L, opt_in);
     hipStream_t st = g.stream;
     size_t n = g.n;
+    const bool edge = d_eb != nullptr;
+    const size_t arcs = g.dev.arcs;
     diag.path_kind = L.kind;
     if (n == 0) return;
     NETBC_HIP(hipMemsetAsync(d_bc, 0, n * sizeof(double), st));
+    if (edge && g.m) NETBC_HIP(hipMemsetAsync(d_eb, 0, g.m * sizeof(double), st));
     if (k == 0) return;
     Workspace& ws = get_workspace(g);
-    int slots = choose_slots(g, L, k, opt);
+    int slots = choose_slots(g, L, k, opt, edge);
     diag.slots = slots;
     size_t qcap = std::max<size_t>(queue_capacity(slots, n), ws.qcap);
     ensure_workspace(ws, g, L, slots, opt.precision, qcap);
@@ -2397,7 +2573,13 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         djk = DjkState{djk_buf.p, djk_buf.p + T, djk_buf.p + 2 * T, djk_u32.p, djk_u32.p + T,
                        djk_buf.p + 3 * T, djk_u32.p + 2 * T, djk_u32.p + 3 * T};
     }
-    Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k};
+    int rows = 0;  // edge credit rows (Workspace::eacc)
+    if (edge) {
+        rows = L.kind == kReal ? slots * kLanes : slots;
+        ws.eacc.alloc((size_t)rows * arcs);
+        NETBC_HIP(hipMemsetAsync(ws.eacc.p, 0, (size_t)rows * arcs * sizeof(double), st));
+    }
+    Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k, edge};
     int64_t gsize = (k + 63) / 64;
     NETBC_HIP(hipEventRecord(ws.ev0, st));
     for (int64_t item0 = 0; item0 < k; item0 += (int64_t)slots * kLanes) {
@@ -2412,13 +2594,8 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                 eng.clear_rows();
                 grow_queue(ws);
             }
-            if (opt.precision) {
-                if (L.kind == kUniform) eng.backward_bfs<float>(ws.coef32.p);
-                else eng.backward_dial<float>(ws.coef32.p);
-            } else {
-                if (L.kind == kUniform) eng.backward_bfs<double>(ws.coef64.p);
-                else eng.backward_dial<double>(ws.coef64.p);
-            }
+            if (opt.precision) eng.backward(ws.coef32.p);
+            else eng.backward(ws.coef64.p);
         }
         if (free_order) {
             const uint32_t nsrc = eng.lv.size() > 1 ? eng.lv[1] : 0;
@@ -2438,6 +2615,11 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     }
     finish_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(d_bc, ws.acc.p, n);
     NETBC_HIP(hipGetLastError());
+    if (edge && arcs) {
+        fold_arcs_kernel<<<blocks_for_threads(std::min<size_t>(arcs, (size_t)1 << 22)), kBlock, 0,
+                           st>>>(ws.eacc.p, rows, arcs, g.dev.arc_edge.p, d_eb);
+        NETBC_HIP(hipGetLastError());
+    }
     NETBC_HIP(hipEventRecord(ws.ev1, st));
     read_ctrl(ws, st);
     ws.rows_dirty = false;
@@ -2473,6 +2655,24 @@ void finish_scores(Graph& g, bool normalized, double* d_bc, double* out) {
         NETBC_HIP(hipGetLastError());
     }
     NETBC_HIP(hipMemcpyAsync(out, d_bc, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipStreamSynchronize(st));
+}
+
+void finish_edge_scores(Graph& g, bool normalized, double* d_eb, double* out) {
+    hipStream_t st = g.stream;
+    const uint64_t n = g.n, m = g.m;
+    if (m == 0) return;
+    // reach.rs:207-224
+    double scale = g.directed ? 1.0 : 0.5;
+    if (normalized) {
+        const double pairs = (double)n * ((double)n - 1.0);
+        scale = n < 2 ? NAN : g.directed ? scale / pairs : scale * 2.0 / pairs;
+    }
+    if (scale != 1.0) {
+        scale_kernel<<<blocks_for_threads(m), kBlock, 0, st>>>(d_eb, m, scale);
+        NETBC_HIP(hipGetLastError());
+    }
+    NETBC_HIP(hipMemcpyAsync(out, d_eb, m * sizeof(double), hipMemcpyDeviceToHost, st));
     NETBC_HIP(hipStreamSynchronize(st));
 }
 
@@ -2544,13 +2744,8 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                     }
                 }
         }
-        if (o.precision) {
-            if (L.kind == kUniform) eng.backward_bfs<float>(ws.coef32.p);
-            else eng.backward_dial<float>(ws.coef32.p);
-        } else {
-            if (L.kind == kUniform) eng.backward_bfs<double>(ws.coef64.p);
-            else eng.backward_dial<double>(ws.coef64.p);
-        }
+        if (o.precision) eng.backward(ws.coef32.p);
+        else eng.backward(ws.coef64.p);
         NETBC_HIP(hipMemcpyAsync(rows.data(), ws.sig.p, rows.size() * 8, hipMemcpyDeviceToHost, st));
         eng.clear_rows();
         NETBC_HIP(hipStreamSynchronize(st));

@@ -138,9 +138,10 @@ void merge_diag(netbc::RunDiag& into, const netbc::RunDiag& d) {
 
 // Betweenness of one handle over a source list (sources == nullptr: item i
 // is node i), through the batched engine; returns the run's diagnostics.
+// With edge, out receives the m edge scores instead of the n node scores.
 netbc::RunDiag run_graph(netbc::Graph& g, const int64_t* sources, int64_t k, int32_t weighted,
                          int32_t weights_as, int32_t normalized, const netbc_options* opt,
-                         double* out, double* build_ms) {
+                         double* out, double* build_ms, bool edge = false) {
     std::lock_guard<std::mutex> lock(g.mu);
     if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
     double b = prepare(g);
@@ -148,20 +149,35 @@ netbc::RunDiag run_graph(netbc::Graph& g, const int64_t* sources, int64_t k, int
     netbc::Lengths L;
     netbc::classify_lengths(g, weighted != 0, weights_as, L);
     netbc::RunDiag d;
-    netbc::DevBuf<double> bc;
+    netbc::DevBuf<double> bc, eb;
     bc.alloc(g.n);
-    netbc::run_batched(g, L, sources, k, options(opt), bc.p, d);
-    netbc::finish_scores(g, normalized != 0, bc.p, out);
+    if (edge) eb.alloc(g.m);
+    netbc::run_batched(g, L, sources, k, options(opt), bc.p, d, edge ? eb.p : nullptr);
+    if (edge) netbc::finish_edge_scores(g, normalized != 0, eb.p, out);
+    else netbc::finish_scores(g, normalized != 0, bc.p, out);
     return d;
 }
 
 void run_sources(netbc::Graph& g, const int64_t* sources, int64_t k, int32_t weighted,
                  int32_t weights_as, int32_t normalized, const netbc_options* opt, double* out,
-                 netbc_diagnostics* diag) {
+                 netbc_diagnostics* diag, bool edge = false) {
     double build_ms = 0.0;
     netbc::RunDiag d = run_graph(g, sources, k, weighted, weights_as, normalized, opt, out,
-                                 &build_ms);
+                                 &build_ms, edge);
     fill_diag(diag, d, build_ms);
+}
+
+// Checks a caller's source list; returns the item count (all n nodes
+// when sources is null).
+int64_t check_sources(const netbc::Graph& g, const int64_t* sources, int64_t k) {
+    if (!sources) return (int64_t)g.n;
+    if (k < 0) throw netbc::Error("source count must be non-negative");
+    for (int64_t i = 0; i < k; i++)
+        if (sources[i] < 0 || (uint64_t)sources[i] >= g.n)
+            throw netbc::Error("source " + std::to_string(i) + " is node index " +
+                               std::to_string(sources[i]) + ", but valid indices are 0.." +
+                               std::to_string(g.n));
+    return k;
 }
 
 // The reference's scale for one graph (betweenness.rs:100-113).
@@ -259,18 +275,36 @@ NETBC_EXPORT int32_t netbc_betweenness_sources(const GraphHandle* h, const int64
     return guard(err, err_len, [&] {
         netbc::Graph& g = handle(h);
         check_kind(weights_as);
-        if (sources) {
-            if (k < 0) throw netbc::Error("source count must be non-negative");
-            for (int64_t i = 0; i < k; i++)
-                if (sources[i] < 0 || (uint64_t)sources[i] >= g.n)
-                    throw netbc::Error("source " + std::to_string(i) + " is node index " +
-                                       std::to_string(sources[i]) +
-                                       ", but valid indices are 0.." + std::to_string(g.n));
-        } else {
-            k = (int64_t)g.n;
-        }
+        k = check_sources(g, sources, k);
         check_options(opt);
         run_sources(g, sources, k, weighted, weights_as, normalized, opt, out, diag);
+    });
+}
+
+NETBC_EXPORT int32_t net_edge_betweenness(const GraphHandle* h, int32_t weighted,
+                                          int32_t weights_as, int32_t normalized, int32_t threads,
+                                          double* out, char* err, size_t err_len) {
+    (void)threads;
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        check_kind(weights_as);
+        run_sources(g, nullptr, (int64_t)g.n, weighted, weights_as, normalized, nullptr, out,
+                    nullptr, true);
+    });
+}
+
+NETBC_EXPORT int32_t netbc_edge_betweenness_sources(const GraphHandle* h, const int64_t* sources,
+                                                    int64_t k, int32_t weighted,
+                                                    int32_t weights_as, int32_t normalized,
+                                                    const netbc_options* opt, double* out,
+                                                    netbc_diagnostics* diag, char* err,
+                                                    size_t err_len) {
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        check_kind(weights_as);
+        k = check_sources(g, sources, k);
+        check_options(opt);
+        run_sources(g, sources, k, weighted, weights_as, normalized, opt, out, diag, true);
     });
 }
 

@@ -247,7 +247,7 @@ inline unsigned grid_for(size_t items, int block = kBlock) {
 void build_one(uint64_t n, uint64_t m, bool directed, bool reverse, const uint32_t* ds,
                const uint32_t* dt, const double* dw, const uint32_t* before, uint64_t arcs,
                DevBuf<uint32_t>& off, DevBuf<uint32_t>& idx, DevBuf<double>& wout,
-               hipStream_t st) {
+               DevBuf<uint32_t>* arc_edge, hipStream_t st) {
     int b = bits_for(n);
     DevBuf<uint64_t> k0, k1;
     DevBuf<uint32_t> v0, v1;
@@ -276,6 +276,13 @@ void build_one(uint64_t n, uint64_t m, bool directed, bool reverse, const uint32
         unpack_kernel<<<grid_for(arcs), kBlock, 0, st>>>(key, val, arcs, (1ull << b) - 1, dw,
                                                          idx.p, dw ? wout.p : nullptr);
         NETBC_HIP(hipGetLastError());
+    }
+    if (arc_edge) {
+        // the sorted values are the input edge of every arc (reach.rs:234-253)
+        arc_edge->alloc(arcs);
+        if (arcs)
+            NETBC_HIP(hipMemcpyAsync(arc_edge->p, val, arcs * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, st));
     }
     NETBC_HIP(hipStreamSynchronize(st));
 }
@@ -423,10 +430,10 @@ void build_device_graph(Graph& g) {
     }
     d.arcs = arcs;
     build_one(g.n, g.m, g.directed, false, ds.p, dt.p, g.has_weights ? dw.p : nullptr,
-              before.p, arcs, d.out_off, d.out_idx, d.out_w, st);
+              before.p, arcs, d.out_off, d.out_idx, d.out_w, &d.arc_edge, st);
     if (g.directed)
         build_one(g.n, g.m, true, true, ds.p, dt.p, g.has_weights ? dw.p : nullptr, nullptr,
-                  arcs, d.in_off, d.in_idx, d.in_w, st);
+                  arcs, d.in_off, d.in_idx, d.in_w, nullptr, st);
     NETBC_HIP(hipStreamSynchronize(st));
     // Hub tables from the in-degrees.
     std::vector<uint32_t> ioff(g.n + 1);

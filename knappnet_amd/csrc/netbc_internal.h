@@ -81,6 +81,7 @@ struct DeviceGraph {
     bool weighted = false;
     DevBuf<uint32_t> out_off, out_idx, in_off, in_idx;
     DevBuf<double> out_w, in_w;  // arc weights in CSR / CSC order
+    DevBuf<uint32_t> arc_edge;   // input edge of every out-CSR arc
     // Hubs: nodes with more than kHubIn in-arcs. Their path counts (and,
     // bottom-up, their new lanes) are gathered in chunks of kHubChunk
     // in-arcs, one task per chunk and slot.
@@ -162,9 +163,12 @@ void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L);
 
 // Runs the batched engine over item positions [0, k) with sources
 // `sources` (nullptr: item i is source i) and writes unscaled sums into
-// d_bc (device, n doubles) in the reference's group order.
+// d_bc (device, n doubles) in the reference's group order. With d_eb
+// (device, m doubles) it also writes the unscaled edge sums (reach.rs:
+// 174-232): every arc's credit summed over the sources, then the arcs of
+// each input edge added.
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt, double* d_bc, RunDiag& diag);
+                 const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb = nullptr);
 
 // Debug: per-source distances (hops or integer distances; UINT64_MAX when
 // unreached), sigma and delta for k sources, each n long, host buffers.
@@ -173,6 +177,9 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
 
 // Scales d_bc in place as betweenness.rs:100-113 and copies it to host out.
 void finish_scores(Graph& g, bool normalized, double* d_bc, double* out);
+
+// Scales d_eb in place as reach.rs:207-224 and copies it to host out.
+void finish_edge_scores(Graph& g, bool normalized, double* d_eb, double* out);
 
 // Ensures g has a stream on the current device.
 void ensure_stream(Graph& g);

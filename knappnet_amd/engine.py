@@ -79,12 +79,22 @@ class Graph:
             raise EngineError(err.value.decode())
         return out[: self.n]
 
-    def betweenness_sources(self, sources=None, weighted: bool = False,
-                            normalized: bool = False, weights_as: str = "distance",
-                            precision: int = 0, slots: int = 0, direction: int = 0,
-                            alpha: float = 0.0, exact: bool = False, sweep_group: int = 0):
-        """``netbc_betweenness_sources``: returns (scores, diagnostics dict)."""
-        out = np.zeros(max(self.n, 1), dtype=np.float64)
+    def edge_betweenness(self, weighted: bool = False, normalized: bool = False,
+                         threads: int = 0, weights_as: str = "distance") -> np.ndarray:
+        """``net_edge_betweenness``: scores of every edge, in input order."""
+        m = self.edge_count()
+        out = np.zeros(max(m, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if self.lib.net_edge_betweenness(self.handle, int(bool(weighted)), kind,
+                                         int(bool(normalized)), int(threads),
+                                         _ptr(out, c_double), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return out[:m]
+
+    def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, precision,
+                      slots, direction, alpha, exact, sweep_group):
+        out = np.zeros(max(length, 1), dtype=np.float64)
         err = ctypes.create_string_buffer(ERR_LEN)
         opt = _native.Options(precision, slots, direction, int(bool(exact)), alpha,
                               sweep_group, 0)
@@ -95,12 +105,28 @@ class Graph:
         else:
             srcs = _i64(sources)
             src_p, k = _ptr(srcs, c_int64), len(srcs)
-        if self.lib.netbc_betweenness_sources(self.handle, src_p, k, int(bool(weighted)), kind,
-                                              int(bool(normalized)), ctypes.byref(opt),
-                                              _ptr(out, c_double), ctypes.byref(diag), err,
-                                              ERR_LEN):
+        if fn(self.handle, src_p, k, int(bool(weighted)), kind, int(bool(normalized)),
+              ctypes.byref(opt), _ptr(out, c_double), ctypes.byref(diag), err, ERR_LEN):
             raise EngineError(err.value.decode())
-        return out[: self.n], diag.as_dict()
+        return out[:length], diag.as_dict()
+
+    def betweenness_sources(self, sources=None, weighted: bool = False,
+                            normalized: bool = False, weights_as: str = "distance",
+                            precision: int = 0, slots: int = 0, direction: int = 0,
+                            alpha: float = 0.0, exact: bool = False, sweep_group: int = 0):
+        """``netbc_betweenness_sources``: returns (scores, diagnostics dict)."""
+        return self._sources_call(self.lib.netbc_betweenness_sources, self.n, sources,
+                                  weighted, normalized, weights_as, precision, slots, direction,
+                                  alpha, exact, sweep_group)
+
+    def edge_betweenness_sources(self, sources=None, weighted: bool = False,
+                                 normalized: bool = False, weights_as: str = "distance",
+                                 precision: int = 0, slots: int = 0, direction: int = 0,
+                                 alpha: float = 0.0, exact: bool = False, sweep_group: int = 0):
+        """``netbc_edge_betweenness_sources``: returns (edge scores, diagnostics dict)."""
+        return self._sources_call(self.lib.netbc_edge_betweenness_sources, self.edge_count(),
+                                  sources, weighted, normalized, weights_as, precision, slots,
+                                  direction, alpha, exact, sweep_group)
 
     def state_dump(self, sources, weighted: bool = False, weights_as: str = "distance",
                    precision: int = 0):

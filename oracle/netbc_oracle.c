@@ -1006,27 +1006,37 @@ static uint32_t *arc_edges(const OGraph *g) {
  * reach.rs:174-232: per-edge sums over all sources in 64 ordered groups,
  * then scale = (directed ? 1 : 0.5), divided by n(n - 1) as a double when
  * normalized (times 2 undirected; NaN below 2 nodes). out: m doubles in
- * input edge order. */
-int32_t orc_edge_betweenness(const OGraph *g, int32_t weighted, int32_t weights_as,
-                             int32_t normalized, int32_t threads, double *out, char *err,
-                             size_t err_len) {
+ * input edge order. The _sources form runs over a list of k sources
+ * (sources == NULL: all n nodes), items being list positions, with the
+ * same scaling; it is the comparator for sampled-source runs. */
+int32_t orc_edge_betweenness_sources(const OGraph *g, const int64_t *sources, int64_t k,
+                                     int32_t weighted, int32_t weights_as, int32_t normalized,
+                                     int32_t threads, double *out, char *err, size_t err_len) {
     if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
     if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    for (int64_t i = 0; sources && i < k; i++) {
+        if (sources[i] < 0 || (uint64_t)sources[i] >= g->n) {
+            write_err(err, err_len, "source out of range");
+            return ORC_FAILED;
+        }
+    }
     Lengths L;
     if (path_lengths(g, weighted, weights_as, 0, &L, err, err_len)) return ORC_FAILED;
     uint64_t n = g->n, m = g->m;
+    uint64_t items = sources ? (uint64_t)k : n;
     memset(out, 0, m * sizeof *out);
     uint32_t *map = arc_edges(g);
     SumCtx c;
     memset(&c, 0, sizeof c);
     c.g = g;
     c.L = &L;
-    c.items = n;
+    c.items = items;
+    c.sources = sources;
     c.bc = out;
     c.edge = 1;
     c.arc_edge = map;
-    uint64_t t = threads_for(threads > 0 ? (uint64_t)threads : 0, n,
-                             search_work(n, n, g->out.arcs));
+    uint64_t t = threads_for(threads > 0 ? (uint64_t)threads : 0, items,
+                             search_work(items, n, g->out.arcs));
     sum_in_order(&c, t);
     double scale = g->directed ? 1.0 : 0.5;
     if (normalized) {
@@ -1040,6 +1050,13 @@ int32_t orc_edge_betweenness(const OGraph *g, int32_t weighted, int32_t weights_
     free(map);
     lengths_free(&L);
     return ORC_OK;
+}
+
+int32_t orc_edge_betweenness(const OGraph *g, int32_t weighted, int32_t weights_as,
+                             int32_t normalized, int32_t threads, double *out, char *err,
+                             size_t err_len) {
+    return orc_edge_betweenness_sources(g, NULL, 0, weighted, weights_as, normalized, threads,
+                                        out, err, err_len);
 }
 
 /* One search from s, exposing its state: hops (BFS) or integer distance

@@ -49,6 +49,10 @@ def lib() -> ctypes.CDLL:
         L.orc_edge_betweenness.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32,
                                            POINTER(c_double), ctypes.c_char_p, ctypes.c_size_t]
         L.orc_edge_betweenness.restype = c_int32
+        L.orc_edge_betweenness_sources.argtypes = [c_void_p, POINTER(c_int64), c_int64, c_int32,
+                                                   c_int32, c_int32, c_int32, POINTER(c_double),
+                                                   c_char_p, c_size_t]
+        L.orc_edge_betweenness_sources.restype = c_int32
         L.orc_betweenness_sources.argtypes = [c_void_p, POINTER(c_int64), c_int64, c_int32,
                                               c_int32, c_int32, c_int32, POINTER(c_double),
                                               c_char_p, c_size_t]
@@ -138,6 +142,17 @@ class OracleGraph:
         err = ctypes.create_string_buffer(ERR_LEN)
         if self.L.orc_edge_betweenness(self.h, int(weighted), KIND[weights_as], int(normalized),
                                        int(threads), _p(out, c_double), err, ERR_LEN):
+            raise OracleError(err.value.decode())
+        return out[: self.m]
+
+    def edge_betweenness_sources(self, sources, weighted=False, normalized=False, threads=1,
+                                 weights_as="distance") -> np.ndarray:
+        s = np.ascontiguousarray(np.asarray(sources, dtype=np.int64))
+        out = np.zeros(max(self.m, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        if self.L.orc_edge_betweenness_sources(self.h, _p(s, c_int64), len(s), int(weighted),
+                                               KIND[weights_as], int(normalized), int(threads),
+                                               _p(out, c_double), err, ERR_LEN):
             raise OracleError(err.value.decode())
         return out[: self.m]
 
