@@ -16,6 +16,11 @@ The printed JSON line carries:
 - ``cpu_baseline``: the CPU oracle (the reference's algorithm restated in
   C, all host cores) timed on a strided sample of the same sources, and a
   bit-exact check of the device result on that sample.
+
+``--edge`` times edge betweenness (``netbc_edge_betweenness_sources``)
+on the same workload instead; its roofline keeps the node sweep's byte
+model, a lower bound for the edge sweep (which also updates one 8-byte
+credit per slot and arc it meets).
 """
 from __future__ import annotations
 
@@ -33,6 +38,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = ("GTEPS for exact betweenness (sources×edges/s), FP32-δ vs FP64 at rel err "
           "≤1e-6")
+METRIC_EDGE = "GTEPS for exact edge betweenness (sources×edges/s)"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -176,6 +182,8 @@ def main() -> None:
                     help="time only the first K sources (experiments; not the headline)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target CPU time of the sampled baseline (0 = skip)")
+    ap.add_argument("--edge", action="store_true",
+                    help="edge betweenness (net_edge_betweenness) instead of node betweenness")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -200,7 +208,10 @@ def main() -> None:
         pool = np.arange(el.n) if sources is None else sources
         sources = np.asarray(pool[: args.max_sources])
         config = dict(config, workload=config["workload"] + f"_first{len(sources)}")
+    if args.edge:
+        config = dict(config, workload=config["workload"] + "_edges")
     g = Graph(el.n, el.src, el.tgt, el.weights, el.directed)
+    run = g.edge_betweenness_sources if args.edge else g.betweenness_sources
     k = el.n if sources is None else len(sources)
     print(f"# device: {device_info()}; workload {config['workload']} n={el.n} m={el.m} "
           f"sources={k} (generated in {gen_s:.1f}s)", flush=True)
@@ -209,14 +220,14 @@ def main() -> None:
               direction=args.direction, alpha=args.alpha)
     for i in range(args.warmup):
         t = time.perf_counter()
-        res, diag = g.betweenness_sources(**kw)
+        res, diag = run(**kw)
         print(f"# warmup {i}: {time.perf_counter() - t:.3f}s build_ms={diag['build_ms']:.1f}",
               flush=True)
 
     times, diags = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        res, diag = g.betweenness_sources(**kw)
+        res, diag = run(**kw)
         dt = time.perf_counter() - t
         times.append(dt)
         diags.append(diag)
@@ -236,8 +247,9 @@ def main() -> None:
     achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "bfs_backward_kernel<double>" if args.precision == 0 else
-                  "bfs_backward_kernel<float>",
+        "kernel": f"{'dial' if weighted else 'bfs'}_backward_kernel<"
+                  f"{'double' if args.precision == 0 else 'float'}, "
+                  f"{'true' if args.edge else 'false'}>",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -258,6 +270,7 @@ def main() -> None:
     if args.cpu_seconds > 0:
         from oracle.oracle import OracleGraph, available_cores
         o = OracleGraph(el.n, el.src, el.tgt, el.weights, el.directed)
+        orun = o.edge_betweenness_sources if args.edge else o.betweenness_sources
         pool = np.arange(el.n) if sources is None else np.asarray(sources)
         cores = available_cores()
         take = min(len(pool), 256)
@@ -265,16 +278,16 @@ def main() -> None:
             stride = max(1, len(pool) // take)
             sample = pool[::stride][:take]
             t = time.perf_counter()
-            want = o.betweenness_sources(sample, weighted=weighted, threads=0)
+            want = orun(sample, weighted=weighted, threads=0)
             ct = time.perf_counter() - t
             if ct >= args.cpu_seconds / 4 or take >= len(pool):
                 break
             take = min(len(pool), int(take * max(2.0, args.cpu_seconds / 2 / max(ct, 1e-3))))
         # best of two timed runs of the final sample (the host may be shared)
         t = time.perf_counter()
-        o.betweenness_sources(sample, weighted=weighted, threads=0)
+        orun(sample, weighted=weighted, threads=0)
         ct = min(ct, time.perf_counter() - t)
-        got, _ = g.betweenness_sources(sample, weighted=weighted, precision=args.precision)
+        got, _ = run(sample, weighted=weighted, precision=args.precision)
         exact = bool(np.array_equal(got, want))
         nz = (got != 0) | (want != 0)
         rel = float(np.max(np.abs(got - want)[nz] / np.maximum(np.abs(want[nz]), 1e-300))) \
@@ -289,7 +302,7 @@ def main() -> None:
               f"device vs oracle on sample: {cpu['check']}", flush=True)
 
     precision_check = None
-    if not args.no_compare and args.precision == 0:
+    if not args.no_compare and args.precision == 0 and not args.edge:
         # The batched FP32-dependency form against FP64 (error) and against
         # the plain FP64 form: sources in the reference's order and sums in
         # its order, bit-identical to the CPU engine.
@@ -318,7 +331,7 @@ def main() -> None:
         print(f"# precision check: {precision_check}", flush=True)
 
     line = {
-        "metric": METRIC,
+        "metric": METRIC_EDGE if args.edge else METRIC,
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,

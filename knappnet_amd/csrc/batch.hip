@@ -864,37 +864,83 @@ struct Dep<float> {
     }
 };
 
+// Cross-lane moves for edge_flush. v_permlane32_swap_b32 exchanges the
+// upper 32 lanes of its first operand with the lower 32 lanes of its
+// second, v_permlane16_swap_b32 the upper 16 with the lower 16 lanes of
+// each half-wave, so after the swap the two registers hold, in lane L, the
+// operand pair's values of lanes L and L + d (L below d within each 2d
+// group: first operand) or L - d and L (second operand). swap_add returns
+// their sum. DPP quad_perm and row_ror move values within rows of 16 lanes
+// (row_ror:k: lane L reads lane L - k mod 16).
+template <int d>
+__device__ __forceinline__ double swap_add(double a, double b) {
+    static_assert(d == 16 || d == 32, "half-wave or quarter-wave swap");
+    const unsigned alo = (unsigned)__double2loint(a), ahi = (unsigned)__double2hiint(a);
+    const unsigned blo = (unsigned)__double2loint(b), bhi = (unsigned)__double2hiint(b);
+    if constexpr (d == 32) {
+        const auto lo = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+        return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+    } else {
+        const auto lo = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+        return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+    }
+}
+
+constexpr int kDppXor1 = 0xB1;   // quad_perm [1,0,3,2]: lane L reads L ^ 1
+constexpr int kDppXor2 = 0x4E;   // quad_perm [2,3,0,1]: lane L reads L ^ 2
+constexpr int kDppRor8 = 0x128;  // row_ror:8: lane L reads L ^ 8
+constexpr int kDppRor12 = 0x12C; // row_ror:12: lane L reads L + 4 (mod 16)
+
+template <int kCtrl>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), kCtrl, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Edge credit of up to K arcs taken by one wave (reach.rs:287-309: arc
 // v -> w on a shortest path carries sigma_v / sigma_w * (1 + delta_w) per
 // source). c[h] holds lane l's credit of hit h (0 where the lane's source
 // does not use the arc), ak[h] the arc (kNone: no hit) and bk[h] its slot.
 // The 64 lanes of every hit are added by a fixed reduce-scatter: log2(K)
-// halving exchanges leave lane L with hit L >> (6 - log2 K) summed over the
-// lanes that differ from L in the top log2(K) bits, and a butterfly over
-// the low bits completes it (10 exchanges for 8 hits instead of 48). The
-// totals are added to eacc[slot * arcs + arc]: one sweep launch meets a
-// (slot, arc) pair at most once, so the update needs no atomic.
+// halving exchanges (lane bits 5, 4, 3) leave lane L with hit
+// L >> (6 - log2 K) summed over the lanes that differ from L in those bits,
+// and sums over the low bits complete it in the lanes whose low bits are
+// zero, which write it. The exchanges use register swaps and DPP moves,
+// not LDS. The totals are added to eacc[slot * arcs + arc]: one sweep
+// launch meets a (slot, arc) pair at most once, so the update needs no
+// atomic.
 template <int K>
 __device__ __forceinline__ void edge_flush(double (&c)[K], const uint32_t (&ak)[K],
                                            const uint32_t (&bk)[K], double* __restrict__ eacc,
                                            size_t arcs) {
-    static_assert(K >= 1 && K <= 32 && (K & (K - 1)) == 0, "hits per flush: a power of two");
-    constexpr int kLog = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : K == 16 ? 4 : 5;
+    static_assert(K == 1 || K == 2 || K == 4 || K == 8, "hits per flush: 1, 2, 4 or 8");
+    constexpr int kLog = K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : 3;
     const int lane = lane_id();
+    if constexpr (kLog >= 1) {
 #pragma unroll
-    for (int s = 0; s < kLog; s++) {
-        const int d = 32 >> s;
-        const int half = K >> (s + 1);
-        const bool hi = (lane & d) != 0;
-#pragma unroll
-        for (int i = 0; i < half; i++) {
-            const double keep = hi ? c[i + half] : c[i];
-            const double send = hi ? c[i] : c[i + half];
-            c[i] = keep + __shfl_xor(send, d, 64);
-        }
+        for (int i = 0; i < K / 2; i++) c[i] = swap_add<32>(c[i], c[i + K / 2]);
     }
+    if constexpr (kLog >= 2) {
 #pragma unroll
-    for (int o = 32 >> kLog; o > 0; o >>= 1) c[0] += __shfl_xor(c[0], o, 64);
+        for (int i = 0; i < K / 4; i++) c[i] = swap_add<16>(c[i], c[i + K / 4]);
+    }
+    if constexpr (kLog >= 3) {
+        const bool hi = (lane & 8) != 0;
+        const double keep = hi ? c[1] : c[0];
+        const double send = hi ? c[0] : c[1];
+        c[0] = keep + dpp_f64<kDppRor8>(send);
+    }
+    double x = c[0];
+    x += dpp_f64<kDppXor1>(x);                       // lane bit 0
+    x += dpp_f64<kDppXor2>(x);                       // bit 1
+    x += dpp_f64<kDppRor12>(x);                      // bit 2 (exact where bit 2 is 0)
+    if constexpr (kLog <= 2) x += dpp_f64<kDppRor8>(x);  // bit 3
+    if constexpr (kLog <= 1) x = swap_add<16>(x, x);    // bit 4
+    if constexpr (kLog == 0) x = swap_add<32>(x, x);    // bit 5
+    c[0] = x;
     const int h = lane >> (6 - kLog);
     uint32_t a = kNone, b = 0;
 #pragma unroll
