@@ -23,6 +23,7 @@
 #include <cstddef>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 
 #include "netbc_internal.h"
@@ -44,13 +45,16 @@ struct Ctrl {
     uint32_t tlen[2];
     uint32_t qoverflow;
     uint32_t hlen;       // heavy-row entries listed (positions in the level queue)
-    uint32_t left[2];    // integer lengths: tentative distances still pending
+    uint32_t left[2];    // bucketed searches: tentative distances still pending
     uint32_t hchunks;    // chunk tasks of the heavy rows listed last
     // Direction choice: out-arcs of the entries of a level (alternating by
     // level parity) and in-arcs of the (slot, node) pairs whose active lanes
     // are all reached.
     unsigned long long front_arcs[2];
     unsigned long long closed_arcs;
+    // Bucketed searches: the lowest bucket still pending after a step
+    // (alternating by step parity; all ones when none).
+    unsigned long long next_bucket[2];
     // Work tickets of dynamically scheduled kernels; the last wavefront to
     // finish resets both for the next launch.
     uint32_t ticket, exits;
@@ -114,6 +118,10 @@ __device__ __forceinline__ uint64_t rl64(uint64_t x, int j) {
     uint32_t lo = rl32((uint32_t)x, j), hi = rl32((uint32_t)(x >> 32), j);
     return (uint64_t)hi << 32 | lo;
 }
+__device__ __forceinline__ uint32_t rl(uint32_t x, int j) { return rl32(x, j); }
+__device__ __forceinline__ double rl(double x, int j) {
+    return __longlong_as_double((long long)rl64((uint64_t)__double_as_longlong(x), j));
+}
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
@@ -144,11 +152,13 @@ __device__ __forceinline__ void note_sigma(Ctrl* ctrl, double s) {
 
 // ---- initialisation -------------------------------------------------------
 
-// Marks the source of every active lane: visited, sigma = 1, level 0.
+// Marks the source of every active lane: visited, sigma = 1, level 0
+// (distance 0 in the bucketed searches).
+template <class Dist>
 __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t item0, int64_t k,
                                     int slots, size_t n, uint64_t* __restrict__ vis,
                                     uint64_t* __restrict__ g0, double* __restrict__ sig,
-                                    uint32_t* __restrict__ dist, uint64_t* __restrict__ pend,
+                                    Dist* __restrict__ dist, uint64_t* __restrict__ pend,
                                     int words) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
@@ -161,7 +171,7 @@ __global__ void init_sources_kernel(const int64_t* __restrict__ list, int64_t it
     atomicOr((unsigned long long*)&vis[mi], bit);
     if (g0) atomicOr((unsigned long long*)&g0[mi], bit);
     sig[mi * kLanes + l] = 1.0;
-    if (dist) dist[mi * kLanes + l] = 0;
+    if (dist) dist[mi * kLanes + l] = Dist(0);
     if (pend) atomicOr((unsigned long long*)&pend[mi * words], 1ull);  // bucket of distance 0
 }
 
@@ -1242,6 +1252,55 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
     note_sigma(ctrl, smax);
 }
 
+// ---- bucketed searches: integer lengths (Dial) and real lengths -------------
+
+// Distances and lengths of the bucketed searches. Integer lengths: a bucket
+// per distance (Dial, betweenness.rs:214-295), on the path iff equal. Real
+// lengths: buckets of width delta <= half the shortest arc, so an arc always
+// leads to a later bucket and every tentative distance in the lowest
+// pending bucket is final; a distance is the least offer it received, and
+// an arc is on a shortest path iff its offer ties that distance within a
+// relative 1e-10 (approx_eq, betweenness.rs:117-120, 297-368).
+struct IntPath {
+    using Dist = uint32_t;
+    using Len = uint32_t;
+    using Sum = uint64_t;  // distance + length without wrapping
+    static constexpr uint64_t kInfBits = kNone;
+    __device__ static Sum add(Dist d, Len l) { return (Sum)d + l; }
+    __device__ static bool reached(Dist d) { return d != kNone; }
+    __device__ static uint64_t bucket(Sum x, double) { return x; }
+    __device__ static bool tie(Sum via, Dist d) { return via == (Sum)d; }
+    __device__ static bool offer(Dist* p, Sum v) {
+        if (v < (Sum)*p) {
+            atomicMin(p, (uint32_t)v);
+            return true;
+        }
+        return false;
+    }
+};
+
+struct RealPath {
+    using Dist = double;
+    using Len = double;
+    using Sum = double;
+    static constexpr uint64_t kInfBits = 0x7FF0000000000000ull;  // +infinity
+    __device__ static Sum add(Dist d, Len l) { return d + l; }
+    __device__ static bool reached(Dist d) { return d < INFINITY; }
+    __device__ static uint64_t bucket(Sum x, double delta) { return (uint64_t)(x / delta); }
+    __device__ static bool tie(Sum via, Dist d) {
+        const double m = fabs(via) > fabs(d) ? fabs(via) : fabs(d);
+        return fabs(via - d) <= 1e-10 * m;
+    }
+    // Non-negative doubles order as their bit patterns.
+    __device__ static bool offer(Dist* p, Sum v) {
+        if (v < *p) {
+            atomicMin((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
+            return true;
+        }
+        return false;
+    }
+};
+
 // ---- integer lengths (Dial / delta-stepping with unit buckets) -------------
 
 // Integer lengths, forward step D (Dial's algorithm with exact integer
@@ -1251,16 +1310,23 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
 // distances lie in [D, D + W), so bucket D mod W holds exactly distance D.
 //
 // dial_scan_kernel lists the (slot, node) pairs with bucket D mod W set and
-// clears it, and records whether any bucket is still set anywhere.
-__global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int kb, size_t count,
-                                 uint2* __restrict__ list, uint32_t* __restrict__ llen,
-                                 uint32_t* __restrict__ llen_next, uint32_t* __restrict__ left,
-                                 uint32_t* __restrict__ left_next, size_t n) {
+// clears it, records whether any bucket is still set anywhere, and lowers
+// next_bucket to the first bucket after D still set at some pair (the
+// pending buckets of a pair lie in (D, D + W), so a residue r names bucket
+// D + ((r - D) mod W)); the step adds the buckets it offers to.
+__global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, uint32_t D,
+                                 size_t count, uint2* __restrict__ list,
+                                 uint32_t* __restrict__ llen, uint32_t* __restrict__ llen_next,
+                                 uint32_t* __restrict__ left, uint32_t* __restrict__ left_next,
+                                 unsigned long long* __restrict__ next_bucket,
+                                 unsigned long long* __restrict__ next_bucket_next, size_t n) {
+    const int kb = (int)(D % (uint32_t)W);
     __shared__ uint2 sbuf[kBlock / 64][kTouchBuf];
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *llen_next = 0;
         *left_next = 0;
+        *next_bucket_next = ~0ull;
     }
     uint2* mybuf = sbuf[threadIdx.x >> 6];
     uint32_t cnt = 0;
@@ -1273,19 +1339,35 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int kb,
     };
     const uint64_t lt = lanemask_lt(lane);
     const unsigned long long bit = 1ull << (kb & 63);
+    const int kw = kb >> 6;
+    // residues above kb within its word, and below it
+    const uint64_t above = (kb & 63) == 63 ? 0ull : (~0ull << ((kb & 63) + 1));
+    const uint64_t below = bit - 1;
     int any = 0;
+    uint32_t ahead = kNone;  // least (r - kb) mod W over pending residues r
     for (size_t w0 = global_wave() * 64; w0 < count; w0 += wave_count() * 64) {
         size_t i = w0 + lane;
         bool hit = false;
         if (i < count) {
             uint64_t* p = &pend[i * words];
-            uint64_t word = p[kb >> 6];
+            uint64_t word = p[kw];
             hit = (word & bit) != 0;
-            if (hit) p[kb >> 6] = word & ~bit;
-            uint64_t rest = hit ? (word & ~bit) : word;
-            for (int j = 0; j < words && !rest; j++)
-                if (j != (kb >> 6)) rest |= p[j];
-            any |= rest != 0;
+            if (hit) p[kw] = word & ~bit;
+            word &= ~bit;
+            uint32_t hi = kNone, lo = kNone;  // first residue above kb / at all below it
+            if (word & above) hi = (uint32_t)(kw * 64 + __ffsll((unsigned long long)(word & above)) - 1);
+            if (word & below) lo = (uint32_t)(kw * 64 + __ffsll((unsigned long long)(word & below)) - 1);
+            for (int j = 0; j < words; j++) {
+                if (j == kw) continue;
+                const uint64_t wj = p[j];
+                if (!wj) continue;
+                const uint32_t r = (uint32_t)(j * 64 + __ffsll((unsigned long long)wj) - 1);
+                if (j > kw) hi = min(hi, r);
+                else lo = min(lo, r);
+            }
+            if (hi != kNone) ahead = min(ahead, hi - (uint32_t)kb);
+            else if (lo != kNone) ahead = min(ahead, lo + (uint32_t)W - (uint32_t)kb);
+            any |= (hi != kNone || lo != kNone);
         }
         uint64_t bal = __ballot(hit);
         if (bal) {
@@ -1296,29 +1378,37 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int kb,
         }
     }
     if (cnt) flush();
+    for (int o = 32; o > 0; o >>= 1) ahead = min(ahead, (uint32_t)__shfl_xor((int)ahead, o, 64));
+    if (lane == 0 && ahead != kNone) atomicMin(next_bucket, (unsigned long long)D + ahead);
     if (__syncthreads_or(any) && threadIdx.x == 0) atomicOr(left, 1u);
 }
 
-// Settles and relaxes the listed pairs at distance D. The lanes of x whose
-// tentative distance is D are final (M). Their path counts are pulled over
-// in-arcs u->x of length len with dist[u] + len == D, and (x, b, M) joins
-// the level queue (both skipped for the sources at D = 0). Then every
-// out-arc x->y of length len offers D + len to the lanes of y not yet
-// settled, by atomic minimum, and marks y's bucket (D + len) mod W.
-// Settled distances (<= D) never change here, since offers are > D.
+// Settles and relaxes the listed pairs of bucket D (P: IntPath or
+// RealPath). The lanes of x whose tentative distance lies in bucket D are
+// final (M). Their path counts are pulled over in-arcs u->x of length len
+// from settled lanes whose offer dist[u] + len ties dist[x], and (x, b, M)
+// joins the level queue (both skipped for the sources at D = 0). Then every
+// out-arc x->y offers dist[x] + len to the lanes of y not yet settled, by
+// atomic minimum, and marks the offer's bucket mod W at y. Offers land in
+// later buckets, so settled distances never change here.
+template <class P>
 __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t* __restrict__ llen,
-                                 uint32_t D, int W, int words, int first_step,
+                                 uint32_t D, int W, int words, int first_step, double delta,
                                  uint64_t* __restrict__ pend,
                                  const uint32_t* __restrict__ in_off,
                                  const uint32_t* __restrict__ in_idx,
-                                 const uint32_t* __restrict__ in_len,
+                                 const typename P::Len* __restrict__ in_len,
                                  const uint32_t* __restrict__ off,
                                  const uint32_t* __restrict__ idx,
-                                 const uint32_t* __restrict__ len_arr,
-                                 uint32_t* __restrict__ dist, uint64_t* __restrict__ vis,
+                                 const typename P::Len* __restrict__ len_arr,
+                                 typename P::Dist* __restrict__ dist, uint64_t* __restrict__ vis,
                                  double* __restrict__ sig, QEnt* __restrict__ q, uint32_t qcap,
-                                 Ctrl* ctrl, size_t n) {
+                                 Ctrl* ctrl, unsigned long long* __restrict__ next_bucket,
+                                 size_t n) {
+    using Dist = typename P::Dist;
+    using Len = typename P::Len;
     __shared__ QEnt qbuf[kBlock / 64][kTouchBuf / 2];
+    unsigned long long least = ~0ull;  // least bucket offered by this lane
     const int lane = lane_id();
     QEnt* mybuf = qbuf[threadIdx.x >> 6];
     uint32_t cnt = 0;
@@ -1339,13 +1429,15 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         const uint2 te = list[t];
         const uint32_t x = te.x;
         const size_t base = (size_t)te.y * n;
-        const uint64_t M = __ballot(dist[(base + x) * kLanes + lane] == D);
+        const Dist dx = dist[(base + x) * kLanes + lane];
+        const uint64_t M = __ballot(P::reached(dx) && P::bucket(dx, delta) == D);
         if (M == 0) continue;
         if (lane == 0) {
             NETBC_COUNT(c0, 1);
             NETBC_COUNT(c1, in_off[x + 1] - in_off[x]);
             NETBC_COUNT(c2, off[x + 1] - off[x]);
         }
+        const bool mine = (M >> lane) & 1;
         if (!first_step) {
             double acc = 0.0;
             const uint32_t a0 = in_off[x], a1 = in_off[x + 1];
@@ -1353,22 +1445,22 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                 const uint32_t a = c + lane;
                 const bool ok = a < a1;
                 const uint32_t u = ok ? in_idx[a] : 0;
-                const uint32_t ln = ok ? in_len[a] : 0;
+                const Len ln = ok ? in_len[a] : Len(0);
                 const uint64_t mm = ok ? (vis[base + u] & M) : 0;
                 uint64_t bal = __ballot(mm != 0);
                 while (bal) {
                     const int j = __ffsll((unsigned long long)bal) - 1;
                     bal &= bal - 1;
                     const uint32_t uu = rl32(u, j);
-                    const uint32_t ll = rl32(ln, j);
+                    const Len ll = rl(ln, j);
                     const uint64_t m2 = rl64(mm, j);
                     if ((m2 >> lane) & 1) {
                         const size_t r = (base + uu) * kLanes + lane;
-                        if ((uint64_t)dist[r] + ll == D) acc += sig[r];
+                        if (P::tie(P::add(dist[r], ll), dx)) acc += sig[r];
                     }
                 }
             }
-            if ((M >> lane) & 1) {
+            if (mine) {
                 sig[(base + x) * kLanes + lane] = acc;
                 smax = fmax(smax, acc);
             }
@@ -1382,7 +1474,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             const uint32_t a = c + lane;
             const bool ok = a < a1;
             const uint32_t y = ok ? idx[a] : 0;
-            const uint32_t ln = ok ? len_arr[a] : 0;
+            const Len ln = ok ? len_arr[a] : Len(0);
             const uint64_t mm = ok ? (M & ~vis[base + y]) : 0;
             NETBC_COUNT(c3, mm != 0);
             uint64_t bal = __ballot(mm != 0);
@@ -1390,25 +1482,36 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                 const int j = __ffsll((unsigned long long)bal) - 1;
                 bal &= bal - 1;
                 const uint32_t yy = rl32(y, j);
-                const uint32_t nd = D + rl32(ln, j);
+                const Len ll = rl(ln, j);
                 const uint64_t m2 = rl64(mm, j);
-                bool offer = false;
+                bool offered = false;
+                uint32_t kk = 0;
                 if ((m2 >> lane) & 1) {
-                    uint32_t* p = &dist[(base + yy) * kLanes + lane];
-                    if (nd < *p) {
-                        atomicMin(p, nd);
-                        offer = true;
-                    }
+                    const typename P::Sum nd = P::add(dx, ll);
+                    offered = P::offer(&dist[(base + yy) * kLanes + lane], nd);
+                    const uint64_t bnd = P::bucket(nd, delta);
+                    kk = (uint32_t)(bnd % (uint64_t)W);
+                    if (offered) least = min(least, (unsigned long long)bnd);
                 }
-                if (__ballot(offer) && lane == 0) {
-                    const uint32_t kk = nd % (uint32_t)W;
-                    atomicOr((unsigned long long*)&pend[(base + yy) * words + (kk >> 6)],
-                             1ull << (kk & 63));
+                // mark the offered buckets (one atomic per distinct bucket)
+                uint64_t pending = __ballot(offered);
+                while (pending) {
+                    const int f = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t kf = rl32(kk, f);
+                    pending &= ~__ballot(offered && kk == kf);
+                    if (lane == f)
+                        atomicOr((unsigned long long*)&pend[(base + yy) * words + (kf >> 6)],
+                                 1ull << (kf & 63));
                 }
             }
         }
     }
     if (cnt) flush();
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = (unsigned long long)__shfl_xor((long long)least, o, 64);
+        least = min(least, other);
+    }
+    if (lane == 0 && least != ~0ull) atomicMin(next_bucket, least);
     note_sigma(ctrl, smax);
     NETBC_FLUSH(ctrl, 0, c0);
     NETBC_FLUSH(ctrl, 1, c1);
@@ -1416,20 +1519,23 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
     NETBC_FLUSH(ctrl, 3, c3);
 }
 
-// Dependencies of the entries settled at distance D: a wavefront per entry
-// walks its row 64 arcs at a time; for every arc whose head some lane of
-// the entry reached, the lanes that reached it at exactly D + length add
-// its coefficient, in CSR order (up to kAhead rows loaded before they are
-// added). With kEdge every arc's credit also goes to eacc (edge_flush).
-template <class C, bool kEdge>
+// Dependencies of the entries settled in one bucket (P: IntPath or
+// RealPath): a wavefront per entry walks its row 64 arcs at a time; for
+// every arc whose head some lane of the entry reached, the lanes whose
+// offer dist[v] + length ties the head's distance add its coefficient, in
+// CSR order (up to kAhead rows loaded before they are added). With kEdge
+// every arc's credit also goes to eacc (edge_flush).
+template <class C, bool kEdge, class P>
 __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
-                                     uint32_t D, const uint32_t* __restrict__ off,
+                                     const uint32_t* __restrict__ off,
                                      const uint32_t* __restrict__ idx,
-                                     const uint32_t* __restrict__ len_arr,
-                                     const uint32_t* __restrict__ dist,
+                                     const typename P::Len* __restrict__ len_arr,
+                                     const typename P::Dist* __restrict__ dist,
                                      const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                      C* __restrict__ coef, size_t n, double* __restrict__ eacc,
                                      size_t arcs) {
+    using Dist = typename P::Dist;
+    using Len = typename P::Len;
     const int lane = lane_id();
     for (size_t e = qbeg + global_wave(); e < qend; e += wave_count()) {
         QEnt en = q[e];
@@ -1437,13 +1543,14 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
         uint64_t M = en.m;
         const bool mine = (M >> lane) & 1;
         const double s = mine ? sig[(base + en.v) * kLanes + lane] : 0.0;
+        const Dist dv0 = mine ? dist[(base + en.v) * kLanes + lane] : Dist(0);
         Dep<C> dv;
         uint32_t a0 = off[en.v], a1 = off[en.v + 1];
         for (uint32_t c = a0; c < a1; c += 64) {
             uint32_t a = c + lane;
             bool ok = a < a1;
             uint32_t x = ok ? idx[a] : 0;
-            uint32_t len = ok ? len_arr[a] : 0;
+            Len len = ok ? len_arr[a] : Len(0);
             uint64_t mm = ok ? (vis[base + x] & M) : 0;
             uint64_t bal = __ballot(mm != 0);
             while (bal) {
@@ -1460,12 +1567,12 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                         const int j = __ffsll((unsigned long long)bal) - 1;
                         bal &= bal - 1;
                         const uint32_t xx = rl32(x, j);
-                        const uint64_t want = (uint64_t)D + rl32(len, j);
+                        const Len ll = rl(len, j);
                         const uint64_t m2 = rl64(mm, j);
                         ak[u] = c + (uint32_t)j;
                         if ((m2 >> lane) & 1) {
                             const size_t r = (base + xx) * kLanes + lane;
-                            if ((uint64_t)dist[r] == want) {
+                            if (P::tie(P::add(dv0, ll), dist[r])) {
                                 use[u] = true;
                                 val[u] = coef[r];
                             }
@@ -2006,7 +2113,8 @@ struct Workspace {
     DevBuf<uint64_t> inb;  // integer lengths: pending-bucket bitmasks per (slot, node)
     DevBuf<double> sig, coef64, acc;
     DevBuf<float> coef32;
-    DevBuf<uint32_t> dist;
+    DevBuf<uint32_t> dist;   // integer lengths: distance rows u32[n][64]
+    DevBuf<double> rdist;    // real lengths (bucketed): distance rows f64[n][64]
     DevBuf<int64_t> gcur, list, list2;
     DevBuf<QEnt> q;
     DevBuf<uint32_t> hq;    // positions of heavy-row entries in q
@@ -2055,15 +2163,50 @@ void read_ctrl(Workspace& ws, hipStream_t st) {
     NETBC_HIP(hipStreamSynchronize(st));
 }
 
+// Real lengths run as a bucketed search when buckets of half the shortest
+// arc keep the window (longest arc / width + 2 buckets) within this many
+// and every bucket index below 2^32; otherwise (very unequal lengths) as
+// the reference's Dijkstra, one source per thread.
+constexpr double kRealWindowMax = 4096.0;
+
+struct Buckets {
+    int W = 1;           // buckets in the window (tentative distances span < W)
+    int words = 1;       // 64-bit words of a pending mask
+    double delta = 1.0;  // bucket width
+};
+
+bool real_bucketed(const Lengths& L, uint64_t n) {
+    if (L.kind != kReal || !(L.rmin > 0.0)) return false;
+    const double delta = L.rmin / 2.0;
+    const double window = std::floor(L.rmax / delta) + 2.0;
+    const double last = (double)std::max<uint64_t>(n, 1) * L.rmax / delta + window;
+    return window <= kRealWindowMax && last < 4.0e9;
+}
+
+Buckets buckets_for(const Lengths& L) {
+    Buckets b;
+    if (L.kind == kInteger) {
+        b.W = (int)L.max + 1;
+    } else if (L.kind == kReal) {
+        b.delta = L.rmin / 2.0;
+        b.W = (int)(std::floor(L.rmax / b.delta) + 2.0);
+    }
+    b.words = (b.W + 63) / 64;
+    return b;
+}
+
 // Sizes (in bytes per node and slot) of the per-slot arrays.
 // bitmaps (visited, offers, successors, the level buffers), the touched
 // list, about four queue entries with their heavy-row slots, and the
 // per-lane rows.
-size_t slot_bytes_per_node(const Lengths& L, int precision) {
+size_t slot_bytes_per_node(const Lengths& L, int precision, uint64_t n) {
     size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
     if (L.kind == kUniform) b += kLevelBufs * 8;
-    if (L.kind == kInteger) b += kLanes * 4 + 8 * ((L.max + 1 + 63) / 64) + 8;
-    if (L.kind == kReal) b += kLanes * (4 * 8 + 4 * 4);  // per-thread Dijkstra arrays
+    if (L.kind == kInteger) b += kLanes * 4 + 8 * buckets_for(L).words + 8;
+    if (L.kind == kReal) {
+        if (real_bucketed(L, n)) b += kLanes * 8 + 8 * buckets_for(L).words + 8;
+        else b += kLanes * (4 * 8 + 4 * 4);  // per-thread Dijkstra arrays
+    }
     return b;
 }
 
@@ -2089,7 +2232,8 @@ const TuneEntry* tuning_for(const Graph& g, const Lengths& L) {
 RunOptions tuned_options(const Graph& g, const Lengths& L, const RunOptions& opt) {
     RunOptions o = opt;
     const TuneEntry* t = tuning_for(g, L);
-    if (o.slots <= 0) o.slots = t ? t->slots : (L.kind == kReal ? 4 : 16);
+    if (o.slots <= 0)
+        o.slots = t ? t->slots : (L.kind == kReal && !real_bucketed(L, g.n) ? 4 : 16);
     if (!(o.alpha > 0.0)) o.alpha = t ? t->alpha : kBottomUpAlpha;
     if (o.sweep_group <= 0) o.sweep_group = t ? t->sweep_group : (int)kSweepGroup;
     return o;
@@ -2102,8 +2246,9 @@ int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& 
                  bool edge = false) {
     int64_t batches = (k + kLanes - 1) / kLanes;
     int want = std::min(std::max(opt.slots, 1), 1024);
-    size_t per = slot_bytes_per_node(L, opt.precision) * std::max<uint64_t>(g.n, 1);
-    if (edge) per += (size_t)g.dev.arcs * sizeof(double) * (L.kind == kReal ? kLanes : 1);
+    size_t per = slot_bytes_per_node(L, opt.precision, g.n) * std::max<uint64_t>(g.n, 1);
+    const bool per_thread = L.kind == kReal && !real_bucketed(L, g.n);
+    if (edge) per += (size_t)g.dev.arcs * sizeof(double) * (per_thread ? kLanes : 1);
     size_t budget = (size_t)96 << 30;
     int by_mem = (int)std::max<size_t>(1, budget / std::max<size_t>(per, 1));
     int s = std::min<int64_t>({(int64_t)want, batches, (int64_t)by_mem});
@@ -2144,7 +2289,6 @@ struct Engine {
                                               g.dev.out_off.p, ws.hpre.p, &ws.ctrl.p->hchunks);
         launch_check();
     }
-    std::vector<uint32_t> lv_dist;  // Dial: distance of each level
 
     void launch_check() { NETBC_HIP(hipGetLastError()); }
 
@@ -2169,13 +2313,15 @@ struct Engine {
             ws.zero_words = 0;
             launch_check();
         }
-        if (L.kind == kInteger) {
-            fill_u32_kernel<<<blocks_for_threads(std::min<size_t>(nm * kLanes, (size_t)1 << 22)),
-                              kBlock, 0, st>>>(ws.dist.p, nm * kLanes, kNone);
-            int words = (int)((L.max + 1 + 63) / 64);
-            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.inb.p, nm * words, 0);
-            launch_check();
-        }
+        const unsigned rb = blocks_for_threads(std::min<size_t>(nm * kLanes, (size_t)1 << 22));
+        if (L.kind == kInteger)
+            fill_u32_kernel<<<rb, kBlock, 0, st>>>(ws.dist.p, nm * kLanes, kNone);
+        else if (L.kind == kReal && real_bucketed(L, n))
+            fill_u64_kernel<<<rb, kBlock, 0, st>>>((uint64_t*)ws.rdist.p, nm * kLanes,
+                                                   RealPath::kInfBits);
+        if (L.kind == kInteger || (L.kind == kReal && real_bucketed(L, n)))
+            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.inb.p, nm * buckets_for(L).words, 0);
+        launch_check();
         NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, offsetof(Ctrl, sig_overflow), st));
     }
 
@@ -2185,7 +2331,7 @@ struct Engine {
     bool run_bfs(int64_t item0) {
         const DeviceGraph& d = g.dev;
         reset_slots();
-        init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+        init_sources_kernel<uint32_t><<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, level_bits(0), ws.sig.p, nullptr, nullptr, 0);
         launch_check();
         source_entries_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
@@ -2312,10 +2458,30 @@ struct Engine {
         if (L.kind == kUniform) {
             if (edge) backward_bfs<C, true>(coef);
             else backward_bfs<C, false>(coef);
+        } else if (L.kind == kInteger) {
+            if (edge) backward_dial<C, true, IntPath>(coef);
+            else backward_dial<C, false, IntPath>(coef);
         } else {
-            if (edge) backward_dial<C, true>(coef);
-            else backward_dial<C, false>(coef);
+            if (edge) backward_dial<C, true, RealPath>(coef);
+            else backward_dial<C, false, RealPath>(coef);
         }
+    }
+
+    // Distance rows and lengths of a bucketed search.
+    template <class P>
+    typename P::Dist* dist_rows() {
+        if constexpr (std::is_same_v<P, IntPath>) return ws.dist.p;
+        else return ws.rdist.p;
+    }
+    template <class P>
+    const typename P::Len* out_lengths() {
+        if constexpr (std::is_same_v<P, IntPath>) return L.out_len.p;
+        else return L.out_real.p;
+    }
+    template <class P>
+    const typename P::Len* in_lengths() {
+        if constexpr (std::is_same_v<P, IntPath>) return g.dev.directed ? L.in_len.p : L.out_len.p;
+        else return g.dev.directed ? L.in_real.p : L.out_real.p;
     }
 
     template <class C, bool kEdge>
@@ -2361,13 +2527,19 @@ struct Engine {
         }
     }
 
-    bool run_dial(int64_t item0) {
+    // Bucketed search (Dial for integer lengths, P = IntPath; buckets of
+    // half the shortest arc for real lengths, P = RealPath): one scan and
+    // one step per bucket, one host sync per bucket.
+    template <class P>
+    bool run_bucketed(int64_t item0) {
         const DeviceGraph& d = g.dev;
         reset_slots();
-        const int W = (int)L.max + 1;
-        const int words = (W + 63) / 64;
+        const Buckets bk = buckets_for(L);
+        const int W = bk.W;
+        const int words = bk.words;
+        typename P::Dist* dist = dist_rows<P>();
         init_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
-            d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, ws.dist.p, ws.inb.p, words);
+            d_list, item0, k, slots, n, ws.vis.p, nullptr, ws.sig.p, dist, ws.inb.p, words);
         launch_check();
         source_entries_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, ws.q.p, ws.ctrl.p, d.heavy_flag.p, nullptr,
@@ -2377,47 +2549,54 @@ struct Engine {
         ws.sweep.collect();
         lv.assign(1, 0);
         lv.push_back(ws.host_ctrl->qlen);
-        lv_dist.assign(1, 0);
         const size_t pairs = (size_t)slots * n;
         const unsigned scan_blocks = resident_blocks(dial_scan_kernel, (pairs + 63) / 64);
-        const unsigned step_blocks = resident_blocks(dial_step_kernel, pairs);
-        const uint32_t* in_len = d.directed ? L.in_len.p : L.out_len.p;
-        for (uint32_t D = 0;; D++) {
-            const int p = D & 1;
+        const unsigned step_blocks = resident_blocks(dial_step_kernel<P>, pairs);
+        // the first step's bucket minimum starts empty; later ones are reset
+        // by the scan before them
+        NETBC_HIP(hipMemsetAsync(&ws.ctrl.p->next_bucket[0], 0xFF, sizeof(unsigned long long),
+                                 st));
+        uint32_t D = 0;
+        for (uint32_t it = 0;; it++) {
+            const int p = it & 1;
             dial_scan_kernel<<<scan_blocks, kBlock, 0, st>>>(
-                ws.inb.p, words, (int)(D % (uint32_t)W), pairs, ws.touched.p,
-                &ws.ctrl.p->tlen[p], &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p],
-                &ws.ctrl.p->left[p ^ 1], n);
+                ws.inb.p, words, W, D, pairs, ws.touched.p, &ws.ctrl.p->tlen[p],
+                &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p], &ws.ctrl.p->left[p ^ 1],
+                &ws.ctrl.p->next_bucket[p], &ws.ctrl.p->next_bucket[p ^ 1], n);
             launch_check();
-            dial_step_kernel<<<step_blocks, kBlock, 0, st>>>(
-                ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, ws.inb.p, d.in_off_p(),
-                d.in_idx_p(), in_len, d.out_off.p, d.out_idx.p, L.out_len.p, ws.dist.p,
-                ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p, n);
+            dial_step_kernel<P><<<step_blocks, kBlock, 0, st>>>(
+                ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
+                d.in_off_p(), d.in_idx_p(), in_lengths<P>(), d.out_off.p, d.out_idx.p,
+                out_lengths<P>(), dist, ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p,
+                &ws.ctrl.p->next_bucket[p], n);
             launch_check();
             read_ctrl(ws, st);
             if (ws.host_ctrl->qoverflow) return false;
             const uint32_t ql = ws.host_ctrl->qlen;
             if (ql > lv.back()) {
                 lv.push_back(ql);
-                lv_dist.push_back(D);
                 diag.levels++;
             }
             // Done when this step listed nothing and no bucket is pending.
             if (ws.host_ctrl->tlen[p] == 0 && ws.host_ctrl->left[p] == 0) break;
+            // otherwise on to the lowest pending bucket (empty ones skipped)
+            const unsigned long long nb = ws.host_ctrl->next_bucket[p];
+            if (nb > 0xFFFFFFFFull && nb != ~0ull) throw Error("bucket index exceeds 2^32");
+            D = nb != ~0ull && nb > D ? (uint32_t)nb : D + 1;
         }
         return true;
     }
 
-    template <class C, bool kEdge>
+    template <class C, bool kEdge, class P>
     void backward_dial(C* coef) {
         const DeviceGraph& d = g.dev;
         int D = (int)lv.size() - 2;
         for (int dl = D; dl >= (kEdge ? 0 : 1); dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             ws.sweep.mark(st);
-            dial_backward_kernel<C, kEdge>
-                <<<resident_blocks(dial_backward_kernel<C, kEdge>, qe - qb), kBlock, 0, st>>>(
-                    ws.q.p, qb, qe, lv_dist[dl], d.out_off.p, d.out_idx.p, L.out_len.p, ws.dist.p,
+            dial_backward_kernel<C, kEdge, P>
+                <<<resident_blocks(dial_backward_kernel<C, kEdge, P>, qe - qb), kBlock, 0, st>>>(
+                    ws.q.p, qb, qe, d.out_off.p, d.out_idx.p, out_lengths<P>(), dist_rows<P>(),
                     ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs);
             launch_check();
             ws.sweep.mark(st);
@@ -2473,7 +2652,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
         if (ws.levels.p != lv_before) ws.zero_words = ws.levels.n;
         ws.touched.alloc(nm);
     }
-    if (L.kind != kReal) {
+    const bool per_thread = L.kind == kReal && !real_bucketed(L, g.n);
+    if (!per_thread) {
         if (precision) ws.coef32.alloc(nm * kLanes);
         else ws.coef64.alloc(nm * kLanes);
         if (qcap > ws.qcap || !ws.q.p) {
@@ -2486,11 +2666,10 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
             ws.qcap = qcap;
         }
     }
-    if (L.kind == kInteger) {
-        int W = (int)L.max + 1;
-        int words = (W + 63) / 64;
-        ws.dist.alloc(nm * kLanes);
-        ws.inb.alloc(nm * words);
+    if (L.kind == kInteger || (L.kind == kReal && !per_thread)) {
+        if (L.kind == kInteger) ws.dist.alloc(nm * kLanes);
+        else ws.rdist.alloc(nm * kLanes);
+        ws.inb.alloc(nm * buckets_for(L).words);
         ws.touched.alloc(nm);
     }
     ws.acc.alloc(n);
@@ -2594,7 +2773,9 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                                  st));
         d_list = ws.list.p;
     }
-    const bool free_order = !opt.exact && k >= kReorderMin && L.kind != kReal;
+    // real lengths with very unequal arcs: the reference's Dijkstra per thread
+    const bool per_thread = L.kind == kReal && !real_bucketed(L, n);
+    const bool free_order = !opt.exact && k >= kReorderMin && !per_thread;
     if (free_order) {
         // Batch sources whose searches look alike (order_sources). Items are
         // summed in the new order, which changes only the association of
@@ -2612,7 +2793,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     DevBuf<double> djk_buf;
     DevBuf<uint32_t> djk_u32;
     DjkState djk{};
-    if (L.kind == kReal) {
+    if (per_thread) {
         size_t T = (size_t)slots * kLanes * n;
         djk_buf.alloc(4 * T);
         djk_u32.alloc(4 * T);
@@ -2621,7 +2802,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     }
     int rows = 0;  // edge credit rows (Workspace::eacc)
     if (edge) {
-        rows = L.kind == kReal ? slots * kLanes : slots;
+        rows = per_thread ? slots * kLanes : slots;
         ws.eacc.alloc((size_t)rows * arcs);
         NETBC_HIP(hipMemsetAsync(ws.eacc.p, 0, (size_t)rows * arcs * sizeof(double), st));
     }
@@ -2629,11 +2810,13 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     int64_t gsize = (k + 63) / 64;
     NETBC_HIP(hipEventRecord(ws.ev0, st));
     for (int64_t item0 = 0; item0 < k; item0 += (int64_t)slots * kLanes) {
-        if (L.kind == kReal) {
+        if (per_thread) {
             eng.run_dijkstra(item0, djk);
         } else {
             for (;;) {
-                bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_dial(item0);
+                bool ok = L.kind == kUniform   ? eng.run_bfs(item0)
+                          : L.kind == kInteger ? eng.run_bucketed<IntPath>(item0)
+                                               : eng.run_bucketed<RealPath>(item0);
                 if (ok) break;
                 // Level queue overflow: grow and repeat the super-batch from
                 // clean rows.
@@ -2748,7 +2931,7 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     std::vector<uint64_t> vis((size_t)slots * n);
     for (int64_t item0 = 0; item0 < k; item0 += (int64_t)slots * kLanes) {
         for (;;) {
-            bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_dial(item0);
+            bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_bucketed<IntPath>(item0);
             if (ok) break;
             eng.clear_rows();
             grow_queue(ws);

@@ -325,10 +325,19 @@ __global__ void int_lengths_kernel(const double* __restrict__ w, size_t arcs, in
     if (a < arcs) len[a] = (uint32_t)arc_length(w[a], strength);
 }
 
+// Real lengths in CSR (or CSC) order; with range, the shortest and longest
+// as bit patterns (positive doubles order as their unsigned bits).
 __global__ void real_lengths_kernel(const double* __restrict__ w, size_t arcs, int strength,
-                                    double* __restrict__ len) {
+                                    double* __restrict__ len,
+                                    unsigned long long* __restrict__ range) {
     size_t a = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < arcs) len[a] = arc_length(w[a], strength);
+    if (a >= arcs) return;
+    const double x = arc_length(w[a], strength);
+    len[a] = x;
+    if (range) {
+        atomicMin(&range[0], (unsigned long long)__double_as_longlong(x));
+        atomicMax(&range[1], (unsigned long long)__double_as_longlong(x));
+    }
 }
 
 // Rust's Display for f64 (error.rs:81-85 prints the weight this way):
@@ -540,10 +549,26 @@ void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L) {
         }
     } else {
         L.kind = kReal;
+        DevBuf<unsigned long long> range;
+        range.alloc(2);
+        const unsigned long long init_range[2] = {~0ull, 0ull};
+        NETBC_HIP(hipMemcpyAsync(range.p, init_range, sizeof init_range, hipMemcpyHostToDevice,
+                                 st));
         L.out_real.alloc(d.arcs);
         real_lengths_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.out_w.p, d.arcs, strength,
-                                                                 L.out_real.p);
+                                                                 L.out_real.p, range.p);
         NETBC_HIP(hipGetLastError());
+        if (d.directed) {
+            L.in_real.alloc(d.arcs);
+            real_lengths_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.in_w.p, d.arcs, strength,
+                                                                     L.in_real.p, nullptr);
+            NETBC_HIP(hipGetLastError());
+        }
+        unsigned long long got[2];
+        NETBC_HIP(hipMemcpyAsync(got, range.p, sizeof got, hipMemcpyDeviceToHost, st));
+        NETBC_HIP(hipStreamSynchronize(st));
+        memcpy(&L.rmin, &got[0], 8);
+        memcpy(&L.rmax, &got[1], 8);
     }
     NETBC_HIP(hipStreamSynchronize(st));
 }

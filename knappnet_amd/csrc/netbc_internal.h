@@ -124,7 +124,8 @@ struct Lengths {
     LengthKind kind = kUniform;
     uint32_t max = 0;
     DevBuf<uint32_t> out_len, in_len;  // integer lengths, CSR / CSC order
-    DevBuf<double> out_real;           // real lengths, CSR order
+    DevBuf<double> out_real, in_real;  // real lengths, CSR / CSC order
+    double rmin = 0.0, rmax = 0.0;     // real lengths: shortest and longest arc
 };
 
 // Options and diagnostics of one betweenness run.
