@@ -54,6 +54,9 @@ def build_workload(name: str):
     if name == "ba200k":
         el = gen.ba_weighted()
         return el, None, dict(workload="ba_200000_m4_int_weights_1_16_all_sources", seed=3)
+    if name == "ba200k-real":
+        el = gen.ba_real()
+        return el, None, dict(workload="ba_200000_m4_real_weights_1_16_all_sources", seed=3)
     if name == "rmat22":
         el = gen.rmat(22, 16, directed=False, seed=1)
         src = gen.sample_sources(el.n, 8192, 7)

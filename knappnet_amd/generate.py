@@ -143,6 +143,14 @@ def ba_weighted(n: int = 200_000, m: int = 4, seed: int = 3, wmax: int = 16) -> 
     return el
 
 
+def ba_real(n: int = 200_000, m: int = 4, seed: int = 3, lo: float = 1.0,
+            hi: float = 16.0) -> EdgeList:
+    """Barabasi-Albert graph with real weights uniform in [lo, hi) (stream 2)."""
+    el = preferential(n, m, seed, 0)
+    el.weights = lo + (hi - lo) * stream_values(seed, 2, "uniform", el.m)
+    return el
+
+
 def ws_sizes(count: int = 4096, base: int = 11) -> np.ndarray:
     lib = _native.generators()
     out = np.zeros(count, dtype=np.int64)
