@@ -1270,13 +1270,7 @@ struct IntPath {
     __device__ static bool reached(Dist d) { return d != kNone; }
     __device__ static uint64_t bucket(Sum x, double) { return x; }
     __device__ static bool tie(Sum via, Dist d) { return via == (Sum)d; }
-    __device__ static bool offer(Dist* p, Sum v) {
-        if (v < (Sum)*p) {
-            atomicMin(p, (uint32_t)v);
-            return true;
-        }
-        return false;
-    }
+    __device__ static void offer(Dist* p, Sum v) { atomicMin(p, (uint32_t)v); }
 };
 
 struct RealPath {
@@ -1292,12 +1286,8 @@ struct RealPath {
         return fabs(via - d) <= 1e-10 * m;
     }
     // Non-negative doubles order as their bit patterns.
-    __device__ static bool offer(Dist* p, Sum v) {
-        if (v < *p) {
-            atomicMin((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
-            return true;
-        }
-        return false;
+    __device__ static void offer(Dist* p, Sum v) {
+        atomicMin((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
     }
 };
 
@@ -1449,15 +1439,34 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                 const uint64_t mm = ok ? (vis[base + u] & M) : 0;
                 uint64_t bal = __ballot(mm != 0);
                 while (bal) {
-                    const int j = __ffsll((unsigned long long)bal) - 1;
-                    bal &= bal - 1;
-                    const uint32_t uu = rl32(u, j);
-                    const Len ll = rl(ln, j);
-                    const uint64_t m2 = rl64(mm, j);
-                    if ((m2 >> lane) & 1) {
-                        const size_t r = (base + uu) * kLanes + lane;
-                        if (P::tie(P::add(dist[r], ll), dx)) acc += sig[r];
+                    // up to kAhead predecessors loaded before they are added
+                    Dist du[kAhead];
+                    double su[kAhead];
+                    Len lu[kAhead];
+                    bool use[kAhead];
+#pragma unroll
+                    for (int h = 0; h < kAhead; h++) {
+                        use[h] = false;
+                        su[h] = 0.0;
+                        du[h] = Dist(0);
+                        lu[h] = Len(0);
+                        if (bal) {
+                            const int j = __ffsll((unsigned long long)bal) - 1;
+                            bal &= bal - 1;
+                            const uint32_t uu = rl32(u, j);
+                            lu[h] = rl(ln, j);
+                            const uint64_t m2 = rl64(mm, j);
+                            if ((m2 >> lane) & 1) {
+                                const size_t r = (base + uu) * kLanes + lane;
+                                use[h] = true;
+                                du[h] = dist[r];
+                                su[h] = sig[r];
+                            }
+                        }
                     }
+#pragma unroll
+                    for (int h = 0; h < kAhead; h++)
+                        if (use[h] && P::tie(P::add(du[h], lu[h]), dx)) acc += su[h];
                 }
             }
             if (mine) {
@@ -1479,29 +1488,51 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             NETBC_COUNT(c3, mm != 0);
             uint64_t bal = __ballot(mm != 0);
             while (bal) {
-                const int j = __ffsll((unsigned long long)bal) - 1;
-                bal &= bal - 1;
-                const uint32_t yy = rl32(y, j);
-                const Len ll = rl(ln, j);
-                const uint64_t m2 = rl64(mm, j);
-                bool offered = false;
-                uint32_t kk = 0;
-                if ((m2 >> lane) & 1) {
-                    const typename P::Sum nd = P::add(dx, ll);
-                    offered = P::offer(&dist[(base + yy) * kLanes + lane], nd);
-                    const uint64_t bnd = P::bucket(nd, delta);
-                    kk = (uint32_t)(bnd % (uint64_t)W);
-                    if (offered) least = min(least, (unsigned long long)bnd);
+                // up to kAhead heads: their distances loaded first, then offered
+                uint32_t yh[kAhead];
+                Dist cur[kAhead];
+                typename P::Sum nd[kAhead];
+                bool use[kAhead];
+#pragma unroll
+                for (int h = 0; h < kAhead; h++) {
+                    yh[h] = kNone;
+                    use[h] = false;
+                    cur[h] = Dist(0);
+                    nd[h] = typename P::Sum(0);
+                    if (bal) {
+                        const int j = __ffsll((unsigned long long)bal) - 1;
+                        bal &= bal - 1;
+                        yh[h] = rl32(y, j);
+                        const uint64_t m2 = rl64(mm, j);
+                        if ((m2 >> lane) & 1) {
+                            use[h] = true;
+                            nd[h] = P::add(dx, rl(ln, j));
+                            cur[h] = dist[(base + yh[h]) * kLanes + lane];
+                        }
+                    }
                 }
-                // mark the offered buckets (one atomic per distinct bucket)
-                uint64_t pending = __ballot(offered);
-                while (pending) {
-                    const int f = __ffsll((unsigned long long)pending) - 1;
-                    const uint32_t kf = rl32(kk, f);
-                    pending &= ~__ballot(offered && kk == kf);
-                    if (lane == f)
-                        atomicOr((unsigned long long*)&pend[(base + yy) * words + (kf >> 6)],
-                                 1ull << (kf & 63));
+#pragma unroll
+                for (int h = 0; h < kAhead; h++) {
+                    if (yh[h] == kNone) continue;
+                    bool offered = false;
+                    uint32_t kk = 0;
+                    if (use[h] && nd[h] < (typename P::Sum)cur[h]) {
+                        P::offer(&dist[(base + yh[h]) * kLanes + lane], nd[h]);
+                        offered = true;
+                        const uint64_t bnd = P::bucket(nd[h], delta);
+                        kk = (uint32_t)(bnd % (uint64_t)W);
+                        least = min(least, (unsigned long long)bnd);
+                    }
+                    // mark the offered buckets (one atomic per distinct bucket)
+                    uint64_t pending = __ballot(offered);
+                    while (pending) {
+                        const int f = __ffsll((unsigned long long)pending) - 1;
+                        const uint32_t kf = rl32(kk, f);
+                        pending &= ~__ballot(offered && kk == kf);
+                        if (lane == f)
+                            atomicOr((unsigned long long*)&pend[(base + yh[h]) * words + (kf >> 6)],
+                                     1ull << (kf & 63));
+                    }
                 }
             }
         }
