@@ -1156,3 +1156,510 @@ double orc_ordered_sum_test(uint64_t items, uint64_t threads) {
 }
 
 uint64_t orc_available_cores(void) { return available_cores(); }
+
+/* ---- betweenness for sources and targets: path_centrality.rs:421-689 --- */
+
+/* PathGraph::new, paths/mod.rs:173-299 (paths follow arcs, mode Out): arcs
+ * generated edge by edge (for undirected non-loop edges the reverse right
+ * after), stably sorted by (tail, head); with collapse, one arc per
+ * (tail, head), of the least length, the first edge of that length. */
+typedef struct {
+    uint64_t n, arcs;
+    uint64_t *off;
+    uint32_t *head, *tail, *edge;
+    double *len;
+    int unit, integral;
+    double min_len, max_len;
+} OPathGraph;
+
+static void pg_free(OPathGraph *pg) {
+    free(pg->off);
+    free(pg->head);
+    free(pg->tail);
+    free(pg->edge);
+    free(pg->len);
+    memset(pg, 0, sizeof *pg);
+}
+
+static int pg_cmp(const void *pa, const void *pb) {
+    const uint64_t *a = pa, *b = pb;
+    for (int i = 0; i < 3; i++)
+        if (a[i] != b[i]) return a[i] < b[i] ? -1 : 1;
+    return 0;
+}
+
+/* path_graph, path_centrality.rs:198-216, with PathGraph::new's length
+ * checks (paths/mod.rs:175-192) and require_positive (paths/mod.rs:416-425). */
+static int pg_build(const OGraph *g, int weighted, int kind, int collapse, OPathGraph *pg,
+                    char *err, size_t err_len) {
+    char msg[512], num[64];
+    memset(pg, 0, sizeof *pg);
+    uint64_t n = g->n, m = g->m;
+    double *elen = malloc((m ? m : 1) * sizeof *elen);
+    for (uint64_t e = 0; e < m; e++) elen[e] = 1.0;
+    if (weighted) {
+        if (!g->w) {
+            free(elen);
+            write_err(err, err_len, "the graph has no edge weights");
+            return ORC_FAILED;
+        }
+        for (uint64_t e = 0; e < m; e++) {
+            if (kind == 1 && g->w[e] <= 0.0) {
+                fmt_f64(g->w[e], num, sizeof num);
+                snprintf(msg, sizeof msg,
+                         "edge %llu has weight %s; weights read as strengths become lengths "
+                         "1/w and must be strictly positive",
+                         (unsigned long long)e, num);
+                free(elen);
+                write_err(err, err_len, msg);
+                return ORC_FAILED;
+            }
+            elen[e] = kind == 1 ? 1.0 / g->w[e] : g->w[e];
+        }
+    }
+    uint64_t *t = malloc((2 * m + 1) * 4 * sizeof *t); /* tail, head, gen, edge */
+    uint64_t na = 0;
+    for (uint64_t e = 0; e < m; e++) {
+        uint64_t s = g->src[e], h = g->tgt[e];
+        uint64_t *r = t + 4 * na;
+        r[0] = s; r[1] = h; r[2] = na; r[3] = e;
+        na++;
+        if (!g->directed && s != h) {
+            r = t + 4 * na;
+            r[0] = h; r[1] = s; r[2] = na; r[3] = e;
+            na++;
+        }
+    }
+    qsort(t, na, 4 * sizeof *t, pg_cmp);
+    pg->n = n;
+    pg->head = malloc((na ? na : 1) * sizeof *pg->head);
+    pg->tail = malloc((na ? na : 1) * sizeof *pg->tail);
+    pg->edge = malloc((na ? na : 1) * sizeof *pg->edge);
+    pg->len = malloc((na ? na : 1) * sizeof *pg->len);
+    uint64_t k = 0;
+    for (uint64_t i = 0; i < na; i++) {
+        uint32_t tl = (uint32_t)t[4 * i], hd = (uint32_t)t[4 * i + 1];
+        uint32_t e = (uint32_t)t[4 * i + 3];
+        double len = elen[e];
+        if (collapse && k > 0 && pg->tail[k - 1] == tl && pg->head[k - 1] == hd) {
+            if (len < pg->len[k - 1]) {
+                pg->len[k - 1] = len;
+                pg->edge[k - 1] = e;
+            }
+            continue;
+        }
+        pg->head[k] = hd;
+        pg->tail[k] = tl;
+        pg->len[k] = len;
+        pg->edge[k] = e;
+        k++;
+    }
+    pg->arcs = k;
+    free(t);
+    free(elen);
+    pg->off = calloc(n + 1, sizeof *pg->off);
+    for (uint64_t a = 0; a < k; a++) pg->off[(uint64_t)pg->tail[a] + 1]++;
+    for (uint64_t v = 0; v < n; v++) pg->off[v + 1] += pg->off[v];
+    pg->unit = 1;
+    pg->integral = 1;
+    pg->min_len = INFINITY;
+    pg->max_len = -INFINITY;
+    for (uint64_t a = 0; a < k; a++) {
+        double x = pg->len[a];
+        if (x != 1.0) pg->unit = 0;
+        if (!(x - trunc(x) == 0.0 && fabs(x) < 4503599627370496.0)) pg->integral = 0;
+        if (x < pg->min_len) pg->min_len = x;
+        if (x > pg->max_len) pg->max_len = x;
+    }
+    if (weighted && pg->min_len <= 0.0) {
+        uint64_t a = 0;
+        while (a < k && !(pg->len[a] <= 0.0)) a++;
+        fmt_f64(pg->len[a], num, sizeof num);
+        snprintf(msg, sizeof msg,
+                 "edge %u has length %s; shortest-path betweenness needs strictly positive "
+                 "lengths",
+                 pg->edge[a], num);
+        pg_free(pg);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    return ORC_OK;
+}
+
+/* Tie::equal, paths/mod.rs:96-104 (tie 0: relative 1e-10, 1: exact). */
+static int tie_equal(int tie, double a, double b) {
+    if (a == b) return 1;
+    if (tie != 0 || !isfinite(a) || !isfinite(b)) return 0;
+    double m = fabs(a) > fabs(b) ? fabs(a) : fabs(b);
+    return fabs(a - b) <= 1e-10 * m;
+}
+
+/* The shortest-path DAG from one source, path_centrality.rs:225-409. */
+typedef struct {
+    uint64_t n;
+    double *dist, *sigma;
+    uint32_t *rank, *order;
+    uint64_t norder;
+    U32Vec *preds;
+    Heap heap;
+    U32Vec *buckets;
+    uint64_t nbuckets;
+} ODag;
+
+static void dag_init(ODag *d, uint64_t n) {
+    memset(d, 0, sizeof *d);
+    d->n = n;
+    d->dist = malloc((n ? n : 1) * sizeof *d->dist);
+    d->sigma = malloc((n ? n : 1) * sizeof *d->sigma);
+    d->rank = malloc((n ? n : 1) * sizeof *d->rank);
+    d->order = malloc((n ? n : 1) * sizeof *d->order);
+    d->preds = calloc(n ? n : 1, sizeof *d->preds);
+    d->heap.items = malloc((n ? n : 1) * sizeof *d->heap.items);
+    d->heap.pos = malloc((n ? n : 1) * sizeof *d->heap.pos);
+    for (uint64_t v = 0; v < n; v++) {
+        d->dist[v] = INFINITY;
+        d->sigma[v] = 0.0;
+        d->rank[v] = U32_MAX_;
+        d->heap.pos[v] = U32_MAX_;
+    }
+}
+
+static void dag_free(ODag *d) {
+    for (uint64_t v = 0; v < d->n; v++) free(d->preds[v].a);
+    for (uint64_t b = 0; b < d->nbuckets; b++) free(d->buckets[b].a);
+    free(d->buckets);
+    free(d->dist);
+    free(d->sigma);
+    free(d->rank);
+    free(d->order);
+    free(d->preds);
+    free(d->heap.items);
+    free(d->heap.pos);
+}
+
+static void dag_reset(ODag *d) {
+    for (uint64_t v = 0; v < d->n; v++) {
+        d->dist[v] = INFINITY;
+        d->sigma[v] = 0.0;
+        d->rank[v] = U32_MAX_;
+        d->preds[v].len = 0;
+        d->heap.pos[v] = U32_MAX_;
+    }
+    d->norder = 0;
+    d->heap.len = 0;
+}
+
+static double pred_sigma(const ODag *d, const OPathGraph *pg, uint32_t v) {
+    double s = 0.0;
+    for (uint64_t i = 0; i < d->preds[v].len; i++) s += d->sigma[pg->tail[d->preds[v].a[i]]];
+    return s;
+}
+
+/* Dag::run without a cutoff: breadth-first (unit lengths), Dial's buckets
+ * (integer lengths up to DIAL_MAX; buckets are stacks), or the 4-ary heap
+ * with the tie rule. */
+static void dag_run(ODag *d, const OPathGraph *pg, uint32_t s, int tie) {
+    dag_reset(d);
+    d->dist[s] = 0.0;
+    d->sigma[s] = 1.0;
+    if (pg->unit) {
+        d->rank[s] = 0;
+        d->order[d->norder++] = s;
+        uint64_t head = 0;
+        while (head < d->norder) {
+            uint32_t v = d->order[head++];
+            if (v != s) d->sigma[v] = pred_sigma(d, pg, v);
+            double next = d->dist[v] + 1.0;
+            for (uint64_t a = pg->off[v]; a < pg->off[v + 1]; a++) {
+                uint32_t w = pg->head[a];
+                if (d->rank[w] == U32_MAX_) {
+                    d->dist[w] = next;
+                    d->rank[w] = (uint32_t)d->norder;
+                    d->order[d->norder++] = w;
+                }
+                if (d->dist[w] == next) vpush(&d->preds[w], (uint32_t)a);
+            }
+        }
+        return;
+    }
+    if (!pg->unit && pg->integral && pg->min_len >= 0.0 && pg->max_len <= (double)DIAL_MAX) {
+        uint64_t width = (uint64_t)(pg->max_len > 0.0 ? pg->max_len : 0.0) + 1;
+        if (d->nbuckets != width) {
+            for (uint64_t b = 0; b < d->nbuckets; b++) free(d->buckets[b].a);
+            free(d->buckets);
+            d->buckets = calloc(width, sizeof *d->buckets);
+            d->nbuckets = width;
+        }
+        for (uint64_t b = 0; b < width; b++) d->buckets[b].len = 0;
+        vpush(&d->buckets[0], s);
+        uint64_t pending = 1, cur = 0;
+        while (pending > 0) {
+            U32Vec *bk = &d->buckets[cur % width];
+            if (bk->len == 0) {
+                cur++;
+                continue;
+            }
+            uint32_t v = bk->a[--bk->len];
+            pending--;
+            if (d->rank[v] != U32_MAX_ || d->dist[v] != (double)cur) continue;
+            d->rank[v] = (uint32_t)d->norder;
+            d->order[d->norder++] = v;
+            if (v != s) d->sigma[v] = pred_sigma(d, pg, v);
+            for (uint64_t a = pg->off[v]; a < pg->off[v + 1]; a++) {
+                uint32_t w = pg->head[a];
+                if (d->rank[w] != U32_MAX_) continue;
+                uint64_t alt = cur + (uint64_t)pg->len[a];
+                double altf = (double)alt;
+                double dw = d->dist[w];
+                if (isinf(dw) || altf < dw) {
+                    d->dist[w] = altf;
+                    d->preds[w].len = 0;
+                    vpush(&d->preds[w], (uint32_t)a);
+                    vpush(&d->buckets[alt % width], w);
+                    pending++;
+                } else if (altf == dw) {
+                    vpush(&d->preds[w], (uint32_t)a);
+                }
+            }
+        }
+        return;
+    }
+    h_push_or_decrease(&d->heap, s, 0.0);
+    HItem top;
+    while (h_pop(&d->heap, &top)) {
+        uint32_t v = top.v;
+        double dv = top.d;
+        d->rank[v] = (uint32_t)d->norder;
+        d->order[d->norder++] = v;
+        if (v != s) d->sigma[v] = pred_sigma(d, pg, v);
+        for (uint64_t a = pg->off[v]; a < pg->off[v + 1]; a++) {
+            uint32_t w = pg->head[a];
+            if (d->rank[w] != U32_MAX_) continue;
+            double alt = dv + pg->len[a];
+            double cur = d->dist[w];
+            if (isinf(cur)) {
+                d->dist[w] = alt;
+                vpush(&d->preds[w], (uint32_t)a);
+                h_push_or_decrease(&d->heap, w, alt);
+            } else if (tie_equal(tie, alt, cur)) {
+                vpush(&d->preds[w], (uint32_t)a);
+            } else if (alt < cur) {
+                d->dist[w] = alt;
+                d->preds[w].len = 0;
+                vpush(&d->preds[w], (uint32_t)a);
+                h_push_or_decrease(&d->heap, w, alt);
+            }
+        }
+    }
+}
+
+/* networkx_rescale, path_centrality.rs:596-649. sampled < 0: no sample. */
+static void nx_rescale(double *values, uint64_t len, uint64_t n, int normalized, int directed,
+                       int endpoints, int64_t sampled, const uint8_t *is_source) {
+    double big_n = endpoints ? (double)n : (double)n - 1.0;
+    if (big_n < 2.0) return;
+    double correction = directed ? 1.0 : 2.0;
+    double k_source = sampled >= 0 ? (double)sampled : big_n;
+    if (sampled < 0 || endpoints) {
+        double scale = normalized ? 1.0 / (k_source * (big_n - 1.0))
+                                  : big_n / (k_source * correction);
+        if (scale != 1.0)
+            for (uint64_t v = 0; v < len; v++) values[v] *= scale;
+        return;
+    }
+    double ss, so;
+    if (normalized) {
+        ss = k_source > 1.0 ? 1.0 / ((k_source - 1.0) * (big_n - 1.0)) : NAN;
+        so = 1.0 / (k_source * (big_n - 1.0));
+    } else {
+        ss = k_source > 1.0 ? big_n / ((k_source - 1.0) * correction) : NAN;
+        so = big_n / (k_source * correction);
+    }
+    for (uint64_t v = 0; v < len; v++) values[v] *= (is_source && is_source[v]) ? ss : so;
+}
+
+/* share_among_parallel, path_centrality.rs:651-689: each adjacent pair's
+ * value is shared equally by its parallel edges of least length; other
+ * parallel edges and loops get 0. Groups by (a, b) (sorted undirected). */
+static void share_among_parallel(const OGraph *g, const double *lengths, double *values) {
+    uint64_t m = g->m;
+    uint64_t *t = malloc((m ? m : 1) * 3 * sizeof *t);
+    for (uint64_t e = 0; e < m; e++) {
+        uint64_t a = g->src[e], b = g->tgt[e];
+        if (!g->directed && a > b) { uint64_t x = a; a = b; b = x; }
+        t[3 * e] = a; t[3 * e + 1] = b; t[3 * e + 2] = e;
+    }
+    qsort(t, m, 3 * sizeof *t, pg_cmp);
+    double *out = calloc(m ? m : 1, sizeof *out);
+    for (uint64_t i = 0; i < m;) {
+        uint64_t j = i;
+        while (j < m && t[3 * j] == t[3 * i] && t[3 * j + 1] == t[3 * i + 1]) j++;
+        if (t[3 * i] != t[3 * i + 1]) {
+            double total = 0.0, shortest = INFINITY;
+            for (uint64_t q = i; q < j; q++) total += values[t[3 * q + 2]];  /* input order */
+            for (uint64_t q = i; q < j; q++)
+                if (lengths[t[3 * q + 2]] < shortest) shortest = lengths[t[3 * q + 2]];
+            uint64_t ties = 0;
+            for (uint64_t q = i; q < j; q++) ties += lengths[t[3 * q + 2]] == shortest;
+            for (uint64_t q = i; q < j; q++)
+                if (lengths[t[3 * q + 2]] == shortest) out[t[3 * q + 2]] = total / (double)ties;
+        }
+        i = j;
+    }
+    memcpy(values, out, m * sizeof *values);
+    free(out);
+    free(t);
+}
+
+/* net_betweenness_subset, crates/capi/src/centrality.rs:89-144 ->
+ * betweenness_subset, path_centrality.rs:421-593. sources / targets: NULL
+ * (or a negative count) for every node. rescale 0 unordered, 1 NetworkX,
+ * 2 NetworkX subset; parallel 0 distinct, 1 collapse; edge_split 0
+ * proportional, 1 NetworkX; tie 0 relative, 1 exact. Sources are summed
+ * in 64 ordered groups (sum_in_order), single-threaded here: the result
+ * does not depend on the thread count. nodes: n doubles; edges: NULL or m. */
+int32_t orc_betweenness_subset(const OGraph *g, const int64_t *sources, int64_t n_sources,
+                               const int64_t *targets, int64_t n_targets, int32_t weighted,
+                               int32_t weights_as, int32_t endpoints, int32_t normalized,
+                               int32_t rescale, int32_t parallel, int32_t edge_split,
+                               int32_t tie, double *nodes, double *edges, char *err,
+                               size_t err_len) {
+    char msg[256];
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    uint64_t n = g->n, m = g->m;
+    const int64_t *lists[2] = {sources, targets};
+    int64_t counts[2] = {n_sources, n_targets};
+    for (int l = 0; l < 2; l++) {
+        if (!lists[l] || counts[l] < 0) continue;
+        for (int64_t i = 0; i < counts[l]; i++) {
+            int64_t v = lists[l][i];
+            if (v < 0 || (uint64_t)v >= n) {
+                snprintf(msg, sizeof msg,
+                         "node index %lld is out of range for a network with %llu nodes",
+                         (long long)v, (unsigned long long)n);
+                write_err(err, err_len, msg);
+                return ORC_FAILED;
+            }
+        }
+    }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    if (rescale < 0 || rescale > 2) {
+        snprintf(msg, sizeof msg, "unknown rescaling code %d", rescale);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (parallel < 0 || parallel > 1) {
+        snprintf(msg, sizeof msg, "unknown code %d for parallel edges", parallel);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (edge_split < 0 || edge_split > 1) {
+        snprintf(msg, sizeof msg, "unknown edge split code %d", edge_split);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (tie < 0 || tie > 1) {
+        snprintf(msg, sizeof msg, "unknown tie code %d", tie);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    int collapse = parallel == 1;
+    OPathGraph pg;
+    if (pg_build(g, weighted, weights_as, collapse, &pg, err, err_len)) return ORC_FAILED;
+    int all_src = !sources || n_sources < 0;
+    uint64_t ns = all_src ? n : (uint64_t)n_sources;
+    uint32_t *src = malloc((ns ? ns : 1) * sizeof *src);
+    for (uint64_t i = 0; i < ns; i++) src[i] = all_src ? (uint32_t)i : (uint32_t)sources[i];
+    uint8_t *is_target = calloc(n ? n : 1, 1);
+    if (!targets || n_targets < 0) {
+        for (uint64_t v = 0; v < n; v++) is_target[v] = 1;
+    } else {
+        for (int64_t i = 0; i < n_targets; i++) is_target[targets[i]] = 1;
+    }
+    int quirk = edge_split == 1;
+    double *acc_n = calloc(n ? n : 1, sizeof *acc_n), *acc_e = calloc(m ? m : 1, sizeof *acc_e);
+    double *tot_n = calloc(n ? n : 1, sizeof *tot_n), *tot_e = calloc(m ? m : 1, sizeof *tot_e);
+    double *delta = calloc(n ? n : 1, sizeof *delta), *delta_e = calloc(n ? n : 1, sizeof *delta_e);
+    ODag d;
+    dag_init(&d, n);
+    uint64_t size = (ns + SUM_GROUPS - 1) / SUM_GROUPS;
+    uint64_t groups = size ? (ns + size - 1) / size : 0;
+    for (uint64_t grp = 0; grp < groups; grp++) {
+        uint64_t lo = grp * size, hi = lo + size < ns ? lo + size : ns;
+        for (uint64_t k = lo; k < hi; k++) {
+            uint32_t s = src[k];
+            dag_run(&d, &pg, s, tie);
+            uint64_t reached_targets = 0;
+            for (uint64_t i = d.norder; i-- > 0;) {
+                uint32_t w = d.order[i];
+                double tw = (is_target[w] && w != s) ? 1.0 : 0.0;
+                double coeff = (tw + delta[w]) / d.sigma[w];
+                double np = (double)d.preds[w].len;
+                for (uint64_t q = 0; q < d.preds[w].len; q++) {
+                    uint32_t a = d.preds[w].a[q];
+                    uint32_t v = pg.tail[a];
+                    double c = d.sigma[v] * coeff;
+                    delta[v] += c;
+                    double ce = c;
+                    if (quirk) {
+                        ce = is_target[w] ? d.sigma[v] / d.sigma[w] * (1.0 + delta_e[w])
+                                          : delta_e[w] / np;
+                        delta_e[v] += ce;
+                    }
+                    acc_e[pg.edge[a]] += ce;
+                }
+                if (w != s) {
+                    acc_n[w] += delta[w];
+                    if (endpoints && tw > 0.0) {
+                        acc_n[w] += 1.0;
+                        reached_targets++;
+                    }
+                }
+            }
+            if (endpoints) acc_n[s] += (double)reached_targets;
+            for (uint64_t i = 0; i < d.norder; i++) {
+                delta[d.order[i]] = 0.0;
+                delta_e[d.order[i]] = 0.0;
+            }
+        }
+        for (uint64_t v = 0; v < n; v++) { tot_n[v] += acc_n[v]; acc_n[v] = 0.0; }
+        for (uint64_t e = 0; e < m; e++) { tot_e[e] += acc_e[e]; acc_e[e] = 0.0; }
+    }
+    /* scaling, path_centrality.rs:527-574 */
+    uint8_t *source_set = calloc(n ? n : 1, 1);
+    for (uint64_t i = 0; i < ns; i++) source_set[src[i]] = 1;
+    int64_t sampled = all_src ? -1 : (int64_t)ns;
+    if (rescale == 0) {
+        double half = g->directed ? 1.0 : 0.5;
+        double np_ = 1.0, ep_ = 1.0;
+        if (normalized) {
+            double nf = (double)n;
+            np_ = endpoints ? nf * (nf - 1.0) : (nf - 1.0) * (nf - 2.0);
+            ep_ = nf * (nf - 1.0);
+        }
+        double fn = np_ > 0.0 ? half / np_ : half;
+        double fe = ep_ > 0.0 ? half / ep_ : half;
+        for (uint64_t v = 0; v < n; v++) tot_n[v] *= fn;
+        for (uint64_t e = 0; e < m; e++) tot_e[e] *= fe;
+    } else if (rescale == 1) {
+        nx_rescale(tot_n, n, n, normalized, g->directed, endpoints, sampled, source_set);
+        nx_rescale(tot_e, m, n, normalized, g->directed, 1, sampled, NULL);
+    } else {
+        nx_rescale(tot_n, n, n, normalized, g->directed, 0, -1, NULL);
+        nx_rescale(tot_e, m, n, normalized, g->directed, 1, -1, NULL);
+    }
+    if (collapse) {
+        double *lengths = malloc((m ? m : 1) * sizeof *lengths);
+        for (uint64_t e = 0; e < m; e++)
+            lengths[e] = (weighted && g->w) ? (weights_as == 1 ? 1.0 / g->w[e] : g->w[e]) : 1.0;
+        share_among_parallel(g, lengths, tot_e);
+        free(lengths);
+    }
+    memcpy(nodes, tot_n, n * sizeof *nodes);
+    if (edges) memcpy(edges, tot_e, m * sizeof *edges);
+    dag_free(&d);
+    pg_free(&pg);
+    free(src); free(is_target); free(source_set);
+    free(acc_n); free(acc_e); free(tot_n); free(tot_e); free(delta); free(delta_e);
+    return ORC_OK;
+}

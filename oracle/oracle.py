@@ -57,6 +57,12 @@ def lib() -> ctypes.CDLL:
                                               c_int32, c_int32, c_int32, POINTER(c_double),
                                               c_char_p, c_size_t]
         L.orc_betweenness_sources.restype = c_int32
+        L.orc_betweenness_subset.argtypes = [c_void_p, POINTER(c_int64), c_int64,
+                                             POINTER(c_int64), c_int64, c_int32, c_int32,
+                                             c_int32, c_int32, c_int32, c_int32, c_int32,
+                                             c_int32, POINTER(c_double), POINTER(c_double),
+                                             c_char_p, c_size_t]
+        L.orc_betweenness_subset.restype = c_int32
         L.orc_single_source.argtypes = [c_void_p, c_int32, c_int32, c_int64, POINTER(c_int64),
                                         POINTER(c_int32), POINTER(c_uint64), POINTER(c_double),
                                         POINTER(c_double), POINTER(c_double), c_char_p,
@@ -166,6 +172,30 @@ class OracleGraph:
                                           _p(out, c_double), err, ERR_LEN):
             raise OracleError(err.value.decode())
         return out[: self.n]
+
+    def betweenness_subset(self, sources=None, targets=None, weighted=False,
+                           weights_as="distance", endpoints=False, normalized=False, rescale=0,
+                           parallel=0, edge_split=0, tie=0):
+        """path_centrality.rs:421-593: (node values, edge values). sources /
+        targets None: every node."""
+        def lst(x):
+            if x is None:
+                return None, -1
+            a = np.ascontiguousarray(np.asarray(x, dtype=np.int64))
+            return a, len(a)
+        s, ns = lst(sources)
+        t, nt = lst(targets)
+        nodes = np.zeros(max(self.n, 1), dtype=np.float64)
+        edges = np.zeros(max(self.m, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if self.L.orc_betweenness_subset(
+                self.h, None if s is None else _p(s, c_int64), ns,
+                None if t is None else _p(t, c_int64), nt, int(weighted), kind, int(endpoints),
+                int(normalized), int(rescale), int(parallel), int(edge_split), int(tie),
+                _p(nodes, c_double), _p(edges, c_double), err, ERR_LEN):
+            raise OracleError(err.value.decode())
+        return nodes[: self.n], edges[: self.m]
 
     def single_source(self, s, weighted=False, weights_as="distance"):
         """(kind, dist_u64, dist_f64, sigma, delta, reached)."""

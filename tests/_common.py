@@ -69,3 +69,63 @@ def networkx_documented(fx, variant):
     if variant == "edge_betweenness_normalized" and fx["n"] < 2:
         return True
     return False
+
+
+# Betweenness for sources and targets (validation/manifest.d/centrality_extra.yaml:
+# cx_betweenness_subset; node sets as validation/bindings/centrality_extra.py:
+# 60-63, 140-155). rescale 0 unordered, 1 NetworkX, 2 NetworkX subset;
+# parallel 0 distinct, 1 collapse; edge_split 0 proportional, 1 NetworkX;
+# tie 0 relative, 1 exact.
+_NX_SUBSET = dict(sources="even", targets="thirds", rescale=2, parallel=1, tie=1)
+_NX_SAMPLE = dict(sample=7, seed=3, rescale=1, parallel=1, tie=1)
+SUBSET_VARIANTS = {
+    "cx_bc_all_distinct": dict(out="nodes", rescale=0, parallel=0),
+    "cx_bc_all_distinct_weighted": dict(out="nodes", rescale=0, parallel=0, weighted=True),
+    "cx_ebc_all_distinct": dict(out="edges", rescale=0, parallel=0),
+    "cx_bc_subset_nx": dict(out="nodes", **_NX_SUBSET),
+    "cx_bc_subset_nx_normalized": dict(out="nodes", normalized=True, **_NX_SUBSET),
+    "cx_bc_subset_nx_weighted": dict(out="nodes", weighted=True, **_NX_SUBSET),
+    "cx_ebc_subset_nx": dict(out="edges", edge_split=1, **_NX_SUBSET),
+    "cx_ebc_subset_nx_weighted": dict(out="edges", edge_split=1, weighted=True, **_NX_SUBSET),
+    "cx_ebc_subset_proportional": dict(out="edges", edge_split=0, **_NX_SUBSET),
+    "cx_bc_sampled": dict(out="nodes", normalized=True, **_NX_SAMPLE),
+    "cx_bc_sampled_raw": dict(out="nodes", **_NX_SAMPLE),
+    "cx_bc_sampled_endpoints": dict(out="nodes", normalized=True, endpoints=True, **_NX_SAMPLE),
+    "cx_bc_sampled_weighted": dict(out="nodes", normalized=True, weighted=True, **_NX_SAMPLE),
+    "cx_ebc_sampled": dict(out="edges", normalized=True, **_NX_SAMPLE),
+}
+
+
+def subset_call(n, spec):
+    """(keyword arguments of a betweenness_subset call, which output) for a spec."""
+    import random
+    kw = {k: v for k, v in spec.items() if k not in ("out", "sample", "seed", "sources",
+                                                       "targets")}
+
+    def node_set(kind):
+        if kind == "even":
+            return [v for v in range(n) if v % 2 == 0]
+        if kind == "thirds":
+            return [v for v in range(n) if v % 3 == 0]
+        return None
+
+    if "sample" in spec:
+        kw["sources"] = random.Random(spec["seed"]).sample(list(range(n)), min(spec["sample"], n))
+    else:
+        kw["sources"] = node_set(spec.get("sources", "all"))
+    kw["targets"] = node_set(spec.get("targets", "all"))
+    return kw, spec["out"]
+
+
+def golden_values(vals):
+    """The centrality_extra goldens write NaN as -999 and infinity as -1000."""
+    out = []
+    for x in vals:
+        out.append(float("nan") if x is None or x == -999 else (float("inf") if x == -1000 else x))
+    return out
+
+
+def subset_networkx_documented(fx, variant):
+    # centrality_extra.yaml:16-58: the all_distinct variants count parallel
+    # edges as distinct paths, NetworkX once.
+    return "multi_edges" in set(fx["tags"]) and "all_distinct" in variant

@@ -4,9 +4,10 @@ The reference keeps 28 small networks (validation/fixtures/<name>/edges.csv
 and meta.json) and the values its oracles produced for them
 (validation/golden/{definition,networkx}/<name>/<variant>.csv, one
 ``%.17g`` value per line under a ``value`` header; see
-validation/common.py:341-352). This script copies the edge lists, the
-metadata and the betweenness goldens into one JSON file, as data. Nothing
-of the reference is imported or run.
+validation/common.py:341-352; the centrality_extra variants write NaN as
+-999). This script copies the edge lists, the metadata and the goldens of
+betweenness, edge betweenness and betweenness for sources and targets
+into one JSON file, as data. Nothing of the reference is imported or run.
 
 Run once in a tree where the reference is checked out:
 
@@ -30,6 +31,21 @@ VARIANTS = [
     "edge_betweenness_normalized",
     "edge_betweenness_weighted",
     "edge_betweenness_weighted_strength",
+    # betweenness for sources and targets (validation/manifest.d/centrality_extra.yaml)
+    "cx_bc_all_distinct",
+    "cx_bc_all_distinct_weighted",
+    "cx_ebc_all_distinct",
+    "cx_bc_subset_nx",
+    "cx_bc_subset_nx_normalized",
+    "cx_bc_subset_nx_weighted",
+    "cx_ebc_subset_nx",
+    "cx_ebc_subset_nx_weighted",
+    "cx_ebc_subset_proportional",
+    "cx_bc_sampled",
+    "cx_bc_sampled_raw",
+    "cx_bc_sampled_endpoints",
+    "cx_bc_sampled_weighted",
+    "cx_ebc_sampled",
 ]
 ORACLES = ["definition", "networkx"]
 

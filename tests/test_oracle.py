@@ -17,7 +17,8 @@ from scipy.sparse import csr_matrix
 from scipy.sparse.csgraph import shortest_path
 
 import definition
-from _common import (EDGE_VARIANTS, VARIANTS, close, networkx_documented, random_case,
+from _common import (EDGE_VARIANTS, SUBSET_VARIANTS, VARIANTS, close, golden_values,
+                     networkx_documented, random_case, subset_call, subset_networkx_documented,
                      tolerance)
 from oracle.oracle import (OracleError, OracleGraph, available_cores, ordered_sum_bits,
                            threads_for)
@@ -83,6 +84,25 @@ def test_edge_goldens_networkx(fixtures, variant):
         if networkx_documented(fx, variant):
             continue  # documented difference; the definition golden rules
         assert ok, (name, variant)
+
+
+@pytest.mark.parametrize("variant", list(SUBSET_VARIANTS))
+def test_subset_goldens(fixtures, variant):
+    # centrality_extra.yaml: rel 1e-9, abs 1e-12 for every variant
+    checked = 0
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"]:
+            continue
+        kw, out = subset_call(fx["n"], SUBSET_VARIANTS[variant])
+        nodes, edges = og(fx).betweenness_subset(**kw)
+        got = nodes if out == "nodes" else edges
+        for oracle, want in fx["golden"][variant].items():
+            ok = close(got, golden_values(want), 1e-9, 1e-12)
+            if oracle == "networkx" and subset_networkx_documented(fx, variant):
+                continue
+            assert ok, (name, variant, oracle)
+        checked += 1
+    assert checked >= 7
 
 
 # ---- unit known answers: betweenness.rs:387-517 -------------------------------
@@ -302,6 +322,65 @@ def test_edge_definition_random(seed):
     assert close(g.edge_betweenness(weighted=True, weights_as="strength"), want, 1e-9, 1e-9)
     # results never depend on the thread count
     assert np.array_equal(g.edge_betweenness(threads=1), g.edge_betweenness(threads=3))
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_subset_definition_random(seed):
+    # sources with repeats, a target set, endpoints, parallel edges distinct
+    # or collapsed, unit and integer lengths, against exact rationals
+    rng = np.random.default_rng(3000 + seed)
+    n, s, t, directed = random_case(rng, max_n=11)
+    w = rng.integers(1, 4, size=len(s)).astype(np.float64)
+    srcs = list(rng.integers(0, n, size=int(rng.integers(1, n + 2))))
+    tg = list(rng.choice(n, size=int(rng.integers(1, n + 1)), replace=False))
+    half = 1.0 if directed else 0.5
+    g = OracleGraph(n, s, t, w, directed)
+    for collapse in (0, 1):
+        for weighted in (False, True):
+            for endpoints in (False, True):
+                wn, we = definition.subset_raw(n, s, t, w, directed, weighted=weighted,
+                                               sources=srcs, targets=tg, endpoints=endpoints,
+                                               collapse=bool(collapse))
+                gn, ge = g.betweenness_subset(sources=srcs, targets=tg, weighted=weighted,
+                                              endpoints=endpoints, parallel=collapse)
+                assert close(gn, np.array(wn) * half, 1e-12, 1e-12)
+                assert close(ge, np.array(we) * half, 1e-12, 1e-12)
+
+
+def test_subset_with_every_pair_is_betweenness():
+    # path_centrality.rs:32-35: S = T = every node, unordered pairs, distinct
+    # parallel edges give betweenness and edge betweenness
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        n, s, t, directed = random_case(rng, max_n=30)
+        w = rng.random(len(s)) + 0.5
+        g = OracleGraph(n, s, t, w, directed)
+        for kw in ({}, {"weighted": True}):
+            nodes, edges = g.betweenness_subset(**kw)
+            assert close(nodes, g.betweenness(**kw), 1e-12, 1e-12)
+            assert close(edges, g.edge_betweenness(**kw), 1e-12, 1e-12)
+
+
+def test_subset_errors():
+    g = OracleGraph(3, [0, 1], [1, 2], [1.0, -2.0], True)
+    with pytest.raises(OracleError, match=r"^node index 5 is out of range for a network "
+                                          r"with 3 nodes$"):
+        g.betweenness_subset(targets=[0, 5])
+    for kw, msg in (({"rescale": 3}, "unknown rescaling code 3"),
+                    ({"parallel": 2}, "unknown code 2 for parallel edges"),
+                    ({"edge_split": -1}, "unknown edge split code -1"),
+                    ({"tie": 2}, "unknown tie code 2"),
+                    ({"weights_as": 4}, "unknown weight kind code 4")):
+        with pytest.raises(OracleError, match="^" + msg + "$"):
+            g.betweenness_subset(**kw)
+    with pytest.raises(OracleError, match=r"^edge 1 has length -2; shortest-path betweenness "
+                                          r"needs strictly positive lengths$"):
+        g.betweenness_subset(weighted=True)
+    with pytest.raises(OracleError, match=r"^edge 1 has weight -2; weights read as strengths "
+                                          r"become lengths 1/w and must be strictly positive$"):
+        g.betweenness_subset(weighted=True, weights_as="strength")
+    with pytest.raises(OracleError, match=r"^the graph has no edge weights$"):
+        OracleGraph(2, [0], [1], None, True).betweenness_subset(weighted=True)
 
 
 # ---- closed forms and the distance-sum identity -------------------------------
