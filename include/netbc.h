@@ -70,6 +70,26 @@ int32_t net_edge_betweenness(const GraphHandle *h, int32_t weighted,
                              int32_t threads, double *out, char *err,
                              size_t err_len);
 
+/* Replaces net_betweenness_subset, netcapi.h:1957-1979 (crates/capi/src/
+ * centrality.rs:89-144, crates/engine/src/metrics/path_centrality.rs:
+ * 421-689). Betweenness for sources and targets: node values (n doubles)
+ * and, when `edges` is not NULL, edge values (m doubles). Node lists: NULL
+ * or a negative count means every node. rescale: 0 unordered pairs, 1
+ * NetworkX's betweenness_centrality (a source list is a sample), 2
+ * NetworkX's *_subset; parallel: 0 distinct paths, 1 collapsed (NetworkX);
+ * edge_split: 0 proportional, 1 NetworkX's subset edge split (edge values
+ * with 1 are not available on the device path and fail with a message);
+ * tie: 0 relative 1e-10, 1 exact. Node values are within 1e-12 of the
+ * reference's. `threads` is accepted and ignored. */
+int32_t net_betweenness_subset(const GraphHandle *h, const int64_t *sources,
+                               int64_t n_sources, const int64_t *targets,
+                               int64_t n_targets, int32_t weighted,
+                               int32_t weights_as, int32_t endpoints,
+                               int32_t normalized, int32_t rescale,
+                               int32_t parallel, int32_t edge_split,
+                               int32_t tie, int32_t threads, double *nodes,
+                               double *edges, char *err, size_t err_len);
+
 /* ---- device-path entry points --------------------------------------- */
 
 /* Run options. Zero-initialised means: FP64 dependencies, automatic batch

@@ -122,6 +122,10 @@ __device__ __forceinline__ uint32_t rl(uint32_t x, int j) { return rl32(x, j); }
 __device__ __forceinline__ double rl(double x, int j) {
     return __longlong_as_double((long long)rl64((uint64_t)__double_as_longlong(x), j));
 }
+// 1 for a target node (target == nullptr: every node), else 0.
+__device__ __forceinline__ double target_of(const uint8_t* __restrict__ target, uint32_t v) {
+    return target ? (double)target[v] : 1.0;
+}
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
@@ -853,8 +857,10 @@ struct Dep<double> {
     double s = 0.0;
     __device__ __forceinline__ void add(double x) { s += x; }
     __device__ __forceinline__ double value() const { return s; }
-    __device__ __forceinline__ static double coef(double delta, double sigma) {
-        return (1.0 + delta) / sigma;
+    // (t + delta) / sigma, t = 1 when the node is a target (every node is,
+    // except in runs for sources and targets, path_centrality.rs:457-459)
+    __device__ __forceinline__ static double coef(double t, double delta, double sigma) {
+        return (t + delta) / sigma;
     }
 };
 
@@ -869,8 +875,8 @@ struct Dep<float> {
         s = t;
     }
     __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-    __device__ __forceinline__ static float coef(double delta, double sigma) {
-        return (float)((1.0 + delta) / sigma);
+    __device__ __forceinline__ static float coef(double t, double delta, double sigma) {
+        return (float)((t + delta) / sigma);
     }
 };
 
@@ -1041,7 +1047,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     const uint64_t* __restrict__ succ,
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
                                     uint32_t gsz, size_t n, double* __restrict__ eacc,
-                                    size_t arcs) {
+                                    size_t arcs, const uint8_t* __restrict__ target) {
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
@@ -1063,7 +1069,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
         if ((M >> lane) & 1) {
             const size_t r = ((size_t)b * n + v) * kLanes + lane;
             const double delta = s * dvv;
-            coef[r] = Dep<C>::coef(delta, s);
+            coef[r] = Dep<C>::coef(target_of(target, v), delta, s);
             sig[r] = delta;
             smax = fmax(smax, s);
         }
@@ -1229,7 +1235,8 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
                                    uint32_t hbeg, uint32_t hend, const uint32_t* __restrict__ hpre,
                                    uint32_t hchunks, const double* __restrict__ part,
                                    const uint32_t* __restrict__ off, double* __restrict__ sig,
-                                   C* __restrict__ coef, Ctrl* ctrl, size_t n) {
+                                   C* __restrict__ coef, Ctrl* ctrl, size_t n,
+                                   const uint8_t* __restrict__ target) {
     const int lane = lane_id();
     double smax = 0.0;
     for (size_t h = global_wave(); h < hend - hbeg; h += wave_count()) {
@@ -1244,7 +1251,7 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
             size_t r = ((size_t)en.b * n + en.v) * kLanes + lane;
             double s = sig[r];
             double delta = s * dv;
-            coef[r] = Dep<C>::coef(delta, s);
+            coef[r] = Dep<C>::coef(target_of(target, en.v), delta, s);
             sig[r] = delta;
             smax = fmax(smax, s);
         }
@@ -1289,6 +1296,12 @@ struct RealPath {
     __device__ static void offer(Dist* p, Sum v) {
         atomicMin((unsigned long long*)p, (unsigned long long)__double_as_longlong(v));
     }
+};
+
+// Real lengths with ties only between equal lengths (Tie::Exact,
+// paths/mod.rs:96-104, NetworkX's rule).
+struct RealPathExact : RealPath {
+    __device__ static bool tie(Sum via, Dist d) { return via == d; }
 };
 
 // ---- integer lengths (Dial / delta-stepping with unit buckets) -------------
@@ -1564,7 +1577,7 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                                      const typename P::Dist* __restrict__ dist,
                                      const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                      C* __restrict__ coef, size_t n, double* __restrict__ eacc,
-                                     size_t arcs) {
+                                     size_t arcs, const uint8_t* __restrict__ target) {
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     const int lane = lane_id();
@@ -1624,7 +1637,7 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
         if (mine) {
             size_t r = (base + en.v) * kLanes + lane;
             double delta = s * dv.value();
-            coef[r] = Dep<C>::coef(delta, s);
+            coef[r] = Dep<C>::coef(target_of(target, en.v), delta, s);
             sig[r] = delta;
         }
     }
@@ -1655,7 +1668,8 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
                                 int slots, size_t n, const uint32_t* __restrict__ off,
                                 const uint32_t* __restrict__ idx, const double* __restrict__ lens,
                                 DjkState st, uint64_t* __restrict__ vis, double* __restrict__ sig,
-                                Ctrl* ctrl, double* __restrict__ eacc, size_t arcs) {
+                                Ctrl* ctrl, double* __restrict__ eacc, size_t arcs,
+                                const uint8_t* __restrict__ target) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= slots * kLanes) return;
     int64_t item = item0 + t;
@@ -1780,7 +1794,7 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
             }
         }
         double delta = sigma[v] * dv;
-        coef[v] = (1.0 + delta) / sigma[v];
+        coef[v] = (target_of(target, v) + delta) / sigma[v];
         size_t mi = (size_t)b * n + v;
         atomicOr((unsigned long long*)&vis[mi], 1ull << l);
         sig[mi * kLanes + l] = delta;
@@ -2054,6 +2068,73 @@ __global__ void fold_arcs_kernel(const double* __restrict__ eacc, int rows, size
     }
 }
 
+// Path endpoints (path_centrality.rs:480-492): every target t reached from
+// a source s != t adds 1 to t and 1 to s. endpoint_nodes_kernel adds to
+// ep[v], for a target v, the lanes of the super-batch that reached it
+// (including a lane whose source is v); endpoint_lanes_kernel counts, per
+// (slot, lane), the targets the lane reached (including its source when
+// that is a target), 64 nodes per wavefront: lane l tests bit l of the 64
+// loaded words in turn. endpoint_sources_kernel adds each lane's count to
+// its source, less 2 when the source is a target (the two self counts).
+// The counts are integers, so the order of the atomic adds does not matter.
+__global__ void endpoint_nodes_kernel(const uint64_t* __restrict__ vis, int slots, size_t n,
+                                      const uint8_t* __restrict__ target,
+                                      unsigned long long* __restrict__ ep) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        if (target && !target[v]) continue;
+        unsigned long long c = 0;
+        for (int b = 0; b < slots; b++) c += (unsigned long long)__popcll(vis[(size_t)b * n + v]);
+        if (c) ep[v] += c;
+    }
+}
+
+__global__ void endpoint_lanes_kernel(const uint64_t* __restrict__ vis, int slots, size_t n,
+                                      const uint8_t* __restrict__ target,
+                                      unsigned long long* __restrict__ lane_count) {
+    const int lane = lane_id();
+    const size_t chunks = (n + 63) / 64;
+    for (size_t w = global_wave(); w < chunks * (size_t)slots; w += wave_count()) {
+        const size_t b = w / chunks, v = (w % chunks) * 64 + lane;
+        uint64_t word = 0;
+        if (v < n && (!target || target[v])) word = vis[b * n + v];
+        unsigned long long c = 0;
+        for (int j = 0; j < 64; j++) c += (rl64(word, j) >> lane) & 1;
+        if (c) atomicAdd(&lane_count[b * kLanes + lane], c);
+    }
+}
+
+__global__ void endpoint_sources_kernel(const int64_t* __restrict__ list, int64_t item0,
+                                        int64_t k, int slots, const uint8_t* __restrict__ target,
+                                        unsigned long long* __restrict__ lane_count,
+                                        unsigned long long* __restrict__ ep) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots * kLanes) return;
+    const int64_t i = item0 + t;
+    if (i < k) {
+        const int64_t s = list ? list[i] : i;
+        const unsigned long long self = (!target || target[s]) ? 2ull : 0ull;
+        atomicAdd(&ep[s], lane_count[t] - self);  // modular: the sum stays exact
+    }
+    lane_count[t] = 0;
+}
+
+__global__ void add_counts_kernel(double* __restrict__ bc, const unsigned long long* __restrict__ ep,
+                                  size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        if (ep[v]) bc[v] += (double)ep[v];
+}
+
+// vals[v] *= flag[v] ? f_flagged : f_other (flag == nullptr: f_other).
+__global__ void scale_flagged_kernel(double* __restrict__ vals, size_t n,
+                                     const uint8_t* __restrict__ flag, double f_flagged,
+                                     double f_other) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        vals[v] *= (flag && flag[v]) ? f_flagged : f_other;
+}
+
 inline unsigned blocks_for_threads(size_t threads) {
     size_t b = (threads + kBlock - 1) / kBlock;
     return (unsigned)std::max<size_t>(1, std::min<size_t>(b, 1u << 20));
@@ -2289,7 +2370,8 @@ int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& 
 }  // namespace
 
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb);
+                 const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb,
+                 const SubsetRun* sub);
 
 namespace {
 
@@ -2308,6 +2390,8 @@ struct Engine {
     const int64_t* d_list;
     int64_t k;
     bool edge = false;  // add every arc's credit to ws.eacc
+    const uint8_t* target = nullptr;  // target flags (nullptr: every node)
+    bool tie_exact = false;           // real lengths tie only when equal
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -2492,6 +2576,9 @@ struct Engine {
         } else if (L.kind == kInteger) {
             if (edge) backward_dial<C, true, IntPath>(coef);
             else backward_dial<C, false, IntPath>(coef);
+        } else if (tie_exact) {
+            if (edge) backward_dial<C, true, RealPathExact>(coef);
+            else backward_dial<C, false, RealPathExact>(coef);
         } else {
             if (edge) backward_dial<C, true, RealPath>(coef);
             else backward_dial<C, false, RealPath>(coef);
@@ -2545,13 +2632,12 @@ struct Engine {
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
-                    n, ws.eacc.p, d.arcs);
+                    n, ws.eacc.p, d.arcs, target);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
                     ws.q.p, ws.hq.p, hb, he, ws.hpre.p, hc[dl], ws.part.p, d.out_off.p, ws.sig.p,
-                    coef,
-                    ws.ctrl.p, n);
+                    coef, ws.ctrl.p, n, target);
                 launch_check();
             }
             ws.sweep.mark(st);
@@ -2628,7 +2714,7 @@ struct Engine {
             dial_backward_kernel<C, kEdge, P>
                 <<<resident_blocks(dial_backward_kernel<C, kEdge, P>, qe - qb), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, d.out_off.p, d.out_idx.p, out_lengths<P>(), dist_rows<P>(),
-                    ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs);
+                    ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs, target);
             launch_check();
             ws.sweep.mark(st);
         }
@@ -2640,7 +2726,7 @@ struct Engine {
         ws.sweep.mark(st);
         dijkstra_kernel<<<slots, kLanes, 0, st>>>(
             d_list, item0, k, slots, n, d.out_off.p, d.out_idx.p, L.out_real.p, dst, ws.vis.p,
-            ws.sig.p, ws.ctrl.p, edge ? ws.eacc.p : nullptr, d.arcs);
+            ws.sig.p, ws.ctrl.p, edge ? ws.eacc.p : nullptr, d.arcs, target);
         launch_check();
         ws.sweep.mark(st);
     }
@@ -2779,7 +2865,8 @@ void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hi
 }  // namespace
 
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt_in, double* d_bc, RunDiag& diag, double* d_eb) {
+                 const RunOptions& opt_in, double* d_bc, RunDiag& diag, double* d_eb,
+                 const SubsetRun* sub) {
     const RunOptions opt = tuned_options(g, L, opt_in);
     hipStream_t st = g.stream;
     size_t n = g.n;
@@ -2806,6 +2893,9 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     }
     // real lengths with very unequal arcs: the reference's Dijkstra per thread
     const bool per_thread = L.kind == kReal && !real_bucketed(L, n);
+    if (per_thread && sub && sub->tie_exact)
+        throw Error("exact ties on real lengths whose longest arc exceeds 2,047 times the "
+                    "shortest are not available on the device path");
     const bool free_order = !opt.exact && k >= kReorderMin && !per_thread;
     if (free_order) {
         // Batch sources whose searches look alike (order_sources). Items are
@@ -2838,6 +2928,17 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         NETBC_HIP(hipMemsetAsync(ws.eacc.p, 0, (size_t)rows * arcs * sizeof(double), st));
     }
     Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k, edge};
+    const uint8_t* d_target = sub ? sub->d_target : nullptr;
+    const bool endpoints = sub && sub->endpoints;
+    eng.target = d_target;
+    eng.tie_exact = sub && sub->tie_exact;
+    DevBuf<unsigned long long> ep, lane_count;  // endpoint counts
+    if (endpoints) {
+        ep.alloc(n);
+        lane_count.alloc((size_t)slots * kLanes);
+        NETBC_HIP(hipMemsetAsync(ep.p, 0, n * sizeof(unsigned long long), st));
+        NETBC_HIP(hipMemsetAsync(lane_count.p, 0, (size_t)slots * kLanes * 8, st));
+    }
     int64_t gsize = (k + 63) / 64;
     NETBC_HIP(hipEventRecord(ws.ev0, st));
     for (int64_t item0 = 0; item0 < k; item0 += (int64_t)slots * kLanes) {
@@ -2847,6 +2948,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             for (;;) {
                 bool ok = L.kind == kUniform   ? eng.run_bfs(item0)
                           : L.kind == kInteger ? eng.run_bucketed<IntPath>(item0)
+                          : eng.tie_exact      ? eng.run_bucketed<RealPathExact>(item0)
                                                : eng.run_bucketed<RealPath>(item0);
                 if (ok) break;
                 // Level queue overflow: grow and repeat the super-batch from
@@ -2856,6 +2958,18 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             }
             if (opt.precision) eng.backward(ws.coef32.p);
             else eng.backward(ws.coef64.p);
+        }
+        if (endpoints) {
+            // before the row sums, which leave vis alone but come last
+            endpoint_nodes_kernel<<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22)),
+                                    kBlock, 0, st>>>(ws.vis.p, slots, n, d_target, ep.p);
+            endpoint_lanes_kernel<<<resident_blocks(endpoint_lanes_kernel,
+                                                    (n + 63) / 64 * (size_t)slots),
+                                    kBlock, 0, st>>>(ws.vis.p, slots, n, d_target, lane_count.p);
+            endpoint_sources_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0,
+                                      st>>>(d_list, item0, k, slots, d_target, lane_count.p,
+                                            ep.p);
+            NETBC_HIP(hipGetLastError());
         }
         if (free_order) {
             const uint32_t nsrc = eng.lv.size() > 1 ? eng.lv[1] : 0;
@@ -2875,6 +2989,11 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     }
     finish_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(d_bc, ws.acc.p, n);
     NETBC_HIP(hipGetLastError());
+    if (endpoints) {
+        add_counts_kernel<<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22)), kBlock, 0,
+                            st>>>(d_bc, ep.p, n);
+        NETBC_HIP(hipGetLastError());
+    }
     if (edge && arcs) {
         fold_arcs_kernel<<<blocks_for_threads(std::min<size_t>(arcs, (size_t)1 << 22)), kBlock, 0,
                            st>>>(ws.eacc.p, rows, arcs, g.dev.arc_edge.p, d_eb);
@@ -2916,6 +3035,14 @@ void finish_scores(Graph& g, bool normalized, double* d_bc, double* out) {
     }
     NETBC_HIP(hipMemcpyAsync(out, d_bc, n * sizeof(double), hipMemcpyDeviceToHost, st));
     NETBC_HIP(hipStreamSynchronize(st));
+}
+
+void scale_flagged(Graph& g, double* d_vals, size_t count, const uint8_t* d_flag,
+                   double f_flagged, double f_other) {
+    if (count == 0) return;
+    scale_flagged_kernel<<<blocks_for_threads(std::min<size_t>(count, (size_t)1 << 22)), kBlock,
+                           0, g.stream>>>(d_vals, count, d_flag, f_flagged, f_other);
+    NETBC_HIP(hipGetLastError());
 }
 
 void finish_edge_scores(Graph& g, bool normalized, double* d_eb, double* out) {

@@ -190,6 +190,250 @@ double graph_scale(uint64_t n, bool directed, bool normalized) {
     return scale;
 }
 
+// ---- betweenness for sources and targets ----------------------------------
+
+// A node list of net_betweenness_subset (centrality.rs:45-63): NULL or a
+// negative count means every node.
+void check_node_list(const netbc::Graph& g, const int64_t* nodes, int64_t k) {
+    if (!nodes || k < 0) return;
+    for (int64_t i = 0; i < k; i++)
+        if (nodes[i] < 0 || (uint64_t)nodes[i] >= g.n)
+            throw netbc::Error("node index " + std::to_string(nodes[i]) +
+                               " is out of range for a network with " + std::to_string(g.n) +
+                               " nodes");
+}
+
+// Edge lengths of the path graph (paths/mod.rs:175-192): 1, w or 1/w, with
+// PathGraph::new's strength check and require_positive's (paths/mod.rs:
+// 416-425), which names the first arc in row order whose length is not
+// positive (with collapse, the arc kept for its adjacent pair).
+std::vector<double> subset_lengths(const netbc::Graph& g, bool weighted, int weights_as,
+                                   bool collapse) {
+    const uint64_t m = g.m;
+    std::vector<double> len(m, 1.0);
+    if (!weighted) return len;
+    if (!g.has_weights) throw netbc::Error("the graph has no edge weights");
+    bool bad = false;
+    for (uint64_t e = 0; e < m; e++) {
+        const double w = g.w[e];
+        if (weights_as == 1) {
+            if (w <= 0.0)
+                throw netbc::Error("edge " + std::to_string(e) + " has weight " +
+                                   netbc::format_f64(w) +
+                                   "; weights read as strengths become lengths 1/w and must be "
+                                   "strictly positive");
+            len[e] = 1.0 / w;
+        } else {
+            len[e] = w;
+        }
+        bad |= !(len[e] > 0.0);
+    }
+    if (!bad) return len;
+    // arcs in row order: (tail, head, generation), generation = edge order,
+    // an undirected edge's reverse right after it
+    struct Arc { uint32_t t, h; uint64_t gen; uint32_t e; };
+    std::vector<Arc> arcs;
+    for (uint64_t e = 0; e < m; e++) {
+        arcs.push_back({g.src[e], g.tgt[e], arcs.size(), (uint32_t)e});
+        if (!g.directed && g.src[e] != g.tgt[e])
+            arcs.push_back({g.tgt[e], g.src[e], arcs.size(), (uint32_t)e});
+    }
+    std::sort(arcs.begin(), arcs.end(), [](const Arc& a, const Arc& b) {
+        return a.t != b.t ? a.t < b.t : a.h != b.h ? a.h < b.h : a.gen < b.gen;
+    });
+    for (size_t i = 0; i < arcs.size();) {
+        size_t j = i + 1;
+        if (collapse)
+            while (j < arcs.size() && arcs[j].t == arcs[i].t && arcs[j].h == arcs[i].h) j++;
+        uint32_t keep = arcs[i].e;  // the first arc of least length
+        for (size_t q = i + 1; q < j; q++)
+            if (len[arcs[q].e] < len[keep]) keep = arcs[q].e;
+        if (!(len[keep] > 0.0))
+            throw netbc::Error("edge " + std::to_string(keep) + " has length " +
+                               netbc::format_f64(len[keep]) +
+                               "; shortest-path betweenness needs strictly positive lengths");
+        i = j;
+    }
+    return len;
+}
+
+// The path graph with parallel arcs collapsed (paths/mod.rs:203-215): one
+// edge per adjacent pair (unordered in undirected graphs), the first of
+// least length; kept[i] is the input edge behind edge i.
+void collapse_graph(const netbc::Graph& g, const std::vector<double>& len, netbc::Graph& out,
+                    std::vector<uint32_t>& kept) {
+    const uint64_t m = g.m;
+    std::vector<std::pair<uint64_t, uint32_t>> key(m);
+    for (uint64_t e = 0; e < m; e++) {
+        uint64_t a = g.src[e], b = g.tgt[e];
+        if (!g.directed && a > b) std::swap(a, b);
+        key[e] = {a << 32 | b, (uint32_t)e};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    kept.clear();
+    for (size_t i = 0; i < m;) {
+        size_t j = i;
+        uint32_t best = key[i].second;
+        while (j < m && key[j].first == key[i].first) {
+            if (len[key[j].second] < len[best]) best = key[j].second;
+            j++;
+        }
+        kept.push_back(best);
+        i = j;
+    }
+    std::sort(kept.begin(), kept.end());
+    out.n = g.n;
+    out.m = kept.size();
+    out.directed = g.directed;
+    out.has_weights = g.has_weights;
+    out.src.resize(out.m);
+    out.tgt.resize(out.m);
+    if (g.has_weights) out.w.resize(out.m);
+    for (size_t i = 0; i < kept.size(); i++) {
+        out.src[i] = g.src[kept[i]];
+        out.tgt[i] = g.tgt[kept[i]];
+        if (g.has_weights) out.w[i] = g.w[kept[i]];
+    }
+}
+
+// NetworkX's _rescale (path_centrality.rs:596-649) as factors: values of
+// sources get f_src, others f_other (equal unless a sample is rescaled);
+// sampled < 0: no sample.
+void nx_factors(uint64_t n, bool normalized, bool directed, bool endpoints, int64_t sampled,
+                double& f_src, double& f_other) {
+    f_src = f_other = 1.0;
+    const double big_n = endpoints ? (double)n : (double)n - 1.0;
+    if (big_n < 2.0) return;
+    const double correction = directed ? 1.0 : 2.0;
+    const double k = sampled >= 0 ? (double)sampled : big_n;
+    if (sampled < 0 || endpoints) {
+        f_src = f_other = normalized ? 1.0 / (k * (big_n - 1.0)) : big_n / (k * correction);
+        return;
+    }
+    if (normalized) {
+        f_src = k > 1.0 ? 1.0 / ((k - 1.0) * (big_n - 1.0)) : NAN;
+        f_other = 1.0 / (k * (big_n - 1.0));
+    } else {
+        f_src = k > 1.0 ? big_n / ((k - 1.0) * correction) : NAN;
+        f_other = big_n / (k * correction);
+    }
+}
+
+// betweenness_subset, path_centrality.rs:421-593, through the batched
+// engine: searches, target-aware dependencies, endpoint counts, the
+// cross-source sums and the scaling run on the device; the collapsed
+// path graph is built on the host, and edge values of collapsed pairs
+// are shared among their parallel edges there (path_centrality.rs:651-689).
+void run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
+                const int64_t* targets, int64_t n_targets, int32_t weighted, int32_t weights_as,
+                int32_t endpoints, int32_t normalized, int32_t rescale, int32_t parallel,
+                int32_t tie, double* nodes, double* edges) {
+    std::lock_guard<std::mutex> lock(g.mu);
+    const bool collapse = parallel == 1;
+    const std::vector<double> len = subset_lengths(g, weighted != 0, weights_as, collapse);
+    const uint64_t n = g.n;
+    netbc::Graph derived;
+    std::vector<uint32_t> kept;
+    netbc::Graph* pg = &g;
+    if (collapse) {
+        collapse_graph(g, len, derived, kept);
+        pg = &derived;
+    }
+    prepare(*pg);
+    netbc::Lengths L;
+    netbc::classify_lengths(*pg, weighted != 0, weights_as, L);
+    const bool all_src = !sources || n_sources < 0;
+    const int64_t k = all_src ? (int64_t)n : n_sources;
+    netbc::DevBuf<uint8_t> tflag, sflag;
+    netbc::SubsetRun sub;
+    sub.endpoints = endpoints != 0;
+    sub.tie_exact = tie == 1;
+    if (targets && n_targets >= 0) {
+        std::vector<uint8_t> f(std::max<uint64_t>(n, 1), 0);
+        for (int64_t i = 0; i < n_targets; i++) f[targets[i]] = 1;
+        tflag.alloc(f.size());
+        NETBC_HIP(hipMemcpyAsync(tflag.p, f.data(), f.size(), hipMemcpyHostToDevice, pg->stream));
+        sub.d_target = tflag.p;
+    }
+    netbc::DevBuf<double> bc, eb;
+    bc.alloc(n);
+    if (edges) eb.alloc(pg->m);
+    netbc::RunDiag d;
+    netbc::run_batched(*pg, L, all_src ? nullptr : sources, k, netbc::RunOptions(), bc.p, d,
+                       edges ? eb.p : nullptr, &sub);
+    // scaling, path_centrality.rs:527-574
+    double fn_src, fn_other, fe, fe_unused;
+    const bool directed = g.directed;
+    if (rescale == 0) {
+        const double half = directed ? 1.0 : 0.5;
+        double np = 1.0, ep = 1.0;
+        if (normalized) {
+            const double nf = (double)n;
+            np = endpoints ? nf * (nf - 1.0) : (nf - 1.0) * (nf - 2.0);
+            ep = nf * (nf - 1.0);
+        }
+        fn_src = fn_other = np > 0.0 ? half / np : half;
+        fe = ep > 0.0 ? half / ep : half;
+    } else {
+        const int64_t sampled = rescale == 1 && !all_src ? k : -1;
+        nx_factors(n, normalized != 0, directed, rescale == 1 && endpoints, sampled, fn_src,
+                   fn_other);
+        nx_factors(n, normalized != 0, directed, true, sampled, fe, fe_unused);
+    }
+    if (n) {
+        if (fn_src != fn_other) {
+            std::vector<uint8_t> f(n, 0);
+            for (int64_t i = 0; i < k; i++) f[all_src ? i : sources[i]] = 1;
+            sflag.alloc(n);
+            NETBC_HIP(hipMemcpyAsync(sflag.p, f.data(), n, hipMemcpyHostToDevice, pg->stream));
+        }
+        netbc::scale_flagged(*pg, bc.p, n, sflag.p, fn_src, fn_other);
+        NETBC_HIP(hipMemcpyAsync(nodes, bc.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 pg->stream));
+    }
+    std::vector<double> ev(edges ? pg->m : 0);
+    if (edges && pg->m) {
+        netbc::scale_flagged(*pg, eb.p, pg->m, nullptr, fe, fe);
+        NETBC_HIP(hipMemcpyAsync(ev.data(), eb.p, pg->m * sizeof(double), hipMemcpyDeviceToHost,
+                                 pg->stream));
+    }
+    NETBC_HIP(hipStreamSynchronize(pg->stream));
+    if (!edges) return;
+    if (!collapse) {
+        std::copy(ev.begin(), ev.end(), edges);
+        return;
+    }
+    // share_among_parallel: a pair's value equally among its parallel edges
+    // of least length; other parallels and loops get 0
+    std::vector<double> vals(g.m, 0.0);
+    for (size_t i = 0; i < kept.size(); i++) vals[kept[i]] = ev[i];
+    std::vector<std::pair<uint64_t, uint32_t>> key(g.m);
+    for (uint64_t e = 0; e < g.m; e++) {
+        uint64_t a = g.src[e], b = g.tgt[e];
+        if (!directed && a > b) std::swap(a, b);
+        key[e] = {a << 32 | b, (uint32_t)e};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::fill(edges, edges + g.m, 0.0);
+    for (size_t i = 0; i < g.m;) {
+        size_t j = i;
+        while (j < g.m && key[j].first == key[i].first) j++;
+        const uint64_t a = key[i].first >> 32, b = key[i].first & 0xFFFFFFFFull;
+        if (a != b) {
+            double total = 0.0, shortest = INFINITY;
+            for (size_t q = i; q < j; q++) total += vals[key[q].second];
+            for (size_t q = i; q < j; q++) shortest = std::min(shortest, len[key[q].second]);
+            double ties = 0.0;
+            for (size_t q = i; q < j; q++) ties += len[key[q].second] == shortest;
+            for (size_t q = i; q < j; q++)
+                if (len[key[q].second] == shortest) edges[key[q].second] = total / ties;
+        }
+        i = j;
+    }
+}
+
 // Fills a host graph from edges, with net_graph_new's checks and texts
 // (graph.rs:142-234); `prefix` names the graph within a batch.
 void fill_graph(netbc::Graph& g, int64_t n, const int64_t* src, const int64_t* tgt,
@@ -290,6 +534,36 @@ NETBC_EXPORT int32_t net_edge_betweenness(const GraphHandle* h, int32_t weighted
         check_kind(weights_as);
         run_sources(g, nullptr, (int64_t)g.n, weighted, weights_as, normalized, nullptr, out,
                     nullptr, true);
+    });
+}
+
+NETBC_EXPORT int32_t net_betweenness_subset(const GraphHandle* h, const int64_t* sources,
+                                            int64_t n_sources, const int64_t* targets,
+                                            int64_t n_targets, int32_t weighted,
+                                            int32_t weights_as, int32_t endpoints,
+                                            int32_t normalized, int32_t rescale,
+                                            int32_t parallel, int32_t edge_split, int32_t tie,
+                                            int32_t threads, double* nodes, double* edges,
+                                            char* err, size_t err_len) {
+    (void)threads;
+    return guard(err, err_len, [&] {
+        netbc::Graph& g = handle(h);
+        check_node_list(g, sources, n_sources);
+        check_node_list(g, targets, n_targets);
+        check_kind(weights_as);
+        if (rescale < 0 || rescale > 2)
+            throw netbc::Error("unknown rescaling code " + std::to_string(rescale));
+        if (parallel < 0 || parallel > 1)
+            throw netbc::Error("unknown code " + std::to_string(parallel) + " for parallel edges");
+        if (edge_split < 0 || edge_split > 1)
+            throw netbc::Error("unknown edge split code " + std::to_string(edge_split));
+        if (tie < 0 || tie > 1) throw netbc::Error("unknown tie code " + std::to_string(tie));
+        if (edges && edge_split == 1)
+            throw netbc::Error("NetworkX's edge split of subset edge betweenness (edge_split 1) "
+                               "is not available on the device path");
+        if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
+        run_subset(g, sources, n_sources, targets, n_targets, weighted, weights_as, endpoints,
+                   normalized, rescale, parallel, tie, nodes, edges);
     });
 }
 

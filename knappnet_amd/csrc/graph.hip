@@ -377,6 +377,8 @@ std::string rust_f64(double x) {
 
 }  // namespace
 
+std::string format_f64(double x) { return rust_f64(x); }
+
 int sort_pairs_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t len, int bits,
                    hipStream_t st) {
     return radix_sort_pairs(k0, v0, k1, v1, len, bits, st);

@@ -162,6 +162,16 @@ void build_device_graph(Graph& g);
 // Throws Error with the reference's message on bad weights.
 void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L);
 
+// Runs for sources and targets (path_centrality.rs:421-593) on top of the
+// batched engine: the dependency of a node counts 1 for itself only when it
+// is a target, path endpoints can be counted, and real lengths can tie only
+// when equal (Tie::Exact, paths/mod.rs:96-104).
+struct SubsetRun {
+    const uint8_t* d_target = nullptr;  // device, n flags; nullptr: every node
+    bool endpoints = false;
+    bool tie_exact = false;
+};
+
 // Runs the batched engine over item positions [0, k) with sources
 // `sources` (nullptr: item i is source i) and writes unscaled sums into
 // d_bc (device, n doubles) in the reference's group order. With d_eb
@@ -169,7 +179,13 @@ void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L);
 // 174-232): every arc's credit summed over the sources, then the arcs of
 // each input edge added.
 void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
-                 const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb = nullptr);
+                 const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb = nullptr,
+                 const SubsetRun* sub = nullptr);
+
+// vals[v] *= flag[v] ? f_flagged : f_other on the graph's stream (flag:
+// device, may be nullptr).
+void scale_flagged(Graph& g, double* d_vals, size_t count, const uint8_t* d_flag,
+                   double f_flagged, double f_other);
 
 // Debug: per-source distances (hops or integer distances; UINT64_MAX when
 // unreached), sigma and delta for k sources, each n long, host buffers.
@@ -189,6 +205,10 @@ void ensure_stream(Graph& g);
 // keys (graph.hip). Returns 0 if the result is in (k0, v0), 1 if in (k1, v1).
 int sort_pairs_u64(uint64_t* k0, uint32_t* v0, uint64_t* k1, uint32_t* v1, size_t len, int bits,
                    hipStream_t st);
+
+// A double as Rust's Display prints it (shortest round trip, positional),
+// for error messages that quote a weight or a length.
+std::string format_f64(double x);
 
 // Bits needed for node ids 0..n-1 (at least 1).
 int node_bits(uint64_t n);

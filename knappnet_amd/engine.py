@@ -92,6 +92,31 @@ class Graph:
             raise EngineError(err.value.decode())
         return out[:m]
 
+    def betweenness_subset(self, sources=None, targets=None, weighted: bool = False,
+                           weights_as: str = "distance", endpoints: bool = False,
+                           normalized: bool = False, rescale: int = 0, parallel: int = 0,
+                           edge_split: int = 0, tie: int = 0, edges: bool = True):
+        """``net_betweenness_subset``: (node values, edge values or None).
+        sources / targets None: every node."""
+        def lst(x):
+            if x is None:
+                return None, -1, None
+            a = _i64(x)
+            return _ptr(a, c_int64), len(a), a
+        sp, ns, _sa = lst(sources)
+        tp, nt, _ta = lst(targets)
+        m = self.edge_count()
+        nodes = np.zeros(max(self.n, 1), dtype=np.float64)
+        ev = np.zeros(max(m, 1), dtype=np.float64) if edges else None
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if self.lib.net_betweenness_subset(
+                self.handle, sp, ns, tp, nt, int(bool(weighted)), kind, int(bool(endpoints)),
+                int(bool(normalized)), int(rescale), int(parallel), int(edge_split), int(tie), 0,
+                _ptr(nodes, c_double), None if ev is None else _ptr(ev, c_double), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return nodes[: self.n], None if ev is None else ev[:m]
+
     def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, precision,
                       slots, direction, alpha, exact, sweep_group):
         out = np.zeros(max(length, 1), dtype=np.float64)
