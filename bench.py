@@ -20,7 +20,11 @@ The printed JSON line carries:
 ``--edge`` times edge betweenness (``netbc_edge_betweenness_sources``)
 on the same workload instead; its roofline keeps the node sweep's byte
 model, a lower bound for the edge sweep (which also updates one 8-byte
-credit per slot and arc it meets).
+credit per slot and arc it meets). ``--subset`` times betweenness for
+sources and targets (``netbc_betweenness_subset_ex``: the workload's
+sources, targets every third node, NetworkX's subset rescaling, node
+values only); its CPU baseline is the oracle's single-threaded
+restatement.
 """
 from __future__ import annotations
 
@@ -39,6 +43,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("GTEPS for exact betweenness (sources×edges/s), FP32-δ vs FP64 at rel err "
           "≤1e-6")
 METRIC_EDGE = "GTEPS for exact edge betweenness (sources×edges/s)"
+METRIC_SUBSET = "GTEPS for exact betweenness for sources and targets (sources×edges/s)"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -187,6 +192,8 @@ def main() -> None:
                     help="target CPU time of the sampled baseline (0 = skip)")
     ap.add_argument("--edge", action="store_true",
                     help="edge betweenness (net_edge_betweenness) instead of node betweenness")
+    ap.add_argument("--subset", action="store_true",
+                    help="betweenness for sources and targets (every third node a target)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -213,8 +220,19 @@ def main() -> None:
         config = dict(config, workload=config["workload"] + f"_first{len(sources)}")
     if args.edge:
         config = dict(config, workload=config["workload"] + "_edges")
+    if args.subset:
+        config = dict(config, workload=config["workload"] + "_targets_every_third",
+                      rescale="networkx_subset")
     g = Graph(el.n, el.src, el.tgt, el.weights, el.directed)
     run = g.edge_betweenness_sources if args.edge else g.betweenness_sources
+    targets = np.arange(0, el.n, 3)
+    if args.subset:
+        def run(sources=None, weighted=False, precision=0, slots=0, direction=0, alpha=0.0):
+            nodes, _, d = g.betweenness_subset(
+                sources=sources, targets=targets, weighted=weighted, rescale=2, edges=False,
+                options=dict(precision=precision, slots=slots, direction=direction,
+                             alpha=alpha))
+            return nodes, d
     k = el.n if sources is None else len(sources)
     print(f"# device: {device_info()}; workload {config['workload']} n={el.n} m={el.m} "
           f"sources={k} (generated in {gen_s:.1f}s)", flush=True)
@@ -274,9 +292,13 @@ def main() -> None:
         from oracle.oracle import OracleGraph, available_cores
         o = OracleGraph(el.n, el.src, el.tgt, el.weights, el.directed)
         orun = o.edge_betweenness_sources if args.edge else o.betweenness_sources
+        if args.subset:
+            def orun(sample, weighted=False, threads=0):
+                return o.betweenness_subset(sources=sample, targets=targets, weighted=weighted,
+                                            rescale=2)[0]
         pool = np.arange(el.n) if sources is None else np.asarray(sources)
-        cores = available_cores()
-        take = min(len(pool), 256)
+        cores = 1 if args.subset else available_cores()
+        take = min(len(pool), 256 if not args.subset else 16)
         while True:
             stride = max(1, len(pool) // take)
             sample = pool[::stride][:take]
@@ -298,14 +320,14 @@ def main() -> None:
         cpu_gteps = len(sample) * el.m / ct / 1e9
         cpu = {"value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": cores,
                "kind": "port", "seconds": round(ct, 3),
-               "sample": f"{len(sample)} of {k} sources (every {stride}th), all {cores} host "
-                         f"threads, CPU: {cpu_model()}",
+               "sample": f"{len(sample)} of {k} sources (every {stride}th), {cores} host "
+                         f"thread{'s' if cores > 1 else ''}, CPU: {cpu_model()}",
                "check": "bit-exact" if exact else f"max rel err {rel:.3e}"}
         print(f"# cpu sample: {len(sample)} sources in {ct:.2f}s -> {cpu_gteps:.3f} GTEPS; "
               f"device vs oracle on sample: {cpu['check']}", flush=True)
 
     precision_check = None
-    if not args.no_compare and args.precision == 0 and not args.edge:
+    if not args.no_compare and args.precision == 0 and not args.edge and not args.subset:
         # The batched FP32-dependency form against FP64 (error) and against
         # the plain FP64 form: sources in the reference's order and sums in
         # its order, bit-identical to the CPU engine.
@@ -334,7 +356,7 @@ def main() -> None:
         print(f"# precision check: {precision_check}", flush=True)
 
     line = {
-        "metric": METRIC_EDGE if args.edge else METRIC,
+        "metric": METRIC_EDGE if args.edge else METRIC_SUBSET if args.subset else METRIC,
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,

@@ -163,6 +163,19 @@ int32_t netbc_edge_betweenness_sources(const GraphHandle *h,
                                        netbc_diagnostics *diag, char *err,
                                        size_t err_len);
 
+/* net_betweenness_subset with run options and diagnostics (both may be
+ * NULL). No counterpart in netcapi.h. */
+int32_t netbc_betweenness_subset_ex(const GraphHandle *h,
+                                    const int64_t *sources, int64_t n_sources,
+                                    const int64_t *targets, int64_t n_targets,
+                                    int32_t weighted, int32_t weights_as,
+                                    int32_t endpoints, int32_t normalized,
+                                    int32_t rescale, int32_t parallel,
+                                    int32_t edge_split, int32_t tie,
+                                    const netbc_options *opt, double *nodes,
+                                    double *edges, netbc_diagnostics *diag,
+                                    char *err, size_t err_len);
+
 /* Betweenness of many independent graphs in one call (null-model and
  * permutation replicates). Graph i has node_counts[i] nodes and
  * edge_counts[i] edges, listed one graph after another in src/tgt (node

@@ -82,6 +82,11 @@ def device() -> ctypes.CDLL:
             c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(c_double),
             POINTER(c_double), c_char_p, c_size_t]
         lib.net_betweenness_subset.restype = c_int32
+        lib.netbc_betweenness_subset_ex.argtypes = [
+            c_void_p, POINTER(c_int64), c_int64, POINTER(c_int64), c_int64, c_int32, c_int32,
+            c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(Options),
+            POINTER(c_double), POINTER(c_double), POINTER(Diagnostics), c_char_p, c_size_t]
+        lib.netbc_betweenness_subset_ex.restype = c_int32
         lib.netbc_edge_betweenness_sources.argtypes = [
             c_void_p, POINTER(c_int64), c_int64, c_int32, c_int32, c_int32,
             POINTER(Options), POINTER(c_double), POINTER(Diagnostics), c_char_p, c_size_t]

@@ -325,10 +325,11 @@ void nx_factors(uint64_t n, bool normalized, bool directed, bool endpoints, int6
 // cross-source sums and the scaling run on the device; the collapsed
 // path graph is built on the host, and edge values of collapsed pairs
 // are shared among their parallel edges there (path_centrality.rs:651-689).
-void run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
-                const int64_t* targets, int64_t n_targets, int32_t weighted, int32_t weights_as,
-                int32_t endpoints, int32_t normalized, int32_t rescale, int32_t parallel,
-                int32_t tie, double* nodes, double* edges) {
+netbc::RunDiag run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
+                          const int64_t* targets, int64_t n_targets, int32_t weighted,
+                          int32_t weights_as, int32_t endpoints, int32_t normalized,
+                          int32_t rescale, int32_t parallel, int32_t tie,
+                          const netbc_options* opt, double* nodes, double* edges) {
     std::lock_guard<std::mutex> lock(g.mu);
     const bool collapse = parallel == 1;
     const std::vector<double> len = subset_lengths(g, weighted != 0, weights_as, collapse);
@@ -360,7 +361,7 @@ void run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
     bc.alloc(n);
     if (edges) eb.alloc(pg->m);
     netbc::RunDiag d;
-    netbc::run_batched(*pg, L, all_src ? nullptr : sources, k, netbc::RunOptions(), bc.p, d,
+    netbc::run_batched(*pg, L, all_src ? nullptr : sources, k, options(opt), bc.p, d,
                        edges ? eb.p : nullptr, &sub);
     // scaling, path_centrality.rs:527-574
     double fn_src, fn_other, fe, fe_unused;
@@ -399,10 +400,10 @@ void run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
                                  pg->stream));
     }
     NETBC_HIP(hipStreamSynchronize(pg->stream));
-    if (!edges) return;
+    if (!edges) return d;
     if (!collapse) {
         std::copy(ev.begin(), ev.end(), edges);
-        return;
+        return d;
     }
     // share_among_parallel: a pair's value equally among its parallel edges
     // of least length; other parallels and loops get 0
@@ -432,6 +433,36 @@ void run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
         }
         i = j;
     }
+    return d;
+}
+
+// The checks of net_betweenness_subset in the reference's order
+// (centrality.rs:108-136), then the run.
+void subset_entry(const GraphHandle* h, const int64_t* sources, int64_t n_sources,
+                  const int64_t* targets, int64_t n_targets, int32_t weighted, int32_t weights_as,
+                  int32_t endpoints, int32_t normalized, int32_t rescale, int32_t parallel,
+                  int32_t edge_split, int32_t tie, const netbc_options* opt, double* nodes,
+                  double* edges, netbc_diagnostics* diag) {
+    netbc::Graph& g = handle(h);
+    check_node_list(g, sources, n_sources);
+    check_node_list(g, targets, n_targets);
+    check_kind(weights_as);
+    if (rescale < 0 || rescale > 2)
+        throw netbc::Error("unknown rescaling code " + std::to_string(rescale));
+    if (parallel < 0 || parallel > 1)
+        throw netbc::Error("unknown code " + std::to_string(parallel) + " for parallel edges");
+    if (edge_split < 0 || edge_split > 1)
+        throw netbc::Error("unknown edge split code " + std::to_string(edge_split));
+    if (tie < 0 || tie > 1) throw netbc::Error("unknown tie code " + std::to_string(tie));
+    check_options(opt);
+    if (edges && edge_split == 1)
+        throw netbc::Error("NetworkX's edge split of subset edge betweenness (edge_split 1) "
+                           "is not available on the device path");
+    if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
+    netbc::RunDiag d = run_subset(g, sources, n_sources, targets, n_targets, weighted,
+                                  weights_as, endpoints, normalized, rescale, parallel, tie, opt,
+                                  nodes, edges);
+    fill_diag(diag, d, 0.0);
 }
 
 // Fills a host graph from edges, with net_graph_new's checks and texts
@@ -547,23 +578,21 @@ NETBC_EXPORT int32_t net_betweenness_subset(const GraphHandle* h, const int64_t*
                                             char* err, size_t err_len) {
     (void)threads;
     return guard(err, err_len, [&] {
-        netbc::Graph& g = handle(h);
-        check_node_list(g, sources, n_sources);
-        check_node_list(g, targets, n_targets);
-        check_kind(weights_as);
-        if (rescale < 0 || rescale > 2)
-            throw netbc::Error("unknown rescaling code " + std::to_string(rescale));
-        if (parallel < 0 || parallel > 1)
-            throw netbc::Error("unknown code " + std::to_string(parallel) + " for parallel edges");
-        if (edge_split < 0 || edge_split > 1)
-            throw netbc::Error("unknown edge split code " + std::to_string(edge_split));
-        if (tie < 0 || tie > 1) throw netbc::Error("unknown tie code " + std::to_string(tie));
-        if (edges && edge_split == 1)
-            throw netbc::Error("NetworkX's edge split of subset edge betweenness (edge_split 1) "
-                               "is not available on the device path");
-        if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
-        run_subset(g, sources, n_sources, targets, n_targets, weighted, weights_as, endpoints,
-                   normalized, rescale, parallel, tie, nodes, edges);
+        subset_entry(h, sources, n_sources, targets, n_targets, weighted, weights_as, endpoints,
+                     normalized, rescale, parallel, edge_split, tie, nullptr, nodes, edges,
+                     nullptr);
+    });
+}
+
+NETBC_EXPORT int32_t netbc_betweenness_subset_ex(
+    const GraphHandle* h, const int64_t* sources, int64_t n_sources, const int64_t* targets,
+    int64_t n_targets, int32_t weighted, int32_t weights_as, int32_t endpoints,
+    int32_t normalized, int32_t rescale, int32_t parallel, int32_t edge_split, int32_t tie,
+    const netbc_options* opt, double* nodes, double* edges, netbc_diagnostics* diag, char* err,
+    size_t err_len) {
+    return guard(err, err_len, [&] {
+        subset_entry(h, sources, n_sources, targets, n_targets, weighted, weights_as, endpoints,
+                     normalized, rescale, parallel, edge_split, tie, opt, nodes, edges, diag);
     });
 }
 

@@ -95,9 +95,11 @@ class Graph:
     def betweenness_subset(self, sources=None, targets=None, weighted: bool = False,
                            weights_as: str = "distance", endpoints: bool = False,
                            normalized: bool = False, rescale: int = 0, parallel: int = 0,
-                           edge_split: int = 0, tie: int = 0, edges: bool = True):
-        """``net_betweenness_subset``: (node values, edge values or None).
-        sources / targets None: every node."""
+                           edge_split: int = 0, tie: int = 0, edges: bool = True,
+                           options: dict | None = None):
+        """``net_betweenness_subset``: (node values, edge values or None);
+        with ``options`` (precision, slots, ...), ``netbc_betweenness_subset_ex``
+        and a third item, the diagnostics. sources / targets None: every node."""
         def lst(x):
             if x is None:
                 return None, -1, None
@@ -110,12 +112,22 @@ class Graph:
         ev = np.zeros(max(m, 1), dtype=np.float64) if edges else None
         err = ctypes.create_string_buffer(ERR_LEN)
         kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
-        if self.lib.net_betweenness_subset(
-                self.handle, sp, ns, tp, nt, int(bool(weighted)), kind, int(bool(endpoints)),
-                int(bool(normalized)), int(rescale), int(parallel), int(edge_split), int(tie), 0,
-                _ptr(nodes, c_double), None if ev is None else _ptr(ev, c_double), err, ERR_LEN):
+        args = (self.handle, sp, ns, tp, nt, int(bool(weighted)), kind, int(bool(endpoints)),
+                int(bool(normalized)), int(rescale), int(parallel), int(edge_split), int(tie))
+        outs = (_ptr(nodes, c_double), None if ev is None else _ptr(ev, c_double))
+        if options is None:
+            if self.lib.net_betweenness_subset(*args, 0, *outs, err, ERR_LEN):
+                raise EngineError(err.value.decode())
+            return nodes[: self.n], None if ev is None else ev[:m]
+        o = dict(precision=0, slots=0, direction=0, exact=False, alpha=0.0, sweep_group=0)
+        o.update(options)
+        opt = _native.Options(o["precision"], o["slots"], o["direction"], int(bool(o["exact"])),
+                              o["alpha"], o["sweep_group"], 0)
+        diag = _native.Diagnostics()
+        if self.lib.netbc_betweenness_subset_ex(*args, ctypes.byref(opt), *outs,
+                                                ctypes.byref(diag), err, ERR_LEN):
             raise EngineError(err.value.decode())
-        return nodes[: self.n], None if ev is None else ev[:m]
+        return nodes[: self.n], None if ev is None else ev[:m], diag.as_dict()
 
     def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, precision,
                       slots, direction, alpha, exact, sweep_group):
