@@ -77,10 +77,12 @@ int32_t net_edge_betweenness(const GraphHandle *h, int32_t weighted,
  * or a negative count means every node. rescale: 0 unordered pairs, 1
  * NetworkX's betweenness_centrality (a source list is a sample), 2
  * NetworkX's *_subset; parallel: 0 distinct paths, 1 collapsed (NetworkX);
- * edge_split: 0 proportional, 1 NetworkX's subset edge split (edge values
- * with 1 are not available on the device path and fail with a message);
- * tie: 0 relative 1e-10, 1 exact. Node values are within 1e-12 of the
- * reference's. `threads` is accepted and ignored. */
+ * edge_split: 0 proportional, 1 NetworkX's subset edge split (a second
+ * search pass carries its edge dependencies); tie: 0 relative 1e-10, 1
+ * exact. Values are within 1e-12 of the reference's. `threads` is accepted
+ * and ignored. Real lengths whose longest arc exceeds about 2,047 times
+ * the shortest support neither exact ties nor the NetworkX edge split on
+ * the device path (the call fails with a message). */
 int32_t net_betweenness_subset(const GraphHandle *h, const int64_t *sources,
                                int64_t n_sources, const int64_t *targets,
                                int64_t n_targets, int32_t weighted,

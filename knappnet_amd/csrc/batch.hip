@@ -386,7 +386,8 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
                                                   uint64_t* __restrict__ succ,
                                                   double* __restrict__ sig, size_t n,
                                                   unsigned long long& arcs_seen,
-                                                  unsigned long long& hits_seen) {
+                                                  unsigned long long& hits_seen,
+                                                  uint32_t* __restrict__ npr = nullptr) {
     const int lane = lane_id();
     uint32_t T;
     const uint32_t pre = wave_prefix(deg, T);
@@ -394,16 +395,23 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
     int curc = -1;  // pair whose hits are being summed (wave-uniform)
     uint64_t ncur = 0;
     double acc = 0.0;
+    uint32_t cnt = 0;  // predecessor arcs in this lane (npr != nullptr)
     auto close_pair = [&]() {
         if (curc < 0) return;
         const uint32_t xs = rl32(x, curc), bs = rl32(b, curc);
         if ((ncur >> lane) & 1) {
-            double* dst = &sig[((size_t)bs * n + xs) * kLanes + lane];
+            const size_t r = ((size_t)bs * n + xs) * kLanes + lane;
+            double* dst = &sig[r];
             if (kAdd) unsafeAtomicAdd(dst, acc);
             else *dst = acc;
+            if (npr) {
+                if (kAdd) atomicAdd(&npr[r], cnt);
+                else npr[r] = cnt;
+            }
         }
         if (lane == curc) mine = ncur;
         acc = 0.0;
+        cnt = 0;
         ncur = 0;
     };
     if (lane == 0) NETBC_COUNT(arcs_seen, T);
@@ -452,6 +460,7 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
                 }
                 ncur |= hm[k];
                 acc += val[k];
+                cnt += (uint32_t)((hm[k] >> lane) & 1);
             }
         }
     }
@@ -618,7 +627,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
                                const uint64_t* __restrict__ vis, uint64_t* __restrict__ succ,
                                uint2* __restrict__ touched, uint32_t* __restrict__ tlen,
                                const uint64_t* __restrict__ flanes, Ctrl* ctrl, uint32_t gsz,
-                               size_t n) {
+                               size_t n, uint32_t* __restrict__ npr) {
     const int lane = lane_id();
     const uint64_t lt = lanemask_lt(lane);
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
@@ -640,7 +649,8 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
         const uint32_t deg = N ? ht.z - ht.y : 0;
         NETBC_COUNT(c6, N != 0);
         const uint64_t got = gather_counts<true>(ht.x, b, N, ht.y, deg, in_idx, cur,
-                                                 bottom_up ? succ : nullptr, sig, n, c4, c5);
+                                                 bottom_up ? succ : nullptr, sig, n, c4, c5,
+                                                 npr);
         if (bottom_up) {
             bool first = false;
             if (got)
@@ -698,7 +708,7 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
                                   const uint8_t* __restrict__ heavy_flag, QEnt* __restrict__ q,
                                   uint32_t qcap, uint32_t* __restrict__ hq, Ctrl* ctrl,
                                   int64_t left, unsigned long long* __restrict__ front_next,
-                                  size_t n) {
+                                  size_t n, uint32_t* __restrict__ npr) {
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) *tlen_next = 0;
     const uint32_t T = *tlen_cur;
@@ -719,7 +729,8 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
             i1 = in_off[te.x + 1];
             if (!hub_flag[te.x]) deg = i1 - i0;
         }
-        (void)gather_counts<false>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig, n, c4, c5);
+        (void)gather_counts<false>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig, n, c4, c5,
+                                   npr);
         if (ok) {
             cand[p] = 0;
             const uint64_t vn = vis[p] | N;
@@ -776,7 +787,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
                                 unsigned long long* __restrict__ front_next,
                                 unsigned long long* __restrict__ front_done,
                                 const uint64_t* __restrict__ flanes, int slots, int64_t left,
-                                size_t n) {
+                                size_t n, uint32_t* __restrict__ npr) {
     const int lane = lane_id();
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nth = (size_t)gridDim.x * blockDim.x;
@@ -813,7 +824,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         }
         NETBC_COUNT(c6, deg != 0);
         const uint64_t mine =
-            gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5);
+            gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5, npr);
         if (mine) {
             const uint64_t vn = V | mine;
             vis[p] = vn;
@@ -1033,8 +1044,12 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
 // dependency (delta = 0). Heavy rows (listed in hq) are split into chunk
 // tasks that each write a per-lane partial sum (part), added in chunk order
 // by bfs_combine_kernel; with split == 0 every row is summed whole. With
-// kEdge every arc's credit is also added to eacc (edge_flush).
-template <class C, bool kEdge>
+// kEdge every arc's credit is also added to eacc (edge_flush). With kQuirk
+// (NetworkX's subset edge split, path_centrality.rs:465-474) the rows hold
+// E_w instead of coefficients, E_w = (1 + de_w) / sigma_w for a target and
+// de_w / np_w otherwise (np_w: w's predecessor arcs, npr), the arc v -> w
+// carries sigma_v E_w or E_w, and de_v is their sum; rows are whole.
+template <class C, bool kEdge, bool kQuirk = false>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level, int rebuild,
                                     const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
@@ -1047,7 +1062,8 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     const uint64_t* __restrict__ succ,
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
                                     uint32_t gsz, size_t n, double* __restrict__ eacc,
-                                    size_t arcs, const uint8_t* __restrict__ target) {
+                                    size_t arcs, const uint8_t* __restrict__ target,
+                                    uint32_t* __restrict__ npr) {
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
@@ -1063,14 +1079,24 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             atomicAnd((unsigned long long*)&W[(size_t)e.b * n + e.v], ~e.m);
         }
     }
-    split = split && !last_level;
+    split = split && !last_level && !kQuirk;
     // delta, coefficient and path-count bookkeeping of lane l of entry (v, b, M)
-    auto finish = [&](uint32_t v, uint32_t b, uint64_t M, double s, double dvv) {
+    auto finish = [&](uint32_t v, uint32_t b, uint64_t M, double s, double dvv,
+                      [[maybe_unused]] double dqv) {
         if ((M >> lane) & 1) {
             const size_t r = ((size_t)b * n + v) * kLanes + lane;
-            const double delta = s * dvv;
-            coef[r] = Dep<C>::coef(target_of(target, v), delta, s);
-            sig[r] = delta;
+            if constexpr (kQuirk) {
+                const double de = s * dvv + dqv;
+                const uint32_t np = npr[r];
+                npr[r] = 0;  // rows start at zero for the next super-batch
+                coef[r] = (C)(target_of(target, v) != 0.0 ? (1.0 + de) / s
+                                                          : (np ? de / (double)np : 0.0));
+                sig[r] = de;
+            } else {
+                const double delta = s * dvv;
+                coef[r] = Dep<C>::coef(target_of(target, v), delta, s);
+                sig[r] = delta;
+            }
             smax = fmax(smax, s);
         }
     };
@@ -1120,13 +1146,15 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
         if (lane == 0) NETBC_COUNT(c5, T);
         bool done = !live;
         int cur = -1;       // entry whose successors are being summed (wave-uniform)
-        Dep<C> dv;
+        Dep<C> dv, dq;      // dq: kQuirk, successors that are not targets
         double sv = 0.0;    // its path counts, loaded when it starts
         auto close_entry = [&]() {
             if (cur < 0) return;
-            finish(rl32(en.v, cur), rl32(en.b, cur), rl64(en.m, cur), sv, dv.value());
+            finish(rl32(en.v, cur), rl32(en.b, cur), rl64(en.m, cur), sv, dv.value(),
+                   dq.value());
             if (lane == cur) done = true;
             dv = Dep<C>();
+            dq = Dep<C>();
         };
         auto open_entry = [&](int c) {
             cur = c;
@@ -1156,6 +1184,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                 bool use[kAhead];
                 int hc[kAhead];
                 uint32_t ak[kAhead], bk[kAhead];
+                [[maybe_unused]] bool tk[kAhead];
 #pragma unroll
                 for (int k = 0; k < kAhead; k++) {
                     use[k] = false;
@@ -1163,6 +1192,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                     hc[k] = -1;
                     ak[k] = kNone;
                     bk[k] = 0;
+                    tk[k] = true;
                     if (bal) {
                         const int j = __ffsll((unsigned long long)bal) - 1;
                         bal &= bal - 1;
@@ -1171,6 +1201,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                         hc[k] = (int)rl32((uint32_t)c, j);
                         ak[k] = rl32(arc, j);
                         bk[k] = bb;
+                        if constexpr (kQuirk) tk[k] = target_of(target, xx) != 0.0;
                         use[k] = (m2 >> lane) & 1;
                         if (use[k]) val[k] = coef[((size_t)bb * n + xx) * kLanes + lane];
                     }
@@ -1185,8 +1216,13 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                         open_entry(hc[k]);
                     }
                     if (use[k]) {
-                        dv.add(val[k]);
-                        if constexpr (kEdge) cv[k] = sv * (double)val[k];
+                        if (!kQuirk || tk[k]) {
+                            dv.add(val[k]);
+                            if constexpr (kEdge) cv[k] = sv * (double)val[k];
+                        } else {
+                            dq.add(val[k]);
+                            cv[k] = (double)val[k];
+                        }
                     }
                 }
                 if constexpr (kEdge) edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
@@ -1216,7 +1252,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             for (int k = 0; k < kAhead; k++) {
                 if (ec[k] < 0) break;
                 finish(rl32(en.v, ec[k]), rl32(en.b, ec[k]), rl64(en.m, ec[k]), s[k],
-                       Dep<C>().value());
+                       Dep<C>().value(), 0.0);
             }
         }
     }
@@ -1407,7 +1443,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                                  typename P::Dist* __restrict__ dist, uint64_t* __restrict__ vis,
                                  double* __restrict__ sig, QEnt* __restrict__ q, uint32_t qcap,
                                  Ctrl* ctrl, unsigned long long* __restrict__ next_bucket,
-                                 size_t n) {
+                                 size_t n, uint32_t* __restrict__ npr) {
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     __shared__ QEnt qbuf[kBlock / 64][kTouchBuf / 2];
@@ -1443,6 +1479,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         const bool mine = (M >> lane) & 1;
         if (!first_step) {
             double acc = 0.0;
+            uint32_t npred = 0;  // predecessor arcs in this lane
             const uint32_t a0 = in_off[x], a1 = in_off[x + 1];
             for (uint32_t c = a0; c < a1; c += 64) {
                 const uint32_t a = c + lane;
@@ -1479,11 +1516,15 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                     }
 #pragma unroll
                     for (int h = 0; h < kAhead; h++)
-                        if (use[h] && P::tie(P::add(du[h], lu[h]), dx)) acc += su[h];
+                        if (use[h] && P::tie(P::add(du[h], lu[h]), dx)) {
+                            acc += su[h];
+                            npred++;
+                        }
                 }
             }
             if (mine) {
                 sig[(base + x) * kLanes + lane] = acc;
+                if (npr) npr[(base + x) * kLanes + lane] = npred;
                 smax = fmax(smax, acc);
             }
             if (lane == 0) vis[base + x] |= M;
@@ -1569,7 +1610,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
 // offer dist[v] + length ties the head's distance add its coefficient, in
 // CSR order (up to kAhead rows loaded before they are added). With kEdge
 // every arc's credit also goes to eacc (edge_flush).
-template <class C, bool kEdge, class P>
+template <class C, bool kEdge, class P, bool kQuirk = false>
 __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                      const uint32_t* __restrict__ off,
                                      const uint32_t* __restrict__ idx,
@@ -1577,7 +1618,8 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                                      const typename P::Dist* __restrict__ dist,
                                      const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                      C* __restrict__ coef, size_t n, double* __restrict__ eacc,
-                                     size_t arcs, const uint8_t* __restrict__ target) {
+                                     size_t arcs, const uint8_t* __restrict__ target,
+                                     uint32_t* __restrict__ npr) {
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     const int lane = lane_id();
@@ -1588,7 +1630,7 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
         const bool mine = (M >> lane) & 1;
         const double s = mine ? sig[(base + en.v) * kLanes + lane] : 0.0;
         const Dist dv0 = mine ? dist[(base + en.v) * kLanes + lane] : Dist(0);
-        Dep<C> dv;
+        Dep<C> dv, dq;  // dq: kQuirk (as bfs_backward_kernel), non-target successors
         uint32_t a0 = off[en.v], a1 = off[en.v + 1];
         for (uint32_t c = a0; c < a1; c += 64) {
             uint32_t a = c + lane;
@@ -1601,12 +1643,14 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                 C val[kAhead];
                 bool use[kAhead];
                 uint32_t ak[kAhead], bk[kAhead];
+                [[maybe_unused]] bool tk[kAhead];
 #pragma unroll
                 for (int u = 0; u < kAhead; u++) {
                     use[u] = false;
                     val[u] = 0;
                     ak[u] = kNone;
                     bk[u] = en.b;
+                    tk[u] = true;
                     if (bal) {
                         const int j = __ffsll((unsigned long long)bal) - 1;
                         bal &= bal - 1;
@@ -1614,6 +1658,7 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                         const Len ll = rl(len, j);
                         const uint64_t m2 = rl64(mm, j);
                         ak[u] = c + (uint32_t)j;
+                        if constexpr (kQuirk) tk[u] = target_of(target, xx) != 0.0;
                         if ((m2 >> lane) & 1) {
                             const size_t r = (base + xx) * kLanes + lane;
                             if (P::tie(P::add(dv0, ll), dist[r])) {
@@ -1623,22 +1668,36 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                         }
                     }
                 }
+                [[maybe_unused]] double cv[kAhead];
 #pragma unroll
-                for (int u = 0; u < kAhead; u++)
-                    if (use[u]) dv.add(val[u]);
-                if constexpr (kEdge) {
-                    double cv[kAhead];
-#pragma unroll
-                    for (int u = 0; u < kAhead; u++) cv[u] = use[u] ? s * (double)val[u] : 0.0;
-                    edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
+                for (int u = 0; u < kAhead; u++) {
+                    cv[u] = 0.0;
+                    if (!use[u]) continue;
+                    if (!kQuirk || tk[u]) {
+                        dv.add(val[u]);
+                        cv[u] = s * (double)val[u];
+                    } else {
+                        dq.add(val[u]);
+                        cv[u] = (double)val[u];
+                    }
                 }
+                if constexpr (kEdge) edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
             }
         }
         if (mine) {
             size_t r = (base + en.v) * kLanes + lane;
-            double delta = s * dv.value();
-            coef[r] = Dep<C>::coef(target_of(target, en.v), delta, s);
-            sig[r] = delta;
+            if constexpr (kQuirk) {
+                const double de = s * dv.value() + dq.value();
+                const uint32_t np = npr[r];
+                npr[r] = 0;
+                coef[r] = (C)(target_of(target, en.v) != 0.0 ? (1.0 + de) / s
+                                                             : (np ? de / (double)np : 0.0));
+                sig[r] = de;
+            } else {
+                double delta = s * dv.value();
+                coef[r] = Dep<C>::coef(target_of(target, en.v), delta, s);
+                sig[r] = delta;
+            }
         }
     }
 }
@@ -2235,6 +2294,7 @@ struct Workspace {
     // Edge runs: per-arc credit, one row of arcs per slot (per thread for
     // real lengths), summed over the rows at the end (fold_arcs_kernel).
     DevBuf<double> eacc;
+    DevBuf<uint32_t> npr;  // quirk runs: predecessor arcs per (node, lane), zero between runs
     size_t qcap = 0;
     DevBuf<uint2> touched;
     DevBuf<Ctrl> ctrl;
@@ -2392,6 +2452,8 @@ struct Engine {
     bool edge = false;  // add every arc's credit to ws.eacc
     const uint8_t* target = nullptr;  // target flags (nullptr: every node)
     bool tie_exact = false;           // real lengths tie only when equal
+    bool quirk = false;               // NetworkX's subset edge split (edge runs)
+    uint32_t* npr = nullptr;          // predecessor-arc rows (quirk runs)
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -2494,7 +2556,7 @@ struct Engine {
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
-                                        frontier_lanes(dl), slots, left, n);
+                                        frontier_lanes(dl), slots, left, n, npr);
                 launch_check();
             } else {
                 const uint32_t gsz =
@@ -2516,7 +2578,7 @@ struct Engine {
                                                   cur, ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
                                                   ws.vis.p, ws.succ.p, ws.touched.p,
                                                   &ws.ctrl.p->tlen[dl & 1],
-                                                  frontier_lanes(dl), ws.ctrl.p, gsz, n);
+                                                  frontier_lanes(dl), ws.ctrl.p, gsz, n, npr);
                 launch_check();
             }
             bfs_gather_kernel<<<resident_blocks(bfs_gather_kernel, (pairs + 63) / 64), kBlock, 0,
@@ -2525,7 +2587,7 @@ struct Engine {
                                       d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
                                       ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.q.p,
                                       (uint32_t)ws.qcap, ws.hq.p, ws.ctrl.p, left,
-                                      &front[(dl + 1) & 1], n);
+                                      &front[(dl + 1) & 1], n, npr);
             launch_check();
             scan_heavy(hv.back());
             read_ctrl(ws, st);
@@ -2570,7 +2632,12 @@ struct Engine {
     // The backward sweep of the super-batch just searched.
     template <class C>
     void backward(C* coef) {
-        if (L.kind == kUniform) {
+        if (quirk) {
+            if (L.kind == kUniform) backward_bfs<C, true, true>(coef);
+            else if (L.kind == kInteger) backward_dial<C, true, IntPath, true>(coef);
+            else if (tie_exact) backward_dial<C, true, RealPathExact, true>(coef);
+            else backward_dial<C, true, RealPath, true>(coef);
+        } else if (L.kind == kUniform) {
             if (edge) backward_bfs<C, true>(coef);
             else backward_bfs<C, false>(coef);
         } else if (L.kind == kInteger) {
@@ -2602,11 +2669,11 @@ struct Engine {
         else return g.dev.directed ? L.in_real.p : L.out_real.p;
     }
 
-    template <class C, bool kEdge>
+    template <class C, bool kEdge, bool kQuirk = false>
     void backward_bfs(C* coef) {
         const DeviceGraph& d = g.dev;
         int D = (int)lv.size() - 2;  // last level with entries
-        const int split = opt.exact ? 0 : 1;
+        const int split = opt.exact || kQuirk ? 0 : 1;
         // edge runs also sweep the sources: their out-arcs carry credit
         for (int dl = D; dl >= (kEdge ? 0 : 1); dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
@@ -2618,7 +2685,7 @@ struct Engine {
             uint32_t ae = rebuild && dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
             const uint32_t gsz =
-                group_size(qe - qb, resident_waves(bfs_backward_kernel<C, kEdge>),
+                group_size(qe - qb, resident_waves(bfs_backward_kernel<C, kEdge, kQuirk>),
                            (uint32_t)std::min(std::max(opt.sweep_group, 1), 64));
             size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
             size_t waves = std::max<size_t>(tasks, rebuild ? (qe - qb + ae - ab + 63) / 64 : 0);
@@ -2627,12 +2694,12 @@ struct Engine {
                 ws.part.alloc((size_t)hc[dl] * kLanes * 2);
             }
             ws.sweep.mark(st);
-            bfs_backward_kernel<C, kEdge>
-                <<<resident_blocks(bfs_backward_kernel<C, kEdge>, waves), kBlock, 0, st>>>(
+            bfs_backward_kernel<C, kEdge, kQuirk>
+                <<<resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
-                    n, ws.eacc.p, d.arcs, target);
+                    n, ws.eacc.p, d.arcs, target, npr);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
@@ -2685,7 +2752,7 @@ struct Engine {
                 ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
                 d.in_off_p(), d.in_idx_p(), in_lengths<P>(), d.out_off.p, d.out_idx.p,
                 out_lengths<P>(), dist, ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p,
-                &ws.ctrl.p->next_bucket[p], n);
+                &ws.ctrl.p->next_bucket[p], n, npr);
             launch_check();
             read_ctrl(ws, st);
             if (ws.host_ctrl->qoverflow) return false;
@@ -2704,17 +2771,18 @@ struct Engine {
         return true;
     }
 
-    template <class C, bool kEdge, class P>
+    template <class C, bool kEdge, class P, bool kQuirk = false>
     void backward_dial(C* coef) {
         const DeviceGraph& d = g.dev;
         int D = (int)lv.size() - 2;
         for (int dl = D; dl >= (kEdge ? 0 : 1); dl--) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             ws.sweep.mark(st);
-            dial_backward_kernel<C, kEdge, P>
-                <<<resident_blocks(dial_backward_kernel<C, kEdge, P>, qe - qb), kBlock, 0, st>>>(
-                    ws.q.p, qb, qe, d.out_off.p, d.out_idx.p, out_lengths<P>(), dist_rows<P>(),
-                    ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs, target);
+            dial_backward_kernel<C, kEdge, P, kQuirk>
+                <<<resident_blocks(dial_backward_kernel<C, kEdge, P, kQuirk>, qe - qb), kBlock, 0,
+                   st>>>(ws.q.p, qb, qe, d.out_off.p, d.out_idx.p, out_lengths<P>(),
+                         dist_rows<P>(), ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs, target,
+                         npr);
             launch_check();
             ws.sweep.mark(st);
         }
@@ -2932,6 +3000,16 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     const bool endpoints = sub && sub->endpoints;
     eng.target = d_target;
     eng.tie_exact = sub && sub->tie_exact;
+    if (sub && sub->edge_split_nx) {
+        if (!edge) throw Error("NetworkX's edge split needs an edge run");
+        if (per_thread)
+            throw Error("NetworkX's subset edge split on real lengths whose longest arc exceeds "
+                        "2,047 times the shortest is not available on the device path");
+        eng.quirk = true;
+        ws.npr.alloc((size_t)slots * n * kLanes);
+        NETBC_HIP(hipMemsetAsync(ws.npr.p, 0, (size_t)slots * n * kLanes * 4, st));
+        eng.npr = ws.npr.p;
+    }
     DevBuf<unsigned long long> ep, lane_count;  // endpoint counts
     if (endpoints) {
         ep.alloc(n);

@@ -328,7 +328,7 @@ void nx_factors(uint64_t n, bool normalized, bool directed, bool endpoints, int6
 netbc::RunDiag run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sources,
                           const int64_t* targets, int64_t n_targets, int32_t weighted,
                           int32_t weights_as, int32_t endpoints, int32_t normalized,
-                          int32_t rescale, int32_t parallel, int32_t tie,
+                          int32_t rescale, int32_t parallel, int32_t edge_split, int32_t tie,
                           const netbc_options* opt, double* nodes, double* edges) {
     std::lock_guard<std::mutex> lock(g.mu);
     const bool collapse = parallel == 1;
@@ -361,8 +361,21 @@ netbc::RunDiag run_subset(netbc::Graph& g, const int64_t* sources, int64_t n_sou
     bc.alloc(n);
     if (edges) eb.alloc(pg->m);
     netbc::RunDiag d;
+    const bool quirk = edges && edge_split == 1;
     netbc::run_batched(*pg, L, all_src ? nullptr : sources, k, options(opt), bc.p, d,
-                       edges ? eb.p : nullptr, &sub);
+                       edges && !quirk ? eb.p : nullptr, &sub);
+    if (quirk) {
+        // NetworkX's edge split: a second run whose sweep carries the edge
+        // dependencies instead (its node sums are not betweenness)
+        netbc::SubsetRun sq = sub;
+        sq.endpoints = false;
+        sq.edge_split_nx = true;
+        netbc::DevBuf<double> scratch;
+        scratch.alloc(n);
+        netbc::RunDiag d2;
+        netbc::run_batched(*pg, L, all_src ? nullptr : sources, k, options(opt), scratch.p, d2,
+                           eb.p, &sq);
+    }
     // scaling, path_centrality.rs:527-574
     double fn_src, fn_other, fe, fe_unused;
     const bool directed = g.directed;
@@ -455,13 +468,10 @@ void subset_entry(const GraphHandle* h, const int64_t* sources, int64_t n_source
         throw netbc::Error("unknown edge split code " + std::to_string(edge_split));
     if (tie < 0 || tie > 1) throw netbc::Error("unknown tie code " + std::to_string(tie));
     check_options(opt);
-    if (edges && edge_split == 1)
-        throw netbc::Error("NetworkX's edge split of subset edge betweenness (edge_split 1) "
-                           "is not available on the device path");
     if (weighted && !g.has_weights) throw netbc::Error("the graph has no edge weights");
     netbc::RunDiag d = run_subset(g, sources, n_sources, targets, n_targets, weighted,
-                                  weights_as, endpoints, normalized, rescale, parallel, tie, opt,
-                                  nodes, edges);
+                                  weights_as, endpoints, normalized, rescale, parallel,
+                                  edge_split, tie, opt, nodes, edges);
     fill_diag(diag, d, 0.0);
 }
 

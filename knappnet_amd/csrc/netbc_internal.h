@@ -170,6 +170,10 @@ struct SubsetRun {
     const uint8_t* d_target = nullptr;  // device, n flags; nullptr: every node
     bool endpoints = false;
     bool tie_exact = false;
+    // edge values by NetworkX's subset edge split (path_centrality.rs:465-474)
+    // instead of in proportion to path counts; node values of such a run are
+    // not betweenness
+    bool edge_split_nx = false;
 };
 
 // Runs the batched engine over item positions [0, k) with sources

@@ -4,9 +4,8 @@ against the CPU oracle's restatement of path_centrality.rs:421-689.
 The bar: within 1e-12 relative of the oracle (the reference adds each
 source's dependency terms per predecessor arc in settle order, the device
 per node in row order), and within the reference's tolerance (rel 1e-9,
-abs 1e-12) of the committed definition and NetworkX goldens. Edge values
-with NetworkX's subset edge split are not produced on the device: the call
-fails with a message, tested here.
+abs 1e-12) of the committed definition and NetworkX goldens, including
+edge values with NetworkX's subset edge split.
 """
 import sys
 
@@ -48,8 +47,6 @@ def test_subset_goldens(fixtures):
             if variant not in fx["golden"]:
                 continue
             kw, out = subset_call(fx["n"], spec)
-            if out == "edges" and kw.get("edge_split") == 1:
-                continue  # not produced on the device (test_subset_errors)
             say("fixture", name, variant)
             g, o = both(fx["n"], fx["src"], fx["tgt"], fx["weights"], fx["directed"])
             nodes, edges = g.betweenness_subset(**kw)
@@ -79,7 +76,10 @@ def test_subset_random(seed):
                        dict(targets=tg, parallel=1, tie=1, rescale=2),
                        dict(sources=srcs[:5], parallel=1, tie=1, rescale=1, normalized=True),
                        dict(sources=srcs[:5], parallel=1, rescale=1, endpoints=True),
-                       dict(sources=srcs, tie=1, rescale=0, normalized=True, endpoints=True)):
+                       dict(sources=srcs, tie=1, rescale=0, normalized=True, endpoints=True),
+                       dict(sources=srcs, targets=tg, edge_split=1),
+                       dict(sources=srcs[:9], targets=tg, edge_split=1, parallel=1, tie=1,
+                            rescale=2)):
                 kw = dict(kw, weighted=weighted)
                 nodes, edges = g.betweenness_subset(**kw)
                 onodes, oedges = o.betweenness_subset(**kw)
@@ -104,7 +104,8 @@ def test_subset_large_sampled():
     tg = rng.choice(el.n, el.n // 3, replace=False)
     say("rmat13 sampled", el.n, el.m)
     for kw in (dict(sources=srcs, targets=tg, endpoints=True),
-               dict(sources=srcs, rescale=1, normalized=True, parallel=1, tie=1)):
+               dict(sources=srcs, rescale=1, normalized=True, parallel=1, tie=1),
+               dict(sources=srcs, targets=tg, edge_split=1)):
         nodes, edges = g.betweenness_subset(**kw)
         onodes, oedges = o.betweenness_subset(**kw)
         assert_rel(nodes, onodes, f"nodes {kw}")
@@ -131,8 +132,11 @@ def test_subset_errors():
         g.betweenness_subset(weighted=True, weights_as="strength")
     with pytest.raises(EngineError, match=r"^the graph has no edge weights$"):
         Graph(2, [0], [1], None, True).betweenness_subset(weighted=True)
-    with pytest.raises(EngineError, match=r"edge_split 1\) is not available on the device"):
-        Graph(3, [0, 1], [1, 2], None, True).betweenness_subset(edge_split=1)
-    # node values alone do not depend on the edge split
-    nodes, _ = Graph(3, [0, 1], [1, 2], None, True).betweenness_subset(edge_split=1, edges=False)
-    assert list(nodes) == [0.0, 1.0, 0.0]
+    # node values do not depend on the edge split
+    nodes, edges = Graph(3, [0, 1], [1, 2], None, True).betweenness_subset(edge_split=1,
+                                                                          targets=[2])
+    assert list(nodes) == [0.0, 1.0, 0.0] and list(edges) == [1.0, 2.0]
+    w = [1e-4, 1.0, 10.0]
+    with pytest.raises(EngineError, match=r"exceeds 2,047 times the shortest is not available"):
+        Graph(4, [0, 1, 2], [1, 2, 3], w, True).betweenness_subset(weighted=True,
+                                                                  edge_split=1)
