@@ -1663,3 +1663,262 @@ int32_t orc_betweenness_subset(const OGraph *g, const int64_t *sources, int64_t 
     free(acc_n); free(acc_e); free(tot_n); free(tot_e); free(delta); free(delta_e);
     return ORC_OK;
 }
+
+/* ---- closeness and harmonic centrality: closeness.rs:131-481 ----------- */
+
+/* Profile, closeness.rs:212-223: other nodes reached, sum of lengths, sum
+ * of reciprocal lengths, longest length. */
+typedef struct { uint64_t reached; double total, harmonic, longest; } OProfile;
+
+/* MultiBfs's per-level bookkeeping (closeness.rs:268-293), one source:
+ * reached += c, total += c * level (an integer), harmonic += c / level. */
+static OProfile profile_bfs(const OGraph *g, uint32_t s, uint32_t *hops, uint32_t *queue) {
+    const OCsr *adj = &g->out;
+    OProfile p = {0, 0.0, 0.0, 0.0};
+    uint64_t total = 0, head = 0, tail = 0;
+    hops[s] = 0;
+    queue[tail++] = s;
+    uint64_t level_start = tail;
+    uint32_t level = 0;
+    uint64_t count = 0;
+    while (head < tail) {
+        uint32_t v = queue[head++];
+        for (uint64_t a = adj->offsets[v]; a < adj->offsets[v + 1]; a++) {
+            uint32_t w = adj->targets[a];
+            if (hops[w] != U32_MAX_) continue;
+            hops[w] = hops[v] + 1;
+            queue[tail++] = w;
+        }
+        if (head == level_start) {
+            /* level `level` + 1 is complete */
+            count = tail - level_start;
+            level++;
+            if (count) {
+                p.reached += count;
+                total += count * (uint64_t)level;
+                p.harmonic += (double)count / (double)level;
+                p.longest = (double)level;
+            }
+            level_start = tail;
+        }
+    }
+    p.total = (double)total;
+    for (uint64_t i = 0; i < tail; i++) hops[queue[i]] = U32_MAX_;
+    return p;
+}
+
+/* Search::dial, closeness.rs:364-425: buckets are stacks. */
+static OProfile profile_dial(const OGraph *g, const uint32_t *lens, uint32_t max, uint32_t s,
+                             uint64_t *idist, uint8_t *done, U32Vec *buckets, U32Vec *touched) {
+    const OCsr *adj = &g->out;
+    uint64_t width = (uint64_t)max + 1;
+    touched->len = 0;
+    vpush(touched, s);
+    idist[s] = 0;
+    for (uint64_t b = 0; b < width; b++) buckets[b].len = 0;
+    vpush(&buckets[0], s);
+    uint64_t pending = 1, cur = 0, reached = 0, total = 0, longest = 0;
+    double harmonic = 0.0;
+    while (pending > 0) {
+        U32Vec *bk = &buckets[cur % width];
+        if (bk->len == 0) { cur++; continue; }
+        uint32_t v = bk->a[--bk->len];
+        pending--;
+        if (done[v] || idist[v] != cur) continue;
+        done[v] = 1;
+        if (v != s) {
+            reached++;
+            total += cur;
+            harmonic += 1.0 / (double)cur;
+            if (cur > longest) longest = cur;
+        }
+        for (uint64_t a = adj->offsets[v]; a < adj->offsets[v + 1]; a++) {
+            uint32_t w = adj->targets[a];
+            if (done[w]) continue;
+            uint64_t alt = cur + lens[a];
+            if (idist[w] == UINT64_MAX) vpush(touched, w);
+            if (alt < idist[w]) {
+                idist[w] = alt;
+                vpush(&buckets[alt % width], w);
+                pending++;
+            }
+        }
+    }
+    for (uint64_t i = 0; i < touched->len; i++) {
+        idist[touched->a[i]] = UINT64_MAX;
+        done[touched->a[i]] = 0;
+    }
+    OProfile p = {reached, (double)total, harmonic, (double)longest};
+    return p;
+}
+
+/* Search::dijkstra, closeness.rs:427-470 (the 4-ary heap, no tie rule). */
+static OProfile profile_dijkstra(const OGraph *g, const double *lens, uint32_t s, double *dist,
+                                 uint8_t *done, Heap *heap, U32Vec *touched) {
+    const OCsr *adj = &g->out;
+    touched->len = 0;
+    vpush(touched, s);
+    dist[s] = 0.0;
+    h_push_or_decrease(heap, s, 0.0);
+    OProfile p = {0, 0.0, 0.0, 0.0};
+    HItem top;
+    while (h_pop(heap, &top)) {
+        uint32_t v = top.v;
+        double d = top.d;
+        done[v] = 1;
+        if (v != s) {
+            p.reached++;
+            p.total += d;
+            p.harmonic += 1.0 / d;
+            if (d > p.longest) p.longest = d;
+        }
+        for (uint64_t a = adj->offsets[v]; a < adj->offsets[v + 1]; a++) {
+            uint32_t w = adj->targets[a];
+            if (done[w]) continue;
+            double alt = d + lens[a];
+            if (isinf(dist[w])) vpush(touched, w);
+            if (alt < dist[w]) {
+                dist[w] = alt;
+                h_push_or_decrease(heap, w, alt);
+            }
+        }
+    }
+    for (uint64_t i = 0; i < touched->len; i++) {
+        dist[touched->a[i]] = INFINITY;
+        done[touched->a[i]] = 0;
+    }
+    return p;
+}
+
+/* net_closeness, crates/capi/src/topography.rs:167-199 -> closeness /
+ * harmonic_centrality (closeness.rs:131-209) over distance_profiles
+ * (closeness.rs:240-345). mode: 1 out, 2 in, 3 all; unreachable: 0 ignore,
+ * 1 Wasserman-Faust, 2 zero, 3 max plus one. */
+int32_t orc_closeness(const OGraph *g0, int32_t harmonic, int32_t mode, int32_t weighted,
+                      int32_t weights_as, int32_t normalized, int32_t unreachable,
+                      double *out, char *err, size_t err_len) {
+    char msg[256];
+    if (!g0) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (mode < 1 || mode > 3) {
+        snprintf(msg, sizeof msg, "unknown degree mode code %d", mode);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    if (unreachable < 0 || unreachable > 3) {
+        snprintf(msg, sizeof msg, "unknown code %d for unreachable nodes", unreachable);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (!harmonic && unreachable == 1 && !normalized) {
+        write_err(err, err_len,
+                  "the Wasserman-Faust closeness index is defined only in normalized form");
+        return ORC_FAILED;
+    }
+    if (!harmonic && unreachable == 3 && weighted) {
+        write_err(err, err_len,
+                  "closeness with unreachable lengths set to the longest length plus 1 is "
+                  "defined only for unweighted networks");
+        return ORC_FAILED;
+    }
+    /* oriented, closeness.rs:225-238: paths along in-arcs, or ignoring
+     * direction, as the out-paths of a derived graph */
+    const OGraph *g = g0;
+    OGraph *h = NULL;
+    if (g0->directed && mode != 1) {
+        uint64_t m = g0->m;
+        int64_t *s = malloc((m ? m : 1) * sizeof *s), *t = malloc((m ? m : 1) * sizeof *t);
+        for (uint64_t e = 0; e < m; e++) {
+            s[e] = mode == 2 ? g0->tgt[e] : g0->src[e];
+            t[e] = mode == 2 ? g0->src[e] : g0->tgt[e];
+        }
+        int32_t rc = orc_graph_new((int64_t)g0->n, s, t, g0->w, (int64_t)m, mode == 2, &h, err,
+                                   err_len);
+        free(s);
+        free(t);
+        if (rc) return ORC_FAILED;
+        g = h;
+    }
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, 0, &L, err, err_len)) {
+        orc_graph_free(h);
+        return ORC_FAILED;
+    }
+    uint64_t n = g->n;
+    OProfile *prof = calloc(n ? n : 1, sizeof *prof);
+    uint32_t *hops = malloc((n ? n : 1) * sizeof *hops), *queue = malloc((n ? n : 1) * sizeof *queue);
+    uint64_t *idist = malloc((n ? n : 1) * sizeof *idist);
+    double *dist = malloc((n ? n : 1) * sizeof *dist);
+    uint8_t *done = calloc(n ? n : 1, 1);
+    for (uint64_t v = 0; v < n; v++) {
+        hops[v] = U32_MAX_;
+        idist[v] = UINT64_MAX;
+        dist[v] = INFINITY;
+    }
+    U32Vec touched = {0};
+    U32Vec *buckets = L.kind == LEN_INTEGER ? calloc((uint64_t)L.max + 1, sizeof *buckets) : NULL;
+    Heap heap;
+    heap.items = malloc((n ? n : 1) * sizeof *heap.items);
+    heap.pos = malloc((n ? n : 1) * sizeof *heap.pos);
+    heap.len = 0;
+    for (uint64_t v = 0; v < n; v++) heap.pos[v] = U32_MAX_;
+    for (uint64_t s = 0; s < n; s++) {
+        if (L.kind == LEN_UNIFORM) prof[s] = profile_bfs(g, (uint32_t)s, hops, queue);
+        else if (L.kind == LEN_INTEGER)
+            prof[s] = profile_dial(g, L.ilens, L.max, (uint32_t)s, idist, done, buckets, &touched);
+        else prof[s] = profile_dijkstra(g, L.rlens, (uint32_t)s, dist, done, &heap, &touched);
+    }
+    /* uniform weights: hop counts times the common length (closeness.rs:325-344) */
+    if (L.kind == LEN_UNIFORM && weighted && g->out.arcs > 0 && g->out.weights) {
+        double w = g->out.weights[0];
+        double unit = weights_as == 1 ? 1.0 / w : w;
+        if (unit != 1.0)
+            for (uint64_t s = 0; s < n; s++) {
+                prof[s].total *= unit;
+                prof[s].harmonic /= unit;
+                prof[s].longest *= unit;
+            }
+    }
+    if (harmonic) {
+        double scale = normalized && n > 1 ? 1.0 / (double)(n - 1) : 1.0;
+        for (uint64_t v = 0; v < n; v++) out[v] = prof[v].harmonic * scale;
+    } else {
+        double longest = 0.0;
+        for (uint64_t v = 0; v < n; v++) if (prof[v].longest > longest) longest = prof[v].longest;
+        double others = n > 0 ? (double)(n - 1) : 0.0;
+        for (uint64_t v = 0; v < n; v++) {
+            OProfile *p = &prof[v];
+            double r = (double)p->reached, x;
+            switch (unreachable) {
+            case 0:
+                x = p->reached == 0 ? NAN : (normalized ? r / p->total : 1.0 / p->total);
+                break;
+            case 1:
+                x = p->reached == 0 ? 0.0 : (r / p->total) * (r / others);
+                break;
+            case 2:
+                x = n < 2 ? NAN
+                    : p->reached < n - 1 ? 0.0
+                    : normalized ? others / p->total : 1.0 / p->total;
+                break;
+            default: {
+                if (n < 2) { x = NAN; break; }
+                double total = p->total + (others - r) * (longest + 1.0);
+                x = normalized ? others / total : 1.0 / total;
+            }
+            }
+            out[v] = x;
+        }
+    }
+    if (buckets) {
+        for (uint64_t b = 0; b <= L.max; b++) free(buckets[b].a);
+        free(buckets);
+    }
+    free(touched.a);
+    free(heap.items);
+    free(heap.pos);
+    free(prof); free(hops); free(queue); free(idist); free(dist); free(done);
+    lengths_free(&L);
+    orc_graph_free(h);
+    return ORC_OK;
+}

@@ -63,6 +63,9 @@ def lib() -> ctypes.CDLL:
                                              c_int32, POINTER(c_double), POINTER(c_double),
                                              c_char_p, c_size_t]
         L.orc_betweenness_subset.restype = c_int32
+        L.orc_closeness.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
+                                    c_int32, POINTER(c_double), c_char_p, c_size_t]
+        L.orc_closeness.restype = c_int32
         L.orc_single_source.argtypes = [c_void_p, c_int32, c_int32, c_int64, POINTER(c_int64),
                                         POINTER(c_int32), POINTER(c_uint64), POINTER(c_double),
                                         POINTER(c_double), POINTER(c_double), c_char_p,
@@ -196,6 +199,19 @@ class OracleGraph:
                 _p(nodes, c_double), _p(edges, c_double), err, ERR_LEN):
             raise OracleError(err.value.decode())
         return nodes[: self.n], edges[: self.m]
+
+    def closeness(self, harmonic=False, mode=1, weighted=False, weights_as="distance",
+                  normalized=False, unreachable=0):
+        """closeness.rs:131-209 (mode 1 out, 2 in, 3 all; unreachable 0 ignore,
+        1 Wasserman-Faust, 2 zero, 3 max plus one)."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if self.L.orc_closeness(self.h, int(harmonic), int(mode), int(weighted), kind,
+                                int(normalized), int(unreachable), _p(out, c_double), err,
+                                ERR_LEN):
+            raise OracleError(err.value.decode())
+        return out[: self.n]
 
     def single_source(self, s, weighted=False, weights_as="distance"):
         """(kind, dist_u64, dist_f64, sigma, delta, reached)."""

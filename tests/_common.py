@@ -129,3 +129,33 @@ def subset_networkx_documented(fx, variant):
     # centrality_extra.yaml:16-58: the all_distinct variants count parallel
     # edges as distinct paths, NetworkX once.
     return "multi_edges" in set(fx["tags"]) and "all_distinct" in variant
+
+
+# Closeness and harmonic centrality (validation/manifest.d/topography.yaml:
+# 151-300; runner validation/bindings/topography.py:130-151). mode 1 out, 2
+# in, 3 all; unreachable 0 ignore, 1 Wasserman-Faust, 2 zero, 3 max plus
+# one. The `simple: true` variants read the graph as an adjacency matrix,
+# which changes no unweighted distance, so they run on the graph itself.
+# Tolerance rel 1e-12, abs 1e-14; NaN is written as -1.
+CLOSENESS_VARIANTS = {
+    "closeness": dict(mode=1),
+    "closeness_normalized": dict(mode=1, normalized=True),
+    "closeness_in": dict(mode=2, normalized=True),
+    "closeness_all": dict(mode=3, normalized=True),
+    "closeness_wf": dict(mode=2, normalized=True, unreachable=1),
+    "closeness_sna": dict(mode=1, normalized=True, unreachable=2),
+    "closeness_ucinet": dict(mode=1, normalized=True, unreachable=3),
+    "closeness_ucinet_in": dict(mode=2, normalized=True, unreachable=3),
+    "closeness_weighted": dict(mode=1, weighted=True, normalized=True),
+    "closeness_wf_weighted": dict(mode=2, weighted=True, normalized=True, unreachable=1),
+    "closeness_weighted_strength": dict(mode=1, weighted=True, weights_as="strength",
+                                        normalized=True),
+    "harmonic": dict(harmonic=True, mode=1),
+    "harmonic_normalized": dict(harmonic=True, mode=1, normalized=True),
+    "harmonic_in": dict(harmonic=True, mode=2),
+    "harmonic_weighted": dict(harmonic=True, mode=1, weighted=True),
+}
+
+
+def closeness_golden(vals):
+    return [float("nan") if x is None or x == -1 else x for x in vals]

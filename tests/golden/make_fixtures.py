@@ -5,9 +5,10 @@ and meta.json) and the values its oracles produced for them
 (validation/golden/{definition,networkx}/<name>/<variant>.csv, one
 ``%.17g`` value per line under a ``value`` header; see
 validation/common.py:341-352; the centrality_extra variants write NaN as
--999). This script copies the edge lists, the metadata and the goldens of
-betweenness, edge betweenness and betweenness for sources and targets
-into one JSON file, as data. Nothing of the reference is imported or run.
+-999, the topography ones as -1). This script copies the edge lists, the
+metadata and the goldens of betweenness, edge betweenness, betweenness for
+sources and targets, and closeness / harmonic centrality into one JSON
+file, as data. Nothing of the reference is imported or run.
 
 Run once in a tree where the reference is checked out:
 
@@ -46,6 +47,22 @@ VARIANTS = [
     "cx_bc_sampled_endpoints",
     "cx_bc_sampled_weighted",
     "cx_ebc_sampled",
+    # closeness and harmonic centrality (validation/manifest.d/topography.yaml; NaN as -1)
+    "closeness",
+    "closeness_normalized",
+    "closeness_in",
+    "closeness_all",
+    "closeness_wf",
+    "closeness_sna",
+    "closeness_ucinet",
+    "closeness_ucinet_in",
+    "closeness_weighted",
+    "closeness_wf_weighted",
+    "closeness_weighted_strength",
+    "harmonic",
+    "harmonic_normalized",
+    "harmonic_in",
+    "harmonic_weighted",
 ]
 ORACLES = ["definition", "networkx"]
 

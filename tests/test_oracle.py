@@ -17,9 +17,9 @@ from scipy.sparse import csr_matrix
 from scipy.sparse.csgraph import shortest_path
 
 import definition
-from _common import (EDGE_VARIANTS, SUBSET_VARIANTS, VARIANTS, close, golden_values,
-                     networkx_documented, random_case, subset_call, subset_networkx_documented,
-                     tolerance)
+from _common import (CLOSENESS_VARIANTS, EDGE_VARIANTS, SUBSET_VARIANTS, VARIANTS, close,
+                     closeness_golden, golden_values, networkx_documented, random_case,
+                     subset_call, subset_networkx_documented, tolerance)
 from oracle.oracle import (OracleError, OracleGraph, available_cores, ordered_sum_bits,
                            threads_for)
 
@@ -101,6 +101,20 @@ def test_subset_goldens(fixtures, variant):
             if oracle == "networkx" and subset_networkx_documented(fx, variant):
                 continue
             assert ok, (name, variant, oracle)
+        checked += 1
+    assert checked >= 7
+
+
+@pytest.mark.parametrize("variant", list(CLOSENESS_VARIANTS))
+def test_closeness_goldens(fixtures, variant):
+    # topography.yaml: rel 1e-12, abs 1e-14 for every variant
+    checked = 0
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"]:
+            continue
+        got = og(fx).closeness(**CLOSENESS_VARIANTS[variant])
+        for oracle, want in fx["golden"][variant].items():
+            assert close(got, closeness_golden(want), 1e-12, 1e-14), (name, variant, oracle)
         checked += 1
     assert checked >= 7
 
@@ -381,6 +395,53 @@ def test_subset_errors():
         g.betweenness_subset(weighted=True, weights_as="strength")
     with pytest.raises(OracleError, match=r"^the graph has no edge weights$"):
         OracleGraph(2, [0], [1], None, True).betweenness_subset(weighted=True)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_closeness_definition_random(seed):
+    # closeness and harmonic centrality from scipy's all-pairs distances
+    rng = np.random.default_rng(4000 + seed)
+    n, s, t, directed = random_case(rng, max_n=40)
+    w = rng.random(len(s)) * 3 + 0.5
+    for weighted in (False, True):
+        lens = w if weighted else np.ones(len(s))
+        A = csr_matrix((np.full(len(s), np.inf), (s, t)), shape=(n, n)).tolil()
+        for e in range(len(s)):  # the lightest of parallel edges, loops ignored
+            if s[e] != t[e] and lens[e] < A[s[e], t[e]]:
+                A[s[e], t[e]] = lens[e]
+        A = A.tocsr()
+        A.data[np.isinf(A.data)] = 0
+        A.eliminate_zeros()
+        for mode in (1, 2, 3):
+            D = shortest_path(A, directed=directed and mode != 3)
+            if directed and mode == 2:
+                D = D.T
+            off = ~np.eye(n, dtype=bool)
+            fin = np.isfinite(D) & off
+            r = fin.sum(axis=1)
+            tot = np.where(fin, D, 0).sum(axis=1)
+            harm = np.where(fin, 1.0 / np.where(fin, D, 1.0), 0).sum(axis=1)
+            g = OracleGraph(n, s, t, w, directed)
+            kw = dict(mode=mode, weighted=weighted)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                want = np.where(r > 0, r / tot, np.nan)
+            assert close(g.closeness(normalized=True, **kw), want, 1e-12, 1e-14)
+            assert close(g.closeness(harmonic=True, **kw), harm, 1e-12, 1e-14)
+
+
+def test_closeness_errors():
+    g = OracleGraph(3, [0, 1], [1, 2], [1.0, 2.0], True)
+    for kw, msg in (({"mode": 4}, "unknown degree mode code 4"),
+                    ({"weights_as": 2}, "unknown weight kind code 2"),
+                    ({"unreachable": 5}, "unknown code 5 for unreachable nodes"),
+                    ({"unreachable": 1},
+                     "the Wasserman-Faust closeness index is defined only in normalized form"),
+                    ({"unreachable": 3, "weighted": True},
+                     "closeness with unreachable lengths set to the longest length plus 1 is "
+                     "defined only for unweighted networks")):
+        with pytest.raises(OracleError, match="^" + msg + "$"):
+            g.closeness(**kw)
+    assert np.isfinite(g.closeness(harmonic=True, unreachable=1)).all()
 
 
 # ---- closed forms and the distance-sum identity -------------------------------
