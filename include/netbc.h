@@ -92,6 +92,25 @@ int32_t net_betweenness_subset(const GraphHandle *h, const int64_t *sources,
                                int32_t tie, int32_t threads, double *nodes,
                                double *edges, char *err, size_t err_len);
 
+/* Replaces net_closeness, netcapi.h:401-406 (crates/capi/src/
+ * topography.rs:155-199, crates/engine/src/metrics/closeness.rs:131-481).
+ * Closeness (harmonic == 0) or harmonic centrality (harmonic != 0) of every
+ * node from its distances to the others: mode 1 along out-arcs, 2 along
+ * in-arcs, 3 ignoring direction; unreachable (closeness only): 0 ignore
+ * unreachable nodes, 1 Wasserman-Faust (normalized only), 2 zero unless
+ * every other node is reached, 3 unreachable lengths count as the longest
+ * length plus one (unweighted only). normalized != 0 scales closeness by
+ * the reached count and harmonic centrality by 1/(n-1). Breadth-first
+ * values, and real lengths whose longest arc exceeds about 2,047 times the
+ * shortest (the reference's Dijkstra per source), match the reference's
+ * bit for bit; other weighted runs add each source's reciprocal (and real)
+ * lengths in node order instead of settle order (within 1e-12 relative).
+ * `out` holds n doubles; `threads` is accepted and ignored. */
+int32_t net_closeness(const GraphHandle *h, int32_t harmonic, int32_t mode,
+                      int32_t weighted, int32_t weights_as, int32_t normalized,
+                      int32_t unreachable, int32_t threads, double *out,
+                      char *err, size_t err_len);
+
 /* ---- device-path entry points --------------------------------------- */
 
 /* Run options. Zero-initialised means: FP64 dependencies, automatic batch
@@ -177,6 +196,14 @@ int32_t netbc_betweenness_subset_ex(const GraphHandle *h,
                                     const netbc_options *opt, double *nodes,
                                     double *edges, netbc_diagnostics *diag,
                                     char *err, size_t err_len);
+
+/* net_closeness with run options and diagnostics (both may be NULL). No
+ * counterpart in netcapi.h. */
+int32_t netbc_closeness_ex(const GraphHandle *h, int32_t harmonic,
+                           int32_t mode, int32_t weighted, int32_t weights_as,
+                           int32_t normalized, int32_t unreachable,
+                           const netbc_options *opt, double *out,
+                           netbc_diagnostics *diag, char *err, size_t err_len);
 
 /* Betweenness of many independent graphs in one call (null-model and
  * permutation replicates). Graph i has node_counts[i] nodes and

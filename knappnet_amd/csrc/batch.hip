@@ -1719,6 +1719,80 @@ __device__ __forceinline__ bool h_less(double da, uint32_t va, double db, uint32
     return da < db || (da == db && va < vb);
 }
 
+// The reference's 4-ary indexed min-heap keyed by (distance, node)
+// (weights.rs:155-277) over one thread's private arrays: keys hk, nodes hv
+// and every node's heap position (kNone when absent).
+struct Heap4 {
+    double* hk;
+    uint32_t* hv;
+    uint32_t* pos;
+    size_t len;
+
+    __device__ void sift_up(size_t i) {
+        const double kd = hk[i];
+        const uint32_t kv = hv[i];
+        while (i > 0) {
+            const size_t p = (i - 1) / 4;
+            if (!h_less(kd, kv, hk[p], hv[p])) break;
+            hk[i] = hk[p];
+            hv[i] = hv[p];
+            pos[hv[i]] = (uint32_t)i;
+            i = p;
+        }
+        hk[i] = kd;
+        hv[i] = kv;
+        pos[kv] = (uint32_t)i;
+    }
+    __device__ void sift_down(size_t i) {
+        const double kd = hk[i];
+        const uint32_t kv = hv[i];
+        for (;;) {
+            const size_t first = i * 4 + 1;
+            if (first >= len) break;
+            size_t best = first;
+            const size_t end = first + 4 < len ? first + 4 : len;
+            for (size_t c = first + 1; c < end; c++)
+                if (h_less(hk[c], hv[c], hk[best], hv[best])) best = c;
+            if (!h_less(hk[best], hv[best], kd, kv)) break;
+            hk[i] = hk[best];
+            hv[i] = hv[best];
+            pos[hv[i]] = (uint32_t)i;
+            i = best;
+        }
+        hk[i] = kd;
+        hv[i] = kv;
+        pos[kv] = (uint32_t)i;
+    }
+    __device__ void push_or_decrease(uint32_t node, double d) {
+        const uint32_t p = pos[node];
+        size_t i;
+        if (p == kNone) {
+            hk[len] = d;
+            hv[len] = node;
+            i = len++;
+        } else {
+            i = p;
+            if (!h_less(d, node, hk[i], hv[i])) return;
+            hk[i] = d;
+        }
+        sift_up(i);
+    }
+    // Removes the least (distance, node); false when the heap is empty.
+    __device__ bool pop(double& d, uint32_t& v) {
+        if (len == 0) return false;
+        d = hk[0];
+        v = hv[0];
+        pos[v] = kNone;
+        len--;
+        if (len > 0) {
+            hk[0] = hk[len];
+            hv[0] = hv[len];
+            sift_down(0);
+        }
+        return true;
+    }
+};
+
 // Dijkstra with the reference's 4-ary indexed heap ordered by (distance,
 // node) (weights.rs:155-277) and ties within a relative 1e-10
 // (betweenness.rs:117-120, 297-368). Each thread owns private arrays. With
@@ -1741,8 +1815,6 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
     double* coef = st.coef + o;
     uint32_t* rank = st.rank + o;
     uint32_t* order = st.order + o;
-    double* hk = st.hkey + o;
-    uint32_t* hv = st.hnode + o;
     uint32_t* pos = st.pos + o;
     for (size_t v = 0; v < n; v++) {
         dist[v] = INFINITY;
@@ -1751,72 +1823,17 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
         rank[v] = kNone;
         pos[v] = kNone;
     }
-    size_t hlen = 0, norder = 0;
-    auto sift_up = [&](size_t i) {
-        double kd = hk[i];
-        uint32_t kv = hv[i];
-        while (i > 0) {
-            size_t p = (i - 1) / 4;
-            if (!h_less(kd, kv, hk[p], hv[p])) break;
-            hk[i] = hk[p];
-            hv[i] = hv[p];
-            pos[hv[i]] = (uint32_t)i;
-            i = p;
-        }
-        hk[i] = kd;
-        hv[i] = kv;
-        pos[kv] = (uint32_t)i;
-    };
-    auto sift_down = [&](size_t i) {
-        double kd = hk[i];
-        uint32_t kv = hv[i];
-        for (;;) {
-            size_t first = i * 4 + 1;
-            if (first >= hlen) break;
-            size_t best = first;
-            size_t end = first + 4 < hlen ? first + 4 : hlen;
-            for (size_t c = first + 1; c < end; c++)
-                if (h_less(hk[c], hv[c], hk[best], hv[best])) best = c;
-            if (!h_less(hk[best], hv[best], kd, kv)) break;
-            hk[i] = hk[best];
-            hv[i] = hv[best];
-            pos[hv[i]] = (uint32_t)i;
-            i = best;
-        }
-        hk[i] = kd;
-        hv[i] = kv;
-        pos[kv] = (uint32_t)i;
-    };
-    auto push_or_decrease = [&](uint32_t node, double d) {
-        uint32_t p = pos[node];
-        size_t i;
-        if (p == kNone) {
-            hk[hlen] = d;
-            hv[hlen] = node;
-            i = hlen++;
-        } else {
-            i = p;
-            if (!h_less(d, node, hk[i], hv[i])) return;
-            hk[i] = d;
-        }
-        sift_up(i);
-    };
+    size_t norder = 0;
+    Heap4 heap{st.hkey + o, st.hnode + o, pos, 0};
     auto approx_eq = [](double a, double c) {
         return fabs(a - c) <= 1e-10 * fmax(fabs(a), fabs(c));
     };
     dist[s] = 0.0;
     sigma[s] = 1.0;
-    push_or_decrease(s, 0.0);
-    while (hlen > 0) {
-        double d = hk[0];
-        uint32_t v = hv[0];
-        pos[v] = kNone;
-        hlen--;
-        if (hlen > 0) {
-            hk[0] = hk[hlen];
-            hv[0] = hv[hlen];
-            sift_down(0);
-        }
+    heap.push_or_decrease(s, 0.0);
+    double d;
+    uint32_t v;
+    while (heap.pop(d, v)) {
         rank[v] = (uint32_t)norder;
         order[norder++] = v;
         for (uint32_t a = off[v]; a < off[v + 1]; a++) {
@@ -1827,19 +1844,19 @@ __global__ void dijkstra_kernel(const int64_t* __restrict__ list, int64_t item0,
             if (isinf(cur)) {
                 dist[w] = alt;
                 sigma[w] = sigma[v];
-                push_or_decrease(w, alt);
+                heap.push_or_decrease(w, alt);
             } else if (approx_eq(alt, cur)) {
                 sigma[w] += sigma[v];
             } else if (alt < cur) {
                 dist[w] = alt;
                 sigma[w] = sigma[v];
-                push_or_decrease(w, alt);
+                heap.push_or_decrease(w, alt);
             }
         }
     }
     double mx = 0.0;
     for (size_t i = norder; i-- > 0;) {
-        uint32_t v = order[i];
+        v = order[i];
         uint32_t rv = rank[v];
         double dvv = dist[v];
         double dv = 0.0;
@@ -2185,6 +2202,234 @@ __global__ void add_counts_kernel(double* __restrict__ bc, const unsigned long l
         if (ep[v]) bc[v] += (double)ep[v];
 }
 
+// ---- distance profiles (closeness.rs:212-223, 240-345) --------------------
+
+// Per source: other nodes reached, the sum of their lengths (an integer for
+// hop counts and integer lengths, else a double), the sum of reciprocal
+// lengths and the longest length. Five arrays of `count` 8-byte words, one
+// after another from `base` (all zero bits: an empty profile).
+struct DevProfile {
+    unsigned long long* reached;
+    unsigned long long* itotal;
+    double* ftotal;
+    double* harmonic;
+    double* longest;
+};
+
+inline DevProfile profile_view(uint64_t* base, size_t count) {
+    return DevProfile{(unsigned long long*)base, (unsigned long long*)base + count,
+                      (double*)(base + 2 * count), (double*)(base + 3 * count),
+                      (double*)(base + 4 * count)};
+}
+
+// Breadth-first level `level` (per-lane counts from endpoint_lanes_kernel
+// over the level's bitmap): lane t's profile gets reached += c, total += c *
+// level, harmonic += c / level and longest = level, in level order as
+// MultiBfs's per-level bookkeeping (closeness.rs:268-293). The counts are
+// reset for the next level.
+__global__ void level_profile_kernel(unsigned long long* __restrict__ lane_count, int slots,
+                                     uint32_t level, DevProfile lanes) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots * kLanes) return;
+    const unsigned long long c = lane_count[t];
+    if (c == 0) return;
+    lane_count[t] = 0;
+    lanes.reached[t] += c;
+    lanes.itotal[t] += c * level;
+    lanes.harmonic[t] += (double)c / (double)level;
+    lanes.longest[t] = (double)level;
+}
+
+// Bucketed searches: partial profiles per (slot, chunk of kProfileChunk
+// nodes, lane) from the final distance rows, nodes in order; the source
+// itself (distance 0) is left out.
+constexpr uint32_t kProfileChunk = 512;
+
+template <class P>
+__global__ void dist_profile_kernel(const typename P::Dist* __restrict__ dist, int slots,
+                                    size_t n, uint32_t* __restrict__ pc,
+                                    unsigned long long* __restrict__ pit,
+                                    double* __restrict__ pft, double* __restrict__ ph,
+                                    double* __restrict__ pl) {
+    using Dist = typename P::Dist;
+    const int lane = lane_id();
+    const size_t chunks = (n + kProfileChunk - 1) / kProfileChunk;
+    for (size_t w = global_wave(); w < chunks * (size_t)slots; w += wave_count()) {
+        const size_t b = w / chunks, ch = w % chunks;
+        const size_t v0 = ch * kProfileChunk, v1 = min(n, v0 + kProfileChunk);
+        uint32_t cnt = 0;
+        unsigned long long it = 0;
+        double ft = 0.0, h = 0.0, lg = 0.0;
+        for (size_t v = v0; v < v1; v++) {
+            const Dist d = dist[(b * n + v) * kLanes + lane];
+            if (!P::reached(d) || d == Dist(0)) continue;
+            cnt++;
+            if constexpr (std::is_same_v<Dist, uint32_t>) it += d;
+            else ft += d;
+            h += 1.0 / (double)d;
+            lg = fmax(lg, (double)d);
+        }
+        const size_t r = w * kLanes + lane;
+        pc[r] = cnt;
+        pit[r] = it;
+        pft[r] = ft;
+        ph[r] = h;
+        pl[r] = lg;
+    }
+}
+
+// The partial profiles of each (slot, lane) added in chunk order.
+__global__ void dist_profile_reduce_kernel(const uint32_t* __restrict__ pc,
+                                           const unsigned long long* __restrict__ pit,
+                                           const double* __restrict__ pft,
+                                           const double* __restrict__ ph,
+                                           const double* __restrict__ pl, int slots, size_t n,
+                                           DevProfile lanes) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots * kLanes) return;
+    const int b = t / kLanes, l = t % kLanes;
+    const size_t chunks = (n + kProfileChunk - 1) / kProfileChunk;
+    unsigned long long c = 0, it = 0;
+    double ft = 0.0, h = 0.0, lg = 0.0;
+    for (size_t ch = 0; ch < chunks; ch++) {
+        const size_t r = ((size_t)b * chunks + ch) * kLanes + l;
+        c += pc[r];
+        it += pit[r];
+        ft += pft[r];
+        h += ph[r];
+        lg = fmax(lg, pl[r]);
+    }
+    lanes.reached[t] = c;
+    lanes.itotal[t] = it;
+    lanes.ftotal[t] = ft;
+    lanes.harmonic[t] = h;
+    lanes.longest[t] = lg;
+}
+
+// The lane profiles of a super-batch, written to their sources.
+__global__ void commit_profile_kernel(DevProfile lanes, int slots, const int64_t* __restrict__ list,
+                                      int64_t item0, int64_t k, DevProfile prof) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots * kLanes) return;
+    const int64_t i = item0 + t;
+    if (i >= k) return;
+    const int64_t s = list ? list[i] : i;
+    prof.reached[s] = lanes.reached[t];
+    prof.itotal[s] = lanes.itotal[t];
+    prof.ftotal[s] = lanes.ftotal[t];
+    prof.harmonic[s] = lanes.harmonic[t];
+    prof.longest[s] = lanes.longest[t];
+}
+
+// Real lengths with very unequal arcs: the reference's Dijkstra
+// (closeness.rs:427-470), one source per thread on private arrays (st.dist,
+// st.rank as the settled mark, the heap in st.hkey / st.hnode / st.pos).
+// Every settled node other than the source adds to lane t's profile in
+// settle order, as the reference does.
+__global__ void dijkstra_profile_kernel(const int64_t* __restrict__ list, int64_t item0,
+                                        int64_t k, int slots, size_t n,
+                                        const uint32_t* __restrict__ off,
+                                        const uint32_t* __restrict__ idx,
+                                        const double* __restrict__ lens, DjkState st,
+                                        DevProfile lanes) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= slots * kLanes) return;
+    const int64_t item = item0 + t;
+    if (item >= k) return;
+    const uint32_t s = (uint32_t)(list ? list[item] : item);
+    const size_t o = (size_t)t * n;
+    double* dist = st.dist + o;
+    uint32_t* done = st.rank + o;
+    uint32_t* pos = st.pos + o;
+    for (size_t v = 0; v < n; v++) {
+        dist[v] = INFINITY;
+        done[v] = 0;
+        pos[v] = kNone;
+    }
+    Heap4 heap{st.hkey + o, st.hnode + o, pos, 0};
+    dist[s] = 0.0;
+    heap.push_or_decrease(s, 0.0);
+    unsigned long long reached = 0;
+    double total = 0.0, harmonic = 0.0, longest = 0.0;
+    double d;
+    uint32_t v;
+    while (heap.pop(d, v)) {
+        done[v] = 1;
+        if (v != s) {
+            reached++;
+            total += d;
+            harmonic += 1.0 / d;
+            if (d > longest) longest = d;
+        }
+        for (uint32_t a = off[v]; a < off[v + 1]; a++) {
+            const uint32_t w = idx[a];
+            if (done[w]) continue;
+            const double alt = d + lens[a];
+            if (alt < dist[w]) {
+                dist[w] = alt;
+                heap.push_or_decrease(w, alt);
+            }
+        }
+    }
+    lanes.reached[t] = reached;
+    lanes.itotal[t] = 0;
+    lanes.ftotal[t] = total;
+    lanes.harmonic[t] = harmonic;
+    lanes.longest[t] = longest;
+}
+
+// Closeness (harmonic == 0) or harmonic centrality from the profiles,
+// closeness.rs:131-209 (unit: the common length of uniform weights, which
+// hop counts are multiplied by). rule: 0 ignore, 1 Wasserman-Faust, 2 zero,
+// 3 max plus one (longest: the largest finite length of any source).
+__global__ void closeness_kernel(DevProfile prof, size_t n, int use_ftotal, double unit,
+                                 int harmonic, int normalized, int rule,
+                                 const unsigned long long* __restrict__ longest_bits,
+                                 double* __restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        double total = use_ftotal ? prof.ftotal[v] : (double)prof.itotal[v];
+        double h = prof.harmonic[v];
+        if (unit != 1.0) {
+            total *= unit;
+            h /= unit;
+        }
+        if (harmonic) {
+            out[v] = h * (normalized && n > 1 ? 1.0 / (double)(n - 1) : 1.0);
+            continue;
+        }
+        const unsigned long long reached = prof.reached[v];
+        const double r = (double)reached, others = n > 0 ? (double)(n - 1) : 0.0;
+        double x;
+        if (rule == 0) {
+            x = reached == 0 ? NAN : (normalized ? r / total : 1.0 / total);
+        } else if (rule == 1) {
+            x = reached == 0 ? 0.0 : (r / total) * (r / others);
+        } else if (rule == 2) {
+            x = n < 2 ? NAN
+                : reached < n - 1 ? 0.0
+                : normalized ? others / total : 1.0 / total;
+        } else {
+            double longest = __longlong_as_double((long long)*longest_bits);
+            if (unit != 1.0) longest *= unit;
+            const double t2 = total + (others - r) * (longest + 1.0);
+            x = n < 2 ? NAN : normalized ? others / t2 : 1.0 / t2;
+        }
+        out[v] = x;
+    }
+}
+
+// The largest longest length, as bits (non-negative doubles order as bits).
+__global__ void max_bits_kernel(const double* __restrict__ vals, size_t n,
+                                unsigned long long* __restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    unsigned long long m = 0;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+        m = max(m, (unsigned long long)__double_as_longlong(vals[v]));
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned long long)__shfl_xor((long long)m, o, 64));
+    if (lane_id() == 0 && m) atomicMax(out, m);
+}
+
 // vals[v] *= flag[v] ? f_flagged : f_other (flag == nullptr: f_other).
 __global__ void scale_flagged_kernel(double* __restrict__ vals, size_t n,
                                      const uint8_t* __restrict__ flag, double f_flagged,
@@ -2454,6 +2699,11 @@ struct Engine {
     bool tie_exact = false;           // real lengths tie only when equal
     bool quirk = false;               // NetworkX's subset edge split (edge runs)
     uint32_t* npr = nullptr;          // predecessor-arc rows (quirk runs)
+    // Distance-profile runs (breadth-first): per-lane profiles of the
+    // super-batch (5 * slots * kLanes words from lane_buf, cleared at the
+    // start of every search) and per-lane level counts.
+    uint64_t* lane_buf = nullptr;
+    unsigned long long* lane_count = nullptr;
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -2508,6 +2758,11 @@ struct Engine {
     bool run_bfs(int64_t item0) {
         const DeviceGraph& d = g.dev;
         reset_slots();
+        if (lane_buf) {
+            const size_t words = (size_t)5 * slots * kLanes;
+            fill_u64_kernel<<<blocks_for_threads(words), kBlock, 0, st>>>(lane_buf, words, 0);
+            launch_check();
+        }
         init_sources_kernel<uint32_t><<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
             d_list, item0, k, slots, n, ws.vis.p, level_bits(0), ws.sig.p, nullptr, nullptr, 0);
         launch_check();
@@ -2597,6 +2852,7 @@ struct Engine {
             }
             uint32_t ql = ws.host_ctrl->qlen;
             if (ql == qe) break;
+            if (lane_buf) count_level(nxt, (uint32_t)dl + 1);
             lv.push_back(ql);
             hv.push_back(ws.host_ctrl->hlen);
             hc.push_back(ws.host_ctrl->hchunks);
@@ -2607,6 +2863,30 @@ struct Engine {
         const int D = (int)lv.size() - 2;
         ws.zero_words = (size_t)std::min(D + 2, kLevelBufs) * slots * n;
         return true;
+    }
+
+    // Adds breadth-first level `level` (bitmap `bits`) to the lane profiles.
+    void count_level(const uint64_t* bits, uint32_t level) {
+        endpoint_lanes_kernel<<<resident_blocks(endpoint_lanes_kernel,
+                                                (n + 63) / 64 * (size_t)slots),
+                                kBlock, 0, st>>>(bits, slots, n, nullptr, lane_count);
+        launch_check();
+        level_profile_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+            lane_count, slots, level, profile_view(lane_buf, (size_t)slots * kLanes));
+        launch_check();
+    }
+
+    // Lane profiles of a finished bucketed search, from its distance rows.
+    template <class P>
+    void dist_profile(DevProfile lanes, uint32_t* pc, unsigned long long* pit, double* pft,
+                      double* ph, double* pl) {
+        const size_t chunks = (n + kProfileChunk - 1) / kProfileChunk;
+        dist_profile_kernel<P><<<resident_blocks(dist_profile_kernel<P>, chunks * slots), kBlock,
+                                 0, st>>>(dist_rows<P>(), slots, n, pc, pit, pft, ph, pl);
+        launch_check();
+        dist_profile_reduce_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+            pc, pit, pft, ph, pl, slots, n, lanes);
+        launch_check();
     }
 
     // Lanes with entries on level d, per slot (flanes, by level parity).
@@ -3139,6 +3419,123 @@ void finish_edge_scores(Graph& g, bool normalized, double* d_eb, double* out) {
     }
     NETBC_HIP(hipMemcpyAsync(out, d_eb, m * sizeof(double), hipMemcpyDeviceToHost, st));
     NETBC_HIP(hipStreamSynchronize(st));
+}
+
+void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const ClosenessRun& cr,
+                   double* out, RunDiag& diag) {
+    const RunOptions opt = tuned_options(g, L, opt_in);
+    hipStream_t st = g.stream;
+    const size_t n = g.n;
+    diag.path_kind = L.kind;
+    if (n == 0) return;
+    const int64_t k = (int64_t)n;
+    Workspace& ws = get_workspace(g);
+    const bool per_thread = L.kind == kReal && !real_bucketed(L, n);
+    const int slots = choose_slots(g, L, k, opt, false);
+    diag.slots = slots;
+    ensure_workspace(ws, g, L, slots, opt.precision,
+                     std::max<size_t>(queue_capacity(slots, n), ws.qcap));
+    const size_t S = (size_t)slots * kLanes;
+    DevBuf<uint64_t> pbuf, lbuf;  // profiles per node; per lane + level counts
+    pbuf.alloc(5 * n);
+    lbuf.alloc(6 * S);
+    NETBC_HIP(hipMemsetAsync(pbuf.p, 0, 5 * n * 8, st));
+    NETBC_HIP(hipMemsetAsync(lbuf.p, 0, 6 * S * 8, st));
+    const DevProfile prof = profile_view(pbuf.p, n);
+    const DevProfile lanes = profile_view(lbuf.p, S);
+    const int64_t* d_list = nullptr;
+    // Every node is a source once, so the batching order changes nothing
+    // in the results (each profile is summed within its own search).
+    if (!opt.exact && k >= kReorderMin && !per_thread) {
+        order_sources(g, ws, nullptr, k, st);
+        d_list = ws.list2.p;
+    }
+    NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, sizeof(Ctrl), st));
+    ws.clean_rows(st);
+    ws.rows_dirty = true;
+    DevBuf<double> djk_buf;
+    DevBuf<uint32_t> djk_u32;
+    DjkState djk{};
+    if (per_thread) {
+        const size_t T = S * n;
+        djk_buf.alloc(2 * T);
+        djk_u32.alloc(3 * T);
+        djk = DjkState{djk_buf.p, nullptr, nullptr, djk_u32.p, nullptr, djk_buf.p + T,
+                       djk_u32.p + T, djk_u32.p + 2 * T};
+    }
+    const size_t chunks = (n + kProfileChunk - 1) / kProfileChunk;
+    DevBuf<uint32_t> pc;
+    DevBuf<unsigned long long> pit;
+    DevBuf<double> pft, ph, pl;
+    if (!per_thread && L.kind != kUniform) {
+        pc.alloc(chunks * S);
+        pit.alloc(chunks * S);
+        pft.alloc(chunks * S);
+        ph.alloc(chunks * S);
+        pl.alloc(chunks * S);
+    }
+    NETBC_HIP(hipStreamSynchronize(st));
+    Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k, false};
+    if (L.kind == kUniform) {
+        eng.lane_buf = lbuf.p;
+        eng.lane_count = (unsigned long long*)lbuf.p + 5 * S;
+    }
+    const DeviceGraph& dg = g.dev;
+    NETBC_HIP(hipEventRecord(ws.ev0, st));
+    for (int64_t item0 = 0; item0 < k; item0 += (int64_t)S) {
+        if (per_thread) {
+            dijkstra_profile_kernel<<<slots, kLanes, 0, st>>>(d_list, item0, k, slots, n,
+                                                             dg.out_off.p, dg.out_idx.p,
+                                                             L.out_real.p, djk, lanes);
+            NETBC_HIP(hipGetLastError());
+        } else {
+            for (;;) {
+                const bool ok = L.kind == kUniform   ? eng.run_bfs(item0)
+                                : L.kind == kInteger ? eng.run_bucketed<IntPath>(item0)
+                                                     : eng.run_bucketed<RealPath>(item0);
+                if (ok) break;
+                eng.clear_rows();
+                grow_queue(ws);
+            }
+            if (L.kind == kInteger) eng.dist_profile<IntPath>(lanes, pc.p, pit.p, pft.p, ph.p, pl.p);
+            else if (L.kind == kReal)
+                eng.dist_profile<RealPath>(lanes, pc.p, pit.p, pft.p, ph.p, pl.p);
+            eng.clear_rows();  // path counts of the search: rows start at zero
+        }
+        commit_profile_kernel<<<blocks_for_threads(S), kBlock, 0, st>>>(lanes, slots, d_list,
+                                                                        item0, k, prof);
+        NETBC_HIP(hipGetLastError());
+    }
+    // values, closeness.rs:131-209
+    DevBuf<unsigned long long> longest;
+    longest.alloc(1);
+    NETBC_HIP(hipMemsetAsync(longest.p, 0, 8, st));
+    if (!cr.harmonic && cr.unreachable == 3) {
+        max_bits_kernel<<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 20)), kBlock, 0,
+                          st>>>(prof.longest, n, longest.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    DevBuf<double> vals;
+    vals.alloc(n);
+    closeness_kernel<<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22)), kBlock, 0, st>>>(
+        prof, n, L.kind == kReal ? 1 : 0, cr.unit, cr.harmonic ? 1 : 0, cr.normalized ? 1 : 0,
+        cr.unreachable, longest.p, vals.p);
+    NETBC_HIP(hipGetLastError());
+    NETBC_HIP(hipEventRecord(ws.ev1, st));
+    std::vector<unsigned long long> reached(n);
+    NETBC_HIP(hipMemcpyAsync(out, vals.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipMemcpyAsync(reached.data(), prof.reached, n * 8, hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipStreamSynchronize(st));
+    ws.rows_dirty = false;
+    float ms = 0.0f;
+    NETBC_HIP(hipEventElapsedTime(&ms, ws.ev0, ws.ev1));
+    diag.kernel_ms += ms;
+    for (size_t v = 0; v < n; v++) diag.reached_nodes += (int64_t)reached[v];
+    read_ctrl(ws, st);
+    double mx;
+    const unsigned long long bits = ws.host_ctrl->max_sigma_bits;
+    memcpy(&mx, &bits, 8);
+    diag.max_sigma = std::max(diag.max_sigma, mx);
 }
 
 void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,

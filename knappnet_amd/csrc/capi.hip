@@ -475,6 +475,59 @@ void subset_entry(const GraphHandle* h, const int64_t* sources, int64_t n_source
     fill_diag(diag, d, 0.0);
 }
 
+// Closeness or harmonic centrality (topography.rs:167-199 ->
+// closeness.rs:131-209): the option checks in the reference's order, the
+// graph oriented for the mode (closeness.rs:225-238: in-paths are the
+// out-paths of the reversed graph, paths ignoring direction those of the
+// undirected graph), the path lengths, then the profiles and values on the
+// device.
+void closeness_entry(const GraphHandle* h, int32_t harmonic, int32_t mode, int32_t weighted,
+                     int32_t weights_as, int32_t normalized, int32_t unreachable,
+                     const netbc_options* opt, double* out, netbc_diagnostics* diag) {
+    netbc::Graph& g = handle(h);
+    if (mode < 1 || mode > 3)
+        throw netbc::Error("unknown degree mode code " + std::to_string(mode));
+    check_kind(weights_as);
+    if (unreachable < 0 || unreachable > 3)
+        throw netbc::Error("unknown code " + std::to_string(unreachable) +
+                           " for unreachable nodes");
+    if (!harmonic && unreachable == 1 && !normalized)
+        throw netbc::Error(
+            "the Wasserman-Faust closeness index is defined only in normalized form");
+    if (!harmonic && unreachable == 3 && weighted)
+        throw netbc::Error(
+            "closeness with unreachable lengths set to the longest length plus 1 is defined "
+            "only for unweighted networks");
+    check_options(opt);
+    std::lock_guard<std::mutex> lock(g.mu);
+    netbc::Graph derived;
+    netbc::Graph* pg = &g;
+    if (g.directed && mode != 1) {
+        derived.n = g.n;
+        derived.m = g.m;
+        derived.directed = mode == 2;
+        derived.has_weights = g.has_weights;
+        derived.src = mode == 2 ? g.tgt : g.src;
+        derived.tgt = mode == 2 ? g.src : g.tgt;
+        derived.w = g.w;
+        pg = &derived;
+    }
+    const double build_ms = prepare(*pg);
+    netbc::Lengths L;
+    netbc::classify_lengths(*pg, weighted != 0, weights_as, L);
+    netbc::ClosenessRun cr;
+    cr.harmonic = harmonic != 0;
+    cr.normalized = normalized != 0;
+    cr.unreachable = unreachable;
+    // uniform weights: hop counts times the common length
+    // (closeness.rs:325-344)
+    if (L.kind == netbc::kUniform && weighted && pg->m > 0)
+        cr.unit = weights_as == 1 ? 1.0 / pg->w[0] : pg->w[0];
+    netbc::RunDiag d;
+    netbc::run_closeness(*pg, L, options(opt), cr, out, d);
+    fill_diag(diag, d, build_ms);
+}
+
 // Fills a host graph from edges, with net_graph_new's checks and texts
 // (graph.rs:142-234); `prefix` names the graph within a batch.
 void fill_graph(netbc::Graph& g, int64_t n, const int64_t* src, const int64_t* tgt,
@@ -603,6 +656,28 @@ NETBC_EXPORT int32_t netbc_betweenness_subset_ex(
     return guard(err, err_len, [&] {
         subset_entry(h, sources, n_sources, targets, n_targets, weighted, weights_as, endpoints,
                      normalized, rescale, parallel, edge_split, tie, opt, nodes, edges, diag);
+    });
+}
+
+NETBC_EXPORT int32_t net_closeness(const GraphHandle* h, int32_t harmonic, int32_t mode,
+                                   int32_t weighted, int32_t weights_as, int32_t normalized,
+                                   int32_t unreachable, int32_t threads, double* out, char* err,
+                                   size_t err_len) {
+    (void)threads;
+    return guard(err, err_len, [&] {
+        closeness_entry(h, harmonic, mode, weighted, weights_as, normalized, unreachable, nullptr,
+                        out, nullptr);
+    });
+}
+
+NETBC_EXPORT int32_t netbc_closeness_ex(const GraphHandle* h, int32_t harmonic, int32_t mode,
+                                        int32_t weighted, int32_t weights_as, int32_t normalized,
+                                        int32_t unreachable, const netbc_options* opt,
+                                        double* out, netbc_diagnostics* diag, char* err,
+                                        size_t err_len) {
+    return guard(err, err_len, [&] {
+        closeness_entry(h, harmonic, mode, weighted, weights_as, normalized, unreachable, opt,
+                        out, diag);
     });
 }
 

@@ -186,6 +186,23 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                  const RunOptions& opt, double* d_bc, RunDiag& diag, double* d_eb = nullptr,
                  const SubsetRun* sub = nullptr);
 
+// Closeness or harmonic centrality of every node (closeness.rs:131-209)
+// from the distance profiles of every node as a source (closeness.rs:
+// 240-345), paths along the out-arcs of g (the caller orients g for the
+// in and all modes). unreachable: 0 ignore, 1 Wasserman-Faust, 2 zero, 3
+// the longest length plus one; unit: the common length of uniform weights
+// (1 when unweighted), which hop counts are multiplied by.
+struct ClosenessRun {
+    bool harmonic = false;
+    bool normalized = false;
+    int unreachable = 0;
+    double unit = 1.0;
+};
+
+// Writes the n values to host `out`.
+void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt, const ClosenessRun& cr,
+                   double* out, RunDiag& diag);
+
 // vals[v] *= flag[v] ? f_flagged : f_other on the graph's stream (flag:
 // device, may be nullptr).
 void scale_flagged(Graph& g, double* d_vals, size_t count, const uint8_t* d_flag,

@@ -129,6 +129,32 @@ class Graph:
             raise EngineError(err.value.decode())
         return nodes[: self.n], None if ev is None else ev[:m], diag.as_dict()
 
+    def closeness(self, harmonic: bool = False, mode: int = 1, weighted: bool = False,
+                  weights_as: str = "distance", normalized: bool = False, unreachable: int = 0,
+                  options: dict | None = None):
+        """``net_closeness``: closeness (or harmonic centrality) of every node.
+        mode 1 out, 2 in, 3 all; unreachable 0 ignore, 1 Wasserman-Faust,
+        2 zero, 3 the longest length plus one. With ``options``,
+        ``netbc_closeness_ex``: returns (values, diagnostics dict)."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        args = (self.handle, int(bool(harmonic)), int(mode), int(bool(weighted)), kind,
+                int(bool(normalized)), int(unreachable))
+        if options is None:
+            if self.lib.net_closeness(*args, 0, _ptr(out, c_double), err, ERR_LEN):
+                raise EngineError(err.value.decode())
+            return out[: self.n]
+        o = dict(precision=0, slots=0, direction=0, exact=False, alpha=0.0, sweep_group=0)
+        o.update(options)
+        opt = _native.Options(o["precision"], o["slots"], o["direction"], int(bool(o["exact"])),
+                              o["alpha"], o["sweep_group"], 0)
+        diag = _native.Diagnostics()
+        if self.lib.netbc_closeness_ex(*args, ctypes.byref(opt), _ptr(out, c_double),
+                                       ctypes.byref(diag), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return out[: self.n], diag.as_dict()
+
     def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, precision,
                       slots, direction, alpha, exact, sweep_group):
         out = np.zeros(max(length, 1), dtype=np.float64)
