@@ -477,7 +477,7 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
 // bfs_hub_kernel). Light rows are taken 64 entries per wavefront and walked
 // as one concatenation 64 arcs at a time; heavy rows (listed in hq) as
 // tasks of kChunkArcs arcs. Threads also clear the stale level d-1 entries
-// of the next-level bitmap.
+// of the next-level bitmap. succ may be null (searches without a sweep).
 __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                   uint32_t zbeg, uint32_t zend, const uint32_t* __restrict__ hq,
                                   uint32_t hbeg, uint32_t hend, const uint32_t* __restrict__ hpre,
@@ -573,7 +573,7 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
                     if (lane >= o && cy == c) run |= y;
                 }
                 const int cn = __shfl_down(c, 1, 64);
-                if ((lane == 63 || cn != c) && run)
+                if (succ && (lane == 63 || cn != c) && run)
                     atomicOr((unsigned long long*)&succ[(size_t)bc * n + vc], (unsigned long long)run);
             }
         } else {
@@ -598,7 +598,7 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
                 offer(x, en.b, nv);
             }
             offered = wave_or(offered);
-            if (lane == 0 && offered)
+            if (succ && lane == 0 && offered)
                 atomicOr((unsigned long long*)&succ[base + en.v], (unsigned long long)offered);
         }
     }
@@ -670,6 +670,38 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
     NETBC_FLUSH(ctrl, 14, c6);
 }
 
+// Bottom-up hub tasks of a search for reachability only (no path counts):
+// a wavefront per task (slot b, chunk of hub x's in-arcs) ORs, 64 arcs at a
+// time, the lanes of N = F_b & ~vis[x] held at level d by the in-neighbours,
+// and stops once every lane of N is found. The lanes found are ORed into
+// cand[x] and x enters the touched list with its first lanes.
+__global__ void reach_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chunks, int slots,
+                                 const uint32_t* __restrict__ in_idx,
+                                 const uint64_t* __restrict__ cur, uint64_t* __restrict__ cand,
+                                 const uint64_t* __restrict__ vis, uint2* __restrict__ touched,
+                                 uint32_t* __restrict__ tlen, const uint64_t* __restrict__ flanes,
+                                 size_t n) {
+    const int lane = lane_id();
+    const size_t tasks = (size_t)chunks * slots;
+    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+        const uint32_t b = (uint32_t)(w / chunks);
+        const uint4 ht = hub_task[w % chunks];
+        const size_t p = (size_t)b * n + ht.x;
+        const uint64_t N = flanes[b] & ~vis[p];
+        if (!N) continue;
+        const uint64_t* crow = cur + (size_t)b * n;
+        uint64_t got = 0;
+        for (uint32_t a = ht.y; a < ht.z && got != N; a += 64) {
+            uint64_t m = 0;
+            if (a + lane < ht.z) m = crow[in_idx[a + lane]] & N;
+            got |= wave_or(m);
+        }
+        if (lane == 0 && got &&
+            atomicOr((unsigned long long*)&cand[p], (unsigned long long)got) == 0)
+            touched[atomicAdd(tlen, 1u)] = make_uint2(ht.x, b);
+    }
+}
+
 // Lanes of each slot that hold entries of a level (the lanes whose search
 // is still running): fl[b] |= the masks of the level's entries in slot b,
 // ORed per block in LDS and added with one atomic per block and slot.
@@ -695,7 +727,9 @@ __global__ void frontier_lanes_kernel(const QEnt* __restrict__ q, uint32_t qbeg,
 // bottom-up step only hubs are listed, whose counts come from
 // bfs_hub_kernel). Also counts the out-arcs of the new entries and the
 // in-arcs of pairs whose active lanes are now all reached (the direction
-// choice of the next step).
+// choice of the next step). Without kCount (reachability only) no path
+// counts are gathered.
+template <bool kCount>
 __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
                                   const uint32_t* __restrict__ tlen_cur,
                                   uint32_t* __restrict__ tlen_next,
@@ -729,8 +763,9 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
             i1 = in_off[te.x + 1];
             if (!hub_flag[te.x]) deg = i1 - i0;
         }
-        (void)gather_counts<false>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig, n, c4, c5,
-                                   npr);
+        if constexpr (kCount)
+            (void)gather_counts<false>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig, n, c4,
+                                       c5, npr);
         if (ok) {
             cand[p] = 0;
             const uint64_t vn = vis[p] | N;
@@ -774,7 +809,10 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
 // the XCDs (xcd_wave). Hubs are left to bfs_hub_kernel and
 // bfs_gather_kernel. Threads also clear the level d-1 entries of the next
 // bitmap (AND-NOT, commuting with the OR of the new lanes: a node's lanes
-// at levels d-1 and d+1 are disjoint).
+// at levels d-1 and d+1 are disjoint). Without kCount (reachability only)
+// each pair walks its own in-row and stops at the first in-neighbours that
+// cover every lane sought.
+template <bool kCount>
 __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
                                 const uint32_t* __restrict__ in_off,
                                 const uint32_t* __restrict__ in_idx,
@@ -823,8 +861,17 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
             }
         }
         NETBC_COUNT(c6, deg != 0);
-        const uint64_t mine =
-            gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5, npr);
+        uint64_t mine = 0;
+        if constexpr (kCount) {
+            mine = gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5, npr);
+        } else if (deg) {
+            const uint64_t* crow = cur + (size_t)b * n;
+            const uint32_t* row = in_idx + i0;
+            uint32_t a = 0;
+            for (; a + 4 <= deg && mine != U; a += 4)
+                mine |= (crow[row[a]] | crow[row[a + 1]] | crow[row[a + 2]] | crow[row[a + 3]]) & U;
+            for (; a < deg && mine != U; a++) mine |= crow[row[a]] & U;
+        }
         if (mine) {
             const uint64_t vn = V | mine;
             vis[p] = vn;
@@ -1429,8 +1476,9 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
 // joins the level queue (both skipped for the sources at D = 0). Then every
 // out-arc x->y offers dist[x] + len to the lanes of y not yet settled, by
 // atomic minimum, and marks the offer's bucket mod W at y. Offers land in
-// later buckets, so settled distances never change here.
-template <class P>
+// later buckets, so settled distances never change here. Without kCount
+// (distances only) no path counts are pulled.
+template <class P, bool kCount = true>
 __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t* __restrict__ llen,
                                  uint32_t D, int W, int words, int first_step, double delta,
                                  uint64_t* __restrict__ pend,
@@ -1480,7 +1528,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         if (!first_step) {
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
-            const uint32_t a0 = in_off[x], a1 = in_off[x + 1];
+            const uint32_t a0 = in_off[x], a1 = kCount ? in_off[x + 1] : a0;
             for (uint32_t c = a0; c < a1; c += 64) {
                 const uint32_t a = c + lane;
                 const bool ok = a < a1;
@@ -1522,7 +1570,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                         }
                 }
             }
-            if (mine) {
+            if (kCount && mine) {
                 sig[(base + x) * kLanes + lane] = acc;
                 if (npr) npr[(base + x) * kLanes + lane] = npred;
                 smax = fmax(smax, acc);
@@ -2149,8 +2197,8 @@ __global__ void fold_arcs_kernel(const double* __restrict__ eacc, int rows, size
 // ep[v], for a target v, the lanes of the super-batch that reached it
 // (including a lane whose source is v); endpoint_lanes_kernel counts, per
 // (slot, lane), the targets the lane reached (including its source when
-// that is a target), 64 nodes per wavefront: lane l tests bit l of the 64
-// loaded words in turn. endpoint_sources_kernel adds each lane's count to
+// that is a target), 64 nodes per wavefront: lane l tests bit l of each
+// nonzero loaded word in turn. endpoint_sources_kernel adds each lane's count to
 // its source, less 2 when the source is a target (the two self counts).
 // The counts are integers, so the order of the atomic adds does not matter.
 __global__ void endpoint_nodes_kernel(const uint64_t* __restrict__ vis, int slots, size_t n,
@@ -2175,7 +2223,12 @@ __global__ void endpoint_lanes_kernel(const uint64_t* __restrict__ vis, int slot
         uint64_t word = 0;
         if (v < n && (!target || target[v])) word = vis[b * n + v];
         unsigned long long c = 0;
-        for (int j = 0; j < 64; j++) c += (rl64(word, j) >> lane) & 1;
+        uint64_t nz = __ballot(word != 0);  // only the nonzero words
+        while (nz) {
+            const int j = __ffsll((unsigned long long)nz) - 1;
+            nz &= nz - 1;
+            c += (rl64(word, j) >> lane) & 1;
+        }
         if (c) atomicAdd(&lane_count[b * kLanes + lane], c);
     }
 }
@@ -2704,6 +2757,7 @@ struct Engine {
     // start of every search) and per-lane level counts.
     uint64_t* lane_buf = nullptr;
     unsigned long long* lane_count = nullptr;
+    bool reach_only = false;  // distances only, no path counts
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -2806,8 +2860,8 @@ struct Engine {
                                    (opt.direction == 0 && f_arcs * alpha > open_arcs);
             if (bottom_up) {
                 diag.bottom_up_levels++;
-                bfs_pull_kernel<<<xcd_grid(resident_blocks(bfs_pull_kernel, (pairs + 63) / 64)),
-                                  kBlock, 0, st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
+                auto pull = reach_only ? bfs_pull_kernel<false> : bfs_pull_kernel<true>;
+                pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, 0, st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
@@ -2820,11 +2874,18 @@ struct Engine {
                 size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
                 bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
-                    d.out_idx.p, ws.vis.p, ws.cand.p, nxt, ws.succ.p, ws.touched.p,
+                    d.out_idx.p, ws.vis.p, ws.cand.p, nxt, reach_only ? nullptr : ws.succ.p,
+                    ws.touched.p,
                     &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, &front[dl & 1], gsz, n);
                 launch_check();
             }
-            if (d.hubs) {
+            if (d.hubs && reach_only && bottom_up) {
+                const size_t hub_tasks = (size_t)d.hub_chunks * slots;
+                reach_hub_kernel<<<resident_blocks(reach_hub_kernel, hub_tasks), kBlock, 0, st>>>(
+                    d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(), cur, ws.cand.p, ws.vis.p,
+                    ws.touched.p, &ws.ctrl.p->tlen[dl & 1], frontier_lanes(dl), n);
+                launch_check();
+            } else if (d.hubs && !reach_only) {
                 const size_t hub_tasks = (size_t)d.hub_chunks * slots;
                 const uint32_t gsz = group_size(hub_tasks, resident_waves(bfs_hub_kernel));
                 bfs_hub_kernel<<<xcd_grid(resident_blocks(bfs_hub_kernel,
@@ -2836,8 +2897,8 @@ struct Engine {
                                                   frontier_lanes(dl), ws.ctrl.p, gsz, n, npr);
                 launch_check();
             }
-            bfs_gather_kernel<<<resident_blocks(bfs_gather_kernel, (pairs + 63) / 64), kBlock, 0,
-                                st>>>(ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
+            auto gather = reach_only ? bfs_gather_kernel<false> : bfs_gather_kernel<true>;
+            gather<<<resident_blocks(gather, (pairs + 63) / 64), kBlock, 0, st>>>(ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
                                       &ws.ctrl.p->tlen[(dl + 1) & 1], d.in_off_p(), d.in_idx_p(),
                                       d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
                                       ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.q.p,
@@ -3015,7 +3076,8 @@ struct Engine {
         lv.push_back(ws.host_ctrl->qlen);
         const size_t pairs = (size_t)slots * n;
         const unsigned scan_blocks = resident_blocks(dial_scan_kernel, (pairs + 63) / 64);
-        const unsigned step_blocks = resident_blocks(dial_step_kernel<P>, pairs);
+        auto step = reach_only ? dial_step_kernel<P, false> : dial_step_kernel<P, true>;
+        const unsigned step_blocks = resident_blocks(step, pairs);
         // the first step's bucket minimum starts empty; later ones are reset
         // by the scan before them
         NETBC_HIP(hipMemsetAsync(&ws.ctrl.p->next_bucket[0], 0xFF, sizeof(unsigned long long),
@@ -3028,7 +3090,7 @@ struct Engine {
                 &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p], &ws.ctrl.p->left[p ^ 1],
                 &ws.ctrl.p->next_bucket[p], &ws.ctrl.p->next_bucket[p ^ 1], n);
             launch_check();
-            dial_step_kernel<P><<<step_blocks, kBlock, 0, st>>>(
+            step<<<step_blocks, kBlock, 0, st>>>(
                 ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
                 d.in_off_p(), d.in_idx_p(), in_lengths<P>(), d.out_off.p, d.out_idx.p,
                 out_lengths<P>(), dist, ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p,
@@ -3476,6 +3538,7 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     }
     NETBC_HIP(hipStreamSynchronize(st));
     Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k, false};
+    eng.reach_only = true;
     if (L.kind == kUniform) {
         eng.lane_buf = lbuf.p;
         eng.lane_count = (unsigned long long*)lbuf.p + 5 * S;
@@ -3500,7 +3563,13 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
             if (L.kind == kInteger) eng.dist_profile<IntPath>(lanes, pc.p, pit.p, pft.p, ph.p, pl.p);
             else if (L.kind == kReal)
                 eng.dist_profile<RealPath>(lanes, pc.p, pit.p, pft.p, ph.p, pl.p);
-            eng.clear_rows();  // path counts of the search: rows start at zero
+            // only the sources' own path counts were written
+            const uint32_t nsrc = eng.lv[1];
+            if (nsrc) {
+                zero_sources_kernel<<<blocks_for_threads(nsrc), kBlock, 0, st>>>(ws.q.p, nsrc,
+                                                                                ws.sig.p, n);
+                NETBC_HIP(hipGetLastError());
+            }
         }
         commit_profile_kernel<<<blocks_for_threads(S), kBlock, 0, st>>>(lanes, slots, d_list,
                                                                         item0, k, prof);

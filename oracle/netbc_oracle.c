@@ -1922,3 +1922,89 @@ int32_t orc_closeness(const OGraph *g0, int32_t harmonic, int32_t mode, int32_t 
     orc_graph_free(h);
     return ORC_OK;
 }
+
+/* ---- sampled distance profiles (timing baseline for closeness) --------- */
+
+/* The profiles of closeness.rs:240-345 for a list of sources, along the
+ * out-arcs of g, `threads` workers (0: every core) each searching every
+ * threads-th source with private scratch. */
+typedef struct {
+    const OGraph *g;
+    const Lengths *L;
+    const int64_t *src;
+    uint64_t k, stride, first;
+    uint64_t *reached;
+    double *total, *harmonic, *longest;
+} ProfJob;
+
+static void *profile_worker(void *arg) {
+    ProfJob *j = arg;
+    const OGraph *g = j->g;
+    uint64_t n = g->n;
+    uint32_t *hops = malloc((n ? n : 1) * sizeof *hops), *queue = malloc((n ? n : 1) * sizeof *queue);
+    uint64_t *idist = malloc((n ? n : 1) * sizeof *idist);
+    double *dist = malloc((n ? n : 1) * sizeof *dist);
+    uint8_t *done = calloc(n ? n : 1, 1);
+    for (uint64_t v = 0; v < n; v++) {
+        hops[v] = U32_MAX_;
+        idist[v] = UINT64_MAX;
+        dist[v] = INFINITY;
+    }
+    U32Vec touched = {0};
+    U32Vec *buckets = j->L->kind == LEN_INTEGER ? calloc((uint64_t)j->L->max + 1, sizeof *buckets)
+                                                 : NULL;
+    Heap heap;
+    heap.items = malloc((n ? n : 1) * sizeof *heap.items);
+    heap.pos = malloc((n ? n : 1) * sizeof *heap.pos);
+    heap.len = 0;
+    for (uint64_t v = 0; v < n; v++) heap.pos[v] = U32_MAX_;
+    for (uint64_t i = j->first; i < j->k; i += j->stride) {
+        uint32_t s = (uint32_t)j->src[i];
+        OProfile p;
+        if (j->L->kind == LEN_UNIFORM) p = profile_bfs(g, s, hops, queue);
+        else if (j->L->kind == LEN_INTEGER)
+            p = profile_dial(g, j->L->ilens, j->L->max, s, idist, done, buckets, &touched);
+        else p = profile_dijkstra(g, j->L->rlens, s, dist, done, &heap, &touched);
+        j->reached[i] = p.reached;
+        j->total[i] = p.total;
+        j->harmonic[i] = p.harmonic;
+        j->longest[i] = p.longest;
+    }
+    if (buckets) {
+        for (uint64_t b = 0; b <= j->L->max; b++) free(buckets[b].a);
+        free(buckets);
+    }
+    free(touched.a);
+    free(heap.items);
+    free(heap.pos);
+    free(hops); free(queue); free(idist); free(dist); free(done);
+    return NULL;
+}
+
+int32_t orc_distance_profiles(const OGraph *g, int32_t weighted, int32_t weights_as,
+                              const int64_t *sources, int64_t k, int32_t threads,
+                              uint64_t *reached, double *total, double *harmonic,
+                              double *longest, char *err, size_t err_len) {
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (check_kind(weights_as, err, err_len)) return ORC_FAILED;
+    for (int64_t i = 0; i < k; i++)
+        if (sources[i] < 0 || (uint64_t)sources[i] >= g->n) {
+            write_err(err, err_len, "source out of range");
+            return ORC_FAILED;
+        }
+    Lengths L;
+    if (path_lengths(g, weighted, weights_as, 0, &L, err, err_len)) return ORC_FAILED;
+    uint64_t t = threads > 0 ? (uint64_t)threads : available_cores();
+    if (t > (uint64_t)(k > 0 ? k : 1)) t = (uint64_t)(k > 0 ? k : 1);
+    ProfJob *jobs = calloc(t, sizeof *jobs);
+    pthread_t *tid = malloc(t * sizeof *tid);
+    for (uint64_t i = 0; i < t; i++) {
+        jobs[i] = (ProfJob){g, &L, sources, (uint64_t)k, t, i, reached, total, harmonic, longest};
+        pthread_create(&tid[i], NULL, profile_worker, &jobs[i]);
+    }
+    for (uint64_t i = 0; i < t; i++) pthread_join(tid[i], NULL);
+    free(jobs);
+    free(tid);
+    lengths_free(&L);
+    return ORC_OK;
+}

@@ -75,6 +75,10 @@ def lib() -> ctypes.CDLL:
         L.orc_ordered_sum_test.restype = c_double
         L.orc_threads_for.argtypes = [c_int64, c_int64, c_uint64]
         L.orc_threads_for.restype = c_int64
+        L.orc_distance_profiles.argtypes = [
+            c_void_p, c_int32, c_int32, POINTER(c_int64), c_int64, c_int32, POINTER(c_uint64),
+            POINTER(c_double), POINTER(c_double), POINTER(c_double), c_char_p, c_size_t]
+        L.orc_distance_profiles.restype = c_int32
         L.orc_available_cores.argtypes = []
         L.orc_available_cores.restype = c_uint64
         _lib = L
@@ -212,6 +216,21 @@ class OracleGraph:
                                 ERR_LEN):
             raise OracleError(err.value.decode())
         return out[: self.n]
+
+    def distance_profiles(self, sources, weighted=False, weights_as="distance", threads=0):
+        """closeness.rs:240-345 for the listed sources (out-paths): a dict of
+        reached, total, harmonic and longest arrays."""
+        src = np.ascontiguousarray(sources, dtype=np.int64)
+        k = len(src)
+        r = np.zeros(max(k, 1), dtype=np.uint64)
+        t, h, lg = (np.zeros(max(k, 1), dtype=np.float64) for _ in range(3))
+        err = ctypes.create_string_buffer(ERR_LEN)
+        kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
+        if self.L.orc_distance_profiles(self.h, int(weighted), kind, _p(src, c_int64), k,
+                                        int(threads), _p(r, c_uint64), _p(t, c_double),
+                                        _p(h, c_double), _p(lg, c_double), err, ERR_LEN):
+            raise OracleError(err.value.decode())
+        return dict(reached=r[:k], total=t[:k], harmonic=h[:k], longest=lg[:k])
 
     def single_source(self, s, weighted=False, weights_as="distance"):
         """(kind, dist_u64, dist_f64, sigma, delta, reached)."""
