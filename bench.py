@@ -24,7 +24,14 @@ credit per slot and arc it meets). ``--subset`` times betweenness for
 sources and targets (``netbc_betweenness_subset_ex``: the workload's
 sources, targets every third node, NetworkX's subset rescaling, node
 values only); its CPU baseline is the oracle's single-threaded
-restatement.
+restatement. ``--closeness`` times normalized closeness of every node
+(``netbc_closeness_ex``, unreachable nodes ignored: reached / sum of
+distances) on the same workload; the roofline kernel is the
+reachability-only bottom-up step (``bfs_pull_kernel<false>``), whose
+algorithmic bytes the engine counts as it runs (8 per pair examined, 8 per
+in-row looked up, 12 per in-arc read, 40 per pair reached, 24 per stale
+level entry cleared), and the CPU baseline is the oracle's distance
+profiles of a strided sample of sources on every host core.
 """
 from __future__ import annotations
 
@@ -44,6 +51,7 @@ METRIC = ("GTEPS for exact betweenness (sources×edges/s), FP32-δ vs FP64 at re
           "≤1e-6")
 METRIC_EDGE = "GTEPS for exact edge betweenness (sources×edges/s)"
 METRIC_SUBSET = "GTEPS for exact betweenness for sources and targets (sources×edges/s)"
+METRIC_CLOSENESS = "GTEPS for closeness centrality (sources×edges/s)"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -194,6 +202,8 @@ def main() -> None:
                     help="edge betweenness (net_edge_betweenness) instead of node betweenness")
     ap.add_argument("--subset", action="store_true",
                     help="betweenness for sources and targets (every third node a target)")
+    ap.add_argument("--closeness", action="store_true",
+                    help="closeness of every node (net_closeness, normalized) instead")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -223,6 +233,11 @@ def main() -> None:
     if args.subset:
         config = dict(config, workload=config["workload"] + "_targets_every_third",
                       rescale="networkx_subset")
+    if args.closeness:
+        if sources is not None:
+            raise SystemExit("closeness runs take every node as a source")
+        config = dict(config, workload=config["workload"] + "_closeness", normalized=True,
+                      unreachable="ignore")
     g = Graph(el.n, el.src, el.tgt, el.weights, el.directed)
     run = g.edge_betweenness_sources if args.edge else g.betweenness_sources
     targets = np.arange(0, el.n, 3)
@@ -233,6 +248,10 @@ def main() -> None:
                 options=dict(precision=precision, slots=slots, direction=direction,
                              alpha=alpha))
             return nodes, d
+    if args.closeness:
+        def run(sources=None, weighted=False, precision=0, slots=0, direction=0, alpha=0.0):
+            return g.closeness(weighted=weighted, normalized=True,
+                               options=dict(slots=slots, direction=direction, alpha=alpha))
     k = el.n if sources is None else len(sources)
     print(f"# device: {device_info()}; workload {config['workload']} n={el.n} m={el.m} "
           f"sources={k} (generated in {gen_s:.1f}s)", flush=True)
@@ -264,11 +283,14 @@ def main() -> None:
 
     d = diags[-1]
     sweep_bytes = 8 * d["reached_arcs"] + 32 * d["reached_nodes"]
+    if args.closeness:
+        sweep_bytes = d["dominant_bytes"]
     sweep_s = d["dominant_ms"] / 1e3
     achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": f"{'dial' if weighted else 'bfs'}_backward_kernel<"
+        "kernel": "bfs_pull_kernel<false>" if args.closeness else
+                  f"{'dial' if weighted else 'bfs'}_backward_kernel<"
                   f"{'double' if args.precision == 0 else 'float'}, "
                   f"{'true' if args.edge else 'false'}>",
         "achieved": round(achieved, 2),
@@ -296,6 +318,11 @@ def main() -> None:
             def orun(sample, weighted=False, threads=0):
                 return o.betweenness_subset(sources=sample, targets=targets, weighted=weighted,
                                             rescale=2)[0]
+        if args.closeness:
+            def orun(sample, weighted=False, threads=0):
+                p = o.distance_profiles(sample, weighted=weighted, threads=threads)
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    return np.where(p["reached"] > 0, p["reached"] / p["total"], np.nan)
         pool = np.arange(el.n) if sources is None else np.asarray(sources)
         cores = 1 if args.subset else available_cores()
         take = min(len(pool), 256 if not args.subset else 16)
@@ -312,7 +339,11 @@ def main() -> None:
         t = time.perf_counter()
         orun(sample, weighted=weighted, threads=0)
         ct = min(ct, time.perf_counter() - t)
-        got, _ = run(sample, weighted=weighted, precision=args.precision)
+        if args.closeness:
+            got = np.nan_to_num(res[sample], nan=-1.0)
+            want = np.nan_to_num(want, nan=-1.0)
+        else:
+            got, _ = run(sample, weighted=weighted, precision=args.precision)
         exact = bool(np.array_equal(got, want))
         nz = (got != 0) | (want != 0)
         rel = float(np.max(np.abs(got - want)[nz] / np.maximum(np.abs(want[nz]), 1e-300))) \
@@ -327,7 +358,8 @@ def main() -> None:
               f"device vs oracle on sample: {cpu['check']}", flush=True)
 
     precision_check = None
-    if not args.no_compare and args.precision == 0 and not args.edge and not args.subset:
+    if (not args.no_compare and args.precision == 0 and not args.edge and not args.subset
+            and not args.closeness):
         # The batched FP32-dependency form against FP64 (error) and against
         # the plain FP64 form: sources in the reference's order and sums in
         # its order, bit-identical to the CPU engine.
@@ -356,7 +388,8 @@ def main() -> None:
         print(f"# precision check: {precision_check}", flush=True)
 
     line = {
-        "metric": METRIC_EDGE if args.edge else METRIC_SUBSET if args.subset else METRIC,
+        "metric": (METRIC_EDGE if args.edge else METRIC_SUBSET if args.subset
+                   else METRIC_CLOSENESS if args.closeness else METRIC),
         "value": round(gteps, 3),
         "unit": "GTEPS",
         "n_gpus": 1,

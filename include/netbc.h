@@ -148,13 +148,17 @@ typedef struct netbc_diagnostics {
     int64_t reached_arcs;    /* sum over sources of out-arcs of those nodes */
     int64_t levels;          /* search levels run */
     double kernel_ms;        /* device time of the search and reduction */
-    double dominant_ms;      /* device time of the dependency sweep */
+    double dominant_ms;      /* device time of the dependency sweep
+                                (closeness: of the bottom-up steps) */
     int64_t dominant_launches;
     int32_t slots;           /* 64-source batches per launch used */
     int32_t bottom_up_levels; /* search levels run bottom-up */
     double build_ms;         /* device CSR/CSC build time (first call) */
     int64_t work[16];        /* engine work counters; zero unless the library
                                 is built with NETBC_COUNTERS */
+    int64_t dominant_bytes;  /* closeness runs: algorithmic bytes of the
+                                timed kernel (the reachability-only
+                                bottom-up step), summed over launches */
 } netbc_diagnostics;
 
 /* Betweenness restricted to a source list (NULL with k = n: every node,

@@ -36,7 +36,8 @@ class Diagnostics(Structure):
                 ("reached_arcs", c_int64), ("levels", c_int64),
                 ("kernel_ms", c_double), ("dominant_ms", c_double),
                 ("dominant_launches", c_int64), ("slots", c_int32),
-                ("bottom_up_levels", c_int32), ("build_ms", c_double), ("work", c_int64 * 16)]
+                ("bottom_up_levels", c_int32), ("build_ms", c_double), ("work", c_int64 * 16),
+                ("dominant_bytes", c_int64)]
 
     def as_dict(self) -> dict:
         d = {name: getattr(self, name) for name, _ in self._fields_

@@ -64,6 +64,11 @@ struct Ctrl {
     unsigned long long reached_nodes;
     unsigned long long reached_arcs;
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
+    // Bytes moved by reachability-only bottom-up steps (bfs_pull_kernel
+    // without path counts): 8 per pair examined, 8 per in-row looked up
+    // plus 12 per in-arc read, 40 per pair reached (visited and level words,
+    // out-row, queue entry).
+    unsigned long long reach_bytes;
     // Work counters (filled only in builds with NETBC_COUNTERS):
     // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes
     // (integer lengths: 0 settled pairs, 1 their in-arcs, 2 out-arcs, 3 offers),
@@ -835,7 +840,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
     }
     const uint64_t lt = lanemask_lt(lane);
-    unsigned long long fa = 0, closed = 0;
+    unsigned long long fa = 0, closed = 0, bytes = 0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
     const XcdWave xw = xcd_wave();
     const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
@@ -871,7 +876,9 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
             for (; a + 4 <= deg && mine != U; a += 4)
                 mine |= (crow[row[a]] | crow[row[a + 1]] | crow[row[a + 2]] | crow[row[a + 3]]) & U;
             for (; a < deg && mine != U; a++) mine |= crow[row[a]] & U;
+            bytes += 8 + 12ull * a;
         }
+        if (!kCount && ok) bytes += 8 + (mine ? 40 : 0);
         if (mine) {
             const uint64_t vn = V | mine;
             vis[p] = vn;
@@ -899,9 +906,11 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     NETBC_FLUSH(ctrl, 15, c6);
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
+    if constexpr (!kCount) bytes = wave_sum_u64(bytes);
     if (lane == 0) {
         if (fa) atomicAdd(front_next, fa);
         if (closed) atomicAdd(&ctrl->closed_arcs, closed);
+        if (!kCount && bytes) atomicAdd(&ctrl->reach_bytes, bytes);
     }
 }
 
@@ -2758,6 +2767,9 @@ struct Engine {
     uint64_t* lane_buf = nullptr;
     unsigned long long* lane_count = nullptr;
     bool reach_only = false;  // distances only, no path counts
+    // Reachability-only breadth-first runs time their bottom-up steps with
+    // ws.sweep; bytes of the stale level entries those steps clear.
+    unsigned long long stale_bytes = 0;
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -2861,12 +2873,17 @@ struct Engine {
             if (bottom_up) {
                 diag.bottom_up_levels++;
                 auto pull = reach_only ? bfs_pull_kernel<false> : bfs_pull_kernel<true>;
+                if (reach_only) {
+                    ws.sweep.mark(st);
+                    stale_bytes += 24ull * (ze - zb);  // stale entries cleared
+                }
                 pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, 0, st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
                                         frontier_lanes(dl), slots, left, n, npr);
                 launch_check();
+                if (reach_only) ws.sweep.mark(st);
             } else {
                 const uint32_t gsz =
                     group_size(qe - qb, resident_waves(bfs_expand_kernel));
@@ -3515,6 +3532,10 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, sizeof(Ctrl), st));
     ws.clean_rows(st);
     ws.rows_dirty = true;
+    NETBC_HIP(hipStreamSynchronize(st));
+    ws.sweep.collect();
+    ws.sweep.ms = 0.0;
+    ws.sweep.launches = 0;
     DevBuf<double> djk_buf;
     DevBuf<uint32_t> djk_u32;
     DjkState djk{};
@@ -3601,6 +3622,10 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     diag.kernel_ms += ms;
     for (size_t v = 0; v < n; v++) diag.reached_nodes += (int64_t)reached[v];
     read_ctrl(ws, st);
+    ws.sweep.collect();
+    diag.dominant_ms += ws.sweep.ms;
+    diag.dominant_launches += ws.sweep.launches;
+    diag.dominant_bytes += (int64_t)(ws.host_ctrl->reach_bytes + eng.stale_bytes);
     double mx;
     const unsigned long long bits = ws.host_ctrl->max_sigma_bits;
     memcpy(&mx, &bits, 8);

@@ -92,6 +92,7 @@ void fill_diag(netbc_diagnostics* out, const netbc::RunDiag& d, double build_ms)
     out->bottom_up_levels = (int32_t)d.bottom_up_levels;
     out->build_ms = build_ms;
     for (int i = 0; i < 16; i++) out->work[i] = d.work[i];
+    out->dominant_bytes = d.dominant_bytes;
 }
 
 // Rejects option values the engine does not define.
@@ -132,6 +133,7 @@ void merge_diag(netbc::RunDiag& into, const netbc::RunDiag& d) {
     into.kernel_ms += d.kernel_ms;
     into.dominant_ms += d.dominant_ms;
     into.dominant_launches += d.dominant_launches;
+    into.dominant_bytes += d.dominant_bytes;
     into.slots = std::max(into.slots, d.slots);
     for (int i = 0; i < 16; i++) into.work[i] += d.work[i];
 }
