@@ -111,6 +111,35 @@ int32_t net_closeness(const GraphHandle *h, int32_t harmonic, int32_t mode,
                       int32_t unreachable, int32_t threads, double *out,
                       char *err, size_t err_len);
 
+/* Replaces net_centralization, netcapi.h:534-539 (crates/capi/src/
+ * topography.rs:134-159, crates/engine/src/metrics/topography.rs:376-568).
+ * Freeman centralization: n * max - sum of the scores, divided by the
+ * tool's theoretical maximum when normalized != 0. measure 0 degree (mode
+ * 1 out, 2 in, 3 all; loops counted when loops != 0, igraph only), 1
+ * betweenness (unweighted), 2 closeness (normalized; unreachable nodes
+ * ignored, sna: zero; UCINET: NaN unless every node reaches every other);
+ * convention 0 igraph, 1 NetworkX (an error, as in the reference), 2 sna
+ * and 3 UCINET (parallel edges collapsed, loops ignored). Measure 3
+ * (eigenvector) passes the reference's checks and then fails: it is not
+ * part of this library. Writes one double; `threads` is ignored. */
+int32_t net_centralization(const GraphHandle *h, int32_t measure, int32_t mode,
+                           int32_t loops, int32_t normalized,
+                           int32_t convention, int32_t threads, double *out,
+                           char *err, size_t err_len);
+
+/* Replaces net_twomode_centrality, netcapi.h:753-760 (crates/capi/src/
+ * twomode.rs:229-286, crates/engine/src/metrics/bipartite.rs:130-269).
+ * Degree (0), closeness (1) or betweenness (2) of the affiliation graph
+ * (direction ignored, parallel edges collapsed), normalized for two-mode
+ * data when normalized != 0 (Borgatti and Everett 1997). types: n codes, 0
+ * first mode, 1 second; every edge must join the modes. unreachable
+ * (closeness only): 0 ignore, 1 Wasserman-Faust (normalized only). `out`
+ * holds n doubles; `threads` is ignored. */
+int32_t net_twomode_centrality(const GraphHandle *h, const int32_t *types,
+                               int32_t measure, int32_t normalized,
+                               int32_t unreachable, int32_t threads,
+                               double *out, char *err, size_t err_len);
+
 /* ---- device-path entry points --------------------------------------- */
 
 /* Run options. Zero-initialised means: FP64 dependencies, automatic batch

@@ -88,6 +88,14 @@ def device() -> ctypes.CDLL:
             c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, POINTER(Options),
             POINTER(c_double), POINTER(c_double), POINTER(Diagnostics), c_char_p, c_size_t]
         lib.netbc_betweenness_subset_ex.restype = c_int32
+        lib.net_centralization.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32,
+                                           c_int32, c_int32, POINTER(c_double), c_char_p,
+                                           c_size_t]
+        lib.net_centralization.restype = c_int32
+        lib.net_twomode_centrality.argtypes = [c_void_p, POINTER(c_int32), c_int32, c_int32,
+                                               c_int32, c_int32, POINTER(c_double), c_char_p,
+                                               c_size_t]
+        lib.net_twomode_centrality.restype = c_int32
         lib.net_closeness.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
                                       c_int32, c_int32, POINTER(c_double), c_char_p, c_size_t]
         lib.net_closeness.restype = c_int32

@@ -2481,6 +2481,56 @@ __global__ void closeness_kernel(DevProfile prof, size_t n, int use_ftotal, doub
     }
 }
 
+// Closeness normalized for two-mode data (bipartite.rs:176-210): with own
+// and other the sizes of v's mode and of the other mode, (other + 2(own -
+// 1)) / total when normalized, else 1 / total; times reached / (n - 1)
+// under the Wasserman-Faust rule (wf), which gives 0 to a node reaching no
+// other (NaN otherwise).
+__global__ void twomode_closeness_kernel(DevProfile prof, size_t n,
+                                         const uint8_t* __restrict__ types, double first,
+                                         double second, int normalized, int wf,
+                                         double* __restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const unsigned long long r = prof.reached[v];
+        if (r == 0) {
+            out[v] = wf ? 0.0 : NAN;
+            continue;
+        }
+        const double total = (double)prof.itotal[v];
+        const double own = types[v] ? second : first, other = types[v] ? first : second;
+        double c = normalized ? (other + 2.0 * (own - 1.0)) / total : 1.0 / total;
+        if (wf) c *= (double)r / (double)(n - 1);
+        out[v] = c;
+    }
+}
+
+// Degree of every node from the edge list, degree.rs:41-74 (mode 1 out, 2
+// in, 3 all; loops skipped unless counted). Integer counts: the order of
+// the atomic adds does not matter.
+__global__ void degree_kernel(const uint32_t* __restrict__ src, const uint32_t* __restrict__ tgt,
+                              size_t m, int directed, int mode, int loops,
+                              double* __restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+        const uint32_t s = src[e], t = tgt[e];
+        if (s == t && !loops) continue;
+        if (!directed || mode != 2) atomicAdd(&out[s], 1.0);
+        if (!directed || mode != 1) atomicAdd(&out[t], 1.0);
+    }
+}
+
+// vals[v] /= div[types[v]] where that divisor is positive, else NaN.
+__global__ void divide_by_type_kernel(double* __restrict__ vals, size_t n,
+                                      const uint8_t* __restrict__ types, double div0,
+                                      double div1) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        const double d = types[v] ? div1 : div0;
+        vals[v] = d > 0.0 ? vals[v] / d : NAN;
+    }
+}
+
 // The largest longest length, as bits (non-negative doubles order as bits).
 __global__ void max_bits_kernel(const double* __restrict__ vals, size_t n,
                                 unsigned long long* __restrict__ out) {
@@ -3607,9 +3657,17 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     }
     DevBuf<double> vals;
     vals.alloc(n);
-    closeness_kernel<<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22)), kBlock, 0, st>>>(
-        prof, n, L.kind == kReal ? 1 : 0, cr.unit, cr.harmonic ? 1 : 0, cr.normalized ? 1 : 0,
-        cr.unreachable, longest.p, vals.p);
+    const unsigned vb = blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22));
+    if (cr.d_types) {
+        if (L.kind != kUniform) throw Error("two-mode closeness counts steps");
+        twomode_closeness_kernel<<<vb, kBlock, 0, st>>>(prof, n, cr.d_types, (double)cr.first,
+                                                        (double)cr.second, cr.normalized ? 1 : 0,
+                                                        cr.unreachable == 1 ? 1 : 0, vals.p);
+    } else {
+        closeness_kernel<<<vb, kBlock, 0, st>>>(prof, n, L.kind == kReal ? 1 : 0, cr.unit,
+                                                cr.harmonic ? 1 : 0, cr.normalized ? 1 : 0,
+                                                cr.unreachable, longest.p, vals.p);
+    }
     NETBC_HIP(hipGetLastError());
     NETBC_HIP(hipEventRecord(ws.ev1, st));
     std::vector<unsigned long long> reached(n);
@@ -3630,6 +3688,27 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     const unsigned long long bits = ws.host_ctrl->max_sigma_bits;
     memcpy(&mx, &bits, 8);
     diag.max_sigma = std::max(diag.max_sigma, mx);
+}
+
+void degree_scores(Graph& g, int mode, bool loops, double* d_out) {
+    hipStream_t st = g.stream;
+    NETBC_HIP(hipMemsetAsync(d_out, 0, std::max<uint64_t>(g.n, 1) * sizeof(double), st));
+    if (g.m == 0) return;
+    DevBuf<uint32_t> ends;
+    ends.alloc(2 * g.m);
+    NETBC_HIP(hipMemcpyAsync(ends.p, g.src.data(), g.m * 4, hipMemcpyHostToDevice, st));
+    NETBC_HIP(hipMemcpyAsync(ends.p + g.m, g.tgt.data(), g.m * 4, hipMemcpyHostToDevice, st));
+    degree_kernel<<<blocks_for_threads(std::min<size_t>(g.m, (size_t)1 << 22)), kBlock, 0, st>>>(
+        ends.p, ends.p + g.m, g.m, g.directed ? 1 : 0, mode, loops ? 1 : 0, d_out);
+    NETBC_HIP(hipGetLastError());
+    NETBC_HIP(hipStreamSynchronize(st));  // the edge copies must finish first
+}
+
+void divide_by_type(Graph& g, double* d_vals, const uint8_t* d_types, double div0, double div1) {
+    if (g.n == 0) return;
+    divide_by_type_kernel<<<blocks_for_threads(std::min<size_t>(g.n, (size_t)1 << 22)), kBlock, 0,
+                            g.stream>>>(d_vals, g.n, d_types, div0, div1);
+    NETBC_HIP(hipGetLastError());
 }
 
 void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,

@@ -477,6 +477,273 @@ void subset_entry(const GraphHandle* h, const int64_t* sources, int64_t n_source
     fill_diag(diag, d, 0.0);
 }
 
+// Profiles and values of a closeness run (no option checks): the graph
+// oriented for the mode (closeness.rs:225-238: in-paths are the out-paths
+// of the reversed graph, paths ignoring direction those of the undirected
+// graph), the path lengths, then the device run. Returns the device build
+// time of the graph searched.
+double closeness_run(netbc::Graph& g, int32_t mode, int32_t weighted, int32_t weights_as,
+                     netbc::ClosenessRun cr, const netbc_options* opt, double* out,
+                     netbc::RunDiag& d) {
+    std::lock_guard<std::mutex> lock(g.mu);
+    netbc::Graph derived;
+    netbc::Graph* pg = &g;
+    if (g.directed && mode != 1) {
+        derived.n = g.n;
+        derived.m = g.m;
+        derived.directed = mode == 2;
+        derived.has_weights = g.has_weights;
+        derived.src = mode == 2 ? g.tgt : g.src;
+        derived.tgt = mode == 2 ? g.src : g.tgt;
+        derived.w = g.w;
+        pg = &derived;
+    }
+    const double build_ms = prepare(*pg);
+    netbc::Lengths L;
+    netbc::classify_lengths(*pg, weighted != 0, weights_as, L);
+    // uniform weights: hop counts times the common length
+    // (closeness.rs:325-344)
+    if (L.kind == netbc::kUniform && weighted && pg->m > 0)
+        cr.unit = weights_as == 1 ? 1.0 / pg->w[0] : pg->w[0];
+    netbc::run_closeness(*pg, L, options(opt), cr, out, d);
+    return build_ms;
+}
+
+// simplify, topography.rs:105-140: loops dropped, one edge per ordered
+// pair (unordered, smaller id first, when `undirected` or the graph is),
+// the largest weight of a pair kept, edges sorted by (source, target).
+void simplify_graph(const netbc::Graph& g, bool undirected, netbc::Graph& out) {
+    const bool directed = g.directed && !undirected;
+    struct Arc {
+        uint32_t s, t;
+        double x;
+    };
+    std::vector<Arc> a;
+    a.reserve(g.m);
+    for (uint64_t e = 0; e < g.m; e++) {
+        uint32_t s = g.src[e], t = g.tgt[e];
+        if (s == t) continue;
+        const double x = g.has_weights ? g.w[e] : 1.0;
+        if (!directed && s > t) std::swap(s, t);
+        a.push_back({s, t, x});
+    }
+    std::sort(a.begin(), a.end(),
+              [](const Arc& p, const Arc& q) { return p.s != q.s ? p.s < q.s : p.t < q.t; });
+    out.n = g.n;
+    out.directed = directed;
+    out.has_weights = g.has_weights;
+    out.src.clear();
+    out.tgt.clear();
+    out.w.clear();
+    for (const Arc& r : a) {
+        if (!out.src.empty() && out.src.back() == r.s && out.tgt.back() == r.t) {
+            if (g.has_weights) out.w.back() = std::fmax(out.w.back(), r.x);
+            continue;
+        }
+        out.src.push_back(r.s);
+        out.tgt.push_back(r.t);
+        if (g.has_weights) out.w.push_back(r.x);
+    }
+    out.m = out.src.size();
+}
+
+// centralization_max, topography.rs:504-568 (measure 0 degree, 1
+// betweenness, 2 closeness; convention 0 igraph, 2 sna, 3 UCINET).
+double centralization_max(uint64_t n0, bool directed, int32_t measure, int32_t mode, bool loops,
+                          int32_t convention) {
+    if (n0 == 0) return NAN;
+    const double n = (double)n0;
+    if (!directed) mode = 3;
+    loops = loops && convention == 0;
+    if (measure == 0) {
+        if (!directed) return loops ? (n - 1.0) * n : (n - 1.0) * (n - 2.0);
+        if (mode == 3) return loops ? 2.0 * (n - 1.0) * (n - 1.0) : 2.0 * (n - 1.0) * (n - 2.0);
+        return loops ? (n - 1.0) * n : (n - 1.0) * (n - 1.0);
+    }
+    if (measure == 1) {
+        const double t = (n - 1.0) * (n - 1.0) * (n - 2.0);
+        return directed ? t : t / 2.0;
+    }
+    if (convention == 3 || mode == 3) return (n - 1.0) * (n - 2.0) / (2.0 * n - 3.0);
+    return (n - 1.0) * (1.0 - 1.0 / n);
+}
+
+// Freeman centralization (topography.rs:376-475): the measure's scores on
+// the device, then n * max - sum (in node order; NaN if a score is NaN),
+// over the maximum when normalized.
+void centralization_entry(const GraphHandle* h, int32_t measure, int32_t mode, int32_t loops,
+                          int32_t normalized, int32_t convention, const netbc_options* opt,
+                          double* out, netbc_diagnostics* diag) {
+    netbc::Graph& g0 = handle(h);
+    if (measure < 0 || measure > 3)
+        throw netbc::Error("unknown centralization measure code " + std::to_string(measure));
+    if (mode < 1 || mode > 3)
+        throw netbc::Error("unknown degree mode code " + std::to_string(mode));
+    if (convention < 0 || convention > 3)
+        throw netbc::Error("unknown convention code " + std::to_string(convention));
+    if (convention == 1)
+        throw netbc::Error("NetworkX has no centralization index; use the igraph, sna, or "
+                           "UCINET convention");
+    if (measure == 3) {
+        static const char* names[] = {"igraph", "NetworkX", "sna", "UCINET"};
+        if (convention != 0)
+            throw netbc::Error("eigenvector centralization is provided for the igraph "
+                               "convention only, not " + std::string(names[convention]));
+        if (g0.directed && mode != 3)
+            throw netbc::Error("eigenvector centralization is provided with direction ignored "
+                               "(mode \"all\") only");
+        throw netbc::Error("eigenvector centralization is not part of this library");
+    }
+    check_options(opt);
+    netbc::Graph simple;
+    netbc::Graph* pg = &g0;
+    if (convention == 2 || convention == 3) {
+        std::lock_guard<std::mutex> lock(g0.mu);
+        simplify_graph(g0, false, simple);
+        pg = &simple;
+    }
+    netbc::Graph& g = *pg;
+    const uint64_t n = g.n;
+    const bool directed = g.directed;
+    const int32_t emode = directed ? mode : 3;
+    std::vector<double> scores(std::max<uint64_t>(n, 1));
+    netbc::RunDiag d;
+    double build_ms = 0.0;
+    if (measure == 0) {
+        if (convention == 3 && directed && emode == 3)
+            throw netbc::Error("UCINET reports degree centralization for in- and out-degree only");
+        std::lock_guard<std::mutex> lock(g.mu);
+        build_ms = prepare(g);
+        netbc::DevBuf<double> deg;
+        deg.alloc(std::max<uint64_t>(n, 1));
+        netbc::degree_scores(g, emode, loops && convention == 0, deg.p);
+        NETBC_HIP(hipMemcpyAsync(scores.data(), deg.p, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 g.stream));
+        NETBC_HIP(hipStreamSynchronize(g.stream));
+    } else if (measure == 1) {
+        d = run_graph(g, nullptr, (int64_t)n, 0, 0, 0, opt, scores.data(), &build_ms);
+    } else {
+        netbc::ClosenessRun cr;
+        cr.normalized = true;
+        cr.unreachable = convention == 2 ? 2 : 0;
+        build_ms = closeness_run(g, emode, 0, 0, cr, opt, scores.data(), d);
+        // UCINET: NaN unless every node reaches every other
+        if (convention == 3 && d.reached_nodes != (int64_t)(n * (n > 0 ? n - 1 : 0))) {
+            *out = NAN;
+            fill_diag(diag, d, build_ms);
+            return;
+        }
+    }
+    double result = NAN;
+    if (n > 0) {
+        double mx = -INFINITY, total = 0.0;
+        bool any_nan = false;
+        for (uint64_t v = 0; v < n; v++) {
+            const double x = scores[v];
+            if (std::isnan(x)) any_nan = true;
+            else if (x > mx) mx = x;
+            total += x;
+        }
+        if (any_nan) mx = NAN;
+        result = (double)n * mx - total;
+        if (normalized)
+            result /= centralization_max(n, directed, measure, emode, loops != 0, convention);
+    }
+    *out = result;
+    fill_diag(diag, d, build_ms);
+}
+
+// betweenness_max, bipartite.rs:228-241 (Borgatti and Everett 1997): the
+// largest betweenness of a node whose mode has `own` nodes, the other mode
+// `other`; NaN when either is 0.
+double twomode_betweenness_max(uint64_t own, uint64_t other) {
+    if (other == 0 || own == 0) return NAN;
+    const __int128 m = other, s = (own - 1) / other, t = (own - 1) % other;
+    const __int128 twice =
+        m * m * (s + 1) * (s + 1) + m * (s + 1) * (2 * t - s - 1) - t * (2 * s - t + 3);
+    return (double)twice / 2.0;
+}
+
+// Centrality normalized for two-mode data (crates/capi/src/twomode.rs:
+// 237-286 -> bipartite.rs:130-269): the checks in the reference's order,
+// then the measure on the affiliation graph (undirected, loops dropped,
+// parallel edges collapsed) on the device.
+void twomode_entry(const GraphHandle* h, const int32_t* types, int32_t measure, int32_t normalized,
+                   int32_t unreachable, const netbc_options* opt, double* out,
+                   netbc_diagnostics* diag) {
+    netbc::Graph& g = handle(h);
+    if (!types) throw netbc::Error("node types are missing");
+    const uint64_t n = g.n;
+    for (uint64_t v = 0; v < n; v++)
+        if (types[v] != 0 && types[v] != 1)
+            throw netbc::Error("node " + std::to_string(v) + " has type code " +
+                               std::to_string(types[v]) +
+                               "; use 0 for the first mode and 1 for the second");
+    if (measure == 1) {
+        if (unreachable != 0 && unreachable != 1)
+            throw netbc::Error("unknown code " + std::to_string(unreachable) +
+                               " for unreachable nodes");
+    } else if (measure != 0 && measure != 2) {
+        throw netbc::Error("unknown two-mode centrality code " + std::to_string(measure));
+    }
+    for (uint64_t e = 0; e < g.m; e++)
+        if (types[g.src[e]] == types[g.tgt[e]])
+            throw netbc::Error("edge " + std::to_string(e) + " joins nodes " +
+                               std::to_string(g.src[e]) + " and " + std::to_string(g.tgt[e]) +
+                               ", which are in the same mode; in a two-mode network every edge "
+                               "joins the two modes");
+    uint64_t second = 0;
+    for (uint64_t v = 0; v < n; v++) second += types[v] == 1;
+    const uint64_t first = n - second;
+    if (measure == 1 && unreachable == 1 && !normalized)
+        throw netbc::Error("the Wasserman-Faust rule applies to normalized closeness only");
+    check_options(opt);
+    netbc::Graph a;
+    {
+        std::lock_guard<std::mutex> lock(g.mu);
+        simplify_graph(g, true, a);
+    }
+    netbc::RunDiag d;
+    double build_ms = prepare(a);
+    if (n == 0) {
+        fill_diag(diag, d, build_ms);
+        return;
+    }
+    std::vector<uint8_t> tf(n);
+    for (uint64_t v = 0; v < n; v++) tf[v] = (uint8_t)types[v];
+    netbc::DevBuf<uint8_t> dt;
+    dt.alloc(n);
+    NETBC_HIP(hipMemcpyAsync(dt.p, tf.data(), n, hipMemcpyHostToDevice, a.stream));
+    if (measure == 1) {
+        netbc::ClosenessRun cr;
+        cr.normalized = normalized != 0;
+        cr.unreachable = unreachable;
+        cr.d_types = dt.p;
+        cr.first = first;
+        cr.second = second;
+        build_ms += closeness_run(a, 3, 0, 0, cr, opt, out, d);
+        fill_diag(diag, d, build_ms);
+        return;
+    }
+    netbc::DevBuf<double> vals;
+    vals.alloc(n);
+    if (measure == 0) {
+        netbc::degree_scores(a, 3, false, vals.p);
+        if (normalized) netbc::divide_by_type(a, vals.p, dt.p, (double)second, (double)first);
+    } else {
+        netbc::Lengths L;
+        netbc::classify_lengths(a, false, 0, L);
+        netbc::run_batched(a, L, nullptr, (int64_t)n, options(opt), vals.p, d);
+        netbc::scale_flagged(a, vals.p, n, nullptr, 0.5, 0.5);  // undirected pairs once
+        if (normalized)
+            netbc::divide_by_type(a, vals.p, dt.p, twomode_betweenness_max(first, second),
+                                  twomode_betweenness_max(second, first));
+    }
+    NETBC_HIP(hipMemcpyAsync(out, vals.p, n * sizeof(double), hipMemcpyDeviceToHost, a.stream));
+    NETBC_HIP(hipStreamSynchronize(a.stream));
+    fill_diag(diag, d, build_ms);
+}
+
 // Closeness or harmonic centrality (topography.rs:167-199 ->
 // closeness.rs:131-209): the option checks in the reference's order, the
 // graph oriented for the mode (closeness.rs:225-238: in-paths are the
@@ -501,32 +768,12 @@ void closeness_entry(const GraphHandle* h, int32_t harmonic, int32_t mode, int32
             "closeness with unreachable lengths set to the longest length plus 1 is defined "
             "only for unweighted networks");
     check_options(opt);
-    std::lock_guard<std::mutex> lock(g.mu);
-    netbc::Graph derived;
-    netbc::Graph* pg = &g;
-    if (g.directed && mode != 1) {
-        derived.n = g.n;
-        derived.m = g.m;
-        derived.directed = mode == 2;
-        derived.has_weights = g.has_weights;
-        derived.src = mode == 2 ? g.tgt : g.src;
-        derived.tgt = mode == 2 ? g.src : g.tgt;
-        derived.w = g.w;
-        pg = &derived;
-    }
-    const double build_ms = prepare(*pg);
-    netbc::Lengths L;
-    netbc::classify_lengths(*pg, weighted != 0, weights_as, L);
     netbc::ClosenessRun cr;
     cr.harmonic = harmonic != 0;
     cr.normalized = normalized != 0;
     cr.unreachable = unreachable;
-    // uniform weights: hop counts times the common length
-    // (closeness.rs:325-344)
-    if (L.kind == netbc::kUniform && weighted && pg->m > 0)
-        cr.unit = weights_as == 1 ? 1.0 / pg->w[0] : pg->w[0];
     netbc::RunDiag d;
-    netbc::run_closeness(*pg, L, options(opt), cr, out, d);
+    const double build_ms = closeness_run(g, mode, weighted, weights_as, cr, opt, out, d);
     fill_diag(diag, d, build_ms);
 }
 
@@ -680,6 +927,27 @@ NETBC_EXPORT int32_t netbc_closeness_ex(const GraphHandle* h, int32_t harmonic, 
     return guard(err, err_len, [&] {
         closeness_entry(h, harmonic, mode, weighted, weights_as, normalized, unreachable, opt,
                         out, diag);
+    });
+}
+
+NETBC_EXPORT int32_t net_centralization(const GraphHandle* h, int32_t measure, int32_t mode,
+                                        int32_t loops, int32_t normalized, int32_t convention,
+                                        int32_t threads, double* out, char* err,
+                                        size_t err_len) {
+    (void)threads;
+    return guard(err, err_len, [&] {
+        centralization_entry(h, measure, mode, loops, normalized, convention, nullptr, out,
+                             nullptr);
+    });
+}
+
+NETBC_EXPORT int32_t net_twomode_centrality(const GraphHandle* h, const int32_t* types,
+                                            int32_t measure, int32_t normalized,
+                                            int32_t unreachable, int32_t threads, double* out,
+                                            char* err, size_t err_len) {
+    (void)threads;
+    return guard(err, err_len, [&] {
+        twomode_entry(h, types, measure, normalized, unreachable, nullptr, out, nullptr);
     });
 }
 

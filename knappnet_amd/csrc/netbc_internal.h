@@ -197,11 +197,24 @@ struct ClosenessRun {
     bool normalized = false;
     int unreachable = 0;
     double unit = 1.0;
+    // Two-mode closeness instead (bipartite.rs:176-210): node types (device,
+    // n flags, 1 = second mode) and the sizes of the two modes; unreachable
+    // 0 ignore, 1 Wasserman-Faust.
+    const uint8_t* d_types = nullptr;
+    uint64_t first = 0, second = 0;
 };
 
 // Writes the n values to host `out`.
 void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt, const ClosenessRun& cr,
                    double* out, RunDiag& diag);
+
+// Degree of every node (degree.rs:41-74; mode 1 out, 2 in, 3 all) into
+// d_out (device, n doubles).
+void degree_scores(Graph& g, int mode, bool loops, double* d_out);
+
+// vals[v] /= (types[v] ? div1 : div0) where that divisor is positive, else
+// NaN (device arrays, the graph's stream).
+void divide_by_type(Graph& g, double* d_vals, const uint8_t* d_types, double div0, double div1);
 
 // vals[v] *= flag[v] ? f_flagged : f_other on the graph's stream (flag:
 // device, may be nullptr).

@@ -155,6 +155,34 @@ class Graph:
             raise EngineError(err.value.decode())
         return out[: self.n], diag.as_dict()
 
+    def centralization(self, measure: int = 0, mode: int = 3, loops: bool = True,
+                       normalized: bool = True, convention: int = 0) -> float:
+        """``net_centralization``: Freeman centralization. measure 0 degree, 1
+        betweenness, 2 closeness; convention 0 igraph, 2 sna, 3 UCINET."""
+        out = np.zeros(1, dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        if self.lib.net_centralization(self.handle, int(measure), int(mode), int(bool(loops)),
+                                       int(bool(normalized)), int(convention), 0,
+                                       _ptr(out, c_double), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        return float(out[0])
+
+    def twomode_centrality(self, types, measure: int = 0, normalized: bool = True,
+                           unreachable: int = 0):
+        """``net_twomode_centrality``: measure 0 degree, 1 closeness, 2
+        betweenness, normalized for two-mode data; types: 0 / 1 per node."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        tp, _keep = None, None
+        if types is not None:
+            _keep = np.ascontiguousarray(types, dtype=np.int32)
+            tp = _ptr(_keep, ctypes.c_int32)
+        if self.lib.net_twomode_centrality(self.handle, tp, int(measure), int(bool(normalized)),
+                                           int(unreachable), 0, _ptr(out, c_double), err,
+                                           ERR_LEN):
+            raise EngineError(err.value.decode())
+        return out[: self.n]
+
     def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, precision,
                       slots, direction, alpha, exact, sweep_group):
         out = np.zeros(max(length, 1), dtype=np.float64)

@@ -2008,3 +2008,301 @@ int32_t orc_distance_profiles(const OGraph *g, int32_t weighted, int32_t weights
     lengths_free(&L);
     return ORC_OK;
 }
+
+/* ---- centralization and two-mode centrality ----------------------------- */
+
+/* simplify, topography.rs:105-140: loops dropped, one arc per ordered pair
+ * (unordered when the result is undirected, smaller id first), the largest
+ * weight of a pair kept, arcs sorted by (source, target). */
+typedef struct { uint32_t s, t; double x; } SArc;
+
+static int sarc_cmp(const void *a, const void *b) {
+    const SArc *p = a, *q = b;
+    if (p->s != q->s) return p->s < q->s ? -1 : 1;
+    if (p->t != q->t) return p->t < q->t ? -1 : 1;
+    return 0;
+}
+
+static int32_t simplify_max(const OGraph *g, int undirected, OGraph **out, char *err,
+                            size_t err_len) {
+    int directed = g->directed && !undirected;
+    uint64_t m = g->m, k = 0;
+    SArc *a = malloc((m ? m : 1) * sizeof *a);
+    for (uint64_t e = 0; e < m; e++) {
+        uint32_t s = g->src[e], t = g->tgt[e];
+        if (s == t) continue;
+        double x = g->w ? g->w[e] : 1.0;
+        if (directed || s < t) a[k++] = (SArc){s, t, x};
+        else a[k++] = (SArc){t, s, x};
+    }
+    qsort(a, k, sizeof *a, sarc_cmp);
+    uint64_t mm = 0;
+    for (uint64_t i = 0; i < k; i++) {
+        if (mm > 0 && a[mm - 1].s == a[i].s && a[mm - 1].t == a[i].t) {
+            if (a[i].x > a[mm - 1].x || isnan(a[mm - 1].x)) a[mm - 1].x = a[i].x;  /* f64::max */
+        } else {
+            a[mm++] = a[i];
+        }
+    }
+    int64_t *s = malloc((mm ? mm : 1) * sizeof *s), *t = malloc((mm ? mm : 1) * sizeof *t);
+    double *w = g->w ? malloc((mm ? mm : 1) * sizeof *w) : NULL;
+    for (uint64_t i = 0; i < mm; i++) {
+        s[i] = a[i].s;
+        t[i] = a[i].t;
+        if (w) w[i] = a[i].x;
+    }
+    int32_t rc = orc_graph_new((int64_t)g->n, s, t, w, (int64_t)mm, directed, out, err, err_len);
+    free(a); free(s); free(t); free(w);
+    return rc;
+}
+
+/* degree, degree.rs:41-74 (mode 1 out, 2 in, 3 all). */
+static void degree_scores(const OGraph *g, int32_t mode, int loops, double *out) {
+    for (uint64_t v = 0; v < g->n; v++) out[v] = 0.0;
+    for (uint64_t e = 0; e < g->m; e++) {
+        uint32_t s = g->src[e], t = g->tgt[e];
+        if (s == t && !loops) continue;
+        if (!g->directed) {
+            out[s] += 1.0;
+            out[t] += 1.0;
+            continue;
+        }
+        if (mode != 2) out[s] += 1.0;
+        if (mode != 1) out[t] += 1.0;
+    }
+}
+
+/* centralization_max, topography.rs:504-568 (measure 0 degree, 1
+ * betweenness, 2 closeness; convention 0 igraph, 2 sna, 3 UCINET). */
+static double centralization_max(uint64_t n0, int directed, int32_t measure, int32_t mode,
+                                 int loops, int32_t convention) {
+    if (n0 == 0) return NAN;
+    double n = (double)n0;
+    if (!directed) mode = 3;
+    loops = loops && convention == 0;
+    if (measure == 0) {
+        if (!directed) return loops ? (n - 1.0) * n : (n - 1.0) * (n - 2.0);
+        if (mode == 3) return loops ? 2.0 * (n - 1.0) * (n - 1.0) : 2.0 * (n - 1.0) * (n - 2.0);
+        return loops ? (n - 1.0) * n : (n - 1.0) * (n - 1.0);
+    }
+    if (measure == 1) {
+        double t = (n - 1.0) * (n - 1.0) * (n - 2.0);
+        return directed ? t : t / 2.0;
+    }
+    if (convention == 3 || mode == 3) return (n - 1.0) * (n - 2.0) / (2.0 * n - 3.0);
+    return (n - 1.0) * (1.0 - 1.0 / n);
+}
+
+/* freeman_index, topography.rs:451-475: n * max - sum (in node order),
+ * NaN when a score is NaN, over the maximum when normalized. */
+static double freeman_index(uint64_t n, int directed, const double *scores, int32_t measure,
+                            int32_t mode, int loops, int normalized, int32_t convention) {
+    if (n == 0) return NAN;
+    double max = -INFINITY, total = 0.0;
+    int any_nan = 0;
+    for (uint64_t v = 0; v < n; v++) {
+        double x = scores[v];
+        if (isnan(x)) any_nan = 1;
+        else if (x > max) max = x;
+        total += x;
+    }
+    if (any_nan) max = NAN;
+    double raw = (double)n * max - total;
+    if (!normalized) return raw;
+    return raw / centralization_max(n, directed, measure, mode, loops, convention);
+}
+
+/* net_centralization, crates/capi/src/topography.rs:134-159 ->
+ * centralization (topography.rs:376-449). measure 0 degree, 1
+ * betweenness, 2 closeness (3, eigenvector, is checked and then refused);
+ * convention 0 igraph, 1 NetworkX (refused), 2 sna, 3 UCINET. */
+int32_t orc_centralization(const OGraph *g0, int32_t measure, int32_t mode, int32_t loops,
+                           int32_t normalized, int32_t convention, double *out, char *err,
+                           size_t err_len) {
+    char msg[256];
+    if (!g0) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (measure < 0 || measure > 3) {
+        snprintf(msg, sizeof msg, "unknown centralization measure code %d", measure);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (mode < 1 || mode > 3) {
+        snprintf(msg, sizeof msg, "unknown degree mode code %d", mode);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (convention < 0 || convention > 3) {
+        snprintf(msg, sizeof msg, "unknown convention code %d", convention);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    if (convention == 1) {
+        write_err(err, err_len,
+                  "NetworkX has no centralization index; use the igraph, sna, or UCINET "
+                  "convention");
+        return ORC_FAILED;
+    }
+    if (measure == 3) {
+        static const char *names[] = {"igraph", "NetworkX", "sna", "UCINET"};
+        if (convention != 0) {
+            snprintf(msg, sizeof msg,
+                     "eigenvector centralization is provided for the igraph convention only, "
+                     "not %s", names[convention]);
+            write_err(err, err_len, msg);
+        } else if (g0->directed && mode != 3) {
+            write_err(err, err_len,
+                      "eigenvector centralization is provided with direction ignored (mode "
+                      "\"all\") only");
+        } else {
+            write_err(err, err_len, "eigenvector centralization is not part of this library");
+        }
+        return ORC_FAILED;
+    }
+    const OGraph *g = g0;
+    OGraph *simple = NULL;
+    if (convention == 2 || convention == 3) {
+        if (simplify_max(g0, 0, &simple, err, err_len)) return ORC_FAILED;
+        g = simple;
+    }
+    uint64_t n = g->n;
+    int directed = g->directed;
+    int32_t emode = directed ? mode : 3;
+    int eloops = loops && convention == 0;
+    double *scores = malloc((n ? n : 1) * sizeof *scores);
+    int32_t rc = ORC_OK;
+    double result = NAN;
+    if (measure == 0) {
+        if (convention == 3 && directed && emode == 3) {
+            write_err(err, err_len,
+                      "UCINET reports degree centralization for in- and out-degree only");
+            rc = ORC_FAILED;
+        } else {
+            degree_scores(g, emode, eloops, scores);
+        }
+    } else if (measure == 1) {
+        rc = orc_betweenness(g, 0, 0, 0, 0, scores, err, err_len);
+    } else {
+        rc = orc_closeness(g, 0, emode, 0, 0, 1, convention == 2 ? 2 : 0, scores, err, err_len);
+        if (rc == ORC_OK && convention == 3 && n >= 2) {
+            /* UCINET: NaN unless every node reaches every other (the "zero"
+             * rule gives 0 exactly to the nodes that do not) */
+            double *zero = malloc(n * sizeof *zero);
+            rc = orc_closeness(g, 0, emode, 0, 0, 1, 2, zero, err, err_len);
+            int full = 1;
+            for (uint64_t v = 0; rc == ORC_OK && v < n; v++)
+                if (!(zero[v] > 0.0)) full = 0;
+            free(zero);
+            if (rc == ORC_OK && !full) {
+                *out = NAN;
+                free(scores);
+                orc_graph_free(simple);
+                return ORC_OK;
+            }
+        }
+    }
+    if (rc == ORC_OK)
+        result = freeman_index(n, directed, scores, measure, emode, loops, normalized,
+                               convention);
+    if (rc == ORC_OK) *out = result;
+    free(scores);
+    orc_graph_free(simple);
+    return rc;
+}
+
+/* betweenness_max, bipartite.rs:228-241 (Borgatti and Everett 1997). */
+static double twomode_betweenness_max(uint64_t own, uint64_t other) {
+    if (other == 0 || own == 0) return NAN;
+    __int128 m = other, s = (own - 1) / other, t = (own - 1) % other;
+    __int128 twice = m * m * (s + 1) * (s + 1) + m * (s + 1) * (2 * t - s - 1) -
+                     t * (2 * s - t + 3);
+    return (double)twice / 2.0;
+}
+
+/* net_twomode_centrality, crates/capi/src/twomode.rs:237-286 ->
+ * twomode_degree / twomode_closeness / twomode_betweenness
+ * (bipartite.rs:130-269). types: n codes, 0 first mode, 1 second. measure
+ * 0 degree, 1 closeness (unreachable 0 ignore, 1 Wasserman-Faust), 2
+ * betweenness. */
+int32_t orc_twomode_centrality(const OGraph *g, const int32_t *types, int32_t measure,
+                               int32_t normalized, int32_t unreachable, double *out, char *err,
+                               size_t err_len) {
+    char msg[320];
+    if (!g) { write_err(err, err_len, "graph handle is null"); return ORC_FAILED; }
+    if (!types) { write_err(err, err_len, "node types are missing"); return ORC_FAILED; }
+    uint64_t n = g->n;
+    for (uint64_t v = 0; v < n; v++)
+        if (types[v] != 0 && types[v] != 1) {
+            snprintf(msg, sizeof msg,
+                     "node %llu has type code %d; use 0 for the first mode and 1 for the second",
+                     (unsigned long long)v, types[v]);
+            write_err(err, err_len, msg);
+            return ORC_FAILED;
+        }
+    if (measure == 1) {
+        if (unreachable != 0 && unreachable != 1) {
+            snprintf(msg, sizeof msg, "unknown code %d for unreachable nodes", unreachable);
+            write_err(err, err_len, msg);
+            return ORC_FAILED;
+        }
+    } else if (measure != 0 && measure != 2) {
+        snprintf(msg, sizeof msg, "unknown two-mode centrality code %d", measure);
+        write_err(err, err_len, msg);
+        return ORC_FAILED;
+    }
+    for (uint64_t e = 0; e < g->m; e++)
+        if (types[g->src[e]] == types[g->tgt[e]]) {
+            snprintf(msg, sizeof msg,
+                     "edge %llu joins nodes %u and %u, which are in the same mode; in a two-mode "
+                     "network every edge joins the two modes",
+                     (unsigned long long)e, g->src[e], g->tgt[e]);
+            write_err(err, err_len, msg);
+            return ORC_FAILED;
+        }
+    uint64_t second = 0;
+    for (uint64_t v = 0; v < n; v++) second += types[v] == 1;
+    uint64_t first = n - second;
+    if (measure == 1 && unreachable == 1 && !normalized) {
+        write_err(err, err_len, "the Wasserman-Faust rule applies to normalized closeness only");
+        return ORC_FAILED;
+    }
+    OGraph *a = NULL;
+    if (simplify_max(g, 1, &a, err, err_len)) return ORC_FAILED;
+    int32_t rc = ORC_OK;
+    if (measure == 0) {
+        degree_scores(a, 3, 0, out);
+        if (normalized)
+            for (uint64_t v = 0; v < n; v++) out[v] /= (double)(types[v] ? first : second);
+    } else if (measure == 1) {
+        int64_t *all = malloc((n ? n : 1) * sizeof *all);
+        uint64_t *reached = malloc((n ? n : 1) * sizeof *reached);
+        double *total = malloc((n ? n : 1) * sizeof *total), *h = malloc((n ? n : 1) * sizeof *h),
+               *lg = malloc((n ? n : 1) * sizeof *lg);
+        for (uint64_t v = 0; v < n; v++) all[v] = (int64_t)v;
+        rc = orc_distance_profiles(a, 0, 0, all, (int64_t)n, 0, reached, total, h, lg, err,
+                                   err_len);
+        for (uint64_t v = 0; rc == ORC_OK && v < n; v++) {
+            if (reached[v] == 0) {
+                out[v] = unreachable == 1 ? 0.0 : NAN;
+                continue;
+            }
+            double own = (double)(types[v] ? second : first);
+            double other = (double)(types[v] ? first : second);
+            double c = normalized ? (other + 2.0 * (own - 1.0)) / total[v] : 1.0 / total[v];
+            if (unreachable == 1) c *= (double)reached[v] / (double)(n - 1);
+            out[v] = c;
+        }
+        free(all); free(reached); free(total); free(h); free(lg);
+    } else {
+        rc = orc_betweenness(a, 0, 0, 0, 0, out, err, err_len);
+        if (rc == ORC_OK && normalized) {
+            double max_first = twomode_betweenness_max(first, second);
+            double max_second = twomode_betweenness_max(second, first);
+            for (uint64_t v = 0; v < n; v++) {
+                double mx = types[v] ? max_second : max_first;
+                out[v] = mx > 0.0 ? out[v] / mx : NAN;
+            }
+        }
+    }
+    orc_graph_free(a);
+    return rc;
+}

@@ -79,6 +79,12 @@ def lib() -> ctypes.CDLL:
             c_void_p, c_int32, c_int32, POINTER(c_int64), c_int64, c_int32, POINTER(c_uint64),
             POINTER(c_double), POINTER(c_double), POINTER(c_double), c_char_p, c_size_t]
         L.orc_distance_profiles.restype = c_int32
+        L.orc_centralization.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_int32, c_int32,
+                                         POINTER(c_double), c_char_p, c_size_t]
+        L.orc_centralization.restype = c_int32
+        L.orc_twomode_centrality.argtypes = [c_void_p, POINTER(c_int32), c_int32, c_int32,
+                                             c_int32, POINTER(c_double), c_char_p, c_size_t]
+        L.orc_twomode_centrality.restype = c_int32
         L.orc_available_cores.argtypes = []
         L.orc_available_cores.restype = c_uint64
         _lib = L
@@ -214,6 +220,30 @@ class OracleGraph:
         if self.L.orc_closeness(self.h, int(harmonic), int(mode), int(weighted), kind,
                                 int(normalized), int(unreachable), _p(out, c_double), err,
                                 ERR_LEN):
+            raise OracleError(err.value.decode())
+        return out[: self.n]
+
+    def centralization(self, measure=0, mode=3, loops=True, normalized=True, convention=0):
+        """topography.rs:376-568: measure 0 degree, 1 betweenness, 2 closeness;
+        convention 0 igraph, 2 sna, 3 UCINET."""
+        out = np.zeros(1, dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        if self.L.orc_centralization(self.h, int(measure), int(mode), int(bool(loops)),
+                                     int(bool(normalized)), int(convention), _p(out, c_double),
+                                     err, ERR_LEN):
+            raise OracleError(err.value.decode())
+        return float(out[0])
+
+    def twomode_centrality(self, types, measure=0, normalized=True, unreachable=0):
+        """bipartite.rs:130-269: measure 0 degree, 1 closeness, 2 betweenness."""
+        out = np.zeros(max(self.n, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        tp = None
+        if types is not None:
+            ta = np.ascontiguousarray(types, dtype=np.int32)
+            tp = _p(ta, c_int32)
+        if self.L.orc_twomode_centrality(self.h, tp, int(measure), int(bool(normalized)),
+                                         int(unreachable), _p(out, c_double), err, ERR_LEN):
             raise OracleError(err.value.decode())
         return out[: self.n]
 

@@ -159,3 +159,29 @@ CLOSENESS_VARIANTS = {
 
 def closeness_golden(vals):
     return [float("nan") if x is None or x == -1 else x for x in vals]
+
+
+# Freeman centralization (validation/manifest.d/topography.yaml; the
+# validation bindings' defaults: mode all, loops counted, normalized, igraph)
+CENTRALIZATION_VARIANTS = {
+    "centralization_degree": dict(measure=0, mode=3, loops=True, convention=0),
+    "centralization_degree_in": dict(measure=0, mode=2, loops=True, convention=0),
+    "centralization_degree_sna": dict(measure=0, mode=3, convention=2),
+    "centralization_degree_ucinet": dict(measure=0, mode=1, convention=3),
+    "centralization_betweenness": dict(measure=1, convention=0),
+    "centralization_closeness": dict(measure=2, mode=1, convention=0),
+    "centralization_closeness_sna": dict(measure=2, mode=1, convention=2),
+    "centralization_closeness_ucinet": dict(measure=2, mode=1, convention=3),
+}
+
+# centrality normalized for two-mode data (validation/manifest.d/twomode.yaml;
+# defaults normalized, unreachable nodes ignored)
+TWOMODE_VARIANTS = {
+    "twomode_degree": dict(measure=0, normalized=True),
+    "twomode_degree_raw": dict(measure=0, normalized=False),
+    "twomode_closeness": dict(measure=1, normalized=True, unreachable=1),
+    "twomode_closeness_ignore": dict(measure=1, normalized=True, unreachable=0),
+    "twomode_closeness_raw": dict(measure=1, normalized=False, unreachable=0),
+    "twomode_betweenness": dict(measure=2, normalized=True),
+    "twomode_betweenness_raw": dict(measure=2, normalized=False),
+}

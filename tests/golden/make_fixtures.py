@@ -6,9 +6,10 @@ and meta.json) and the values its oracles produced for them
 ``%.17g`` value per line under a ``value`` header; see
 validation/common.py:341-352; the centrality_extra variants write NaN as
 -999, the topography ones as -1). This script copies the edge lists, the
-metadata and the goldens of betweenness, edge betweenness, betweenness for
-sources and targets, and closeness / harmonic centrality into one JSON
-file, as data. Nothing of the reference is imported or run.
+metadata (with the node types of two-mode fixtures) and the goldens of
+betweenness, edge betweenness, betweenness for sources and targets,
+closeness / harmonic centrality, Freeman centralization and centrality
+normalized for two-mode data into one JSON file, as data. Nothing of the reference is imported or run.
 
 Run once in a tree where the reference is checked out:
 
@@ -63,6 +64,23 @@ VARIANTS = [
     "harmonic_normalized",
     "harmonic_in",
     "harmonic_weighted",
+    # Freeman centralization (topography.yaml; one value, NaN as -1)
+    "centralization_degree",
+    "centralization_degree_in",
+    "centralization_degree_sna",
+    "centralization_degree_ucinet",
+    "centralization_betweenness",
+    "centralization_closeness",
+    "centralization_closeness_sna",
+    "centralization_closeness_ucinet",
+    # centrality normalized for two-mode data (validation/manifest.d/twomode.yaml)
+    "twomode_degree",
+    "twomode_degree_raw",
+    "twomode_closeness",
+    "twomode_closeness_ignore",
+    "twomode_closeness_raw",
+    "twomode_betweenness",
+    "twomode_betweenness_raw",
 ]
 ORACLES = ["definition", "networkx"]
 
@@ -106,6 +124,7 @@ def main(ref: Path, dest: Path) -> None:
             "src": src,
             "tgt": tgt,
             "weights": w if meta["weighted"] else None,
+            "types": meta.get("type"),
             "golden": golden,
         }
     version = (val / "golden" / "networkx" / "VERSION").read_text().strip()

@@ -17,9 +17,10 @@ from scipy.sparse import csr_matrix
 from scipy.sparse.csgraph import shortest_path
 
 import definition
-from _common import (CLOSENESS_VARIANTS, EDGE_VARIANTS, SUBSET_VARIANTS, VARIANTS, close,
-                     closeness_golden, golden_values, networkx_documented, random_case,
-                     subset_call, subset_networkx_documented, tolerance)
+from _common import (CENTRALIZATION_VARIANTS, CLOSENESS_VARIANTS, EDGE_VARIANTS,
+                     SUBSET_VARIANTS, TWOMODE_VARIANTS, VARIANTS, close, closeness_golden,
+                     golden_values, networkx_documented, random_case, subset_call,
+                     subset_networkx_documented, tolerance)
 from oracle.oracle import (OracleError, OracleGraph, available_cores, ordered_sum_bits,
                            threads_for)
 
@@ -557,3 +558,109 @@ def test_full_source_list_is_ordinary_betweenness():
         n, s, t, directed = random_case(rng)
         g = OracleGraph(n, s, t, None, directed)
         assert np.array_equal(g.betweenness_sources(np.arange(n)), g.betweenness())
+
+
+# ---- Freeman centralization and two-mode centrality ---------------------------
+
+@pytest.mark.parametrize("variant", list(CENTRALIZATION_VARIANTS))
+def test_centralization_goldens(fixtures, variant):
+    # topography.yaml: rel 1e-12 / 1e-10, abs 1e-14 / 1e-12; NaN written as -1
+    checked = 0
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"]:
+            continue
+        got = og(fx).centralization(**CENTRALIZATION_VARIANTS[variant])
+        for oracle, want in fx["golden"][variant].items():
+            assert close([got], closeness_golden(want), 1e-10, 1e-12), (name, variant, oracle)
+        checked += 1
+    assert checked >= 8
+
+
+@pytest.mark.parametrize("variant", list(TWOMODE_VARIANTS))
+def test_twomode_goldens(fixtures, variant):
+    # twomode.yaml: rel 1e-14 (degree), 1e-12 (closeness), 1e-9 (betweenness)
+    checked = 0
+    for name, fx in fixtures.items():
+        if variant not in fx["golden"]:
+            continue
+        got = og(fx).twomode_centrality(fx["types"], **TWOMODE_VARIANTS[variant])
+        for oracle, want in fx["golden"][variant].items():
+            assert close(got, closeness_golden(want), 1e-9, 1e-12), (name, variant, oracle)
+        checked += 1
+    assert checked >= 3
+
+
+def test_centralization_stars():
+    # a star reaches the theoretical maximum of every index: 1
+    for n in (3, 5, 9):
+        s, t = np.zeros(n - 1, dtype=np.int64), np.arange(1, n)
+        g = OracleGraph(n, s, t, None, False)
+        for kw in (dict(measure=0, loops=False), dict(measure=0, convention=2),
+                   dict(measure=1), dict(measure=1, convention=3), dict(measure=2),
+                   dict(measure=2, convention=2), dict(measure=2, convention=3)):
+            assert abs(g.centralization(**kw) - 1.0) < 1e-14, (n, kw)
+        # igraph with loops counted: the maximum is (n-1) n, the star (n-1)(n-2)
+        assert abs(g.centralization(measure=0) - (n - 2) / n) < 1e-14
+        # raw index: n * max - sum
+        assert g.centralization(measure=0, normalized=False, loops=False) == \
+            n * (n - 1) - 2 * (n - 1)
+        d = OracleGraph(n, s, t, None, True)  # out-star
+        assert abs(d.centralization(measure=0, mode=1, loops=False) - 1.0) < 1e-14
+        assert abs(d.centralization(measure=0, mode=1, convention=3) - 1.0) < 1e-14
+
+
+def test_twomode_betweenness_max_is_attained():
+    # Exhaustive search over small bipartite graphs: the largest normalized
+    # betweenness of a first-mode node is exactly 1, so the maximum of
+    # bipartite.rs:228-241 is attained and never exceeded.
+    import itertools
+    for own, other in ((2, 2), (3, 2), (2, 3), (4, 2), (3, 3), (1, 3), (4, 1)):
+        n = own + other
+        types = [0] * own + [1] * other
+        pairs = [(a, own + b) for a in range(own) for b in range(other)]
+        best = 0.0
+        for mask in range(1, 1 << len(pairs)):
+            es = [pairs[i] for i in range(len(pairs)) if mask >> i & 1]
+            g = OracleGraph(n, [e[0] for e in es], [e[1] for e in es], None, False)
+            b = g.twomode_centrality(types, measure=2)
+            if not np.isnan(b[0]):
+                best = max(best, float(np.nanmax(b[:own])))
+        want = 1.0 if other > 1 or own == 1 else 0.0
+        assert abs(best - want) < 1e-12, (own, other, best)
+
+
+def test_centralization_and_twomode_errors():
+    g = OracleGraph(4, [0, 1, 2], [1, 2, 3], None, True)
+    for kw, msg in (({"measure": 4}, "unknown centralization measure code 4"),
+                    ({"mode": 0}, "unknown degree mode code 0"),
+                    ({"convention": 7}, "unknown convention code 7"),
+                    ({"convention": 1}, "NetworkX has no centralization index; use the igraph, "
+                                        "sna, or UCINET convention"),
+                    ({"measure": 3, "convention": 2},
+                     "eigenvector centralization is provided for the igraph convention only, "
+                     "not sna"),
+                    ({"measure": 3, "mode": 1},
+                     'eigenvector centralization is provided with direction ignored '
+                     '\\(mode "all"\\) only'),
+                    ({"measure": 3}, "eigenvector centralization is not part of this library"),
+                    ({"measure": 0, "convention": 3},
+                     "UCINET reports degree centralization for in- and out-degree only")):
+        with pytest.raises(OracleError, match="^" + msg + "$"):
+            g.centralization(**kw)
+    b = OracleGraph(4, [0, 1], [2, 3], None, False)
+    with pytest.raises(OracleError, match="^node types are missing$"):
+        b.twomode_centrality(None)
+    with pytest.raises(OracleError, match="^node 2 has type code 5; use 0 for the first mode "
+                                          "and 1 for the second$"):
+        b.twomode_centrality([0, 0, 5, 1])
+    with pytest.raises(OracleError, match="^unknown two-mode centrality code 3$"):
+        b.twomode_centrality([0, 0, 1, 1], measure=3)
+    with pytest.raises(OracleError, match="^unknown code 2 for unreachable nodes$"):
+        b.twomode_centrality([0, 0, 1, 1], measure=1, unreachable=2)
+    with pytest.raises(OracleError, match="^edge 1 joins nodes 1 and 3, which are in the same "
+                                          "mode; in a two-mode network every edge joins the two "
+                                          "modes$"):
+        b.twomode_centrality([0, 1, 1, 1])
+    with pytest.raises(OracleError, match="^the Wasserman-Faust rule applies to normalized "
+                                          "closeness only$"):
+        b.twomode_centrality([0, 0, 1, 1], measure=1, unreachable=1, normalized=False)
