@@ -292,7 +292,9 @@ def main() -> None:
         "kernel": "bfs_pull_kernel<false>" if args.closeness else
                   f"{'dial' if weighted else 'bfs'}_backward_kernel<"
                   f"{'double' if args.precision == 0 else 'float'}, "
-                  f"{'true' if args.edge else 'false'}>",
+                  f"{'true' if args.edge else 'false'}, "
+                  f"{('RealPath, ' if args.workload == 'ba200k-real' else 'IntPath, ') if weighted else ''}"
+                  f"false>",
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

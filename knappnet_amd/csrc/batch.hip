@@ -384,7 +384,8 @@ __device__ __forceinline__ int owner_lane(uint32_t pre, uint32_t t) {
 // a successor in lanes m. Returns in lane c the lanes N_c = OR of its hits.
 // With kAdd the sums are added atomically (exact: integer partial sums)
 // instead of stored, for pairs whose in-row is split over several tasks.
-template <bool kAdd>
+// With kNpr each lane's count of predecessor arcs is also written to npr.
+template <bool kAdd, bool kNpr = false>
 __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64_t U, uint32_t i0,
                                                   uint32_t deg, const uint32_t* __restrict__ in_idx,
                                                   const uint64_t* __restrict__ cur,
@@ -409,7 +410,7 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
             double* dst = &sig[r];
             if (kAdd) unsafeAtomicAdd(dst, acc);
             else *dst = acc;
-            if (npr) {
+            if constexpr (kNpr) {
                 if (kAdd) atomicAdd(&npr[r], cnt);
                 else npr[r] = cnt;
             }
@@ -465,7 +466,7 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
                 }
                 ncur |= hm[k];
                 acc += val[k];
-                cnt += (uint32_t)((hm[k] >> lane) & 1);
+                if constexpr (kNpr) cnt += (uint32_t)((hm[k] >> lane) & 1);
             }
         }
     }
@@ -625,6 +626,8 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
 // the in-neighbours record their successor lanes (succ). A wavefront takes
 // gsz tasks of one slot and walks the arcs of those with N != 0 as one
 // concatenation (gather_counts); slots are spread over the XCDs (xcd_wave).
+// kNpr: predecessor-arc counts as well (npr).
+template <bool kNpr>
 __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chunks, int slots,
                                const uint32_t* __restrict__ in_idx,
                                const uint64_t* __restrict__ cur, uint64_t* __restrict__ cand,
@@ -653,7 +656,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
         }
         const uint32_t deg = N ? ht.z - ht.y : 0;
         NETBC_COUNT(c6, N != 0);
-        const uint64_t got = gather_counts<true>(ht.x, b, N, ht.y, deg, in_idx, cur,
+        const uint64_t got = gather_counts<true, kNpr>(ht.x, b, N, ht.y, deg, in_idx, cur,
                                                  bottom_up ? succ : nullptr, sig, n, c4, c5,
                                                  npr);
         if (bottom_up) {
@@ -733,8 +736,8 @@ __global__ void frontier_lanes_kernel(const QEnt* __restrict__ q, uint32_t qbeg,
 // bfs_hub_kernel). Also counts the out-arcs of the new entries and the
 // in-arcs of pairs whose active lanes are now all reached (the direction
 // choice of the next step). Without kCount (reachability only) no path
-// counts are gathered.
-template <bool kCount>
+// counts are gathered; kNpr: predecessor-arc counts as well (npr).
+template <bool kCount, bool kNpr = false>
 __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
                                   const uint32_t* __restrict__ tlen_cur,
                                   uint32_t* __restrict__ tlen_next,
@@ -769,8 +772,8 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
             if (!hub_flag[te.x]) deg = i1 - i0;
         }
         if constexpr (kCount)
-            (void)gather_counts<false>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig, n, c4,
-                                       c5, npr);
+            (void)gather_counts<false, kNpr>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig,
+                                             n, c4, c5, npr);
         if (ok) {
             cand[p] = 0;
             const uint64_t vn = vis[p] | N;
@@ -816,8 +819,8 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
 // bitmap (AND-NOT, commuting with the OR of the new lanes: a node's lanes
 // at levels d-1 and d+1 are disjoint). Without kCount (reachability only)
 // each pair walks its own in-row and stops at the first in-neighbours that
-// cover every lane sought.
-template <bool kCount>
+// cover every lane sought. kNpr: predecessor-arc counts as well (npr).
+template <bool kCount, bool kNpr = false>
 __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
                                 const uint32_t* __restrict__ in_off,
                                 const uint32_t* __restrict__ in_idx,
@@ -868,7 +871,8 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         NETBC_COUNT(c6, deg != 0);
         uint64_t mine = 0;
         if constexpr (kCount) {
-            mine = gather_counts<false>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5, npr);
+            mine = gather_counts<false, kNpr>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5,
+                                              npr);
         } else if (deg) {
             const uint64_t* crow = cur + (size_t)b * n;
             const uint32_t* row = in_idx + i0;
@@ -2922,7 +2926,9 @@ struct Engine {
                                    (opt.direction == 0 && f_arcs * alpha > open_arcs);
             if (bottom_up) {
                 diag.bottom_up_levels++;
-                auto pull = reach_only ? bfs_pull_kernel<false> : bfs_pull_kernel<true>;
+                auto pull = reach_only ? bfs_pull_kernel<false>
+                            : npr      ? bfs_pull_kernel<true, true>
+                                       : bfs_pull_kernel<true, false>;
                 if (reach_only) {
                     ws.sweep.mark(st);
                     stale_bytes += 24ull * (ze - zb);  // stale entries cleared
@@ -2954,8 +2960,9 @@ struct Engine {
                 launch_check();
             } else if (d.hubs && !reach_only) {
                 const size_t hub_tasks = (size_t)d.hub_chunks * slots;
-                const uint32_t gsz = group_size(hub_tasks, resident_waves(bfs_hub_kernel));
-                bfs_hub_kernel<<<xcd_grid(resident_blocks(bfs_hub_kernel,
+                auto hub = npr ? bfs_hub_kernel<true> : bfs_hub_kernel<false>;
+                const uint32_t gsz = group_size(hub_tasks, resident_waves(hub));
+                hub<<<xcd_grid(resident_blocks(hub,
                                                           (hub_tasks + gsz - 1) / gsz)),
                                  kBlock, 0, st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
                                                   cur, ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
@@ -2964,7 +2971,9 @@ struct Engine {
                                                   frontier_lanes(dl), ws.ctrl.p, gsz, n, npr);
                 launch_check();
             }
-            auto gather = reach_only ? bfs_gather_kernel<false> : bfs_gather_kernel<true>;
+            auto gather = reach_only ? bfs_gather_kernel<false>
+                          : npr      ? bfs_gather_kernel<true, true>
+                                     : bfs_gather_kernel<true, false>;
             gather<<<resident_blocks(gather, (pairs + 63) / 64), kBlock, 0, st>>>(ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
                                       &ws.ctrl.p->tlen[(dl + 1) & 1], d.in_off_p(), d.in_idx_p(),
                                       d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
