@@ -289,7 +289,7 @@ def main() -> None:
     achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "bfs_pull_kernel<false>" if args.closeness else
+        "kernel": "bfs_pull_kernel<false, false>" if args.closeness else
                   f"{'dial' if weighted else 'bfs'}_backward_kernel<"
                   f"{'double' if args.precision == 0 else 'float'}, "
                   f"{'true' if args.edge else 'false'}, "
