@@ -104,6 +104,9 @@ constexpr int kLevelBufs = 16;
 constexpr int kKeptLevels = kLevelBufs - 2;
 // Touched nodes staged per wave before one append to the shared list.
 constexpr uint32_t kTouchBuf = 256;
+// Queue entries staged per wavefront by the bottom-up step (16 bytes each).
+constexpr uint32_t kPullBuf = 192;
+constexpr size_t kPullLds = (size_t)(kBlock / 64) * kPullBuf * 16;
 // A level is searched bottom-up when the out-arcs of its entries times
 // this factor exceed the in-arcs of the pairs with unreached lanes.
 #ifndef NETBC_ALPHA
@@ -474,6 +477,42 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
     return mine;
 }
 
+// Level-queue entries staged per wavefront in LDS (kPullBuf of them, from
+// `buf`) and appended a batch at a time: one queue-slot atomic per flush
+// instead of one per 64-pair group. Entries of heavy rows also get their
+// queue positions listed in hq. cnt is wave-uniform; every lane calls.
+struct QueueStage {
+    QEnt* buf;
+    uint32_t cnt;
+    __device__ void flush(Ctrl* ctrl, QEnt* q, uint32_t qcap, uint32_t* hq,
+                          const uint8_t* heavy_flag) {
+        const int lane = lane_id();
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(&ctrl->qlen, cnt);
+        at = rl32(at, 0);
+        for (uint32_t i = lane; i < cnt; i += 64) {
+            const QEnt e = buf[i];
+            if (at + i < qcap) {
+                q[at + i] = e;
+                if (heavy_flag[e.v]) hq[atomicAdd(&ctrl->hlen, 1u)] = at + i;
+            } else {
+                ctrl->qoverflow = 1;
+            }
+        }
+        cnt = 0;
+    }
+    // Stages (v, b, m) for the lanes with `has`.
+    __device__ void add(bool has, uint32_t v, uint32_t b, uint64_t m, Ctrl* ctrl, QEnt* q,
+                        uint32_t qcap, uint32_t* hq, const uint8_t* heavy_flag) {
+        const uint64_t bal = __ballot(has);
+        if (!bal) return;
+        const uint32_t k = (uint32_t)__popcll(bal);
+        if (cnt + k > kPullBuf) flush(ctrl, q, qcap, hq, heavy_flag);
+        if (has) buf[cnt + __popcll(bal & lanemask_lt(lane_id()))] = QEnt{v, b, (unsigned long long)m};
+        cnt += k;
+    }
+};
+
 // Top-down step of level d: discovery. Every frontier entry (v, b, M)
 // offers its lanes to each out-neighbour x not yet reached in them
 // (nv = M & ~vis[x]); offers are ORed into cand[x], and x enters the
@@ -690,6 +729,17 @@ __global__ void reach_hub_kernel(const uint4* __restrict__ hub_task, uint32_t ch
                                  uint32_t* __restrict__ tlen, const uint64_t* __restrict__ flanes,
                                  size_t n) {
     const int lane = lane_id();
+    // touched pairs staged per wavefront, appended 64 at a time
+    __shared__ uint2 tbuf[kBlock / 64][64];
+    uint2* mybuf = tbuf[threadIdx.x >> 6];
+    uint32_t cnt = 0;  // wave-uniform
+    auto flush = [&]() {
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(tlen, cnt);
+        at = rl32(at, 0);
+        if ((uint32_t)lane < cnt) touched[at + lane] = mybuf[lane];
+        cnt = 0;
+    };
     const size_t tasks = (size_t)chunks * slots;
     for (size_t w = global_wave(); w < tasks; w += wave_count()) {
         const uint32_t b = (uint32_t)(w / chunks);
@@ -704,10 +754,16 @@ __global__ void reach_hub_kernel(const uint4* __restrict__ hub_task, uint32_t ch
             if (a + lane < ht.z) m = crow[in_idx[a + lane]] & N;
             got |= wave_or(m);
         }
-        if (lane == 0 && got &&
-            atomicOr((unsigned long long*)&cand[p], (unsigned long long)got) == 0)
-            touched[atomicAdd(tlen, 1u)] = make_uint2(ht.x, b);
+        bool first = false;
+        if (lane == 0 && got)
+            first = atomicOr((unsigned long long*)&cand[p], (unsigned long long)got) == 0;
+        if (__ballot(first)) {
+            if (cnt == 64) flush();
+            if (lane == 0) mybuf[cnt] = make_uint2(ht.x, b);
+            cnt++;
+        }
     }
+    if (cnt) flush();
 }
 
 // Lanes of each slot that hold entries of a level (the lanes whose search
@@ -754,10 +810,12 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) *tlen_next = 0;
     const uint32_t T = *tlen_cur;
-    const uint64_t lt = lanemask_lt(lane);
     unsigned long long fa = 0, closed = 0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
     const uint32_t gsz = group_size(T, wave_count());
+    // new entries staged in dynamic LDS (kPullLds bytes per block)
+    extern __shared__ unsigned long long pull_lds[];
+    QueueStage stage{reinterpret_cast<QEnt*>(pull_lds) + (threadIdx.x >> 6) * kPullBuf, 0};
     for (size_t t0 = global_wave() * gsz; t0 < T; t0 += wave_count() * gsz) {
         const size_t t = t0 + lane;
         const bool ok = (uint32_t)lane < gsz && t < T;
@@ -782,21 +840,9 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
             fa += out_off[te.x + 1] - out_off[te.x];
             if (vn == active_lanes(left, te.y)) closed += i1 - i0;
         }
-        const uint64_t bal = __ballot(ok);
-        if (bal) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&ctrl->qlen, (uint32_t)__popcll(bal));
-            pos = rl32(pos, 0) + (uint32_t)__popcll(bal & lt);
-            if (ok) {
-                if (pos < qcap) {
-                    q[pos] = QEnt{te.x, te.y, (unsigned long long)N};
-                    if (heavy_flag[te.x]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
-                } else {
-                    ctrl->qoverflow = 1;
-                }
-            }
-        }
+        stage.add(ok, te.x, te.y, N, ctrl, q, qcap, hq, heavy_flag);
     }
+    if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
     NETBC_FLUSH(ctrl, 10, c4);
     NETBC_FLUSH(ctrl, 11, c5);
     fa = wave_sum_u64(fa);
@@ -842,9 +888,11 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         QEnt e = qz[i];
         atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
     }
-    const uint64_t lt = lanemask_lt(lane);
     unsigned long long fa = 0, closed = 0, bytes = 0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
+    // new entries staged in dynamic LDS (kPullLds bytes per block)
+    extern __shared__ unsigned long long pull_lds[];
+    QueueStage stage{reinterpret_cast<QEnt*>(pull_lds) + (threadIdx.x >> 6) * kPullBuf, 0};
     const XcdWave xw = xcd_wave();
     const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
@@ -857,15 +905,17 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         const uint32_t x = ok ? x0 + lane : 0;
         const size_t p = (size_t)b * n + x;
         uint64_t U = 0, V = 0;
-        uint32_t i0 = 0, deg = 0;
+        uint32_t i0 = 0, deg = 0, odeg = 0;
         if (ok) {
+            // independent loads, issued together
             V = vis[p];
+            const uint8_t hub = hub_flag[x];
+            const uint32_t a0 = in_off[x], a1 = in_off[x + 1];
+            odeg = out_off[x + 1] - out_off[x];
             U = F & ~V;
-            {
-                if (U && !hub_flag[x]) {
-                    i0 = in_off[x];
-                    deg = in_off[x + 1] - i0;
-                }
+            if (U && !hub) {
+                i0 = a0;
+                deg = a1 - a0;
             }
         }
         NETBC_COUNT(c6, deg != 0);
@@ -887,24 +937,12 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
             const uint64_t vn = V | mine;
             vis[p] = vn;
             atomicOr((unsigned long long*)&nxt[p], (unsigned long long)mine);
-            fa += out_off[x + 1] - out_off[x];
+            fa += odeg;
             if (vn == active_lanes(left, b)) closed += deg;
         }
-        const uint64_t bal = __ballot(mine != 0);
-        if (bal) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&ctrl->qlen, (uint32_t)__popcll(bal));
-            pos = rl32(pos, 0) + (uint32_t)__popcll(bal & lt);
-            if (mine) {
-                if (pos < qcap) {
-                    q[pos] = QEnt{x, b, (unsigned long long)mine};
-                    if (heavy_flag[x]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
-                } else {
-                    ctrl->qoverflow = 1;
-                }
-            }
-        }
+        stage.add(mine != 0, x, b, mine, ctrl, q, qcap, hq, heavy_flag);
     }
+    if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
     NETBC_FLUSH(ctrl, 8, c4);
     NETBC_FLUSH(ctrl, 9, c5);
     NETBC_FLUSH(ctrl, 15, c6);
@@ -2933,7 +2971,8 @@ struct Engine {
                     ws.sweep.mark(st);
                     stale_bytes += 24ull * (ze - zb);  // stale entries cleared
                 }
-                pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, 0, st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
+                pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, kPullLds,
+                       st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
@@ -2974,7 +3013,8 @@ struct Engine {
             auto gather = reach_only ? bfs_gather_kernel<false>
                           : npr      ? bfs_gather_kernel<true, true>
                                      : bfs_gather_kernel<true, false>;
-            gather<<<resident_blocks(gather, (pairs + 63) / 64), kBlock, 0, st>>>(ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
+            gather<<<resident_blocks(gather, (pairs + 63) / 64), kBlock, kPullLds, st>>>(
+                ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
                                       &ws.ctrl.p->tlen[(dl + 1) & 1], d.in_off_p(), d.in_idx_p(),
                                       d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
                                       ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.q.p,
