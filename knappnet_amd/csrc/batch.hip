@@ -64,11 +64,6 @@ struct Ctrl {
     unsigned long long reached_nodes;
     unsigned long long reached_arcs;
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
-    // Bytes moved by reachability-only bottom-up steps (bfs_pull_kernel
-    // without path counts): 8 per pair examined, 8 per in-row looked up
-    // plus 12 per in-arc read, 40 per pair reached (visited and level words,
-    // out-row, queue entry).
-    unsigned long long reach_bytes;
     // Work counters (filled only in builds with NETBC_COUNTERS):
     // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes
     // (integer lengths: 0 settled pairs, 1 their in-arcs, 2 out-arcs, 3 offers),
@@ -888,7 +883,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         QEnt e = qz[i];
         atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
     }
-    unsigned long long fa = 0, closed = 0, bytes = 0;
+    unsigned long long fa = 0, closed = 0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
     // new entries staged in dynamic LDS (kPullLds bytes per block)
     extern __shared__ unsigned long long pull_lds[];
@@ -930,9 +925,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
             for (; a + 4 <= deg && mine != U; a += 4)
                 mine |= (crow[row[a]] | crow[row[a + 1]] | crow[row[a + 2]] | crow[row[a + 3]]) & U;
             for (; a < deg && mine != U; a++) mine |= crow[row[a]] & U;
-            bytes += 8 + 12ull * a;
         }
-        if (!kCount && ok) bytes += 8 + (mine ? 40 : 0);
         if (mine) {
             const uint64_t vn = V | mine;
             vis[p] = vn;
@@ -948,11 +941,9 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     NETBC_FLUSH(ctrl, 15, c6);
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
-    if constexpr (!kCount) bytes = wave_sum_u64(bytes);
     if (lane == 0) {
         if (fa) atomicAdd(front_next, fa);
         if (closed) atomicAdd(&ctrl->closed_arcs, closed);
-        if (!kCount && bytes) atomicAdd(&ctrl->reach_bytes, bytes);
     }
 }
 
@@ -1406,7 +1397,7 @@ struct IntPath {
     using Dist = uint32_t;
     using Len = uint32_t;
     using Sum = uint64_t;  // distance + length without wrapping
-    static constexpr uint64_t kInfBits = kNone;
+    [[maybe_unused]] static constexpr uint64_t kInfBits = kNone;
     __device__ static Sum add(Dist d, Len l) { return (Sum)d + l; }
     __device__ static bool reached(Dist d) { return d != kNone; }
     __device__ static uint64_t bucket(Sum x, double) { return x; }
@@ -2248,8 +2239,9 @@ __global__ void fold_arcs_kernel(const double* __restrict__ eacc, int rows, size
 // ep[v], for a target v, the lanes of the super-batch that reached it
 // (including a lane whose source is v); endpoint_lanes_kernel counts, per
 // (slot, lane), the targets the lane reached (including its source when
-// that is a target), 64 nodes per wavefront: lane l tests bit l of each
-// nonzero loaded word in turn. endpoint_sources_kernel adds each lane's count to
+// that is a target), 64 nodes per load: lane l tests bit l of each nonzero
+// loaded word in turn; a wavefront keeps to one slot and adds its counts
+// once. endpoint_sources_kernel adds each lane's count to
 // its source, less 2 when the source is a target (the two self counts).
 // The counts are integers, so the order of the atomic adds does not matter.
 __global__ void endpoint_nodes_kernel(const uint64_t* __restrict__ vis, int slots, size_t n,
@@ -2269,16 +2261,23 @@ __global__ void endpoint_lanes_kernel(const uint64_t* __restrict__ vis, int slot
                                       unsigned long long* __restrict__ lane_count) {
     const int lane = lane_id();
     const size_t chunks = (n + 63) / 64;
-    for (size_t w = global_wave(); w < chunks * (size_t)slots; w += wave_count()) {
-        const size_t b = w / chunks, v = (w % chunks) * 64 + lane;
-        uint64_t word = 0;
-        if (v < n && (!target || target[v])) word = vis[b * n + v];
+    // wps wavefronts per slot, each taking every wps-th chunk of its slot and
+    // adding its lane counts once at the end
+    const size_t G = wave_count();
+    const size_t wps = std::max<size_t>(1, G / (size_t)slots);
+    for (size_t idx = global_wave(); idx < (size_t)slots * wps; idx += G) {
+        const size_t b = idx / wps;
         unsigned long long c = 0;
-        uint64_t nz = __ballot(word != 0);  // only the nonzero words
-        while (nz) {
-            const int j = __ffsll((unsigned long long)nz) - 1;
-            nz &= nz - 1;
-            c += (rl64(word, j) >> lane) & 1;
+        for (size_t ch = idx % wps; ch < chunks; ch += wps) {
+            const size_t v = ch * 64 + lane;
+            uint64_t word = 0;
+            if (v < n && (!target || target[v])) word = vis[b * n + v];
+            uint64_t nz = __ballot(word != 0);  // only the nonzero words
+            while (nz) {
+                const int j = __ffsll((unsigned long long)nz) - 1;
+                nz &= nz - 1;
+                c += (rl64(word, j) >> lane) & 1;
+            }
         }
         if (c) atomicAdd(&lane_count[b * kLanes + lane], c);
     }
@@ -2859,9 +2858,11 @@ struct Engine {
     uint64_t* lane_buf = nullptr;
     unsigned long long* lane_count = nullptr;
     bool reach_only = false;  // distances only, no path counts
-    // Reachability-only breadth-first runs time their bottom-up steps with
-    // ws.sweep; bytes of the stale level entries those steps clear.
-    unsigned long long stale_bytes = 0;
+    // Reachability-only breadth-first runs time their top-down expansions
+    // with ws.sweep and count their algorithmic bytes: 24 per level entry
+    // (entry and row bounds), 12 per out-arc scanned (column and the head's
+    // visited word), 24 per stale entry cleared.
+    unsigned long long timed_bytes = 0;
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -2967,10 +2968,6 @@ struct Engine {
                 auto pull = reach_only ? bfs_pull_kernel<false>
                             : npr      ? bfs_pull_kernel<true, true>
                                        : bfs_pull_kernel<true, false>;
-                if (reach_only) {
-                    ws.sweep.mark(st);
-                    stale_bytes += 24ull * (ze - zb);  // stale entries cleared
-                }
                 pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, kPullLds,
                        st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
@@ -2978,18 +2975,23 @@ struct Engine {
                                         ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
                                         frontier_lanes(dl), slots, left, n, npr);
                 launch_check();
-                if (reach_only) ws.sweep.mark(st);
             } else {
                 const uint32_t gsz =
                     group_size(qe - qb, resident_waves(bfs_expand_kernel));
                 size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + hc[dl];
                 size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
+                if (reach_only) {
+                    ws.sweep.mark(st);
+                    timed_bytes += 24ull * (qe - qb) + 12ull * ws.host_ctrl->front_arcs[dl & 1] +
+                                   24ull * (ze - zb);
+                }
                 bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
                     d.out_idx.p, ws.vis.p, ws.cand.p, nxt, reach_only ? nullptr : ws.succ.p,
                     ws.touched.p,
                     &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, &front[dl & 1], gsz, n);
                 launch_check();
+                if (reach_only) ws.sweep.mark(st);
             }
             if (d.hubs && reach_only && bottom_up) {
                 const size_t hub_tasks = (size_t)d.hub_chunks * slots;
@@ -3732,7 +3734,7 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     ws.sweep.collect();
     diag.dominant_ms += ws.sweep.ms;
     diag.dominant_launches += ws.sweep.launches;
-    diag.dominant_bytes += (int64_t)(ws.host_ctrl->reach_bytes + eng.stale_bytes);
+    diag.dominant_bytes += (int64_t)eng.timed_bytes;
     double mx;
     const unsigned long long bits = ws.host_ctrl->max_sigma_bits;
     memcpy(&mx, &bits, 8);
