@@ -266,6 +266,22 @@ def test_dial_ba_bitexact():
     assert_same(got, o.betweenness(weighted=True, threads=0), "ba3000")
 
 
+def test_dial_free_order_weighted_landmarks():
+    # 5,000 sources: batched by weighted distances to the landmarks (integer
+    # lengths, undirected); only the association of the FP64 sums changes
+    el = gen.ba_weighted(n=5000, m=4, seed=11)
+    say("ba5000", el.m)
+    g, o = pair(el.n, el.src, el.tgt, el.weights, False)
+    got, diag = g.betweenness_sources(weighted=True)
+    assert diag["path_kind"] == 1
+    assert_rel(got, o.betweenness(weighted=True, threads=0), "ba5000 free order")
+    exact, _ = g.betweenness_sources(weighted=True, exact=True)
+    assert_same(exact, o.betweenness(weighted=True, threads=0), "ba5000 exact order")
+    src = np.concatenate([np.arange(el.n), np.arange(0, el.n, 3)])
+    got, _ = g.betweenness_sources(src, weighted=True)
+    assert_rel(got, o.betweenness_sources(src, weighted=True, threads=0), "ba5000 repeats")
+
+
 @pytest.mark.parametrize("weighted", [False, True])
 def test_state_dump_bitexact(weighted):
     rng = np.random.default_rng(77)
