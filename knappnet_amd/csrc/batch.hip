@@ -2063,10 +2063,10 @@ __global__ void source_key_kernel(const int64_t* __restrict__ list, int64_t k,
     key[i] = kv;
 }
 
-// One Bellman-Ford round for the weighted landmark distances of an
-// undirected graph with integer lengths: every node v takes, per landmark j
-// < kWeightedLandmarks, the least of wd[v][j] and wd[u][j] + len(u, v)
-// over its neighbours u (a wavefront per node, lanes striding its row, then
+// One Bellman-Ford round for the weighted distances from every node to the
+// landmarks (integer lengths): every node v takes, per landmark j <
+// kWeightedLandmarks, the least of wd[v][j] and len(v, u) + wd[u][j] over
+// its out-neighbours u (a wavefront per node, lanes striding its row, then
 // a wavefront minimum). Updates in place (the minima only fall, so the
 // rounds converge to the distances); *changed is set if any fell.
 __global__ void landmark_relax_kernel(const uint32_t* __restrict__ off,
@@ -3384,9 +3384,9 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
 // sorts, by second out-neighbour, by first out-neighbour, and by the
 // distances to the graph's landmarks (capped at 15), the last one deciding
 // first. Sources in one batch then reach most nodes on the same level, so
-// the 64-lane rows carry more lanes per entry. For undirected graphs with
-// integer lengths the last sort uses the weighted distances to the first 8
-// landmarks instead (capped at 255): sources at equal lengths from the hubs
+// the 64-lane rows carry more lanes per entry. With integer lengths the
+// last sort uses the weighted distances to the first 8 landmarks instead
+// (capped at 255): sources at equal lengths from the hubs
 // settle most nodes in the same buckets (on Barabasi-Albert n = 200,000
 // with lengths 1-16, 9-29 lanes per settled entry instead of 4-5 by hop
 // counts). Writes the list to ws.list2.
@@ -3425,9 +3425,9 @@ void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hi
         // the seed buffer must outlive the copies
         NETBC_HIP(hipStreamSynchronize(st));
     }
-    const bool weighted = L && L->kind == kInteger && !dg.directed && lanes > 0;
+    const bool weighted = L && L->kind == kInteger && lanes > 0;
     if (weighted) {
-        // weighted distances from the first landmarks, by Bellman-Ford rounds
+        // weighted distances to the first landmarks, by Bellman-Ford rounds
         const int wl = std::min(lanes, kWeightedLandmarks);
         DevBuf<uint32_t> wd, flag;
         wd.alloc(n * kWeightedLandmarks);

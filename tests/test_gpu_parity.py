@@ -280,6 +280,13 @@ def test_dial_free_order_weighted_landmarks():
     src = np.concatenate([np.arange(el.n), np.arange(0, el.n, 3)])
     got, _ = g.betweenness_sources(src, weighted=True)
     assert_rel(got, o.betweenness_sources(src, weighted=True, threads=0), "ba5000 repeats")
+    # directed: distances to the landmarks along out-arcs
+    el = gen.rmat(12, 8, directed=True, seed=21)
+    w = np.random.default_rng(4).integers(1, 12, el.m).astype(np.float64)
+    g, o = pair(el.n, el.src, el.tgt, w, True)
+    say("rmat12 directed integer", el.m)
+    assert_rel(g.betweenness(weighted=True), o.betweenness(weighted=True, threads=0),
+               "rmat12 directed free order")
 
 
 @pytest.mark.parametrize("weighted", [False, True])
