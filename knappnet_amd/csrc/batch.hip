@@ -55,9 +55,13 @@ struct Ctrl {
     // Bucketed searches: the lowest bucket still pending after a step
     // (alternating by step parity; all ones when none).
     unsigned long long next_bucket[2];
-    // Work tickets of dynamically scheduled kernels; the last wavefront to
-    // finish resets both for the next launch.
-    uint32_t ticket, exits;
+    // Work tickets of the dependency sweep, one counter pair per XCD class of
+    // workgroups (blockIdx.x % 8), each in its own 64-byte line; the last
+    // wavefront of a class to finish resets its pair for the next launch.
+    struct XTicket {
+        uint32_t ticket, exits;
+        uint32_t pad[14];
+    } xticket[8];
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
@@ -328,18 +332,23 @@ __device__ __forceinline__ uint32_t xcd_slots(uint32_t k, int slots) {
     return (uint32_t)slots > k ? ((uint32_t)slots - k + kXcds - 1) / kXcds : 0;
 }
 
-// Next task of a dynamically scheduled kernel (wave-uniform).
-__device__ __forceinline__ uint32_t next_ticket(uint32_t* ticket) {
+// Next task of the dependency sweep (wave-uniform): the wavefronts of XCD
+// class xc take the tasks xc, xc + 8, xc + 16, ... from their own counter,
+// so eight counters share the atomic traffic.
+__device__ __forceinline__ uint32_t next_ticket(Ctrl* ctrl, uint32_t xc) {
     uint32_t t = 0;
-    if (lane_id() == 0) t = atomicAdd(ticket, 1u);
-    return rl32(t, 0);
+    if (lane_id() == 0) t = atomicAdd(&ctrl->xticket[xc].ticket, 1u);
+    return xc + kXcds * rl32(t, 0);
 }
-// Called once per wavefront after its last ticket: the last one out resets
-// the counters for the next launch on the stream.
-__device__ __forceinline__ void retire_tickets(uint32_t* ticket, uint32_t* exits) {
-    if (lane_id() == 0 && atomicAdd(exits, 1u) == (uint32_t)wave_count() - 1) {
-        *ticket = 0;
-        *exits = 0;
+// Called once per wavefront after its last ticket: the last one of its
+// class resets the class's counters for the next launch on the stream.
+// Every class must hold a workgroup (grids of at least 8 workgroups).
+__device__ __forceinline__ void retire_tickets(Ctrl* ctrl, uint32_t xc) {
+    const uint32_t blocks = gridDim.x / kXcds + (xc < gridDim.x % kXcds ? 1u : 0u);
+    const uint32_t waves = blocks * (blockDim.x >> 6);
+    if (lane_id() == 0 && atomicAdd(&ctrl->xticket[xc].exits, 1u) == waves - 1) {
+        ctrl->xticket[xc].ticket = 0;
+        ctrl->xticket[xc].exits = 0;
     }
 }
 
@@ -1194,10 +1203,11 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
     const size_t tasks = groups + (split ? hchunks : 0);
     // Tasks are handed out by ticket, the next one fetched while the
     // current one runs.
-    uint32_t next = next_ticket(&ctrl->ticket);
+    const uint32_t xc = blockIdx.x % kXcds;
+    uint32_t next = next_ticket(ctrl, xc);
     while (next < tasks) {
         const size_t w = next;
-        next = next_ticket(&ctrl->ticket);
+        next = next_ticket(ctrl, xc);
         if (w >= groups) {
             // One chunk of a heavy row: its partial sum.
             uint32_t t = (uint32_t)(w - groups);
@@ -1345,7 +1355,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             }
         }
     }
-    retire_tickets(&ctrl->ticket, &ctrl->exits);
+    retire_tickets(ctrl, xc);
     note_sigma(ctrl, smax);
     NETBC_FLUSH(ctrl, 4, c4);
     NETBC_FLUSH(ctrl, 5, c5);
@@ -3211,7 +3221,8 @@ struct Engine {
             }
             ws.sweep.mark(st);
             bfs_backward_kernel<C, kEdge, kQuirk>
-                <<<resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves), kBlock, 0, st>>>(
+                <<<xcd_grid(resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves)), kBlock, 0,
+                   st>>>(
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
