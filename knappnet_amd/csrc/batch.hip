@@ -55,13 +55,14 @@ struct Ctrl {
     // Bucketed searches: the lowest bucket still pending after a step
     // (alternating by step parity; all ones when none).
     unsigned long long next_bucket[2];
-    // Work tickets of the dependency sweep, one counter pair per XCD class of
-    // workgroups (blockIdx.x % 8), each in its own 64-byte line; the last
-    // wavefront of a class to finish resets its pair for the next launch.
+    // Work tickets of the dependency sweep, one counter pair per class of
+    // workgroups (blockIdx.x % 32, four classes per XCD), each in its own
+    // 64-byte line; the last wavefront of a class to finish resets its pair
+    // for the next launch.
     struct XTicket {
         uint32_t ticket, exits;
         uint32_t pad[14];
-    } xticket[8];
+    } xticket[32];
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
@@ -332,19 +333,23 @@ __device__ __forceinline__ uint32_t xcd_slots(uint32_t k, int slots) {
     return (uint32_t)slots > k ? ((uint32_t)slots - k + kXcds - 1) / kXcds : 0;
 }
 
-// Next task of the dependency sweep (wave-uniform): the wavefronts of XCD
-// class xc take the tasks xc, xc + 8, xc + 16, ... from their own counter,
-// so eight counters share the atomic traffic.
+// Ticket classes of the dependency sweep (a multiple of kXcds).
+constexpr uint32_t kTicketClasses = 32;
+
+// Next task of the dependency sweep (wave-uniform): the wavefronts of
+// class xc (blockIdx.x % kTicketClasses) take the tasks xc, xc + 32, ...
+// from their own counter, so 32 counters share the atomic traffic.
 __device__ __forceinline__ uint32_t next_ticket(Ctrl* ctrl, uint32_t xc) {
     uint32_t t = 0;
     if (lane_id() == 0) t = atomicAdd(&ctrl->xticket[xc].ticket, 1u);
-    return xc + kXcds * rl32(t, 0);
+    return xc + kTicketClasses * rl32(t, 0);
 }
 // Called once per wavefront after its last ticket: the last one of its
 // class resets the class's counters for the next launch on the stream.
-// Every class must hold a workgroup (grids of at least 8 workgroups).
+// Every class must hold a workgroup (ticket_grid).
 __device__ __forceinline__ void retire_tickets(Ctrl* ctrl, uint32_t xc) {
-    const uint32_t blocks = gridDim.x / kXcds + (xc < gridDim.x % kXcds ? 1u : 0u);
+    const uint32_t blocks =
+        gridDim.x / kTicketClasses + (xc < gridDim.x % kTicketClasses ? 1u : 0u);
     const uint32_t waves = blocks * (blockDim.x >> 6);
     if (lane_id() == 0 && atomicAdd(&ctrl->xticket[xc].exits, 1u) == waves - 1) {
         ctrl->xticket[xc].ticket = 0;
@@ -1203,7 +1208,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
     const size_t tasks = groups + (split ? hchunks : 0);
     // Tasks are handed out by ticket, the next one fetched while the
     // current one runs.
-    const uint32_t xc = blockIdx.x % kXcds;
+    const uint32_t xc = blockIdx.x % kTicketClasses;
     uint32_t next = next_ticket(ctrl, xc);
     while (next < tasks) {
         const size_t w = next;
@@ -2696,6 +2701,13 @@ inline unsigned xcd_grid(unsigned blocks) {
     return std::max<unsigned>(kXcds, (blocks + kXcds - 1) / kXcds * kXcds);
 }
 
+// Grid of the ticketed sweep: a multiple of kTicketClasses, at least one
+// workgroup per class.
+inline unsigned ticket_grid(unsigned blocks) {
+    return std::max<unsigned>(kTicketClasses,
+                              (blocks + kTicketClasses - 1) / kTicketClasses * kTicketClasses);
+}
+
 // Wavefronts of `kernel` that fit on the device at once.
 template <class K>
 size_t resident_waves(K kernel) {
@@ -3221,8 +3233,8 @@ struct Engine {
             }
             ws.sweep.mark(st);
             bfs_backward_kernel<C, kEdge, kQuirk>
-                <<<xcd_grid(resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves)), kBlock, 0,
-                   st>>>(
+                <<<ticket_grid(resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves)), kBlock,
+                   0, st>>>(
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
