@@ -40,6 +40,10 @@ struct QEnt {
 
 // Per-super-batch counters come first (cleared before every super-batch);
 // the rest accumulate over the whole run.
+// Counters summed by the host are spread over this many addresses
+// (blockIdx.x % kSpread picks one).
+constexpr int kSpread = 32;
+
 struct Ctrl {
     uint32_t qlen;
     uint32_t tlen[2];
@@ -50,8 +54,10 @@ struct Ctrl {
     // Direction choice: out-arcs of the entries of a level (alternating by
     // level parity) and in-arcs of the (slot, node) pairs whose active lanes
     // are all reached.
-    unsigned long long front_arcs[2];
-    unsigned long long closed_arcs;
+    // Both are spread over kSpread counters (summed by the host) so that
+    // the wavefronts of a launch do not all add into one address.
+    unsigned long long front_arcs[2][32];
+    unsigned long long closed_arcs[32];
     // Bucketed searches: the lowest bucket still pending after a step
     // (alternating by step parity; all ones when none).
     unsigned long long next_bucket[2];
@@ -157,7 +163,10 @@ __device__ __forceinline__ void note_sigma(Ctrl* ctrl, double s) {
     double mx = wave_max(s);
     if (lane_id() == 0 && mx > 0.0) {
         unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
-        atomicMax(&ctrl->max_sigma_bits, bits);
+        // a plain read first: most wavefronts hold no new maximum (a stale
+        // value only lets an atomic through)
+        if (bits > *(volatile unsigned long long*)&ctrl->max_sigma_bits)
+            atomicMax(&ctrl->max_sigma_bits, bits);
         if (mx > kTwo53) ctrl->sig_overflow = 1;
     }
 }
@@ -208,7 +217,8 @@ __global__ void source_entries_kernel(const int64_t* __restrict__ list, int64_t 
     uint32_t pos = atomicAdd(&ctrl->qlen, 1u);
     q[pos] = QEnt{s, (uint32_t)b, (unsigned long long)M};
     if (hq && heavy_flag[s]) hq[atomicAdd(&ctrl->hlen, 1u)] = pos;
-    if (out_off) atomicAdd(&ctrl->front_arcs[0], (unsigned long long)(out_off[s + 1] - out_off[s]));
+    if (out_off)
+        atomicAdd(&ctrl->front_arcs[0][s % kSpread], (unsigned long long)(out_off[s + 1] - out_off[s]));
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t count, uint32_t v) {
@@ -547,7 +557,7 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
     [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nth = (size_t)gridDim.x * blockDim.x;
-    if (tid == 0) *front_done = 0;  // read by the host before this launch
+    if (tid < (size_t)kSpread) front_done[tid] = 0;  // read by the host before this launch
     for (size_t i = zbeg + tid; i < zend; i += nth) {
         QEnt e = q[i];
         nxt[(size_t)e.b * n + e.v] = 0;
@@ -857,8 +867,8 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
     if (lane == 0) {
-        if (fa) atomicAdd(front_next, fa);
-        if (closed) atomicAdd(&ctrl->closed_arcs, closed);
+        if (fa) atomicAdd(&front_next[blockIdx.x % kSpread], fa);
+        if (closed) atomicAdd(&ctrl->closed_arcs[blockIdx.x % kSpread], closed);
     }
 }
 
@@ -892,7 +902,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     const int lane = lane_id();
     size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nth = (size_t)gridDim.x * blockDim.x;
-    if (tid == 0) *front_done = 0;  // read by the host before this launch
+    if (tid < (size_t)kSpread) front_done[tid] = 0;  // read by the host before this launch
     for (size_t i = zbeg + tid; i < zend; i += nth) {
         QEnt e = qz[i];
         atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
@@ -956,8 +966,8 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
     if (lane == 0) {
-        if (fa) atomicAdd(front_next, fa);
-        if (closed) atomicAdd(&ctrl->closed_arcs, closed);
+        if (fa) atomicAdd(&front_next[blockIdx.x % kSpread], fa);
+        if (closed) atomicAdd(&ctrl->closed_arcs[blockIdx.x % kSpread], closed);
     }
 }
 
@@ -2807,6 +2817,12 @@ Workspace& get_workspace(Graph& g) {
     return *static_cast<Workspace*>(g.workspace);
 }
 
+unsigned long long spread_sum(const unsigned long long (&c)[kSpread]) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kSpread; i++) t += c[i];
+    return t;
+}
+
 void read_ctrl(Workspace& ws, hipStream_t st) {
     NETBC_HIP(hipMemcpyAsync(ws.host_ctrl, ws.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
     NETBC_HIP(hipStreamSynchronize(st));
@@ -3024,7 +3040,7 @@ struct Engine {
             (uint64_t)std::min<int64_t>(slots, (left + kLanes - 1) / kLanes);
         const size_t pairs = (size_t)slots * n;
         const double alpha = opt.alpha > 0.0 ? opt.alpha : kBottomUpAlpha;
-        unsigned long long* front = ws.ctrl.p->front_arcs;
+        unsigned long long* front[2] = {ws.ctrl.p->front_arcs[0], ws.ctrl.p->front_arcs[1]};
         for (int dl = 0;; dl++) {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             uint32_t hb = hv[dl], he = hv[dl + 1];
@@ -3037,9 +3053,9 @@ struct Engine {
             // Direction of this step: top-down costs the out-arcs of the
             // level's entries, bottom-up the in-arcs of the pairs with
             // unreached active lanes.
-            const double f_arcs = (double)ws.host_ctrl->front_arcs[dl & 1];
+            const double f_arcs = (double)spread_sum(ws.host_ctrl->front_arcs[dl & 1]);
             const double open_arcs =
-                (double)live_slots * (double)d.arcs - (double)ws.host_ctrl->closed_arcs;
+                (double)live_slots * (double)d.arcs - (double)spread_sum(ws.host_ctrl->closed_arcs);
             const bool bottom_up = opt.direction == 2 ||
                                    (opt.direction == 0 && f_arcs * alpha > open_arcs);
             if (bottom_up) {
@@ -3051,7 +3067,7 @@ struct Engine {
                        st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
-                                        ws.hq.p, ws.ctrl.p, &front[(dl + 1) & 1], &front[dl & 1],
+                                        ws.hq.p, ws.ctrl.p, front[(dl + 1) & 1], front[dl & 1],
                                         frontier_lanes(dl), slots, left, n, npr);
                 launch_check();
             } else {
@@ -3061,14 +3077,14 @@ struct Engine {
                 size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
                 if (reach_only) {
                     ws.sweep.mark(st);
-                    timed_bytes += 24ull * (qe - qb) + 12ull * ws.host_ctrl->front_arcs[dl & 1] +
+                    timed_bytes += 24ull * (qe - qb) + 12ull * spread_sum(ws.host_ctrl->front_arcs[dl & 1]) +
                                    24ull * (ze - zb);
                 }
                 bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
                     d.out_idx.p, ws.vis.p, ws.cand.p, nxt, reach_only ? nullptr : ws.succ.p,
                     ws.touched.p,
-                    &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, &front[dl & 1], gsz, n);
+                    &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, front[dl & 1], gsz, n);
                 launch_check();
                 if (reach_only) ws.sweep.mark(st);
             }
@@ -3100,7 +3116,7 @@ struct Engine {
                                       d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
                                       ws.sig.p, d.hub_flag.p, d.heavy_flag.p, ws.q.p,
                                       (uint32_t)ws.qcap, ws.hq.p, ws.ctrl.p, left,
-                                      &front[(dl + 1) & 1], n, npr);
+                                      front[(dl + 1) & 1], n, npr);
             launch_check();
             scan_heavy(hv.back());
             read_ctrl(ws, st);
