@@ -72,8 +72,8 @@ struct Ctrl {
     uint32_t sig_overflow;
     uint32_t pad2;
     unsigned long long max_sigma_bits;
-    unsigned long long reached_nodes;
-    unsigned long long reached_arcs;
+    unsigned long long reached_nodes[32];  // spread as front_arcs
+    unsigned long long reached_arcs[32];
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
     // Work counters (filled only in builds with NETBC_COUNTERS):
     // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes
@@ -1532,8 +1532,13 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
     }
     if (cnt) flush();
     for (int o = 32; o > 0; o >>= 1) ahead = min(ahead, (uint32_t)__shfl_xor((int)ahead, o, 64));
-    if (lane == 0 && ahead != kNone) atomicMin(next_bucket, (unsigned long long)D + ahead);
-    if (__syncthreads_or(any) && threadIdx.x == 0) atomicOr(left, 1u);
+    // plain reads first: most wavefronts change neither value (a stale read
+    // only lets an atomic through)
+    if (lane == 0 && ahead != kNone &&
+        (unsigned long long)D + ahead < *(volatile unsigned long long*)next_bucket)
+        atomicMin(next_bucket, (unsigned long long)D + ahead);
+    if (__syncthreads_or(any) && threadIdx.x == 0 && !*(volatile uint32_t*)left)
+        atomicOr(left, 1u);
 }
 
 // Settles and relaxes the listed pairs of bucket D (P: IntPath or
@@ -1711,7 +1716,8 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         const unsigned long long other = (unsigned long long)__shfl_xor((long long)least, o, 64);
         least = min(least, other);
     }
-    if (lane == 0 && least != ~0ull) atomicMin(next_bucket, least);
+    if (lane == 0 && least != ~0ull && least < *(volatile unsigned long long*)next_bucket)
+        atomicMin(next_bucket, least);
     note_sigma(ctrl, smax);
     NETBC_FLUSH(ctrl, 0, c0);
     NETBC_FLUSH(ctrl, 1, c1);
@@ -2196,8 +2202,8 @@ __global__ void accumulate_kernel(const uint64_t* __restrict__ vis, double* __re
     rn = wave_sum_u64(rn);
     ra = wave_sum_u64(ra);
     if (lane_id() == 0 && (rn || ra)) {
-        atomicAdd(&ctrl->reached_nodes, rn);
-        atomicAdd(&ctrl->reached_arcs, ra);
+        atomicAdd(&ctrl->reached_nodes[blockIdx.x % kSpread], rn);
+        atomicAdd(&ctrl->reached_arcs[blockIdx.x % kSpread], ra);
     }
 }
 
@@ -2284,8 +2290,8 @@ __global__ void accumulate_rows_kernel(const uint64_t* __restrict__ vis, double*
     rn = wave_sum_u64(rn);
     ra = wave_sum_u64(ra);
     if (lane == 0 && (rn || ra)) {
-        atomicAdd(&ctrl->reached_nodes, rn);
-        atomicAdd(&ctrl->reached_arcs, ra);
+        atomicAdd(&ctrl->reached_nodes[blockIdx.x % kSpread], rn);
+        atomicAdd(&ctrl->reached_arcs[blockIdx.x % kSpread], ra);
     }
 }
 
@@ -3685,8 +3691,8 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     memcpy(&mx, &bits, 8);
     diag.max_sigma = std::max(diag.max_sigma, mx);
     diag.sigma_overflow |= c.sig_overflow ? 1 : 0;
-    diag.reached_nodes += (int64_t)c.reached_nodes;
-    diag.reached_arcs += (int64_t)c.reached_arcs;
+    diag.reached_nodes += (int64_t)spread_sum(c.reached_nodes);
+    diag.reached_arcs += (int64_t)spread_sum(c.reached_arcs);
     for (int i = 0; i < 16; i++) diag.work[i] += (int64_t)c.work[i];
 }
 
