@@ -37,7 +37,7 @@ PKG = Path(__file__).resolve().parent
 CANDIDATES = {
     "slots": [16, 32, 64, 128],
     "alpha": [0.25, 0.5, 1.0, 2.0, 4.0],
-    "sweep_group": [8, 16, 32, 64],
+    "sweep_group": [4, 8, 12, 16, 32, 64],
 }
 START = {"slots": 64, "alpha": 1.0, "sweep_group": 32}
 

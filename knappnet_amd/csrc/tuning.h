@@ -20,7 +20,7 @@ struct TuneEntry {
 };
 
 constexpr TuneEntry kTuning[] = {
-    {0, 1, 0, 40, 0.0, 1e30, 64, 1.0, 32, "R-MAT scale 18 directed, first 32,768 sources"},
+    {0, 1, 0, 40, 0.0, 1e30, 64, 2.0, 4, "R-MAT scale 18 directed, first 32,768 sources"},
     {0, 0, 0, 40, 0.0, 1e30, 16, 2.0, 32, "R-MAT scale 22 undirected, 4,096 sampled sources"},
     {1, -1, 0, 40, 0.0, 1e30, 32, 1.0, 32, "BA n=200,000 integer weights 1-16, first 8,192 sources"},
 };
