@@ -3740,9 +3740,18 @@ void finish_edge_scores(Graph& g, bool normalized, double* d_eb, double* out) {
     NETBC_HIP(hipStreamSynchronize(st));
 }
 
+// Direction factor of reachability-only breadth-first searches when the
+// caller sets none: their bottom-up steps stop at the first parents and
+// take no path counts, so they pay off earlier than in counting searches
+// (R-MAT 18 closeness: 0.285 s at 1, 0.209 s at 2, 0.190 s at 4, 0.188 s at
+// 8, 0.195 s at 16).
+constexpr double kReachAlpha = 4.0;
+
 void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const ClosenessRun& cr,
                    double* out, RunDiag& diag) {
-    const RunOptions opt = tuned_options(g, L, opt_in);
+    RunOptions opt0 = opt_in;
+    if (!(opt0.alpha > 0.0)) opt0.alpha = kReachAlpha;
+    const RunOptions opt = tuned_options(g, L, opt0);
     hipStream_t st = g.stream;
     const size_t n = g.n;
     diag.path_kind = L.kind;
