@@ -26,12 +26,12 @@ sources, targets every third node, NetworkX's subset rescaling, node
 values only); its CPU baseline is the oracle's single-threaded
 restatement. ``--closeness`` times normalized closeness of every node
 (``netbc_closeness_ex``, unreachable nodes ignored: reached / sum of
-distances) on the same workload; the roofline kernel is the top-down
-expansion of the reachability-only search (``bfs_expand_kernel``), whose
-algorithmic bytes the engine counts per launch (24 per level entry, 12 per
-out-arc scanned, 24 per stale level entry cleared), and the CPU baseline
-is the oracle's distance profiles of a strided sample of sources on every
-host core.
+distances) on the same workload; the roofline kernel is the bottom-up step
+of the reachability-only search (``bfs_pull_kernel<false, false>``), whose
+algorithmic bytes the engine counts as it runs (8 per pair examined, 8 per
+in-row looked up, 12 per in-arc read, 40 per pair reached, 24 per stale
+level entry cleared), and the CPU baseline is the oracle's distance
+profiles of a strided sample of sources on every host core.
 """
 from __future__ import annotations
 
@@ -289,7 +289,7 @@ def main() -> None:
     achieved = sweep_bytes / sweep_s / 1e9 if sweep_s > 0 else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "bfs_expand_kernel" if args.closeness else
+        "kernel": "bfs_pull_kernel<false, false>" if args.closeness else
                   f"{'dial' if weighted else 'bfs'}_backward_kernel<"
                   f"{'double' if args.precision == 0 else 'float'}, "
                   f"{'true' if args.edge else 'false'}, "

@@ -178,7 +178,7 @@ typedef struct netbc_diagnostics {
     int64_t levels;          /* search levels run */
     double kernel_ms;        /* device time of the search and reduction */
     double dominant_ms;      /* device time of the dependency sweep
-                                (closeness: of the top-down expansions) */
+                                (closeness: of the bottom-up steps) */
     int64_t dominant_launches;
     int32_t slots;           /* 64-source batches per launch used */
     int32_t bottom_up_levels; /* search levels run bottom-up */
@@ -186,8 +186,8 @@ typedef struct netbc_diagnostics {
     int64_t work[16];        /* engine work counters; zero unless the library
                                 is built with NETBC_COUNTERS */
     int64_t dominant_bytes;  /* closeness runs: algorithmic bytes of the
-                                timed kernel (the top-down expansion of
-                                the breadth-first search), summed over
+                                timed kernel (the bottom-up step of the
+                                breadth-first search), summed over
                                 launches */
 } netbc_diagnostics;
 

@@ -75,6 +75,11 @@ struct Ctrl {
     unsigned long long reached_nodes[32];  // spread as front_arcs
     unsigned long long reached_arcs[32];
     unsigned long long fwd_arcs;  // arcs scanned by top-down expansion
+    // Bytes of the reachability-only bottom-up steps (bfs_pull_kernel
+    // without path counts), spread as front_arcs: 8 per pair examined, 8 per
+    // in-row looked up plus 12 per in-arc read, 40 per pair reached (visited
+    // and level words, out-row bounds, queue entry).
+    unsigned long long reach_bytes[32];
     // Work counters (filled only in builds with NETBC_COUNTERS):
     // 0 expand entries, 1 expand arcs, 2 arcs with offers, 3 offered lanes
     // (integer lengths: 0 settled pairs, 1 their in-arcs, 2 out-arcs, 3 offers),
@@ -907,7 +912,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         QEnt e = qz[i];
         atomicAnd((unsigned long long*)&nxt[(size_t)e.b * n + e.v], ~e.m);
     }
-    unsigned long long fa = 0, closed = 0;
+    unsigned long long fa = 0, closed = 0, bytes = 0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0;
     // new entries staged in dynamic LDS (kPullLds bytes per block)
     extern __shared__ unsigned long long pull_lds[];
@@ -949,7 +954,9 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
             for (; a + 4 <= deg && mine != U; a += 4)
                 mine |= (crow[row[a]] | crow[row[a + 1]] | crow[row[a + 2]] | crow[row[a + 3]]) & U;
             for (; a < deg && mine != U; a++) mine |= crow[row[a]] & U;
+            bytes += 8 + 12ull * a;
         }
+        if (!kCount && ok) bytes += 8 + (mine ? 40 : 0);
         if (mine) {
             const uint64_t vn = V | mine;
             vis[p] = vn;
@@ -965,9 +972,11 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     NETBC_FLUSH(ctrl, 15, c6);
     fa = wave_sum_u64(fa);
     closed = wave_sum_u64(closed);
+    if constexpr (!kCount) bytes = wave_sum_u64(bytes);
     if (lane == 0) {
         if (fa) atomicAdd(&front_next[blockIdx.x % kSpread], fa);
         if (closed) atomicAdd(&ctrl->closed_arcs[blockIdx.x % kSpread], closed);
+        if (!kCount && bytes) atomicAdd(&ctrl->reach_bytes[blockIdx.x % kSpread], bytes);
     }
 }
 
@@ -2100,29 +2109,31 @@ __global__ void source_key_kernel(const int64_t* __restrict__ list, int64_t k,
 // its out-neighbours u (a wavefront per node, lanes striding its row, then
 // a wavefront minimum). Updates in place (the minima only fall, so the
 // rounds converge to the distances); *changed is set if any fell.
+// (Real lengths: FP64 distances, INFINITY when unreached.)
+template <class T>
 __global__ void landmark_relax_kernel(const uint32_t* __restrict__ off,
                                       const uint32_t* __restrict__ idx,
-                                      const uint32_t* __restrict__ len, uint32_t* __restrict__ wd,
+                                      const T* __restrict__ len, T* __restrict__ wd,
                                       size_t n, uint32_t* __restrict__ changed) {
+    constexpr T kFar = std::is_same_v<T, uint32_t> ? (T)kNone : (T)INFINITY;
     const int lane = lane_id();
     for (size_t v = global_wave(); v < n; v += wave_count()) {
-        uint32_t best[kWeightedLandmarks];
+        T best[kWeightedLandmarks];
 #pragma unroll
-        for (int j = 0; j < kWeightedLandmarks; j++) best[j] = kNone;
+        for (int j = 0; j < kWeightedLandmarks; j++) best[j] = kFar;
         for (uint32_t a = off[v] + lane; a < off[v + 1]; a += 64) {
-            const uint32_t* du = wd + (size_t)idx[a] * kWeightedLandmarks;
-            const uint32_t l = len[a];
+            const T* du = wd + (size_t)idx[a] * kWeightedLandmarks;
+            const T l = len[a];
 #pragma unroll
             for (int j = 0; j < kWeightedLandmarks; j++)
-                if (du[j] != kNone) best[j] = min(best[j], du[j] + l);
+                if (du[j] != kFar) best[j] = min(best[j], du[j] + l);
         }
         bool fell = false;
 #pragma unroll
         for (int j = 0; j < kWeightedLandmarks; j++) {
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1)
-                best[j] = min(best[j], (uint32_t)__shfl_xor((int)best[j], o, 64));
-            const uint32_t old = wd[v * kWeightedLandmarks + j];
+            for (int o = 32; o > 0; o >>= 1) best[j] = min(best[j], __shfl_xor(best[j], o, 64));
+            const T old = wd[v * kWeightedLandmarks + j];
             if (best[j] < old) {
                 if (lane == 0) wd[v * kWeightedLandmarks + j] = best[j];
                 fell = true;
@@ -2132,14 +2143,18 @@ __global__ void landmark_relax_kernel(const uint32_t* __restrict__ off,
     }
 }
 
-// min(wd, 255) as bytes into the landmark distance table (first
-// kWeightedLandmarks columns of each node's kLandmarks).
-__global__ void landmark_bytes_kernel(const uint32_t* __restrict__ wd, size_t n,
+// min(wd / unit, 255) as bytes into the landmark distance table (first
+// kWeightedLandmarks columns of each node's kLandmarks); unit 1 for integer
+// lengths, the shortest arc for real ones.
+template <class T>
+__global__ void landmark_bytes_kernel(const T* __restrict__ wd, size_t n, double unit,
                                       uint8_t* __restrict__ dist) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
-        for (int j = 0; j < kWeightedLandmarks; j++)
-            dist[v * kLandmarks + j] = (uint8_t)min(wd[v * kWeightedLandmarks + j], 255u);
+        for (int j = 0; j < kWeightedLandmarks; j++) {
+            const double q = (double)wd[v * kWeightedLandmarks + j] / unit;
+            dist[v * kLandmarks + j] = (uint8_t)(q < 255.0 ? q : 255.0);
+        }
 }
 
 __global__ void iota_kernel(uint32_t* __restrict__ v, int64_t k) {
@@ -2959,11 +2974,9 @@ struct Engine {
     uint64_t* lane_buf = nullptr;
     unsigned long long* lane_count = nullptr;
     bool reach_only = false;  // distances only, no path counts
-    // Reachability-only breadth-first runs time their top-down expansions
-    // with ws.sweep and count their algorithmic bytes: 24 per level entry
-    // (entry and row bounds), 12 per out-arc scanned (column and the head's
-    // visited word), 24 per stale entry cleared.
-    unsigned long long timed_bytes = 0;
+    // Reachability-only breadth-first runs time their bottom-up steps with
+    // ws.sweep; bytes of the stale level entries those steps clear (24 each).
+    unsigned long long stale_bytes = 0;
 
     // Per-level boundaries in the queue (level d = [lv[d], lv[d+1])).
     std::vector<uint32_t> lv;
@@ -3069,6 +3082,10 @@ struct Engine {
                 auto pull = reach_only ? bfs_pull_kernel<false>
                             : npr      ? bfs_pull_kernel<true, true>
                                        : bfs_pull_kernel<true, false>;
+                if (reach_only) {
+                    ws.sweep.mark(st);
+                    stale_bytes += 24ull * (ze - zb);
+                }
                 pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, kPullLds,
                        st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
@@ -3076,23 +3093,18 @@ struct Engine {
                                         ws.hq.p, ws.ctrl.p, front[(dl + 1) & 1], front[dl & 1],
                                         frontier_lanes(dl), slots, left, n, npr);
                 launch_check();
+                if (reach_only) ws.sweep.mark(st);
             } else {
                 const uint32_t gsz =
                     group_size(qe - qb, resident_waves(bfs_expand_kernel));
                 size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + hc[dl];
                 size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
-                if (reach_only) {
-                    ws.sweep.mark(st);
-                    timed_bytes += 24ull * (qe - qb) + 12ull * spread_sum(ws.host_ctrl->front_arcs[dl & 1]) +
-                                   24ull * (ze - zb);
-                }
                 bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
                     ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
                     d.out_idx.p, ws.vis.p, ws.cand.p, nxt, reach_only ? nullptr : ws.succ.p,
                     ws.touched.p,
                     &ws.ctrl.p->tlen[dl & 1], d.heavy_flag.p, ws.ctrl.p, front[dl & 1], gsz, n);
                 launch_check();
-                if (reach_only) ws.sweep.mark(st);
             }
             if (d.hubs && reach_only && bottom_up) {
                 const size_t hub_tasks = (size_t)d.hub_chunks * slots;
@@ -3424,14 +3436,49 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     ws.ctrl.alloc(1);
 }
 
+// Weighted distances from every node to the first kWeightedLandmarks
+// landmarks by Bellman-Ford rounds (until no distance falls), written as
+// bytes (distance / unit, capped at 255) into the landmark table `dist`.
+template <class T>
+void landmark_distances(const DeviceGraph& dg, const T* len, int lanes, double unit,
+                        uint8_t* dist, hipStream_t st) {
+    const size_t n = dg.n;
+    const int wl = std::min(lanes, kWeightedLandmarks);
+    DevBuf<T> wd;
+    DevBuf<uint32_t> flag;
+    wd.alloc(n * kWeightedLandmarks);
+    flag.alloc(1);
+    std::vector<T> init(n * kWeightedLandmarks,
+                        std::is_same_v<T, uint32_t> ? (T)kNone : (T)INFINITY);
+    for (int j = 0; j < wl; j++) init[(size_t)dg.landmarks[j] * kWeightedLandmarks + j] = (T)0;
+    NETBC_HIP(hipMemcpyAsync(wd.p, init.data(), init.size() * sizeof(T), hipMemcpyHostToDevice,
+                             st));
+    const unsigned rb = resident_blocks(landmark_relax_kernel<T>, n);
+    for (size_t round = 0; round < n; round++) {
+        NETBC_HIP(hipMemsetAsync(flag.p, 0, 4, st));
+        landmark_relax_kernel<T><<<rb, kBlock, 0, st>>>(dg.out_off.p, dg.out_idx.p, len, wd.p, n,
+                                                         flag.p);
+        NETBC_HIP(hipGetLastError());
+        uint32_t changed = 0;
+        NETBC_HIP(hipMemcpyAsync(&changed, flag.p, 4, hipMemcpyDeviceToHost, st));
+        NETBC_HIP(hipStreamSynchronize(st));  // also keeps `init` alive for its copy
+        if (!changed) break;
+    }
+    landmark_bytes_kernel<T><<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22)), kBlock, 0,
+                               st>>>(wd.p, n, unit, dist);
+    NETBC_HIP(hipGetLastError());
+    NETBC_HIP(hipStreamSynchronize(st));
+}
+
 // Orders the k sources of a free-order run (list d_list, nullptr: 0..k-1)
 // so that sources whose searches look alike share batches: three stable
 // sorts, by second out-neighbour, by first out-neighbour, and by the
 // distances to the graph's landmarks (capped at 15), the last one deciding
 // first. Sources in one batch then reach most nodes on the same level, so
-// the 64-lane rows carry more lanes per entry. With integer lengths the
-// last sort uses the weighted distances to the first 8 landmarks instead
-// (capped at 255): sources at equal lengths from the hubs
+// the 64-lane rows carry more lanes per entry. With integer lengths (and
+// bucketed real ones, in units of the shortest arc) the last sort uses the
+// weighted distances to the first 8 landmarks instead (capped at 255):
+// sources at equal lengths from the hubs
 // settle most nodes in the same buckets (on Barabasi-Albert n = 200,000
 // with lengths 1-16, 9-29 lanes per settled entry instead of 4-5 by hop
 // counts). Writes the list to ws.list2.
@@ -3470,34 +3517,13 @@ void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hi
         // the seed buffer must outlive the copies
         NETBC_HIP(hipStreamSynchronize(st));
     }
-    const bool weighted = L && L->kind == kInteger && lanes > 0;
+    const bool weighted =
+        L && lanes > 0 && (L->kind == kInteger || (L->kind == kReal && real_bucketed(*L, n)));
     if (weighted) {
-        // weighted distances to the first landmarks, by Bellman-Ford rounds
-        const int wl = std::min(lanes, kWeightedLandmarks);
-        DevBuf<uint32_t> wd, flag;
-        wd.alloc(n * kWeightedLandmarks);
-        flag.alloc(1);
-        fill_u32_kernel<<<blocks_for_threads(n * kWeightedLandmarks), kBlock, 0, st>>>(
-            wd.p, n * kWeightedLandmarks, kNone);
-        NETBC_HIP(hipGetLastError());
-        for (int j = 0; j < wl; j++)
-            NETBC_HIP(hipMemsetAsync(wd.p + (size_t)dg.landmarks[j] * kWeightedLandmarks + j, 0, 4,
-                                     st));
-        const unsigned rb = resident_blocks(landmark_relax_kernel, n);
-        for (size_t round = 0; round < n; round++) {
-            NETBC_HIP(hipMemsetAsync(flag.p, 0, 4, st));
-            landmark_relax_kernel<<<rb, kBlock, 0, st>>>(dg.out_off.p, dg.out_idx.p, L->out_len.p,
-                                                          wd.p, n, flag.p);
-            NETBC_HIP(hipGetLastError());
-            uint32_t changed = 0;
-            NETBC_HIP(hipMemcpyAsync(&changed, flag.p, 4, hipMemcpyDeviceToHost, st));
-            NETBC_HIP(hipStreamSynchronize(st));
-            if (!changed) break;
-        }
-        landmark_bytes_kernel<<<blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22)), kBlock, 0,
-                                st>>>(wd.p, n, dist.p);
-        NETBC_HIP(hipGetLastError());
-        NETBC_HIP(hipStreamSynchronize(st));
+        if (L->kind == kInteger)
+            landmark_distances<uint32_t>(dg, L->out_len.p, lanes, 1.0, dist.p, st);
+        else
+            landmark_distances<double>(dg, L->out_real.p, lanes, L->rmin, dist.p, st);
     }
     DevBuf<uint64_t> k0, k1;
     DevBuf<uint32_t> v0, v1;
@@ -3882,7 +3908,7 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     ws.sweep.collect();
     diag.dominant_ms += ws.sweep.ms;
     diag.dominant_launches += ws.sweep.launches;
-    diag.dominant_bytes += (int64_t)eng.timed_bytes;
+    diag.dominant_bytes += (int64_t)(spread_sum(ws.host_ctrl->reach_bytes) + eng.stale_bytes);
     double mx;
     const unsigned long long bits = ws.host_ctrl->max_sigma_bits;
     memcpy(&mx, &bits, 8);

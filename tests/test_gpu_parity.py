@@ -280,6 +280,12 @@ def test_dial_free_order_weighted_landmarks():
     src = np.concatenate([np.arange(el.n), np.arange(0, el.n, 3)])
     got, _ = g.betweenness_sources(src, weighted=True)
     assert_rel(got, o.betweenness_sources(src, weighted=True, threads=0), "ba5000 repeats")
+    # real lengths (bucketed): keys in units of the shortest arc
+    w = np.random.default_rng(6).random(el.m) * 7 + 1
+    g, o = pair(el.n, el.src, el.tgt, w, False)
+    got, diag = g.betweenness_sources(weighted=True)
+    assert diag["path_kind"] == 2
+    assert_rel(got, o.betweenness(weighted=True, threads=0), "ba5000 real free order")
     # directed: distances to the landmarks along out-arcs
     el = gen.rmat(12, 8, directed=True, seed=21)
     w = np.random.default_rng(4).integers(1, 12, el.m).astype(np.float64)
