@@ -231,9 +231,16 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
         }
         __syncthreads();
     }
-    // Reached nodes and arcs (diagnostics), one atomic per thread.
-    if (reached) atomicAdd(&stats[0], reached);
-    if (reached_arcs) atomicAdd(&stats[1], reached_arcs);
+    // Reached nodes and arcs (diagnostics): summed over the wavefront, one
+    // atomic per wavefront (one per thread made every thread of the launch
+    // add into the same two addresses).
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        reached += (unsigned long long)__shfl_xor((long long)reached, o, 64);
+        reached_arcs += (unsigned long long)__shfl_xor((long long)reached_arcs, o, 64);
+    }
+    if (lane == 0 && reached) atomicAdd(&stats[0], reached);
+    if (lane == 0 && reached_arcs) atomicAdd(&stats[1], reached_arcs);
 }
 
 // bc[v] = sum over groups j (in order) of gsum[j][v], then the per-graph
