@@ -2236,6 +2236,23 @@ __global__ void clear_rows_kernel(const uint64_t* __restrict__ vis, double* __re
     }
 }
 
+// Zeroes the reached lanes of every hub's row (bfs_hub_kernel adds hub
+// path counts atomically, so those rows must start at zero).
+__global__ void zero_hub_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
+                                     const uint8_t* __restrict__ hub_flag, int slots, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < (size_t)slots * n;
+         p += stride) {
+        if (!hub_flag[p % n]) continue;
+        uint64_t M = vis[p];
+        while (M) {
+            const int l = __ffsll((unsigned long long)M) - 1;
+            M &= M - 1;
+            sig[p * kLanes + l] = 0.0;
+        }
+    }
+}
+
 // Zeroes each source's own entry (sigma = 1 at level 0, which the sweep
 // leaves in place) so that the row sums below need no per-lane source test.
 __global__ void zero_sources_kernel(const QEnt* __restrict__ q, uint32_t count,
@@ -2257,7 +2274,9 @@ __global__ void zero_sources_kernel(const QEnt* __restrict__ q, uint32_t count,
 // are read kAhead at a time and added per lane in slot order; one fixed
 // butterfly adds the lanes, and bc[v] += the result (deterministic for a
 // given slot count). Sources' own entries are zero (zero_sources_kernel).
-// The rows are zeroed for the next super-batch.
+// The rows are left as they are: the next super-batch's search stores every
+// count it reads before reading it, except at hubs, whose counts are added
+// atomically and whose rows zero_hub_rows_kernel clears.
 __global__ void accumulate_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                        int slots, size_t n, int64_t item0, int64_t k,
                                        double* __restrict__ bc,
@@ -2285,7 +2304,6 @@ __global__ void accumulate_rows_kernel(const uint64_t* __restrict__ vis, double*
                         if ((Ms >> lane) & 1) {
                             double* r = &sig[((size_t)(b0 + s) * n + v) * kLanes + lane];
                             val[j] = *r;
-                            *r = 0.0;
                         }
                     }
                 }
@@ -3684,6 +3702,13 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             accumulate_rows_kernel<<<resident_blocks(accumulate_rows_kernel, n), kBlock, 0,
                                      st>>>(ws.vis.p, ws.sig.p, slots, n, item0, k, d_bc,
                                            g.dev.out_off.p, ws.ctrl.p);
+            if (g.dev.hubs) {
+                NETBC_HIP(hipGetLastError());
+                zero_hub_rows_kernel<<<blocks_for_threads(std::min<size_t>((size_t)slots * n,
+                                                                           (size_t)1 << 22)),
+                                       kBlock, 0, st>>>(ws.vis.p, ws.sig.p, g.dev.hub_flag.p,
+                                                        slots, n);
+            }
         } else
             accumulate_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
                 ws.vis.p, ws.sig.p, slots, n, item0, k, gsize, d_list, ws.acc.p, d_bc, ws.gcur.p,
