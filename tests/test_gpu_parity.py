@@ -432,3 +432,27 @@ def test_batch_watts_strogatz_and_weighted():
     with pytest.raises(EngineError, match=r"^graph 1: edge 0 refers to node index 5, but "
                                           r"valid indices are 0..3$"):
         betweenness_batch([(2, [0], [1]), (3, [5], [0])])
+
+
+def test_runs_in_sequence_on_one_handle():
+    # Count rows are left in place between super-batches and runs (only hub
+    # rows are cleared): every kind of run on one handle, in turn, must
+    # still match the oracle.
+    el = gen.rmat(13, 8, directed=True, seed=31)
+    w = np.random.default_rng(8).integers(1, 9, el.m).astype(np.float64)
+    g, o = pair(el.n, el.src, el.tgt, w, True)
+    want = o.betweenness(threads=0)
+    say("rmat13 sequence", el.n, el.m)
+    for rnd in range(2):
+        assert_rel(g.betweenness(), want, f"free order {rnd}")
+        exact, _ = g.betweenness_sources(exact=True)
+        assert_same(exact, want, f"exact order {rnd}")
+        assert_rel(g.edge_betweenness(), o.edge_betweenness(threads=0), f"edge {rnd}")
+        nodes, _ = g.betweenness_subset(targets=np.arange(0, el.n, 3))
+        assert_rel(nodes, o.betweenness_subset(targets=np.arange(0, el.n, 3))[0], f"subset {rnd}")
+        c = g.closeness(normalized=True)
+        assert np.array_equal(c, o.closeness(normalized=True), equal_nan=True), f"closeness {rnd}"
+        assert_rel(g.betweenness(weighted=True), o.betweenness(weighted=True, threads=0),
+                   f"weighted {rnd}")
+        f32, _ = g.betweenness_sources(precision=1)
+        assert_rel(f32, want, f"fp32 {rnd}", tol=1e-6)
