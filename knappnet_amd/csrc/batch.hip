@@ -508,17 +508,21 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
 struct QueueStage {
     QEnt* buf;
     uint32_t cnt;
-    __device__ void flush(Ctrl* ctrl, QEnt* q, uint32_t qcap, uint32_t* hq,
-                          const uint8_t* heavy_flag) {
+    __device__ __forceinline__ void flush(Ctrl* ctrl, QEnt* q, uint32_t qcap, uint32_t* hq,
+                                          const uint8_t* heavy_flag) {
         const int lane = lane_id();
         uint32_t at = 0;
         if (lane == 0) at = atomicAdd(&ctrl->qlen, cnt);
         at = rl32(at, 0);
         for (uint32_t i = lane; i < cnt; i += 64) {
-            const QEnt e = buf[i];
+            // field by field: a whole-struct copy goes through scratch
+            const uint32_t v = buf[i].v, b = buf[i].b;
+            const unsigned long long m = buf[i].m;
             if (at + i < qcap) {
-                q[at + i] = e;
-                if (heavy_flag[e.v]) hq[atomicAdd(&ctrl->hlen, 1u)] = at + i;
+                q[at + i].v = v;
+                q[at + i].b = b;
+                q[at + i].m = m;
+                if (heavy_flag[v]) hq[atomicAdd(&ctrl->hlen, 1u)] = at + i;
             } else {
                 ctrl->qoverflow = 1;
             }
@@ -526,8 +530,9 @@ struct QueueStage {
         cnt = 0;
     }
     // Stages (v, b, m) for the lanes with `has`.
-    __device__ void add(bool has, uint32_t v, uint32_t b, uint64_t m, Ctrl* ctrl, QEnt* q,
-                        uint32_t qcap, uint32_t* hq, const uint8_t* heavy_flag) {
+    __device__ __forceinline__ void add(bool has, uint32_t v, uint32_t b, uint64_t m, Ctrl* ctrl,
+                                        QEnt* q, uint32_t qcap, uint32_t* hq,
+                                        const uint8_t* heavy_flag) {
         const uint64_t bal = __ballot(has);
         if (!bal) return;
         const uint32_t k = (uint32_t)__popcll(bal);
@@ -2237,19 +2242,16 @@ __global__ void clear_rows_kernel(const uint64_t* __restrict__ vis, double* __re
 }
 
 // Zeroes the reached lanes of every hub's row (bfs_hub_kernel adds hub
-// path counts atomically, so those rows must start at zero).
+// path counts atomically, so those rows must start at zero): a wavefront
+// per (slot, hub) row, lane l clearing entry l if l reached the hub.
 __global__ void zero_hub_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
-                                     const uint8_t* __restrict__ hub_flag, int slots, size_t n) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < (size_t)slots * n;
-         p += stride) {
-        if (!hub_flag[p % n]) continue;
-        uint64_t M = vis[p];
-        while (M) {
-            const int l = __ffsll((unsigned long long)M) - 1;
-            M &= M - 1;
-            sig[p * kLanes + l] = 0.0;
-        }
+                                     const uint32_t* __restrict__ hub_list, uint32_t hubs,
+                                     int slots, size_t n) {
+    const int lane = lane_id();
+    const size_t rows = (size_t)slots * hubs;
+    for (size_t r = global_wave(); r < rows; r += wave_count()) {
+        const size_t p = (r / hubs) * n + hub_list[r % hubs];
+        if ((vis[p] >> lane) & 1) sig[p * kLanes + lane] = 0.0;
     }
 }
 
@@ -3704,10 +3706,11 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                                            g.dev.out_off.p, ws.ctrl.p);
             if (g.dev.hubs) {
                 NETBC_HIP(hipGetLastError());
-                zero_hub_rows_kernel<<<blocks_for_threads(std::min<size_t>((size_t)slots * n,
+                const size_t rows = (size_t)slots * g.dev.hubs;
+                zero_hub_rows_kernel<<<blocks_for_threads(std::min<size_t>(rows * 64,
                                                                            (size_t)1 << 22)),
-                                       kBlock, 0, st>>>(ws.vis.p, ws.sig.p, g.dev.hub_flag.p,
-                                                        slots, n);
+                                       kBlock, 0, st>>>(ws.vis.p, ws.sig.p, g.dev.hub_list.p,
+                                                        g.dev.hubs, slots, n);
             }
         } else
             accumulate_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(

@@ -451,16 +451,21 @@ void build_device_graph(Graph& g) {
     NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));
     std::vector<uint8_t> flag(std::max<uint64_t>(g.n, 1), 0);
     std::vector<uint4> task;
-    d.hubs = 0;
+    std::vector<uint32_t> hub_list;
     for (uint64_t v = 0; v < g.n; v++) {
         uint32_t deg = ioff[v + 1] - ioff[v];
         if (deg > kHubIn) {
             flag[v] = 1;
-            d.hubs++;
+            hub_list.push_back((uint32_t)v);
             for (uint32_t a = ioff[v]; a < ioff[v + 1]; a += kHubChunk)
                 task.push_back(make_uint4((uint32_t)v, a, std::min(a + kHubChunk, ioff[v + 1]), 0));
         }
     }
+    d.hubs = (uint32_t)hub_list.size();
+    d.hub_list.alloc(hub_list.size());
+    if (!hub_list.empty())
+        NETBC_HIP(hipMemcpy(d.hub_list.p, hub_list.data(), hub_list.size() * 4,
+                            hipMemcpyHostToDevice));
     d.hub_chunks = (uint32_t)task.size();
     d.hub_flag.alloc(flag.size());
     NETBC_HIP(hipMemcpy(d.hub_flag.p, flag.data(), flag.size(), hipMemcpyHostToDevice));

@@ -86,6 +86,7 @@ struct DeviceGraph {
     // bottom-up, their new lanes) are gathered in chunks of kHubChunk
     // in-arcs, one task per chunk and slot.
     DevBuf<uint8_t> hub_flag;          // per node
+    DevBuf<uint32_t> hub_list;         // the hubs, ascending
     DevBuf<uint4> hub_task;            // per chunk: (hub, first in-arc, end in-arc, 0)
     uint32_t hubs = 0, hub_chunks = 0;
     // Heavy rows: out-degree above kHeavyOut, expanded and swept in tasks
