@@ -108,6 +108,11 @@ constexpr double kTwo53 = 9007199254740992.0;
 #define NETBC_AHEAD 8
 #endif
 constexpr int kAhead = NETBC_AHEAD;
+// Predecessor rows loaded at once by the path-count gathers (gather_counts).
+#ifndef NETBC_GATHER_AHEAD
+#define NETBC_GATHER_AHEAD 8
+#endif
+constexpr int kGatherAhead = NETBC_GATHER_AHEAD;
 // Level bitmap buffers per slot: levels below kKeptLevels keep their own
 // buffer from the forward search to the sweep, deeper ones alternate
 // between the last two (then the sweep rebuilds them from the queue).
@@ -466,11 +471,11 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
         NETBC_COUNT(hits_seen, m != 0);
         uint64_t hits = __ballot(m != 0);
         while (hits) {
-            double val[kAhead];
-            int hc[kAhead];
-            uint64_t hm[kAhead];
+            double val[kGatherAhead];
+            int hc[kGatherAhead];
+            uint64_t hm[kGatherAhead];
 #pragma unroll
-            for (int k = 0; k < kAhead; k++) {
+            for (int k = 0; k < kGatherAhead; k++) {
                 val[k] = 0.0;
                 hc[k] = -1;
                 hm[k] = 0;
@@ -485,7 +490,7 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
                 }
             }
 #pragma unroll
-            for (int k = 0; k < kAhead; k++) {
+            for (int k = 0; k < kGatherAhead; k++) {
                 if (hc[k] < 0) break;
                 if (hc[k] != curc) {
                     close_pair();
@@ -1160,6 +1165,17 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
     }
 }
 
+// Adds the sum over the wavefront's lanes of x (one fixed butterfly) to *dst.
+// Every lane calls. Free-order breadth-first runs add each entry's
+// dependencies into one word per (slot, node) this way; a node has one
+// entry per level, and levels are swept one launch at a time, so the adds
+// to a word come in level order and their sum is deterministic.
+__device__ __forceinline__ void add_lane_sum(double* dst, double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane_id() == 0 && x != 0.0) unsafeAtomicAdd(dst, x);
+}
+
 // Dependencies of the level-d entries. R is the level d+1 bitmap. With
 // rebuild, W is turned from the level d+2 bitmap into level d's (lanes are
 // disjoint, so the OR and AND-NOT updates commute); levels kept from the
@@ -1175,7 +1191,10 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
 // (NetworkX's subset edge split, path_centrality.rs:465-474) the rows hold
 // E_w instead of coefficients, E_w = (1 + de_w) / sigma_w for a target and
 // de_w / np_w otherwise (np_w: w's predecessor arcs, npr), the arc v -> w
-// carries sigma_v E_w or E_w, and de_v is their sum; rows are whole.
+// carries sigma_v E_w or E_w, and de_v is their sum; rows are whole. With
+// bsum (free-order runs) each finished entry also adds the sum of its
+// lanes' dependencies to bsum[b][v] (add_lane_sum; bfs_combine_kernel does
+// the same for heavy rows).
 template <class C, bool kEdge, bool kQuirk = false>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level, int rebuild,
@@ -1190,7 +1209,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
                                     uint32_t gsz, size_t n, double* __restrict__ eacc,
                                     size_t arcs, const uint8_t* __restrict__ target,
-                                    uint32_t* __restrict__ npr) {
+                                    uint32_t* __restrict__ npr, double* __restrict__ bsum) {
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
@@ -1207,9 +1226,11 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
         }
     }
     split = split && !last_level && !kQuirk;
-    // delta, coefficient and path-count bookkeeping of lane l of entry (v, b, M)
+    // delta, coefficient and path-count bookkeeping of lane l of entry (v, b, M);
+    // returns the lane's dependency (0 outside M)
     auto finish = [&](uint32_t v, uint32_t b, uint64_t M, double s, double dvv,
-                      [[maybe_unused]] double dqv) {
+                      [[maybe_unused]] double dqv) -> double {
+        double dep = 0.0;
         if ((M >> lane) & 1) {
             const size_t r = ((size_t)b * n + v) * kLanes + lane;
             if constexpr (kQuirk) {
@@ -1219,13 +1240,16 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                 coef[r] = (C)(target_of(target, v) != 0.0 ? (1.0 + de) / s
                                                           : (np ? de / (double)np : 0.0));
                 sig[r] = de;
+                dep = de;
             } else {
                 const double delta = s * dvv;
                 coef[r] = Dep<C>::coef(target_of(target, v), delta, s);
                 sig[r] = delta;
+                dep = delta;
             }
             smax = fmax(smax, s);
         }
+        return dep;
     };
     const size_t nq = qend - qbeg;
     const size_t groups = (nq + gsz - 1) / gsz;
@@ -1278,8 +1302,9 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
         double sv = 0.0;    // its path counts, loaded when it starts
         auto close_entry = [&]() {
             if (cur < 0) return;
-            finish(rl32(en.v, cur), rl32(en.b, cur), rl64(en.m, cur), sv, dv.value(),
-                   dq.value());
+            const uint32_t vv = rl32(en.v, cur), bb = rl32(en.b, cur);
+            const double dep = finish(vv, bb, rl64(en.m, cur), sv, dv.value(), dq.value());
+            if (bsum) add_lane_sum(&bsum[(size_t)bb * n + vv], dep);
             if (lane == cur) done = true;
             dv = Dep<C>();
             dq = Dep<C>();
@@ -1400,7 +1425,8 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
                                    uint32_t hchunks, const double* __restrict__ part,
                                    const uint32_t* __restrict__ off, double* __restrict__ sig,
                                    C* __restrict__ coef, Ctrl* ctrl, size_t n,
-                                   const uint8_t* __restrict__ target) {
+                                   const uint8_t* __restrict__ target,
+                                   double* __restrict__ bsum) {
     const int lane = lane_id();
     double smax = 0.0;
     for (size_t h = global_wave(); h < hend - hbeg; h += wave_count()) {
@@ -1411,14 +1437,17 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
         const uint32_t c1 = hbeg + h + 1 < hend ? hpre[hbeg + h + 1] : hchunks;
         double dv = 0.0;
         for (uint32_t c = c0; c < c1; c++) dv += part[(size_t)c * kLanes + lane];
+        double dep = 0.0;
         if ((en.m >> lane) & 1) {
             size_t r = ((size_t)en.b * n + en.v) * kLanes + lane;
             double s = sig[r];
             double delta = s * dv;
             coef[r] = Dep<C>::coef(target_of(target, en.v), delta, s);
             sig[r] = delta;
+            dep = delta;
             smax = fmax(smax, s);
         }
+        if (bsum) add_lane_sum(&bsum[(size_t)en.b * n + en.v], dep);
     }
     note_sigma(ctrl, smax);
 }
@@ -2270,6 +2299,51 @@ __global__ void zero_sources_kernel(const QEnt* __restrict__ q, uint32_t count,
     }
 }
 
+// Scores of a free-order breadth-first super-batch: a thread per node v
+// adds bsum[b][v] (the sweep's dependency sums) over the active slots b in
+// slot order to bc[v] (deterministic for a given slot count) and clears
+// bsum for the next super-batch; neighbouring threads read neighbouring
+// words of each slot's row, 8 slots' words in flight at a time. Also
+// counts the reached (source, node) pairs and their out-arcs. Sources' own
+// entries (level 0) add nothing: the sweep adds no sums at level 0.
+__global__ void fold_slot_sums_kernel(const uint64_t* __restrict__ vis, double* __restrict__ bsum,
+                                      int active, size_t n, double* __restrict__ bc,
+                                      const uint32_t* __restrict__ off, Ctrl* ctrl) {
+    constexpr int kBatch = 8;
+    unsigned long long rn = 0, ra = 0;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        double x = 0.0;
+        unsigned long long cnt = 0;
+        for (int b0 = 0; b0 < active; b0 += kBatch) {
+            uint64_t M[kBatch];
+            double y[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; j++)
+                M[j] = b0 + j < active ? vis[(size_t)(b0 + j) * n + v] : 0;
+#pragma unroll
+            for (int j = 0; j < kBatch; j++) y[j] = M[j] ? bsum[(size_t)(b0 + j) * n + v] : 0.0;
+#pragma unroll
+            for (int j = 0; j < kBatch; j++) {
+                cnt += (unsigned long long)__popcll(M[j]);
+                if (y[j] != 0.0) {
+                    x += y[j];
+                    bsum[(size_t)(b0 + j) * n + v] = 0.0;
+                }
+            }
+        }
+        if (x != 0.0) bc[v] += x;
+        rn += cnt;
+        ra += cnt * (unsigned long long)(off[v + 1] - off[v]);
+    }
+    rn = wave_sum_u64(rn);
+    ra = wave_sum_u64(ra);
+    if (lane_id() == 0 && (rn || ra)) {
+        atomicAdd(&ctrl->reached_nodes[blockIdx.x % kSpread], rn);
+        atomicAdd(&ctrl->reached_arcs[blockIdx.x % kSpread], ra);
+    }
+}
+
 // The same sum for runs whose source order is free (neighbourhood
 // batching). A wave per node v: lane b reads the reached lanes of v in
 // slot b (64 slots at a time), then the rows of the slots that reached v
@@ -2811,6 +2885,7 @@ struct Workspace {
     DevBuf<uint64_t> flanes;  // per level parity and slot: lanes with entries on the level
     DevBuf<uint64_t> inb;  // integer lengths: pending-bucket bitmasks per (slot, node)
     DevBuf<double> sig, coef64, acc;
+    DevBuf<double> bsum;  // free-order breadth-first runs: dependency sums per (slot, node)
     DevBuf<float> coef32;
     DevBuf<uint32_t> dist;   // integer lengths: distance rows u32[n][64]
     DevBuf<double> rdist;    // real lengths (bucketed): distance rows f64[n][64]
@@ -2988,6 +3063,9 @@ struct Engine {
     bool tie_exact = false;           // real lengths tie only when equal
     bool quirk = false;               // NetworkX's subset edge split (edge runs)
     uint32_t* npr = nullptr;          // predecessor-arc rows (quirk runs)
+    // Free-order breadth-first runs: per (slot, node) sums of the swept
+    // dependencies (ws.bsum), folded into the scores after each super-batch.
+    double* bsum = nullptr;
     // Distance-profile runs (breadth-first): per-lane profiles of the
     // super-batch (5 * slots * kLanes words from lane_buf, cleared at the
     // start of every search) and per-lane level counts.
@@ -3292,12 +3370,12 @@ struct Engine {
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
-                    n, ws.eacc.p, d.arcs, target, npr);
+                    n, ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
                     ws.q.p, ws.hq.p, hb, he, ws.hpre.p, hc[dl], ws.part.p, d.out_off.p, ws.sig.p,
-                    coef, ws.ctrl.p, n, target);
+                    coef, ws.ctrl.p, n, target, dl > 0 ? bsum : nullptr);
                 launch_check();
             }
             ws.sweep.mark(st);
@@ -3430,6 +3508,7 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
         ws.levels.alloc(nm * kLevelBufs);
         if (ws.levels.p != lv_before) ws.zero_words = ws.levels.n;
         ws.touched.alloc(nm);
+        ws.bsum.alloc(nm);
     }
     const bool per_thread = L.kind == kReal && !real_bucketed(L, g.n);
     if (!per_thread) {
@@ -3641,6 +3720,11 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         NETBC_HIP(hipMemsetAsync(ws.eacc.p, 0, (size_t)rows * arcs * sizeof(double), st));
     }
     Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k, edge};
+    const bool fold = free_order && L.kind == kUniform;  // dependency sums from the sweep
+    if (fold) {
+        NETBC_HIP(hipMemsetAsync(ws.bsum.p, 0, (size_t)slots * n * sizeof(double), st));
+        eng.bsum = ws.bsum.p;
+    }
     const uint8_t* d_target = sub ? sub->d_target : nullptr;
     const bool endpoints = sub && sub->endpoints;
     eng.target = d_target;
@@ -3694,7 +3778,20 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                                             ep.p);
             NETBC_HIP(hipGetLastError());
         }
-        if (free_order) {
+        if (fold) {
+            // slots holding sources of this super-batch
+            const int active = (int)std::min<int64_t>(slots, (k - item0 + kLanes - 1) / kLanes);
+            fold_slot_sums_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
+                ws.vis.p, ws.bsum.p, active, n, d_bc, g.dev.out_off.p, ws.ctrl.p);
+            if (g.dev.hubs) {
+                NETBC_HIP(hipGetLastError());
+                const size_t rows = (size_t)slots * g.dev.hubs;
+                zero_hub_rows_kernel<<<blocks_for_threads(std::min<size_t>(rows * 64,
+                                                                           (size_t)1 << 22)),
+                                       kBlock, 0, st>>>(ws.vis.p, ws.sig.p, g.dev.hub_list.p,
+                                                        g.dev.hubs, slots, n);
+            }
+        } else if (free_order) {
             const uint32_t nsrc = eng.lv.size() > 1 ? eng.lv[1] : 0;
             if (nsrc) {
                 zero_sources_kernel<<<blocks_for_threads(nsrc), kBlock, 0, st>>>(ws.q.p, nsrc,
