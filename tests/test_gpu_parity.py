@@ -11,7 +11,7 @@ import sys
 import numpy as np
 import pytest
 from scipy.sparse import csr_matrix
-from scipy.sparse.csgraph import breadth_first_order
+from scipy.sparse.csgraph import breadth_first_order, connected_components
 
 from _common import VARIANTS, close, random_case, tolerance
 from knappnet_amd import generate as gen
@@ -164,7 +164,6 @@ def test_er_2000_bitexact():
     A = csr_matrix((np.ones(el.m), (el.src, el.tgt)), shape=(el.n, el.n))
     A = A + A.T
     deg = np.diff(A.indptr)
-    from scipy.sparse.csgraph import connected_components
     ncomp, lab = connected_components(A, directed=False)
     size = np.bincount(lab)
     arcs = np.bincount(lab, weights=deg)
@@ -353,6 +352,30 @@ def test_default_mode_large_source_sets():
         assert_same(g.betweenness_sources()[0], fast, "deterministic")
         assert_rel(g.betweenness(normalized=True), o.betweenness(normalized=True, threads=0),
                    "normalized")
+
+
+def test_every_node_a_hub_free_order():
+    # 4,096 nodes, each with 70 in-arcs (every node is a hub): all counts
+    # come from the hub tasks, whose rows must be cleared between
+    # super-batches (8 slots: 8 super-batches); scores are the sweep's
+    # per-(slot, node) sums folded in slot order.
+    rng = np.random.default_rng(77)
+    n, k = 4096, 70
+    t = np.repeat(np.arange(n), k)
+    s = rng.integers(0, n - 1, n * k)
+    s = np.where(s >= t, s + 1, s)  # no self-loops
+    g, o = pair(n, s, t, None, True)
+    say("all hubs", n, len(s))
+    want = o.betweenness(threads=0)
+    got, diag = g.betweenness_sources(slots=8)
+    assert_rel(got, want, "slots=8")
+    # the fold also counts reached pairs: every source reaches every node
+    adj = csr_matrix((np.ones(len(s)), (s, t)), shape=(n, n))
+    assert connected_components(adj, directed=True, connection="strong")[0] == 1
+    assert diag["reached_nodes"] == n * n and diag["reached_arcs"] == n * len(s)
+    assert_same(g.betweenness_sources(slots=8)[0], got, "deterministic")
+    assert_rel(g.betweenness_sources()[0], want, "64 slots")
+    assert_rel(g.edge_betweenness_sources(slots=8)[0], o.edge_betweenness(threads=0), "edges")
 
 
 def test_fp32_dependencies_within_1e6():
