@@ -237,6 +237,26 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t count, uint32_t
     for (; i < count; i += stride) p[i] = v;
 }
 
+// Zeroes three buffers of `count` words each in one pass, 16 bytes per
+// store (the buffers are 256-byte aligned): one launch instead of three.
+__global__ void zero3_kernel(uint64_t* __restrict__ a, uint64_t* __restrict__ b,
+                             uint64_t* __restrict__ c, size_t count) {
+    const size_t pairs = count / 2;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const ulonglong2 z = make_ulonglong2(0, 0);
+    for (size_t i = tid; i < pairs; i += stride) {
+        reinterpret_cast<ulonglong2*>(a)[i] = z;
+        reinterpret_cast<ulonglong2*>(b)[i] = z;
+        reinterpret_cast<ulonglong2*>(c)[i] = z;
+    }
+    if ((count & 1) && tid == 0) {
+        a[count - 1] = 0;
+        b[count - 1] = 0;
+        c[count - 1] = 0;
+    }
+}
+
 __global__ void fill_u64_kernel(uint64_t* __restrict__ p, size_t count, uint64_t v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -3099,16 +3119,18 @@ struct Engine {
     void reset_slots() {
         size_t nm = (size_t)slots * n;
         unsigned gb = blocks_for_threads(std::min<size_t>(nm, (size_t)1 << 22));
-        fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.vis.p, nm, 0);
-        launch_check();
         if (L.kind == kUniform) {
-            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.cand.p, nm, 0);
-            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.succ.p, nm, 0);
+            zero3_kernel<<<blocks_for_threads(std::min<size_t>(nm / 2 + 1, (size_t)1 << 22)),
+                           kBlock, 0, st>>>(ws.vis.p, ws.cand.p, ws.succ.p, nm);
+            launch_check();
             if (ws.zero_words > 0)
                 fill_u64_kernel<<<blocks_for_threads(std::min<size_t>(ws.zero_words,
                                                                       (size_t)1 << 22)),
                                   kBlock, 0, st>>>(ws.levels.p, ws.zero_words, 0);
             ws.zero_words = 0;
+            launch_check();
+        } else {
+            fill_u64_kernel<<<gb, kBlock, 0, st>>>(ws.vis.p, nm, 0);
             launch_check();
         }
         const unsigned rb = blocks_for_threads(std::min<size_t>(nm * kLanes, (size_t)1 << 22));
