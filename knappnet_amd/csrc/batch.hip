@@ -473,8 +473,8 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
         ncur = 0;
     };
     if (lane == 0) NETBC_COUNT(arcs_seen, T);
-    for (uint32_t t0 = 0; t0 < T; t0 += 64) {
-        const uint32_t t = t0 + lane;
+    for (uint64_t t0 = 0; t0 < T; t0 += 64) {
+        const uint32_t t = (uint32_t)t0 + lane;
         const int c = owner_lane(pre, t);
         const uint32_t bc = (uint32_t)__shfl((int)b, c, 64);
         const uint32_t ic = (uint32_t)__shfl((int)i0, c, 64);
@@ -647,8 +647,8 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
             uint32_t T;
             const uint32_t pre = wave_prefix(deg, T);
             if (lane == 0) NETBC_COUNT(c1, T);
-            for (uint32_t t0 = 0; t0 < T; t0 += 64) {
-                const uint32_t t = t0 + lane;
+            for (uint64_t t0 = 0; t0 < T; t0 += 64) {
+                const uint32_t t = (uint32_t)t0 + lane;
                 const int c = owner_lane(pre, t);
                 const uint32_t bc = (uint32_t)__shfl((int)en.b, c, 64);
                 const uint32_t vc = (uint32_t)__shfl((int)en.v, c, 64);
@@ -680,13 +680,13 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
             const uint32_t t = (uint32_t)(w - groups);
             const uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
             const QEnt en = q[hq[h]];
-            const uint32_t a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
-            const uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
+            const uint64_t a0 = (uint64_t)off[en.v] + (uint64_t)(t - hpre[h]) * kChunkArcs;
+            const uint64_t a1 = min(a0 + kChunkArcs, (uint64_t)off[en.v + 1]);
             const size_t base = (size_t)en.b * n;
             uint64_t offered = 0;
             if (lane == 0) NETBC_COUNT(c1, a1 - a0);
-            for (uint32_t c = a0; c < a1; c += 64) {
-                const uint32_t a = c + lane;
+            for (uint64_t c = a0; c < a1; c += 64) {
+                const uint64_t a = c + lane;
                 uint32_t x = 0;
                 uint64_t nv = 0;
                 if (a < a1) {
@@ -803,7 +803,7 @@ __global__ void reach_hub_kernel(const uint4* __restrict__ hub_task, uint32_t ch
         if (!N) continue;
         const uint64_t* crow = cur + (size_t)b * n;
         uint64_t got = 0;
-        for (uint32_t a = ht.y; a < ht.z && got != N; a += 64) {
+        for (uint64_t a = ht.y; a < ht.z && got != N; a += 64) {
             uint64_t m = 0;
             if (a + lane < ht.z) m = crow[in_idx[a + lane]] & N;
             got |= wave_or(m);
@@ -1136,7 +1136,7 @@ __device__ __forceinline__ void edge_flush(double (&c)[K], const uint32_t (&ak)[
 // rows are loaded before they are added. With kEdge, each arc's credit
 // sv * coef (sv: the lane's path count of en) goes to eacc.
 template <class C, bool kEdge>
-__device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t a1,
+__device__ __forceinline__ void sweep_arcs(const QEnt& en, uint64_t a0, uint64_t a1,
                                            const uint32_t* __restrict__ idx,
                                            const uint64_t* __restrict__ R,
                                            const C* __restrict__ coef, size_t n, Dep<C>& dv,
@@ -1144,8 +1144,8 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
                                            unsigned long long& c6, unsigned long long& c7) {
     const int lane = lane_id();
     const size_t base = (size_t)en.b * n;
-    for (uint32_t c = a0; c < a1; c += 64) {
-        uint32_t a = c + lane;
+    for (uint64_t c = a0; c < a1; c += 64) {
+        const uint64_t a = c + lane;
         bool ok = a < a1;
         uint32_t x = ok ? idx[a] : 0;
         uint64_t mm = ok ? (R[base + x] & en.m) : 0;
@@ -1167,7 +1167,7 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint32_t a0, uint32_t
                     bal &= bal - 1;
                     uint32_t xx = rl32(x, j);
                     uint64_t m2 = rl64(mm, j);
-                    ak[u] = c + (uint32_t)j;
+                    ak[u] = (uint32_t)(c + (uint32_t)j);
                     use[u] = (m2 >> lane) & 1;
                     if (use[u]) val[u] = coef[(base + xx) * kLanes + lane];
                 }
@@ -1286,8 +1286,8 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             uint32_t t = (uint32_t)(w - groups);
             uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
             QEnt en = q[hq[h]];
-            uint32_t a0 = off[en.v] + (t - hpre[h]) * kChunkArcs;
-            uint32_t a1 = min(a0 + kChunkArcs, off[en.v + 1]);
+            const uint64_t a0 = (uint64_t)off[en.v] + (uint64_t)(t - hpre[h]) * kChunkArcs;
+            const uint64_t a1 = min(a0 + kChunkArcs, (uint64_t)off[en.v + 1]);
             Dep<C> dv;
             if (succ[(size_t)en.b * n + en.v] & en.m) {
                 // the row's path counts stay in place until bfs_combine_kernel
@@ -1335,8 +1335,8 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             const uint64_t mm = rl64(en.m, c);
             sv = ((mm >> lane) & 1) ? sig[((size_t)bb * n + vv) * kLanes + lane] : 0.0;
         };
-        for (uint32_t t0 = 0; t0 < T; t0 += 64) {
-            const uint32_t t = t0 + lane;
+        for (uint64_t t0 = 0; t0 < T; t0 += 64) {
+            const uint32_t t = (uint32_t)t0 + lane;
             const int c = owner_lane(pre, t);
             const uint32_t bc = (uint32_t)__shfl((int)en.b, c, 64);
             const uint32_t ac = (uint32_t)__shfl((int)a0, c, 64);
@@ -1663,9 +1663,9 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         if (!first_step) {
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
-            const uint32_t a0 = in_off[x], a1 = kCount ? in_off[x + 1] : a0;
-            for (uint32_t c = a0; c < a1; c += 64) {
-                const uint32_t a = c + lane;
+            const uint64_t a0 = in_off[x], a1 = kCount ? in_off[x + 1] : a0;
+            for (uint64_t c = a0; c < a1; c += 64) {
+                const uint64_t a = c + lane;
                 const bool ok = a < a1;
                 const uint32_t u = ok ? in_idx[a] : 0;
                 const Len ln = ok ? in_len[a] : Len(0);
@@ -1715,9 +1715,9 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
             cnt += 1;
         }
-        const uint32_t a0 = off[x], a1 = off[x + 1];
-        for (uint32_t c = a0; c < a1; c += 64) {
-            const uint32_t a = c + lane;
+        const uint64_t a0 = off[x], a1 = off[x + 1];
+        for (uint64_t c = a0; c < a1; c += 64) {
+            const uint64_t a = c + lane;
             const bool ok = a < a1;
             const uint32_t y = ok ? idx[a] : 0;
             const Len ln = ok ? len_arr[a] : Len(0);
@@ -1815,9 +1815,9 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
         const double s = mine ? sig[(base + en.v) * kLanes + lane] : 0.0;
         const Dist dv0 = mine ? dist[(base + en.v) * kLanes + lane] : Dist(0);
         Dep<C> dv, dq;  // dq: kQuirk (as bfs_backward_kernel), non-target successors
-        uint32_t a0 = off[en.v], a1 = off[en.v + 1];
-        for (uint32_t c = a0; c < a1; c += 64) {
-            uint32_t a = c + lane;
+        const uint64_t a0 = off[en.v], a1 = off[en.v + 1];
+        for (uint64_t c = a0; c < a1; c += 64) {
+            const uint64_t a = c + lane;
             bool ok = a < a1;
             uint32_t x = ok ? idx[a] : 0;
             Len len = ok ? len_arr[a] : Len(0);
@@ -1841,7 +1841,7 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                         const uint32_t xx = rl32(x, j);
                         const Len ll = rl(len, j);
                         const uint64_t m2 = rl64(mm, j);
-                        ak[u] = c + (uint32_t)j;
+                        ak[u] = (uint32_t)(c + (uint32_t)j);
                         if constexpr (kQuirk) tk[u] = target_of(target, xx) != 0.0;
                         if ((m2 >> lane) & 1) {
                             const size_t r = (base + xx) * kLanes + lane;
@@ -2093,7 +2093,7 @@ __global__ void landmark_offer_kernel(const uint32_t* __restrict__ in_off,
             const uint32_t vj = (uint32_t)(v0 + j);
             const uint32_t Mj = rl32(M, j);
             const uint32_t a1 = in_off[vj + 1];
-            for (uint32_t a = in_off[vj] + lane; a < a1; a += 64) {
+            for (uint64_t a = (uint64_t)in_off[vj] + lane; a < a1; a += 64) {
                 const uint32_t u = in_idx[a];
                 const uint32_t nw = Mj & ~vis[u];
                 if (nw) atomicOr(&nxt[u], nw);
@@ -2175,7 +2175,8 @@ __global__ void landmark_relax_kernel(const uint32_t* __restrict__ off,
         T best[kWeightedLandmarks];
 #pragma unroll
         for (int j = 0; j < kWeightedLandmarks; j++) best[j] = kFar;
-        for (uint32_t a = off[v] + lane; a < off[v + 1]; a += 64) {
+        const uint64_t a1 = off[v + 1];
+        for (uint64_t a = (uint64_t)off[v] + lane; a < a1; a += 64) {
             const T* du = wd + (size_t)idx[a] * kWeightedLandmarks;
             const T l = len[a];
 #pragma unroll
@@ -3002,7 +3003,7 @@ Buckets buckets_for(const Lengths& L) {
 // per-lane rows.
 size_t slot_bytes_per_node(const Lengths& L, int precision, uint64_t n) {
     size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
-    if (L.kind == kUniform) b += kLevelBufs * 8;
+    if (L.kind == kUniform) b += kLevelBufs * 8 + 8;  // level bitmaps, dependency sums (bsum)
     if (L.kind == kInteger) b += kLanes * 4 + 8 * buckets_for(L).words + 8;
     if (L.kind == kReal) {
         if (real_bucketed(L, n)) b += kLanes * 8 + 8 * buckets_for(L).words + 8;
@@ -3376,9 +3377,13 @@ struct Engine {
             uint32_t ab = rebuild && dl + 2 <= D ? lv[dl + 2] : 0;
             uint32_t ae = rebuild && dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
-            const uint32_t gsz =
+            uint32_t gsz =
                 group_size(qe - qb, resident_waves(bfs_backward_kernel<C, kEdge, kQuirk>),
                            (uint32_t)std::min(std::max(opt.sweep_group, 1), 64));
+            // A task walks its entries' rows as one concatenation indexed in
+            // 32 bits: with whole rows (no split) of up to 2^32 - 2 arcs,
+            // one entry per task keeps it in range.
+            if (!split && d.max_out * gsz > 0xFFFFFFFFull - 64) gsz = 1;
             size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
             size_t waves = std::max<size_t>(tasks, rebuild ? (qe - qb + ae - ab + 63) / 64 : 0);
             if (heavy && ws.part.n < (size_t)hc[dl] * kLanes) {

@@ -1127,6 +1127,25 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
 
 namespace {
 // A plain streaming copy (16 bytes per thread per iteration).
+// Throws unless [p, p + bytes) lies inside one device allocation.
+void check_device_range(const void* p, size_t bytes, const char* what) {
+    hipPointerAttribute_t a{};
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();  // a refused query must not surface in a later call
+        throw netbc::Error(std::string("the ") + what + " is not device memory");
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        throw netbc::Error(std::string("the ") + what + " is not device memory");
+    }
+    const size_t used = (size_t)((const char*)p - (const char*)base);
+    if (used > size || bytes > size - used)
+        throw netbc::Error(std::string("the ") + what + " allocation is smaller than " +
+                           std::to_string(bytes) + " bytes");
+}
+
 __global__ void netbc_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
                                   size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1140,6 +1159,8 @@ NETBC_EXPORT int32_t netbc_calibration_copy(void* dst, const void* src, size_t b
     return guard(err, err_len, [&] {
         ensure_device();
         if (bytes % 16) throw netbc::Error("bytes must be a multiple of 16");
+        check_device_range(dst, bytes, "destination");
+        check_device_range(src, bytes, "source");
         size_t count = bytes / 16;
         netbc_copy_kernel<<<2048, 256>>>((uint4*)dst, (const uint4*)src, count);
         NETBC_HIP(hipGetLastError());

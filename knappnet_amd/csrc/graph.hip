@@ -15,6 +15,7 @@
 #include <cstring>
 
 #include "netbc_internal.h"
+#include "row_tasks.h"
 
 namespace netbc {
 namespace {
@@ -449,18 +450,13 @@ void build_device_graph(Graph& g) {
     // Hub tables from the in-degrees.
     std::vector<uint32_t> ioff(g.n + 1);
     NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));
-    std::vector<uint8_t> flag(std::max<uint64_t>(g.n, 1), 0);
-    std::vector<uint4> task;
+    std::vector<uint8_t> flag;
     std::vector<uint32_t> hub_list;
-    for (uint64_t v = 0; v < g.n; v++) {
-        uint32_t deg = ioff[v + 1] - ioff[v];
-        if (deg > kHubIn) {
-            flag[v] = 1;
-            hub_list.push_back((uint32_t)v);
-            for (uint32_t a = ioff[v]; a < ioff[v + 1]; a += kHubChunk)
-                task.push_back(make_uint4((uint32_t)v, a, std::min(a + kHubChunk, ioff[v + 1]), 0));
-        }
-    }
+    std::vector<HubTask> cut;
+    cut_hub_tasks(ioff.data(), g.n, kHubIn, kHubChunk, flag, hub_list, cut);
+    std::vector<uint4> task(cut.size());
+    for (size_t i = 0; i < cut.size(); i++)
+        task[i] = make_uint4(cut[i].hub, cut[i].first, cut[i].end, 0);
     d.hubs = (uint32_t)hub_list.size();
     d.hub_list.alloc(hub_list.size());
     if (!hub_list.empty())
@@ -476,18 +472,9 @@ void build_device_graph(Graph& g) {
     // Heavy rows from the out-degrees.
     std::vector<uint32_t> ooff(g.n + 1);
     NETBC_HIP(hipMemcpy(ooff.data(), d.out_off.p, (g.n + 1) * 4, hipMemcpyDeviceToHost));
-    std::vector<uint8_t> hflag(std::max<uint64_t>(g.n, 1), 0);
-    uint32_t maxdeg = 0;
-    d.heavy = 0;
-    for (uint64_t v = 0; v < g.n; v++) {
-        uint32_t deg = ooff[v + 1] - ooff[v];
-        maxdeg = std::max(maxdeg, deg);
-        if (deg > kHeavyOut) {
-            hflag[v] = 1;
-            d.heavy++;
-        }
-    }
-    d.max_chunks = std::max<uint32_t>(1, (maxdeg + kChunkArcs - 1) / kChunkArcs);
+    std::vector<uint8_t> hflag;
+    d.max_out = scan_heavy_rows(ooff.data(), g.n, kHeavyOut, kChunkArcs, hflag, &d.heavy,
+                                &d.max_chunks);
     // Landmarks: largest in-degree first, ties by id.
     {
         std::vector<uint32_t> cand(g.n);

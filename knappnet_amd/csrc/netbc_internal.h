@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -41,6 +42,17 @@ struct Error : std::runtime_error {
                                  " (" #call ")");                                    \
     } while (0)
 
+// Failure injection (tests): with NETBC_FAIL_ALLOC_BYTES=b in the
+// environment, every device allocation of b bytes or more fails as if the
+// device were out of memory, before hipMalloc is called.
+inline bool alloc_refused(size_t bytes) {
+    const char* s = std::getenv("NETBC_FAIL_ALLOC_BYTES");
+    if (!s || !*s) return false;
+    char* end = nullptr;
+    const unsigned long long b = std::strtoull(s, &end, 10);
+    return end != s && bytes >= b;
+}
+
 // A device allocation owned by host code.
 template <class T>
 struct DevBuf {
@@ -59,7 +71,7 @@ struct DevBuf {
         if (count <= n && p) return;
         release();
         size_t bytes = (count ? count : 1) * sizeof(T);
-        hipError_t e = hipMalloc(&p, bytes);
+        hipError_t e = alloc_refused(bytes) ? hipErrorOutOfMemory : hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             p = nullptr;
             throw Error("device error: out of device memory allocating " +
@@ -93,6 +105,7 @@ struct DeviceGraph {
     // of kChunkArcs arcs.
     DevBuf<uint8_t> heavy_flag;        // per node
     uint32_t heavy = 0, max_chunks = 1;
+    uint64_t max_out = 0;              // largest out-degree
     // Landmarks for batching sources: the nodes of largest in-degree
     // (ties by id), at most kLandmarks of them.
     std::vector<uint32_t> landmarks;

@@ -123,9 +123,9 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
                 uint64_t N = 0;
                 if (U) {
                     double acc = 0.0;
-                    const uint32_t a0 = in_off[base + w], a1 = in_off[base + w + 1];
-                    for (uint32_t c = a0; c < a1; c += 64) {
-                        const uint32_t a = c + lane;
+                    const uint64_t a0 = in_off[base + w], a1 = in_off[base + w + 1];
+                    for (uint64_t c = a0; c < a1; c += 64) {
+                        const uint64_t a = c + lane;
                         const bool ok = a < a1;
                         const uint32_t u = ok ? in_idx[a] - base : 0;
                         const uint64_t mm = ok ? (cur[u] & U) : 0;
@@ -186,9 +186,9 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
                 if (!M) continue;
                 SDep<C> dv;
                 if (d < D) {
-                    const uint32_t a0 = off[base + v], a1 = off[base + v + 1];
-                    for (uint32_t c = a0; c < a1; c += 64) {
-                        const uint32_t a = c + lane;
+                    const uint64_t a0 = off[base + v], a1 = off[base + v + 1];
+                    for (uint64_t c = a0; c < a1; c += 64) {
+                        const uint64_t a = c + lane;
                         const bool ok = a < a1;
                         const uint32_t x = ok ? idx[a] - base : 0;
                         const uint64_t mm = ok ? (fb[x] & M) : 0;
