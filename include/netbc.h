@@ -258,6 +258,34 @@ int32_t netbc_betweenness_batch(int64_t graphs, const int64_t *node_counts,
                                 netbc_diagnostics *diag, char *err,
                                 size_t err_len);
 
+/* netbc_betweenness_batch without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The graphs arrive already on the device as their union graph
+ * in the layout net_graph_new builds and netbc_graph_csr returns: graph i
+ * has node_counts[i] nodes (host array), numbered after those of graphs
+ * 0..i-1; `offsets` (total nodes + 1) and `targets` (`arcs` entries) are the
+ * union's out-CSR, `in_offsets` / `in_targets` its in-CSR (directed only;
+ * NULL when undirected); `out` receives the scores (total nodes doubles) on
+ * the device, equal bit for bit to what netbc_betweenness_batch returns for
+ * the same unweighted graphs. Limits: at most 65,536 graphs per call, each
+ * of at most 2048 nodes and searched within 254 levels (the call fails
+ * with a message otherwise); rows and offsets are checked on the device
+ * (an arc that leaves its graph is an error). Scratch: one static 64 MiB
+ * device array per GPU the library runs on, held until the process ends;
+ * every call on a GPU shares it, so calls that overlap on one GPU race
+ * (serialise them). Runs on `stream` (NULL: the default stream) and waits
+ * for it before returning. */
+int32_t netbc_betweenness_batch_noalloc(int64_t graphs,
+                                        const int64_t *node_counts,
+                                        int32_t directed, int32_t normalized,
+                                        const uint32_t *offsets,
+                                        const uint32_t *targets,
+                                        const uint32_t *in_offsets,
+                                        const uint32_t *in_targets,
+                                        int64_t arcs, double *out, void *stream,
+                                        char *err, size_t err_len);
+
 /* Debug entry: the per-source state of k sources, each row n long:
  * dist = hop count (breadth-first) or integer distance (Dial),
  * UINT64_MAX when unreached; sigma = path counts; delta = dependencies

@@ -112,6 +112,10 @@ def device() -> ctypes.CDLL:
             POINTER(c_double), c_int32, c_int32, c_int32, c_int32, POINTER(Options),
             POINTER(c_double), POINTER(Diagnostics), c_char_p, c_size_t]
         lib.netbc_betweenness_batch.restype = c_int32
+        lib.netbc_betweenness_batch_noalloc.argtypes = [
+            c_int64, POINTER(c_int64), c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
+            c_int64, c_void_p, c_void_p, c_char_p, c_size_t]
+        lib.netbc_betweenness_batch_noalloc.restype = c_int32
         lib.netbc_state_dump.argtypes = [c_void_p, c_int32, c_int32, c_int32, POINTER(c_int64),
                                          c_int64, POINTER(c_uint64), POINTER(c_double),
                                          POINTER(c_double), c_char_p, c_size_t]

@@ -1125,27 +1125,22 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
     });
 }
 
-namespace {
-// A plain streaming copy (16 bytes per thread per iteration).
-// Throws unless [p, p + bytes) lies inside one device allocation.
-void check_device_range(const void* p, size_t bytes, const char* what) {
-    hipPointerAttribute_t a{};
-    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice) {
-        (void)hipGetLastError();  // a refused query must not surface in a later call
-        throw netbc::Error(std::string("the ") + what + " is not device memory");
-    }
-    hipDeviceptr_t base = nullptr;
-    size_t size = 0;
-    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
-        (void)hipGetLastError();
-        throw netbc::Error(std::string("the ") + what + " is not device memory");
-    }
-    const size_t used = (size_t)((const char*)p - (const char*)base);
-    if (used > size || bytes > size - used)
-        throw netbc::Error(std::string("the ") + what + " allocation is smaller than " +
-                           std::to_string(bytes) + " bytes");
+NETBC_EXPORT int32_t netbc_betweenness_batch_noalloc(
+    int64_t graphs, const int64_t* node_counts, int32_t directed, int32_t normalized,
+    const uint32_t* offsets, const uint32_t* targets, const uint32_t* in_offsets,
+    const uint32_t* in_targets, int64_t arcs, double* out, void* stream, char* err,
+    size_t err_len) {
+    return guard(err, err_len, [&] {
+        ensure_device();
+        netbc::RunDiag d;
+        netbc::run_small_batch_noalloc(graphs, node_counts, directed != 0, normalized != 0,
+                                       offsets, targets, in_offsets, in_targets, arcs, out,
+                                       static_cast<hipStream_t>(stream), d);
+    });
 }
 
+namespace {
+// A plain streaming copy (16 bytes per thread per iteration).
 __global__ void netbc_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
                                   size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1159,8 +1154,8 @@ NETBC_EXPORT int32_t netbc_calibration_copy(void* dst, const void* src, size_t b
     return guard(err, err_len, [&] {
         ensure_device();
         if (bytes % 16) throw netbc::Error("bytes must be a multiple of 16");
-        check_device_range(dst, bytes, "destination");
-        check_device_range(src, bytes, "source");
+        netbc::check_device_range(dst, bytes, "destination");
+        netbc::check_device_range(src, bytes, "source");
         size_t count = bytes / 16;
         netbc_copy_kernel<<<2048, 256>>>((uint4*)dst, (const uint4*)src, count);
         NETBC_HIP(hipGetLastError());

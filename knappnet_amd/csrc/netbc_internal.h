@@ -81,6 +81,26 @@ struct DevBuf {
     }
 };
 
+// Throws unless [p, p + bytes) lies inside one device allocation (buffers
+// handed over by the caller; `what` names them in the message).
+inline void check_device_range(const void* p, size_t bytes, const char* what) {
+    hipPointerAttribute_t a{};
+    if (!p || hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();  // a refused query must not surface in a later call
+        throw Error(std::string("the ") + what + " is not device memory");
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(std::string("the ") + what + " is not device memory");
+    }
+    const size_t used = (size_t)((const char*)p - (const char*)base);
+    if (used > size || bytes > size - used)
+        throw Error(std::string("the ") + what + " allocation is smaller than " +
+                    std::to_string(bytes) + " bytes");
+}
+
 // Path-length classes, as the reference's Lengths enum (weights.rs:72-82).
 enum LengthKind : int32_t { kUniform = 0, kInteger = 1, kReal = 2 };
 

@@ -39,4 +39,14 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
                      const std::vector<uint32_t>& node_graph, const RunOptions& opt,
                      double* out_host, std::vector<uint32_t>& too_deep, RunDiag& diag);
 
+// Betweenness of many small unweighted graphs whose union is already on the
+// device (out-CSR off/idx and, directed, in-CSR in_off/in_idx: `arcs`
+// targets each, node ids consecutive across graphs), with no device
+// allocation: scratch comes from a static device array (small.hip). Writes
+// the scaled scores into device `out`. Bit-identical to run_small_batch.
+void run_small_batch_noalloc(int64_t graphs, const int64_t* node_counts, bool directed,
+                             bool normalized, const uint32_t* off, const uint32_t* idx,
+                             const uint32_t* in_off, const uint32_t* in_idx, int64_t arcs,
+                             double* out, hipStream_t st, RunDiag& diag);
+
 }  // namespace netbc

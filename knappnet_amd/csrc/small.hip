@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "netbc_internal.h"
 #include "small.h"
@@ -64,6 +65,192 @@ struct SDep<float> {
     }
 };
 
+// Checked arc reads (graphs handed over as device arrays, which the host
+// cannot validate): a row whose offsets run backwards or past the arc count
+// is taken as empty, an arc leaving its graph's node range as absent, and
+// the graph is reported in *bad (atomic minimum of graph indices).
+struct ArcCheck {
+    uint64_t arcs;
+    uint32_t* bad;
+};
+
+// Row [a0, a1) of a node of graph g, empty (and g reported) when invalid.
+template <bool kChecked>
+__device__ __forceinline__ void row_bounds(const uint32_t* __restrict__ off, uint32_t node,
+                                           uint32_t g, const ArcCheck& chk, uint64_t& a0,
+                                           uint64_t& a1) {
+    a0 = off[node];
+    a1 = off[node + 1];
+    if constexpr (kChecked) {
+        if (a1 < a0 || a1 > chk.arcs) {
+            atomicMin(chk.bad, g);
+            a1 = a0;
+        }
+    }
+}
+
+// Node u - base of an arc's end, or n (no node) for an arc leaving the
+// graph (reported when checked; a validated graph has none).
+template <bool kChecked>
+__device__ __forceinline__ uint32_t arc_end(uint32_t u, uint32_t base, uint32_t n, uint32_t g,
+                                            const ArcCheck& chk) {
+    const uint32_t x = u - base;
+    if constexpr (kChecked) {
+        if (x >= n) {
+            atomicMin(chk.bad, g);
+            return n;
+        }
+    }
+    return x;
+}
+
+// One batch of sources (work item I of graph G) from start to finish in
+// one workgroup: the breadth-first levels (dense pulls over the graph,
+// frontier bitmaps vis/fa/fb in LDS), then the backward dependency sweep.
+// Leaves delta_s(v) of lane l in sig[v * 64 + l] for the reached lanes of
+// vis[v]. Returns false (every thread) when the search passes 254 levels.
+template <class C, bool kChecked>
+__device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
+                               const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx,
+                               const uint32_t* __restrict__ in_off,
+                               const uint32_t* __restrict__ in_idx, double* __restrict__ sig,
+                               C* __restrict__ coef, uint8_t* __restrict__ lvl, uint64_t* vis,
+                               uint64_t* fa, uint64_t* fb, const ArcCheck& chk) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t n = G.n, base = G.node0;
+    const uint32_t first = I.j0 * G.gs;
+    const uint32_t nl = min(I.ngroups * G.gs, n - first);
+    const uint64_t active = nl >= 64 ? ~0ull : ((1ull << nl) - 1);
+
+    for (uint32_t v = tid; v < n; v += kSmallBlock) {
+        vis[v] = 0;
+        fa[v] = 0;
+    }
+    __syncthreads();
+    if (tid < (int)nl) {
+        uint32_t s = first + tid;
+        atomicOr((unsigned long long*)&vis[s], 1ull << tid);
+        atomicOr((unsigned long long*)&fa[s], 1ull << tid);
+        sig[(size_t)s * kLanes + tid] = 1.0;
+        lvl[(size_t)s * kLanes + tid] = 0;
+    }
+    __syncthreads();
+
+    // Forward: level d -> d+1 by a pull over every node with unreached
+    // lanes; sigma[w][l] sums sigma[u][l] over in-arcs u->w with u at
+    // level d in lane l (in CSC order; exact integers).
+    uint64_t* cur = fa;
+    uint64_t* nxt = fb;
+    int D = 0;
+    for (int d = 0;; d++) {
+        int any = 0;
+        for (uint32_t w = wave; w < n; w += kSmallWaves) {
+            const uint64_t U = active & ~vis[w];
+            uint64_t N = 0;
+            if (U) {
+                double acc = 0.0;
+                uint64_t a0, a1;
+                row_bounds<kChecked>(in_off, base + w, I.g, chk, a0, a1);
+                for (uint64_t c = a0; c < a1; c += 64) {
+                    const uint64_t a = c + lane;
+                    const bool ok = a < a1;
+                    const uint32_t u = ok ? arc_end<kChecked>(in_idx[a], base, n, I.g, chk) : n;
+                    const uint64_t mm = u < n ? (cur[u] & U) : 0;
+                    uint64_t bal = __ballot(mm != 0);
+                    while (bal) {
+                        const int j = __ffsll((unsigned long long)bal) - 1;
+                        bal &= bal - 1;
+                        const uint32_t uu = srl32(u, j);
+                        const uint64_t m2 = srl64(mm, j);
+                        N |= m2;
+                        if ((m2 >> lane) & 1) acc += sig[(size_t)uu * kLanes + lane];
+                    }
+                }
+                if ((N >> lane) & 1) {
+                    sig[(size_t)w * kLanes + lane] = acc;
+                    lvl[(size_t)w * kLanes + lane] = (uint8_t)(d + 1);
+                }
+            }
+            if (lane == 0) {
+                nxt[w] = N;
+                if (N) vis[w] |= N;
+            }
+            any |= N != 0;
+        }
+        if (!__syncthreads_or(any)) {
+            D = d;
+            break;
+        }
+        uint64_t* t = cur;
+        cur = nxt;
+        nxt = t;
+        if (d + 2 >= 255) return false;
+    }
+
+    // Backward: levels D..1. fa = lanes at level d, fb = lanes at d+1.
+    for (int d = D; d >= 1; d--) {
+        for (uint32_t v = wave; v < n; v += kSmallWaves) {
+            const uint8_t lv = lvl[(size_t)v * kLanes + lane];
+            const uint64_t r = vis[v];
+            const uint64_t Mv = __ballot(lv == d) & r;
+            const uint64_t Sv = __ballot(lv == d + 1) & r;
+            if (lane == 0) {
+                fa[v] = Mv;
+                fb[v] = Sv;
+            }
+        }
+        __syncthreads();
+        for (uint32_t v = wave; v < n; v += kSmallWaves) {
+            const uint64_t M = fa[v];
+            if (!M) continue;
+            SDep<C> dv;
+            if (d < D) {
+                uint64_t a0, a1;
+                row_bounds<kChecked>(off, base + v, I.g, chk, a0, a1);
+                for (uint64_t c = a0; c < a1; c += 64) {
+                    const uint64_t a = c + lane;
+                    const bool ok = a < a1;
+                    const uint32_t x = ok ? arc_end<kChecked>(idx[a], base, n, I.g, chk) : n;
+                    const uint64_t mm = x < n ? (fb[x] & M) : 0;
+                    uint64_t bal = __ballot(mm != 0);
+                    while (bal) {
+                        const int j = __ffsll((unsigned long long)bal) - 1;
+                        bal &= bal - 1;
+                        const uint32_t xx = srl32(x, j);
+                        const uint64_t m2 = srl64(mm, j);
+                        if ((m2 >> lane) & 1) dv.add(coef[(size_t)xx * kLanes + lane]);
+                    }
+                }
+            }
+            if ((M >> lane) & 1) {
+                const size_t r = (size_t)v * kLanes + lane;
+                const double s = sig[r];
+                const double delta = s * dv.value();
+                coef[r] = SDep<C>::coef(delta, s);
+                sig[r] = delta;
+            }
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// The sum over sources s of group j of item I (in source order) of
+// delta_s(v), v != s: the reference's per-group accumulator
+// (parallel.rs:437-503) for node v. Also counts reached nodes and arcs.
+__device__ __forceinline__ double group_sum(const SmallItem& I, const SmallGraph& G,
+                                            const double* __restrict__ row, uint64_t r,
+                                            uint32_t v, uint32_t j) {
+    const uint32_t first = I.j0 * G.gs;
+    const uint32_t nl = min(I.ngroups * G.gs, G.n - first);
+    const uint32_t l0 = j * G.gs;
+    const uint32_t l1 = min(l0 + G.gs, nl);
+    double acc = 0.0;
+    for (uint32_t l = l0; l < l1; l++)
+        if (((r >> l) & 1) && first + l != v) acc += row[l];
+    return acc;
+}
+
 template <class C>
 __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
     const SmallItem* __restrict__ items, uint32_t nitems, uint32_t* __restrict__ counter,
@@ -76,11 +263,12 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
     __shared__ uint64_t fa[kSmallMaxNodes];
     __shared__ uint64_t fb[kSmallMaxNodes];
     __shared__ uint32_t item_s;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
     double* sig = sig_all + (size_t)blockIdx.x * nmax * kLanes;
     C* coef = coef_all + (size_t)blockIdx.x * nmax * kLanes;
     uint8_t* lvl = lvl_all + (size_t)blockIdx.x * nmax * kLanes;
     unsigned long long reached = 0, reached_arcs = 0;
+    const ArcCheck chk{0, nullptr};
 
     for (;;) {
         if (tid == 0) item_s = atomicAdd(counter, 1u);
@@ -90,143 +278,24 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
         if (it >= nitems) break;
         const SmallItem I = items[it];
         const SmallGraph G = graphs[I.g];
-        const uint32_t n = G.n, base = G.node0;
-        const uint32_t first = I.j0 * G.gs;
-        const uint32_t nl = min(I.ngroups * G.gs, n - first);
-        const uint64_t active = nl >= 64 ? ~0ull : ((1ull << nl) - 1);
-
-        for (uint32_t v = tid; v < n; v += kSmallBlock) {
-            vis[v] = 0;
-            fa[v] = 0;
-        }
-        __syncthreads();
-        if (tid < (int)nl) {
-            uint32_t s = first + tid;
-            atomicOr((unsigned long long*)&vis[s], 1ull << tid);
-            atomicOr((unsigned long long*)&fa[s], 1ull << tid);
-            sig[(size_t)s * kLanes + tid] = 1.0;
-            lvl[(size_t)s * kLanes + tid] = 0;
-        }
-        __syncthreads();
-
-        // Forward: level d -> d+1 by a pull over every node with unreached
-        // lanes; sigma[w][l] sums sigma[u][l] over in-arcs u->w with u at
-        // level d in lane l (in CSC order; exact integers).
-        uint64_t* cur = fa;
-        uint64_t* nxt = fb;
-        int D = 0;
-        bool deep = false;
-        for (int d = 0;; d++) {
-            int any = 0;
-            for (uint32_t w = wave; w < n; w += kSmallWaves) {
-                const uint64_t U = active & ~vis[w];
-                uint64_t N = 0;
-                if (U) {
-                    double acc = 0.0;
-                    const uint64_t a0 = in_off[base + w], a1 = in_off[base + w + 1];
-                    for (uint64_t c = a0; c < a1; c += 64) {
-                        const uint64_t a = c + lane;
-                        const bool ok = a < a1;
-                        const uint32_t u = ok ? in_idx[a] - base : 0;
-                        const uint64_t mm = ok ? (cur[u] & U) : 0;
-                        uint64_t bal = __ballot(mm != 0);
-                        while (bal) {
-                            const int j = __ffsll((unsigned long long)bal) - 1;
-                            bal &= bal - 1;
-                            const uint32_t uu = srl32(u, j);
-                            const uint64_t m2 = srl64(mm, j);
-                            N |= m2;
-                            if ((m2 >> lane) & 1) acc += sig[(size_t)uu * kLanes + lane];
-                        }
-                    }
-                    if ((N >> lane) & 1) {
-                        sig[(size_t)w * kLanes + lane] = acc;
-                        lvl[(size_t)w * kLanes + lane] = (uint8_t)(d + 1);
-                    }
-                }
-                if (lane == 0) {
-                    nxt[w] = N;
-                    if (N) vis[w] |= N;
-                }
-                any |= N != 0;
-            }
-            if (!__syncthreads_or(any)) {
-                D = d;
-                break;
-            }
-            uint64_t* t = cur;
-            cur = nxt;
-            nxt = t;
-            if (d + 2 >= 255) {
-                deep = true;
-                break;
-            }
-        }
-        if (deep) {
+        if (!run_small_item<C, false>(I, G, off, idx, in_off, in_idx, sig, coef, lvl, vis, fa,
+                                      fb, chk)) {
             if (tid == 0) too_deep[I.g] = 1;
             __syncthreads();
             continue;
         }
-
-        // Backward: levels D..1. fa = lanes at level d, fb = lanes at d+1.
-        for (int d = D; d >= 1; d--) {
-            for (uint32_t v = wave; v < n; v += kSmallWaves) {
-                const uint8_t lv = lvl[(size_t)v * kLanes + lane];
-                const uint64_t r = vis[v];
-                const uint64_t Mv = __ballot(lv == d) & r;
-                const uint64_t Sv = __ballot(lv == d + 1) & r;
-                if (lane == 0) {
-                    fa[v] = Mv;
-                    fb[v] = Sv;
-                }
-            }
-            __syncthreads();
-            for (uint32_t v = wave; v < n; v += kSmallWaves) {
-                const uint64_t M = fa[v];
-                if (!M) continue;
-                SDep<C> dv;
-                if (d < D) {
-                    const uint64_t a0 = off[base + v], a1 = off[base + v + 1];
-                    for (uint64_t c = a0; c < a1; c += 64) {
-                        const uint64_t a = c + lane;
-                        const bool ok = a < a1;
-                        const uint32_t x = ok ? idx[a] - base : 0;
-                        const uint64_t mm = ok ? (fb[x] & M) : 0;
-                        uint64_t bal = __ballot(mm != 0);
-                        while (bal) {
-                            const int j = __ffsll((unsigned long long)bal) - 1;
-                            bal &= bal - 1;
-                            const uint32_t xx = srl32(x, j);
-                            const uint64_t m2 = srl64(mm, j);
-                            if ((m2 >> lane) & 1) dv.add(coef[(size_t)xx * kLanes + lane]);
-                        }
-                    }
-                }
-                if ((M >> lane) & 1) {
-                    const size_t r = (size_t)v * kLanes + lane;
-                    const double s = sig[r];
-                    const double delta = s * dv.value();
-                    coef[r] = SDep<C>::coef(delta, s);
-                    sig[r] = delta;
-                }
-            }
-            __syncthreads();
-        }
-
         // Group sums: for each whole group of the batch, in source order.
-        for (uint32_t v = tid; v < n; v += kSmallBlock) {
+        const uint32_t first = I.j0 * G.gs;
+        const uint32_t nl = min(I.ngroups * G.gs, G.n - first);
+        for (uint32_t v = tid; v < G.n; v += kSmallBlock) {
             const uint64_t r = vis[v];
             reached += (unsigned long long)__popcll(r);
-            reached_arcs += (unsigned long long)__popcll(r) * (off[base + v + 1] - off[base + v]);
-            const double* row = sig + (size_t)v * kLanes;
+            reached_arcs += (unsigned long long)__popcll(r) *
+                            (off[G.node0 + v + 1] - off[G.node0 + v]);
             for (uint32_t j = 0; j < I.ngroups; j++) {
-                const uint32_t l0 = j * G.gs;
-                if (l0 >= nl) break;
-                const uint32_t l1 = min(l0 + G.gs, nl);
-                double acc = 0.0;
-                for (uint32_t l = l0; l < l1; l++)
-                    if (((r >> l) & 1) && first + l != v) acc += row[l];
-                gsum[G.gsum_off + (size_t)(I.j0 + j) * n + v] = acc;
+                if (j * G.gs >= nl) break;
+                gsum[G.gsum_off + (size_t)(I.j0 + j) * G.n + v] =
+                    group_sum(I, G, sig + (size_t)v * kLanes, r, v, j);
             }
         }
         __syncthreads();
@@ -262,6 +331,212 @@ __global__ void small_reduce_kernel(const SmallGraph* __restrict__ graphs,
 __global__ void zero_u32_kernel(uint32_t* p, size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) p[i] = 0;
+}
+
+// ---- the allocation-free entry (netbc_betweenness_batch_noalloc) -----------
+
+// Its scratch: one static device array per GPU the library runs on, held
+// until the process ends, laid out as [header | node counts | graph
+// descriptors | graph order | per-workgroup rows]. Host code reaches it only
+// through the address hipGetSymbolAddress returns on each device.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+constexpr uint32_t kNoAllocMaxGraphs = 1u << 16;
+
+struct NoAllocHeader {
+    uint32_t next;        // work queue: graphs handed out so far
+    uint32_t bad_graph;   // least graph with an invalid row or arc (kNone: none)
+    uint32_t deep_graph;  // least graph searched past 254 levels (kNone: none)
+    uint32_t pad;
+    unsigned long long stats[2];  // reached nodes, reached arcs
+};
+constexpr size_t kCountsOff = 256;
+constexpr size_t kDescOff = kCountsOff + (size_t)kNoAllocMaxGraphs * sizeof(int64_t);
+constexpr size_t kOrderOff = kDescOff + (size_t)kNoAllocMaxGraphs * sizeof(SmallGraph);
+constexpr size_t kRowsOff = (kOrderOff + (size_t)kNoAllocMaxGraphs * 4 + 255) / 256 * 256;
+static_assert(sizeof(NoAllocHeader) <= kCountsOff, "header fits its slot");
+static_assert(kRowsOff < kScratchBytes, "rows region left");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Lays a batch out in the scratch at `base`: clears the header and, from
+// the node counts, writes each graph's descriptor (first node, source-group
+// size and count as parallel.rs:437-440, scale as betweenness.rs:100-113)
+// and the order in which workgroups take the graphs (most nodes first).
+// One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void noalloc_plan_kernel(unsigned char* __restrict__ base,
+                                                           uint32_t graphs, int directed,
+                                                           int normalized) {
+    NoAllocHeader* H = reinterpret_cast<NoAllocHeader*>(base);
+    const int64_t* counts = reinterpret_cast<const int64_t*>(base + kCountsOff);
+    SmallGraph* desc = reinterpret_cast<SmallGraph*>(base + kDescOff);
+    uint32_t* order = reinterpret_cast<uint32_t*>(base + kOrderOff);
+    __shared__ uint32_t part[1024];
+    __shared__ uint32_t hist[kSmallMaxNodes + 1];
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        H->next = 0;
+        H->bad_graph = kNone;
+        H->deep_graph = kNone;
+        H->pad = 0;
+        H->stats[0] = 0;
+        H->stats[1] = 0;
+    }
+    for (uint32_t i = t; i <= kSmallMaxNodes; i += blockDim.x) hist[i] = 0;
+    // graphs [g0, g1) of this thread, their first nodes by a block-wide scan
+    const uint32_t per = (graphs + blockDim.x - 1) / blockDim.x;
+    const uint32_t g0 = min(t * per, graphs), g1 = min(g0 + per, graphs);
+    uint32_t local = 0;
+    for (uint32_t g = g0; g < g1; g++) local += (uint32_t)counts[g];
+    part[t] = local;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (uint32_t i = 0; i < blockDim.x; i++) {
+            const uint32_t x = part[i];
+            part[i] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    uint32_t node0 = part[t];
+    for (uint32_t g = g0; g < g1; g++) {
+        const uint32_t n = (uint32_t)counts[g];
+        SmallGraph G;
+        G.n = n;
+        G.node0 = node0;
+        G.gs = n ? (n + 63) / 64 : 1;
+        G.groups = n ? (n + G.gs - 1) / G.gs : 0;
+        G.gsum_off = 0;
+        double scale = directed ? 1.0 : 0.5;
+        if (normalized && n > 2) {
+            const double pairs = (double)((uint64_t)(n - 1) * (uint64_t)(n - 2));
+            scale *= directed ? 1.0 / pairs : 2.0 / pairs;
+        }
+        G.scale = scale;
+        G.device = 1;
+        G.pad = 0;
+        desc[g] = G;
+        node0 += n;
+        atomicAdd(&hist[kSmallMaxNodes - n], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (uint32_t i = 0; i <= kSmallMaxNodes; i++) {
+            const uint32_t x = hist[i];
+            hist[i] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+    for (uint32_t g = g0; g < g1; g++)
+        order[atomicAdd(&hist[kSmallMaxNodes - (uint32_t)counts[g]], 1u)] = g;
+}
+
+// Betweenness of whole graphs, one graph per workgroup at a time: its
+// batches of source groups one after another (run_small_item), each group's
+// sum added into the caller's scores in group order, which is the
+// reference's merge bc = ((g0 + g1) + g2) + ... (parallel.rs:383-503), then
+// the graph's scale. Besides the graph and the scores everything lives in
+// the scratch at `base`; a workgroup's rows take row_bytes from kRowsOff on.
+__global__ __launch_bounds__(kSmallBlock) void small_whole_kernel(
+    unsigned char* __restrict__ base, uint32_t graphs, size_t nmax, size_t row_bytes,
+    const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx,
+    const uint32_t* __restrict__ in_off, const uint32_t* __restrict__ in_idx, uint64_t arcs,
+    double* __restrict__ out) {
+    __shared__ uint64_t vis[kSmallMaxNodes];
+    __shared__ uint64_t fa[kSmallMaxNodes];
+    __shared__ uint64_t fb[kSmallMaxNodes];
+    __shared__ uint32_t item_s;
+    NoAllocHeader* H = reinterpret_cast<NoAllocHeader*>(base);
+    const SmallGraph* desc = reinterpret_cast<const SmallGraph*>(base + kDescOff);
+    const uint32_t* order = reinterpret_cast<const uint32_t*>(base + kOrderOff);
+    unsigned char* rows = base + kRowsOff + (size_t)blockIdx.x * row_bytes;
+    double* sig = reinterpret_cast<double*>(rows);
+    double* coef = sig + nmax * kLanes;
+    uint8_t* lvl = reinterpret_cast<uint8_t*>(coef + nmax * kLanes);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const ArcCheck chk{arcs, &H->bad_graph};
+    unsigned long long reached = 0, reached_arcs = 0;
+    for (;;) {
+        if (tid == 0) item_s = atomicAdd(&H->next, 1u);
+        __syncthreads();
+        const uint32_t it = item_s;
+        __syncthreads();
+        if (it >= graphs) break;
+        const uint32_t g = order[it];
+        const SmallGraph G = desc[g];
+        const uint32_t per = max(1u, 64u / G.gs);
+        bool ok = true;
+        for (uint32_t j0 = 0; j0 < G.groups; j0 += per) {
+            const SmallItem I{g, j0, min(per, G.groups - j0), 0};
+            ok = run_small_item<double, true>(I, G, off, idx, in_off, in_idx, sig, coef, lvl, vis,
+                                              fa, fb, chk);
+            if (!ok) {
+                if (tid == 0) atomicMin(&H->deep_graph, g);
+                break;
+            }
+            const uint32_t first = j0 * G.gs;
+            const uint32_t nl = min(I.ngroups * G.gs, G.n - first);
+            for (uint32_t v = tid; v < G.n; v += kSmallBlock) {
+                const uint64_t r = vis[v];
+                reached += (unsigned long long)__popcll(r);
+                reached_arcs += (unsigned long long)__popcll(r) *
+                                (off[G.node0 + v + 1] - off[G.node0 + v]);
+                double* o = &out[G.node0 + v];
+                for (uint32_t j = 0; j < I.ngroups; j++) {
+                    if (j * G.gs >= nl) break;
+                    const double acc = group_sum(I, G, sig + (size_t)v * kLanes, r, v, j);
+                    *o = j0 + j == 0 ? acc : *o + acc;
+                }
+            }
+            __syncthreads();
+        }
+        // each score was written by this thread in the loop above
+        if (ok && G.scale != 1.0)
+            for (uint32_t v = tid; v < G.n; v += kSmallBlock) out[G.node0 + v] *= G.scale;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        reached += (unsigned long long)__shfl_xor((long long)reached, o, 64);
+        reached_arcs += (unsigned long long)__shfl_xor((long long)reached_arcs, o, 64);
+    }
+    if (lane == 0 && reached) atomicAdd(&H->stats[0], reached);
+    if (lane == 0 && reached_arcs) atomicAdd(&H->stats[1], reached_arcs);
+}
+
+// The scratch's address on each device (looked up once per device, with
+// that device current) and how many whole-graph workgroups fit on it.
+struct ScratchDevice {
+    bool ready = false;
+    unsigned char* ptr = nullptr;
+    unsigned resident = 1;
+};
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+ScratchDevice g_scratch_dev[kMaxDevices];
+
+ScratchDevice scratch_on_current_device() {
+    int dev = 0;
+    NETBC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error("device index beyond the library's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ScratchDevice& s = g_scratch_dev[dev];
+    if (!s.ready) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(std::string("device error: ") + hipGetErrorString(e) +
+                        " (the address of the allocation-free scratch)");
+        int cus = 0, per = 0;
+        NETBC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, small_whole_kernel,
+                                                               kSmallBlock, 0));
+        s.ptr = static_cast<unsigned char*>(p);
+        s.resident = (unsigned)std::max(1, cus * std::max(per, 1));
+        s.ready = true;
+    }
+    return s;
 }
 
 }  // namespace
@@ -376,6 +651,76 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
     diag.dominant_launches += 1;
     diag.reached_nodes += (int64_t)hs[0];
     diag.reached_arcs += (int64_t)hs[1];
+    diag.slots = (int32_t)grid;
+}
+
+}  // namespace netbc
+
+namespace netbc {
+
+void run_small_batch_noalloc(int64_t graphs, const int64_t* node_counts, bool directed,
+                             bool normalized, const uint32_t* off, const uint32_t* idx,
+                             const uint32_t* in_off, const uint32_t* in_idx, int64_t arcs,
+                             double* out, hipStream_t st, RunDiag& diag) {
+    if (graphs < 0) throw Error("negative graph count");
+    if ((uint64_t)graphs > kNoAllocMaxGraphs)
+        throw Error("the allocation-free entry takes at most " +
+                    std::to_string(kNoAllocMaxGraphs) + " graphs per call");
+    if (graphs > 0 && !node_counts) throw Error("null node counts");
+    uint64_t total = 0, nmax = 0;
+    for (int64_t i = 0; i < graphs; i++) {
+        if (node_counts[i] < 0) throw Error("graph " + std::to_string(i) + ": negative node count");
+        if (node_counts[i] > (int64_t)kSmallMaxNodes)
+            throw Error("graph " + std::to_string(i) + ": " + std::to_string(node_counts[i]) +
+                        " nodes; the allocation-free entry takes graphs of at most " +
+                        std::to_string(kSmallMaxNodes) + " nodes");
+        total += (uint64_t)node_counts[i];
+        nmax = std::max<uint64_t>(nmax, (uint64_t)node_counts[i]);
+    }
+    if (total == 0) return;
+    if (!off || !idx || !out) throw Error("null offsets, targets or scores");
+    if (directed && (!in_off || !in_idx))
+        throw Error("a directed batch needs its in-CSR (in_offsets, in_targets)");
+    if (!directed) {
+        in_off = off;
+        in_idx = idx;
+    }
+    if (arcs < 0 || (uint64_t)arcs > 0xFFFFFFFEull) throw Error("arc count outside 0..2^32 - 2");
+    check_device_range(out, total * sizeof(double), "score buffer");
+    check_device_range(off, (total + 1) * sizeof(uint32_t), "offset array");
+    if (arcs) check_device_range(idx, (size_t)arcs * sizeof(uint32_t), "target array");
+    if (directed) {
+        check_device_range(in_off, (total + 1) * sizeof(uint32_t), "in-offset array");
+        if (arcs) check_device_range(in_idx, (size_t)arcs * sizeof(uint32_t), "in-target array");
+    }
+    const size_t row_bytes = ((size_t)nmax * kLanes * (8 + 8 + 1) + 255) / 256 * 256;
+    const size_t fit = (kScratchBytes - kRowsOff) / row_bytes;
+    if (fit == 0) throw Error("the batch's rows exceed the allocation-free scratch");
+    const ScratchDevice sd = scratch_on_current_device();
+    const unsigned grid =
+        (unsigned)std::min<uint64_t>({(uint64_t)fit, (uint64_t)sd.resident, (uint64_t)graphs});
+    NETBC_HIP(hipMemcpyAsync(sd.ptr + kCountsOff, node_counts, (size_t)graphs * sizeof(int64_t),
+                             hipMemcpyHostToDevice, st));
+    noalloc_plan_kernel<<<1, 1024, 0, st>>>(sd.ptr, (uint32_t)graphs, directed ? 1 : 0,
+                                            normalized ? 1 : 0);
+    NETBC_HIP(hipGetLastError());
+    small_whole_kernel<<<grid, kSmallBlock, 0, st>>>(sd.ptr, (uint32_t)graphs, nmax, row_bytes,
+                                                     off, idx, in_off, in_idx, (uint64_t)arcs,
+                                                     out);
+    NETBC_HIP(hipGetLastError());
+    NoAllocHeader h{};
+    NETBC_HIP(hipMemcpyAsync(&h, sd.ptr, sizeof h, hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipStreamSynchronize(st));
+    if (h.bad_graph != kNone)
+        throw Error("graph " + std::to_string(h.bad_graph) +
+                    ": an offset runs backwards or past the arc count, or an arc leaves the "
+                    "graph's nodes");
+    if (h.deep_graph != kNone)
+        throw Error("graph " + std::to_string(h.deep_graph) +
+                    ": a search passed 254 levels; the allocation-free entry does not take "
+                    "such graphs (netbc_betweenness_batch does)");
+    diag.reached_nodes += (int64_t)h.stats[0];
+    diag.reached_arcs += (int64_t)h.stats[1];
     diag.slots = (int32_t)grid;
 }
 
