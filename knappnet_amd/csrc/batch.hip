@@ -61,6 +61,9 @@ struct Ctrl {
     // Bucketed searches: the lowest bucket still pending after a step
     // (alternating by step parity; all ones when none).
     unsigned long long next_bucket[2];
+    // Integer lengths (distance ring): the largest distance offered so far.
+    uint32_t last_offer;
+    uint32_t pad1;
     // Work tickets of the dependency sweep, one counter pair per class of
     // workgroups (blockIdx.x % 32, four classes per XCD), each in its own
     // 64-byte line; the last wavefront of a class to finish resets its pair
@@ -268,12 +271,18 @@ __global__ void fill_u64_kernel(uint64_t* __restrict__ p, size_t count, uint64_t
 // Chunk counts of the heavy rows listed in hq[hbeg, *hend): hpre[i] = first
 // chunk task of entry i (relative to hbeg); *total = all of them. (hpre at
 // *hend is overwritten by the next level's scan.)
+// With hbeg_dev the first entry is read from device memory instead; with
+// total_copy the total is also written there (the per-level table of
+// searches whose level bounds stay on the device).
 __global__ void heavy_scan_kernel(const uint32_t* __restrict__ hq, uint32_t hbeg,
                                   const uint32_t* __restrict__ hend,
                                   const QEnt* __restrict__ q, const uint32_t* __restrict__ off,
-                                  uint32_t* __restrict__ hpre, uint32_t* __restrict__ total) {
+                                  uint32_t* __restrict__ hpre, uint32_t* __restrict__ total,
+                                  const uint32_t* __restrict__ hbeg_dev = nullptr,
+                                  uint32_t* __restrict__ total_copy = nullptr) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t carry;
+    if (hbeg_dev) hbeg = *hbeg_dev;
     const uint32_t he = *hend;
     const int lane = lane_id(), wid = threadIdx.x >> 6;
     if (threadIdx.x == 0) carry = 0;
@@ -306,6 +315,7 @@ __global__ void heavy_scan_kernel(const uint32_t* __restrict__ hq, uint32_t hbeg
     if (threadIdx.x == 0) {
         hpre[he] = carry;
         *total = carry;
+        if (total_copy) *total_copy = carry;
     }
 }
 
@@ -425,10 +435,41 @@ __device__ __forceinline__ int owner_lane(uint32_t pre, uint32_t t) {
     return lo;
 }
 
+// Level lookups of the path-count gathers and the dependency sweeps: the
+// lanes in which the node of a (slot, node) pair sits at the level an arc
+// leads from (gathers) or to (sweeps). Breadth-first searches read one
+// bitmap per level (LevelBits). Integer lengths keep the bitmap of every
+// distance in a ring of W buffers of `stride` words (W = longest arc + 1,
+// slot D mod W for distance D; d0 = D mod W for the current distance D):
+// an arc of length len leads from distance D - len (RingBits<true>) or to
+// distance D + len (RingBits<false>).
+struct LevelBits {
+    const uint64_t* __restrict__ bits;
+    __device__ __forceinline__ uint64_t lanes(size_t pair, uint64_t) const { return bits[pair]; }
+};
+template <bool kBack>
+struct RingBits {
+    const uint64_t* __restrict__ ring;
+    size_t stride;
+    const uint32_t* __restrict__ len;  // arc lengths, in the order of the arcs walked
+    uint32_t d0, W;
+    __device__ __forceinline__ uint64_t lanes(size_t pair, uint64_t arc) const {
+        const uint32_t l = len[arc];
+        uint32_t sl;
+        if constexpr (kBack) {
+            sl = d0 >= l ? d0 - l : d0 + W - l;
+        } else {
+            sl = d0 + l;
+            if (sl >= W) sl -= W;
+        }
+        return ring[(size_t)sl * stride + pair];
+    }
+};
+
 // Path counts of the wavefront's 64 pairs; lane c holds pair c: slot b,
 // node x, the lanes U sought and the in-row [i0, i0 + deg) (deg = 0: none).
 // The in-rows are walked as one concatenation, 64 arcs at a time; an arc
-// u->x hits in the lanes m = cur[u] & U in which u is at level d. Hits are
+// u->x hits in the lanes m = prev(u) & U in which u is at level d. Hits are
 // then taken in order, so each pair's hits are contiguous and its sums stay
 // in registers until its last one: sigma[x][l] = sum of sigma[u][l] over
 // its hits with l in m (integers below 2^53: exact in any order, so equal
@@ -437,10 +478,10 @@ __device__ __forceinline__ int owner_lane(uint32_t pre, uint32_t t) {
 // With kAdd the sums are added atomically (exact: integer partial sums)
 // instead of stored, for pairs whose in-row is split over several tasks.
 // With kNpr each lane's count of predecessor arcs is also written to npr.
-template <bool kAdd, bool kNpr = false>
+template <bool kAdd, bool kNpr = false, class Prev = LevelBits>
 __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64_t U, uint32_t i0,
                                                   uint32_t deg, const uint32_t* __restrict__ in_idx,
-                                                  const uint64_t* __restrict__ cur,
+                                                  const Prev prev,
                                                   uint64_t* __restrict__ succ,
                                                   double* __restrict__ sig, size_t n,
                                                   unsigned long long& arcs_seen,
@@ -483,8 +524,9 @@ __device__ __forceinline__ uint64_t gather_counts(uint32_t x, uint32_t b, uint64
         uint32_t u = 0;
         uint64_t m = 0;
         if (t < T) {
-            u = in_idx[ic + (t - pc)];
-            m = cur[(size_t)bc * n + u] & Uc;
+            const uint32_t arc = ic + (t - pc);
+            u = in_idx[arc];
+            m = prev.lanes((size_t)bc * n + u, arc) & Uc;
             if (succ && m)
                 atomicOr((unsigned long long*)&succ[(size_t)bc * n + u], (unsigned long long)m);
         }
@@ -749,7 +791,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
         }
         const uint32_t deg = N ? ht.z - ht.y : 0;
         NETBC_COUNT(c6, N != 0);
-        const uint64_t got = gather_counts<true, kNpr>(ht.x, b, N, ht.y, deg, in_idx, cur,
+        const uint64_t got = gather_counts<true, kNpr>(ht.x, b, N, ht.y, deg, in_idx, LevelBits{cur},
                                                  bottom_up ? succ : nullptr, sig, n, c4, c5,
                                                  npr);
         if (bottom_up) {
@@ -884,7 +926,7 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
             if (!hub_flag[te.x]) deg = i1 - i0;
         }
         if constexpr (kCount)
-            (void)gather_counts<false, kNpr>(te.x, te.y, N, i0, deg, in_idx, cur, nullptr, sig,
+            (void)gather_counts<false, kNpr>(te.x, te.y, N, i0, deg, in_idx, LevelBits{cur}, nullptr, sig,
                                              n, c4, c5, npr);
         if (ok) {
             cand[p] = 0;
@@ -975,7 +1017,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         NETBC_COUNT(c6, deg != 0);
         uint64_t mine = 0;
         if constexpr (kCount) {
-            mine = gather_counts<false, kNpr>(x, b, U, i0, deg, in_idx, cur, succ, sig, n, c4, c5,
+            mine = gather_counts<false, kNpr>(x, b, U, i0, deg, in_idx, LevelBits{cur}, succ, sig, n, c4, c5,
                                               npr);
         } else if (deg) {
             const uint64_t* crow = cur + (size_t)b * n;
@@ -1131,14 +1173,13 @@ __device__ __forceinline__ void edge_flush(double (&c)[K], const uint32_t (&ak)[
     if ((lane & ((64 >> kLog) - 1)) == 0 && a != kNone) eacc[(size_t)b * arcs + a] += c[0];
 }
 
-// Adds the coefficients of the successors of `en` (R: the level d+1
-// bitmap) among arcs [a0, a1) to dv, per lane in CSR order; up to kAhead
+// Adds the coefficients of the successors of `en` (next: their lanes, see
+// bfs_backward_kernel) among arcs [a0, a1) to dv, per lane in CSR order; up to kAhead
 // rows are loaded before they are added. With kEdge, each arc's credit
 // sv * coef (sv: the lane's path count of en) goes to eacc.
-template <class C, bool kEdge>
+template <class C, bool kEdge, class Next>
 __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint64_t a0, uint64_t a1,
-                                           const uint32_t* __restrict__ idx,
-                                           const uint64_t* __restrict__ R,
+                                           const uint32_t* __restrict__ idx, const Next next,
                                            const C* __restrict__ coef, size_t n, Dep<C>& dv,
                                            double sv, double* __restrict__ eacc, size_t arcs,
                                            unsigned long long& c6, unsigned long long& c7) {
@@ -1148,7 +1189,7 @@ __device__ __forceinline__ void sweep_arcs(const QEnt& en, uint64_t a0, uint64_t
         const uint64_t a = c + lane;
         bool ok = a < a1;
         uint32_t x = ok ? idx[a] : 0;
-        uint64_t mm = ok ? (R[base + x] & en.m) : 0;
+        uint64_t mm = ok ? (next.lanes(base + x, a) & en.m) : 0;
         NETBC_COUNT(c6, mm != 0);
         NETBC_COUNT(c7, __popcll(mm));
         uint64_t bal = __ballot(mm != 0);
@@ -1196,7 +1237,9 @@ __device__ __forceinline__ void add_lane_sum(double* dst, double x) {
     if (lane_id() == 0 && x != 0.0) unsafeAtomicAdd(dst, x);
 }
 
-// Dependencies of the level-d entries. R is the level d+1 bitmap. With
+// Dependencies of the level-d entries; `nx` gives the lanes in which an
+// arc's head is a successor (breadth-first: the level d+1 bitmap; integer
+// lengths: the bitmap of distance D + len in the distance ring). With
 // rebuild, W is turned from the level d+2 bitmap into level d's (lanes are
 // disjoint, so the OR and AND-NOT updates commute); levels kept from the
 // forward search need no rebuild. A wavefront takes up to 64 light entries:
@@ -1215,7 +1258,7 @@ __device__ __forceinline__ void add_lane_sum(double* dst, double x) {
 // bsum (free-order runs) each finished entry also adds the sum of its
 // lanes' dependencies to bsum[b][v] (add_lane_sum; bfs_combine_kernel does
 // the same for heavy rows).
-template <class C, bool kEdge, bool kQuirk = false>
+template <class C, bool kEdge, bool kQuirk = false, class Next = LevelBits>
 __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level, int rebuild,
                                     const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
@@ -1223,9 +1266,8 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     const uint8_t* __restrict__ heavy_flag,
                                     double* __restrict__ part,
                                     const uint32_t* __restrict__ off,
-                                    const uint32_t* __restrict__ idx,
-                                    const uint64_t* __restrict__ R, uint64_t* __restrict__ W,
-                                    const uint64_t* __restrict__ succ,
+                                    const uint32_t* __restrict__ idx, const Next nx,
+                                    uint64_t* __restrict__ W, const uint64_t* __restrict__ succ,
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
                                     uint32_t gsz, size_t n, double* __restrict__ eacc,
                                     size_t arcs, const uint8_t* __restrict__ target,
@@ -1294,7 +1336,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                 double sv = 0.0;
                 if (kEdge && ((en.m >> lane) & 1))
                     sv = sig[((size_t)en.b * n + en.v) * kLanes + lane];
-                sweep_arcs<C, kEdge>(en, a0, a1, idx, R, coef, n, dv, sv, eacc, arcs, c6, c7);
+                sweep_arcs<C, kEdge>(en, a0, a1, idx, nx, coef, n, dv, sv, eacc, arcs, c6, c7);
             }
             part[t * kLanes + lane] = dv.value();
             continue;
@@ -1347,7 +1389,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             if (t < T) {
                 arc = ac + (t - pc);
                 x = idx[arc];
-                mm = R[(size_t)bc * n + x] & mc;
+                mm = nx.lanes((size_t)bc * n + x, arc) & mc;
             }
             NETBC_COUNT(c6, mm != 0);
             NETBC_COUNT(c7, __popcll(mm));
@@ -1741,9 +1783,10 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                         bal &= bal - 1;
                         yh[h] = rl32(y, j);
                         const uint64_t m2 = rl64(mm, j);
+                        const Len lh = rl(ln, j);  // a cross-lane read: every lane takes part
                         if ((m2 >> lane) & 1) {
                             use[h] = true;
-                            nd[h] = P::add(dx, rl(ln, j));
+                            nd[h] = P::add(dx, lh);
                             cur[h] = dist[(base + yh[h]) * kLanes + lane];
                         }
                     }
@@ -1883,6 +1926,201 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                 sig[r] = delta;
             }
         }
+    }
+}
+
+// ---- integer lengths: the distance ring ------------------------------------
+//
+// The breadth-first engine carried over to integer lengths 1..W-1 (Dial's
+// algorithm with exact integer distances, betweenness.rs:214-295): every
+// integer distance D is a level. A ring of W bitmaps (slot D mod W) holds the
+// lanes of each (slot, node) pair settled at the last W distances, and a ring
+// of W offer bitmaps the lanes offered each of the next W - 1 distances. A
+// step at distance D gathers (dial_gather_kernel): the pairs offered D in
+// lanes not yet settled settle there, with path counts pulled over in-arcs
+// from the lanes at distance D - len; then expands (dial_expand_kernel): the
+// entries of distance D offer D + len along their out-arcs. Distances are
+// never stored per lane: the level of a lane is the distance. Queue bounds
+// per distance stay on the device, so steps chain without host waits; the
+// host checks for the end every few steps. The dependency sweep is the
+// breadth-first one (bfs_backward_kernel) with successors looked up at
+// distance D + len (RingBits<false>), the ring rebuilt from the queue as it
+// goes down.
+
+// Settles distance D (d0 = D mod W). A wavefront takes 64 consecutive
+// (slot, node) pairs of one slot (slots spread over the XCDs); a pair
+// offered D takes N = offer & ~vis, clears the offer word, and, when N is
+// not empty, gathers its path counts over its in-row (gather_counts with
+// RingBits<true>, marking successor lanes in succ), joins vis and the
+// distance-D bitmap with N and the level queue as (y, b, N). kNpr:
+// predecessor-arc counts as well (npr).
+template <bool kNpr>
+__global__ void dial_gather_kernel(uint32_t D, uint32_t W, uint64_t* __restrict__ cand_ring,
+                                   uint64_t* __restrict__ ring, size_t stride,
+                                   const uint32_t* __restrict__ in_off,
+                                   const uint32_t* __restrict__ in_idx,
+                                   const uint32_t* __restrict__ in_len, uint64_t* __restrict__ vis,
+                                   uint64_t* __restrict__ succ, double* __restrict__ sig,
+                                   const uint8_t* __restrict__ heavy_flag, QEnt* q, uint32_t qcap,
+                                   uint32_t* __restrict__ hq, Ctrl* ctrl, int slots, size_t n,
+                                   uint32_t* __restrict__ npr) {
+    const int lane = lane_id();
+    const uint32_t d0 = D % W;
+    uint64_t* cand = cand_ring + (size_t)d0 * stride;
+    uint64_t* lev = ring + (size_t)d0 * stride;
+    const RingBits<true> prev{ring, stride, in_len, d0, W};
+    [[maybe_unused]] unsigned long long c4 = 0, c5 = 0;
+    extern __shared__ unsigned long long pull_lds[];
+    QueueStage stage{reinterpret_cast<QEnt*>(pull_lds) + (threadIdx.x >> 6) * kPullBuf, 0};
+    const XcdWave xw = xcd_wave();
+    const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
+    const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
+    for (size_t g = xw.w; g < groups; g += xw.stride) {
+        const uint32_t b = xw.xcd + kXcds * (uint32_t)(g / gps);
+        const uint32_t x0 = (uint32_t)(g % gps) * 64;
+        const bool ok = x0 + (uint32_t)lane < n;
+        const uint32_t x = ok ? x0 + lane : 0;
+        const size_t p = (size_t)b * n + x;
+        const uint64_t c = ok ? cand[p] : 0;
+        if (!__ballot(c != 0)) continue;
+        uint64_t V = 0, N = 0;
+        uint32_t i0 = 0, deg = 0;
+        if (c) {
+            cand[p] = 0;  // the slot serves distance D + W next
+            V = vis[p];
+            N = c & ~V;
+            if (N) {
+                i0 = in_off[x];
+                deg = in_off[x + 1] - i0;
+            }
+        }
+        (void)gather_counts<false, kNpr>(x, b, N, i0, deg, in_idx, prev, succ, sig, n, c4, c5,
+                                         npr);
+        if (N) {
+            vis[p] = V | N;
+            lev[p] = N;
+        }
+        stage.add(N != 0, x, b, N, ctrl, q, qcap, hq, heavy_flag);
+    }
+    if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
+    NETBC_FLUSH(ctrl, 10, c4);
+    NETBC_FLUSH(ctrl, 11, c5);
+}
+
+// Offers from distance D (d0 = D mod W), once dial_gather_kernel settled it
+// and heavy_scan_kernel cut its heavy rows: every entry (x, b, M) offers
+// D + len to the lanes of each out-neighbour y not yet settled,
+// nv = M & ~vis[y], ORed into the offer bitmap of distance D + len (len < W,
+// so a slot of the ring other than d0). Light rows are taken up to 64
+// entries per wavefront and walked as one concatenation, heavy rows as
+// tasks of kChunkArcs arcs (hq / hpre, their count in ctrl->hchunks). Also
+// records where distance D + 1 starts in the level queue and the heavy
+// list, clears the ring slot of distance D + 1 of the entries of distance
+// D + 1 - W (no longer looked back at), and raises ctrl->last_offer to the
+// largest distance offered.
+__global__ void dial_expand_kernel(uint32_t D, uint32_t W, const QEnt* __restrict__ q,
+                                   uint32_t* __restrict__ lvl_q, uint32_t* __restrict__ lvl_h,
+                                   const uint32_t* __restrict__ hq,
+                                   const uint32_t* __restrict__ hpre,
+                                   const uint32_t* __restrict__ off,
+                                   const uint32_t* __restrict__ idx,
+                                   const uint32_t* __restrict__ out_len,
+                                   const uint64_t* __restrict__ vis,
+                                   uint64_t* __restrict__ cand_ring, uint64_t* __restrict__ ring,
+                                   size_t stride, const uint8_t* __restrict__ heavy_flag,
+                                   Ctrl* ctrl, size_t n, uint32_t qcap) {
+    const int lane = lane_id();
+    const uint32_t d0 = D % W;
+    // after an overflow the count runs past the queue (the host restarts)
+    const uint32_t qend = min(ctrl->qlen, qcap);
+    const uint32_t qbeg = min(lvl_q[D], qend);
+    const uint32_t hbeg = lvl_h[D], hend = ctrl->hlen;
+    const uint32_t hchunks = ctrl->hchunks;
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    if (tid == 0) {
+        lvl_q[D + 1] = qend;
+        lvl_h[D + 1] = hend;
+    }
+    if (D + 1 >= W) {
+        uint64_t* lev1 = ring + (size_t)((D + 1) % W) * stride;
+        const uint32_t zb = lvl_q[D + 1 - W], ze = lvl_q[D + 2 - W];
+        for (size_t i = zb + tid; i < ze; i += nth) {
+            const QEnt e = q[i];
+            lev1[(size_t)e.b * n + e.v] = 0;  // the slot holds only distance D + 1 - W
+        }
+    }
+    uint32_t far = 0;  // largest distance this lane offered
+    auto offer = [&](uint32_t y, uint32_t bb, uint64_t nv, uint32_t len) {
+        if (!nv) return;
+        uint32_t sl = d0 + len;
+        if (sl >= W) sl -= W;
+        unsigned long long* cp =
+            (unsigned long long*)&cand_ring[(size_t)sl * stride + (size_t)bb * n + y];
+        if ((*cp & nv) != nv) atomicOr(cp, (unsigned long long)nv);
+        far = max(far, D + len);
+    };
+    const uint32_t nq = qend - qbeg;
+    const uint32_t gsz = group_size(nq, wave_count());
+    const size_t groups = ((size_t)nq + gsz - 1) / gsz;
+    const size_t tasks = groups + hchunks;
+    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+        if (w < groups) {
+            const size_t i = qbeg + w * gsz + lane;
+            QEnt en{0, 0, 0};
+            uint32_t a0 = 0, deg = 0;
+            if ((uint32_t)lane < gsz && i < qend) {
+                en = q[i];
+                if (!heavy_flag[en.v]) {
+                    a0 = off[en.v];
+                    deg = off[en.v + 1] - a0;
+                }
+            }
+            uint32_t T;
+            const uint32_t pre = wave_prefix(deg, T);
+            for (uint64_t t0 = 0; t0 < T; t0 += 64) {
+                const uint32_t t = (uint32_t)t0 + lane;
+                const int c = owner_lane(pre, t);
+                const uint32_t bc = (uint32_t)__shfl((int)en.b, c, 64);
+                const uint32_t ac = (uint32_t)__shfl((int)a0, c, 64);
+                const uint32_t pc = (uint32_t)__shfl((int)pre, c, 64);
+                const uint64_t mc = __shfl(en.m, c, 64);
+                if (t < T) {
+                    const uint32_t arc = ac + (t - pc);
+                    const uint32_t y = idx[arc];
+                    offer(y, bc, mc & ~vis[(size_t)bc * n + y], out_len[arc]);
+                }
+            }
+        } else {
+            // One chunk of a heavy row.
+            const uint32_t t = (uint32_t)(w - groups);
+            const uint32_t h = heavy_of_task(hpre, hbeg, hend, t);
+            const QEnt en = q[hq[h]];
+            const uint64_t a0 = (uint64_t)off[en.v] + (uint64_t)(t - hpre[h]) * kChunkArcs;
+            const uint64_t a1 = min(a0 + kChunkArcs, (uint64_t)off[en.v + 1]);
+            const size_t base = (size_t)en.b * n;
+            for (uint64_t c = a0; c < a1; c += 64) {
+                const uint64_t a = c + lane;
+                if (a < a1) {
+                    const uint32_t y = idx[a];
+                    offer(y, en.b, en.m & ~vis[base + y], out_len[a]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) far = max(far, (uint32_t)__shfl_xor((int)far, o, 64));
+    if (lane == 0 && far > *(volatile uint32_t*)&ctrl->last_offer) atomicMax(&ctrl->last_offer, far);
+}
+
+// Clears the words of the queue entries [qb, qe) in `words` (one level's
+// bitmap, whose words hold that level's lanes only).
+__global__ void zero_entry_words_kernel(const QEnt* __restrict__ q, uint32_t qb, uint32_t qe,
+                                        uint64_t* __restrict__ words, size_t n) {
+    const size_t nth = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = qb + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < qe; i += nth) {
+        const QEnt e = q[i];
+        words[(size_t)e.b * n + e.v] = 0;
     }
 }
 
@@ -2919,6 +3157,14 @@ struct Workspace {
     // real lengths), summed over the rows at the end (fold_arcs_kernel).
     DevBuf<double> eacc;
     DevBuf<uint32_t> npr;  // quirk runs: predecessor arcs per (node, lane), zero between runs
+    // Integer lengths on the distance ring: W bitmaps of settled lanes and W
+    // of offered lanes, slots * n words each (zero between super-batches
+    // unless ring_dirty), and per distance the level's first queue entry,
+    // first heavy-list entry and heavy chunk count (lvl_cap distances).
+    DevBuf<uint64_t> ring, cring;
+    bool ring_dirty = true;
+    DevBuf<uint32_t> lvl_q, lvl_h, hc_lvl;
+    size_t lvl_cap = 0;
     size_t qcap = 0;
     DevBuf<uint2> touched;
     DevBuf<Ctrl> ctrl;
@@ -2997,14 +3243,33 @@ Buckets buckets_for(const Lengths& L) {
     return b;
 }
 
+// Integer lengths run on the distance ring (dial_gather_kernel /
+// dial_expand_kernel) when the ring is short (longest arc below kRingMaxW)
+// and a wavefront's concatenated in-rows stay within 32-bit indices;
+// otherwise, and for distance-only searches, on the per-lane distance rows
+// (dial_step_kernel).
+constexpr uint32_t kRingMaxW = 64;
+// Ring-search steps between checks for its end (one host wait each).
+constexpr uint32_t kRingCheck = 8;
+// Switched off: a GPU run of the random-multigraph parity tests that
+// reached this path ended in a device memory fault whose cause is not
+// established (a CPU execution of these kernels on the same inputs is in
+// bounds and exact), so integer lengths keep the per-lane distance rows.
+constexpr bool kRingEnabled = false;
+bool ring_path(const Graph& g, const Lengths& L) {
+    return kRingEnabled && L.kind == kInteger && L.max + 1 <= kRingMaxW &&
+           g.dev.max_in <= (1ull << 25);
+}
+
 // Sizes (in bytes per node and slot) of the per-slot arrays.
 // bitmaps (visited, offers, successors, the level buffers), the touched
 // list, about four queue entries with their heavy-row slots, and the
 // per-lane rows.
-size_t slot_bytes_per_node(const Lengths& L, int precision, uint64_t n) {
+size_t slot_bytes_per_node(const Lengths& L, int precision, uint64_t n, bool ring = false) {
     size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
     if (L.kind == kUniform) b += kLevelBufs * 8 + 8;  // level bitmaps, dependency sums (bsum)
-    if (L.kind == kInteger) b += kLanes * 4 + 8 * buckets_for(L).words + 8;
+    if (L.kind == kInteger && ring) b += 2 * (size_t)(L.max + 1) * 8 + 8;  // rings, bsum
+    if (L.kind == kInteger && !ring) b += kLanes * 4 + 8 * buckets_for(L).words + 8;
     if (L.kind == kReal) {
         if (real_bucketed(L, n)) b += kLanes * 8 + 8 * buckets_for(L).words + 8;
         else b += kLanes * (4 * 8 + 4 * 4);  // per-thread Dijkstra arrays
@@ -3045,10 +3310,10 @@ RunOptions tuned_options(const Graph& g, const Lengths& L, const RunOptions& opt
 // number of batches and by a 96 GiB share of the device's memory (edge
 // runs add a row of per-arc credit per slot, per thread for real lengths).
 int choose_slots(const Graph& g, const Lengths& L, int64_t k, const RunOptions& opt,
-                 bool edge = false) {
+                 bool edge = false, bool ring = false) {
     int64_t batches = (k + kLanes - 1) / kLanes;
     int want = std::min(std::max(opt.slots, 1), 1024);
-    size_t per = slot_bytes_per_node(L, opt.precision, g.n) * std::max<uint64_t>(g.n, 1);
+    size_t per = slot_bytes_per_node(L, opt.precision, g.n, ring) * std::max<uint64_t>(g.n, 1);
     const bool per_thread = L.kind == kReal && !real_bucketed(L, g.n);
     if (edge) per += (size_t)g.dev.arcs * sizeof(double) * (per_thread ? kLanes : 1);
     size_t budget = (size_t)96 << 30;
@@ -3084,6 +3349,8 @@ struct Engine {
     bool tie_exact = false;           // real lengths tie only when equal
     bool quirk = false;               // NetworkX's subset edge split (edge runs)
     uint32_t* npr = nullptr;          // predecessor-arc rows (quirk runs)
+    bool ring = false;                // integer lengths on the distance ring
+    uint32_t ring_top = 0;            // last distance the ring search stepped
     // Free-order breadth-first runs: per (slot, node) sums of the swept
     // dependencies (ws.bsum), folded into the scores after each super-batch.
     double* bsum = nullptr;
@@ -3325,6 +3592,12 @@ struct Engine {
     // The backward sweep of the super-batch just searched.
     template <class C>
     void backward(C* coef) {
+        if (ring) {
+            if (quirk) backward_ring<C, true, true>(coef);
+            else if (edge) backward_ring<C, true>(coef);
+            else backward_ring<C, false>(coef);
+            return;
+        }
         if (quirk) {
             if (L.kind == kUniform) backward_bfs<C, true, true>(coef);
             else if (L.kind == kInteger) backward_dial<C, true, IntPath, true>(coef);
@@ -3396,7 +3669,7 @@ struct Engine {
                    0, st>>>(
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
-                    level_bits(dl + 1), level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
+                    LevelBits{level_bits(dl + 1)}, level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
                     n, ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr);
             launch_check();
             if (heavy) {
@@ -3487,6 +3760,168 @@ struct Engine {
         }
     }
 
+    // Distance tables of the ring search grown to `need` distances (the
+    // stream is drained first: the old tables are freed).
+    void grow_levels(size_t need) {
+        if (need <= ws.lvl_cap) return;
+        size_t cap = ws.lvl_cap;
+        while (cap < need) cap *= 2;
+        for (DevBuf<uint32_t>* b : {&ws.lvl_q, &ws.lvl_h, &ws.hc_lvl}) {
+            DevBuf<uint32_t> t;
+            t.alloc(cap);
+            NETBC_HIP(hipMemcpyAsync(t.p, b->p, ws.lvl_cap * sizeof(uint32_t),
+                                     hipMemcpyDeviceToDevice, st));
+            NETBC_HIP(hipStreamSynchronize(st));
+            std::swap(b->p, t.p);
+            std::swap(b->n, t.n);
+        }
+        ws.lvl_cap = cap;
+    }
+
+    // Integer lengths on the distance ring (dial_gather_kernel): the
+    // forward search of one super-batch, one step per distance, chained
+    // without host waits and checked for its end every kRingCheck steps.
+    // Returns false if the level queue overflowed.
+    bool run_ring(int64_t item0) {
+        const DeviceGraph& d = g.dev;
+        const uint32_t W = L.max + 1;
+        const size_t stride = (size_t)slots * n;
+        const unsigned fb = blocks_for_threads(std::min<size_t>(stride, (size_t)1 << 22));
+        fill_u64_kernel<<<fb, kBlock, 0, st>>>(ws.vis.p, stride, 0);
+        fill_u64_kernel<<<fb, kBlock, 0, st>>>(ws.succ.p, stride, 0);
+        if (ws.ring_dirty) {
+            const unsigned rb = blocks_for_threads(std::min<size_t>(stride * W, (size_t)1 << 22));
+            fill_u64_kernel<<<rb, kBlock, 0, st>>>(ws.ring.p, stride * W, 0);
+            fill_u64_kernel<<<rb, kBlock, 0, st>>>(ws.cring.p, stride * W, 0);
+            ws.ring_dirty = false;
+        }
+        launch_check();
+        NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, offsetof(Ctrl, sig_overflow), st));
+        init_sources_kernel<uint32_t><<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+            d_list, item0, k, slots, n, ws.vis.p, ws.ring.p, ws.sig.p, nullptr, nullptr, 0);
+        launch_check();
+        source_entries_kernel<<<blocks_for_threads((size_t)slots * kLanes), kBlock, 0, st>>>(
+            d_list, item0, k, slots, n, ws.vis.p, ws.q.p, ws.ctrl.p, d.heavy_flag.p, ws.hq.p,
+            nullptr);
+        launch_check();
+        fill_u32_kernel<<<1, 64, 0, st>>>(ws.lvl_q.p, 1, 0);
+        fill_u32_kernel<<<1, 64, 0, st>>>(ws.lvl_h.p, 1, 0);
+        launch_check();
+        auto gather = npr ? dial_gather_kernel<true> : dial_gather_kernel<false>;
+        const unsigned gb = xcd_grid(resident_blocks(gather, (stride + 63) / 64));
+        const unsigned eb = resident_blocks(dial_expand_kernel, ~(size_t)0 >> 8);
+        uint32_t D = 0;
+        for (;; D++) {
+            if ((size_t)D + 2 > ws.lvl_cap) grow_levels((size_t)D + 2);
+            if (D > 0) {
+                gather<<<gb, kBlock, kPullLds, st>>>(
+                    D, W, ws.cring.p, ws.ring.p, stride, d.in_off_p(), d.in_idx_p(),
+                    in_lengths<IntPath>(), ws.vis.p, ws.succ.p, ws.sig.p, d.heavy_flag.p, ws.q.p,
+                    (uint32_t)ws.qcap, ws.hq.p, ws.ctrl.p, slots, n, npr);
+                launch_check();
+            }
+            heavy_scan_kernel<<<1, 1024, 0, st>>>(ws.hq.p, 0, &ws.ctrl.p->hlen, ws.q.p,
+                                                  d.out_off.p, ws.hpre.p, &ws.ctrl.p->hchunks,
+                                                  ws.lvl_h.p + D, ws.hc_lvl.p + D);
+            launch_check();
+            dial_expand_kernel<<<eb, kBlock, 0, st>>>(
+                D, W, ws.q.p, ws.lvl_q.p, ws.lvl_h.p, ws.hq.p, ws.hpre.p, d.out_off.p,
+                d.out_idx.p, out_lengths<IntPath>(), ws.vis.p, ws.cring.p, ws.ring.p, stride,
+                d.heavy_flag.p, ws.ctrl.p, n, (uint32_t)ws.qcap);
+            launch_check();
+            if (D % kRingCheck == kRingCheck - 1) {
+                read_ctrl(ws, st);
+                if (ws.host_ctrl->qoverflow) {
+                    ws.ring_dirty = true;
+                    return false;
+                }
+                if (ws.host_ctrl->last_offer <= D) break;  // nothing offered beyond D
+            }
+        }
+        std::vector<uint32_t> bq((size_t)D + 2), bh((size_t)D + 2), bc((size_t)D + 1);
+        NETBC_HIP(hipMemcpyAsync(bq.data(), ws.lvl_q.p, bq.size() * 4, hipMemcpyDeviceToHost, st));
+        NETBC_HIP(hipMemcpyAsync(bh.data(), ws.lvl_h.p, bh.size() * 4, hipMemcpyDeviceToHost, st));
+        NETBC_HIP(hipMemcpyAsync(bc.data(), ws.hc_lvl.p, bc.size() * 4, hipMemcpyDeviceToHost,
+                                 st));
+        NETBC_HIP(hipStreamSynchronize(st));
+        uint32_t last = 0;  // last distance with entries
+        for (uint32_t x = 0; x <= D; x++)
+            if (bq[x + 1] > bq[x]) last = x;
+        lv.assign(bq.begin(), bq.begin() + last + 2);
+        hv.assign(bh.begin(), bh.begin() + last + 2);
+        hc.assign(bc.begin(), bc.begin() + last + 1);
+        ring_top = D;
+        diag.levels += last;
+        return true;
+    }
+
+    // Clears the ring words of the entries of distances [from, to] (each
+    // ring slot holds one distance, so a word is that distance's lanes).
+    void clear_ring_levels(int from, int to) {
+        const uint32_t W = L.max + 1;
+        const size_t stride = (size_t)slots * n;
+        for (int x = std::max(from, 0); x <= to && x + 1 < (int)lv.size(); x++) {
+            const uint32_t qb = lv[x], qe = lv[x + 1];
+            if (qe == qb) continue;
+            zero_entry_words_kernel<<<blocks_for_threads(std::min<size_t>(qe - qb, (size_t)1 << 20)),
+                                      kBlock, 0, st>>>(ws.q.p, qb, qe,
+                                                       ws.ring.p + (size_t)(x % W) * stride, n);
+            launch_check();
+        }
+    }
+
+    // The dependency sweep of a ring search: the breadth-first sweep, one
+    // distance at a time from the last, with successors at distance D + len
+    // (RingBits<false>). The ring starts empty and is rebuilt as the sweep
+    // goes down (slot D mod W: distance D in, distance D + W out), so at
+    // distance D it holds distances D + 1 .. D + W - 1; it is left empty.
+    template <class C, bool kEdge, bool kQuirk = false>
+    void backward_ring(C* coef) {
+        const DeviceGraph& d = g.dev;
+        const uint32_t W = L.max + 1;
+        const size_t stride = (size_t)slots * n;
+        const int D = (int)lv.size() - 2;  // last distance with entries
+        clear_ring_levels((int)ring_top - (int)W + 1, D);
+        const int split = opt.exact || kQuirk ? 0 : 1;
+        const int low = kEdge ? 0 : 1;
+        auto kernel = bfs_backward_kernel<C, kEdge, kQuirk, RingBits<false>>;
+        for (int dl = D; dl >= low; dl--) {
+            const uint32_t qb = lv[dl], qe = lv[dl + 1];
+            const uint32_t hb = hv[dl], he = hv[dl + 1];
+            const bool far = dl + (int)W <= D;
+            const uint32_t ab = far ? lv[dl + W] : 0, ae = far ? lv[dl + W + 1] : 0;
+            if (qb == qe && ab == ae) continue;
+            const bool heavy = split && dl < D && he > hb;
+            uint32_t gsz = group_size(qe - qb, resident_waves(kernel),
+                                      (uint32_t)std::min(std::max(opt.sweep_group, 1), 64));
+            if (!split && d.max_out * gsz > 0xFFFFFFFFull - 64) gsz = 1;
+            const size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + (heavy ? hc[dl] : 0);
+            const size_t waves = std::max<size_t>(tasks, (qe - qb + ae - ab + 63) / 64);
+            if (heavy && ws.part.n < (size_t)hc[dl] * kLanes) {
+                NETBC_HIP(hipStreamSynchronize(st));  // the old buffer may be in use
+                ws.part.alloc((size_t)hc[dl] * kLanes * 2);
+            }
+            const uint32_t d0 = (uint32_t)dl % W;
+            ws.sweep.mark(st);
+            kernel<<<ticket_grid(resident_blocks(kernel, waves)), kBlock, 0, st>>>(
+                ws.q.p, qb, qe, ab, ae, dl == D, 1, ws.hq.p, hb, he, ws.hpre.p, hc[dl], split,
+                d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
+                RingBits<false>{ws.ring.p, stride, out_lengths<IntPath>(), d0, W},
+                ws.ring.p + (size_t)d0 * stride, ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz, n,
+                ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr);
+            launch_check();
+            if (heavy) {
+                bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0,
+                                        st>>>(ws.q.p, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
+                                              ws.part.p, d.out_off.p, ws.sig.p, coef, ws.ctrl.p,
+                                              n, target, dl > 0 ? bsum : nullptr);
+                launch_check();
+            }
+            ws.sweep.mark(st);
+        }
+        clear_ring_levels(low, std::min(D, low + (int)W - 1));
+    }
+
     void run_dijkstra(int64_t item0, DjkState& dst) {
         const DeviceGraph& d = g.dev;
         reset_slots();
@@ -3518,7 +3953,7 @@ void grow_queue(Workspace& ws) {
 }
 
 void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots, int precision,
-                      size_t qcap) {
+                      size_t qcap, bool ring = false) {
     size_t n = std::max<uint64_t>(g.n, 1);
     size_t nm = (size_t)slots * n;
     ws.slots = slots;
@@ -3551,7 +3986,22 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
             ws.qcap = qcap;
         }
     }
-    if (L.kind == kInteger || (L.kind == kReal && !per_thread)) {
+    if (ring) {
+        const size_t W = (size_t)L.max + 1;
+        ws.succ.alloc(nm);
+        ws.bsum.alloc(nm);
+        uint64_t* r0 = ws.ring.p;
+        uint64_t* c0 = ws.cring.p;
+        ws.ring.alloc(nm * W);
+        ws.cring.alloc(nm * W);
+        if (ws.ring.p != r0 || ws.cring.p != c0) ws.ring_dirty = true;
+        if (ws.lvl_cap == 0) {
+            ws.lvl_cap = 4096;
+            ws.lvl_q.alloc(ws.lvl_cap);
+            ws.lvl_h.alloc(ws.lvl_cap);
+            ws.hc_lvl.alloc(ws.lvl_cap);
+        }
+    } else if (L.kind == kInteger || (L.kind == kReal && !per_thread)) {
         if (L.kind == kInteger) ws.dist.alloc(nm * kLanes);
         else ws.rdist.alloc(nm * kLanes);
         ws.inb.alloc(nm * buckets_for(L).words);
@@ -3697,10 +4147,11 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     if (edge && g.m) NETBC_HIP(hipMemsetAsync(d_eb, 0, g.m * sizeof(double), st));
     if (k == 0) return;
     Workspace& ws = get_workspace(g);
-    int slots = choose_slots(g, L, k, opt, edge);
+    const bool ring = ring_path(g, L);
+    int slots = choose_slots(g, L, k, opt, edge, ring);
     diag.slots = slots;
     size_t qcap = std::max<size_t>(queue_capacity(slots, n), ws.qcap);
-    ensure_workspace(ws, g, L, slots, opt.precision, qcap);
+    ensure_workspace(ws, g, L, slots, opt.precision, qcap, ring);
     NETBC_HIP(hipMemsetAsync(ws.acc.p, 0, n * sizeof(double), st));
     NETBC_HIP(hipMemsetAsync(ws.gcur.p, 0, n * sizeof(int64_t), st));
     const int64_t* d_list = nullptr;
@@ -3747,7 +4198,9 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         NETBC_HIP(hipMemsetAsync(ws.eacc.p, 0, (size_t)rows * arcs * sizeof(double), st));
     }
     Engine eng{g, L, ws, opt, diag, st, n, slots, d_list, k, edge};
-    const bool fold = free_order && L.kind == kUniform;  // dependency sums from the sweep
+    eng.ring = ring;
+    // dependency sums from the sweep
+    const bool fold = free_order && (L.kind == kUniform || ring);
     if (fold) {
         NETBC_HIP(hipMemsetAsync(ws.bsum.p, 0, (size_t)slots * n * sizeof(double), st));
         eng.bsum = ws.bsum.p;
@@ -3781,6 +4234,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         } else {
             for (;;) {
                 bool ok = L.kind == kUniform   ? eng.run_bfs(item0)
+                          : ring               ? eng.run_ring(item0)
                           : L.kind == kInteger ? eng.run_bucketed<IntPath>(item0)
                           : eng.tie_exact      ? eng.run_bucketed<RealPathExact>(item0)
                                                : eng.run_bucketed<RealPath>(item0);
@@ -3810,7 +4264,7 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
             const int active = (int)std::min<int64_t>(slots, (k - item0 + kLanes - 1) / kLanes);
             fold_slot_sums_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
                 ws.vis.p, ws.bsum.p, active, n, d_bc, g.dev.out_off.p, ws.ctrl.p);
-            if (g.dev.hubs) {
+            if (g.dev.hubs && L.kind == kUniform) {  // hub rows hold atomic partial sums
                 NETBC_HIP(hipGetLastError());
                 const size_t rows = (size_t)slots * g.dev.hubs;
                 zero_hub_rows_kernel<<<blocks_for_threads(std::min<size_t>(rows * 64,
@@ -4099,9 +4553,10 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     if (L.kind == kReal) throw Error("the state dump covers breadth-first and Dial searches");
     Workspace& ws = get_workspace(g);
     RunOptions o = tuned_options(g, L, opt);
-    int slots = choose_slots(g, L, k, o);
+    const bool ring = ring_path(g, L);
+    int slots = choose_slots(g, L, k, o, false, ring);
     size_t qcap = std::max<size_t>(queue_capacity(slots, n), ws.qcap);
-    ensure_workspace(ws, g, L, slots, o.precision, qcap);
+    ensure_workspace(ws, g, L, slots, o.precision, qcap, ring);
     ws.list.alloc(k);
     NETBC_HIP(hipMemcpyAsync(ws.list.p, sources, k * sizeof(int64_t), hipMemcpyHostToDevice, st));
     NETBC_HIP(hipMemsetAsync(ws.ctrl.p, 0, sizeof(Ctrl), st));
@@ -4109,12 +4564,15 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     ws.rows_dirty = true;
     RunDiag diag;
     Engine eng{g, L, ws, o, diag, st, n, slots, ws.list.p, k};
+    eng.ring = ring;
     std::vector<double> rows((size_t)slots * n * kLanes);
     std::vector<uint32_t> drows;
     std::vector<uint64_t> vis((size_t)slots * n);
     for (int64_t item0 = 0; item0 < k; item0 += (int64_t)slots * kLanes) {
         for (;;) {
-            bool ok = L.kind == kUniform ? eng.run_bfs(item0) : eng.run_bucketed<IntPath>(item0);
+            bool ok = L.kind == kUniform ? eng.run_bfs(item0)
+                      : ring               ? eng.run_ring(item0)
+                                           : eng.run_bucketed<IntPath>(item0);
             if (ok) break;
             eng.clear_rows();
             grow_queue(ws);
@@ -4122,7 +4580,7 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
         int64_t cnt = std::min<int64_t>((int64_t)slots * kLanes, k - item0);
         NETBC_HIP(hipMemcpyAsync(rows.data(), ws.sig.p, rows.size() * 8, hipMemcpyDeviceToHost, st));
         NETBC_HIP(hipMemcpyAsync(vis.data(), ws.vis.p, vis.size() * 8, hipMemcpyDeviceToHost, st));
-        if (L.kind == kInteger) {
+        if (L.kind == kInteger && !ring) {
             drows.resize((size_t)slots * n * kLanes);
             NETBC_HIP(hipMemcpyAsync(drows.data(), ws.dist.p, drows.size() * 4,
                                      hipMemcpyDeviceToHost, st));
@@ -4139,11 +4597,12 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                 bool r = (vis[(size_t)b * n + v] >> l) & 1;
                 sigma[o2 + v] = r ? rows[((size_t)b * n + v) * kLanes + l] : 0.0;
                 dist[o2 + v] = UINT64_MAX;
-                if (r && L.kind == kInteger) dist[o2 + v] = drows[((size_t)b * n + v) * kLanes + l];
+                if (r && L.kind == kInteger && !ring)
+                    dist[o2 + v] = drows[((size_t)b * n + v) * kLanes + l];
                 delta[o2 + v] = 0.0;
             }
         }
-        if (L.kind == kUniform) {
+        if (L.kind == kUniform || ring) {  // the level of an entry is its distance
             for (size_t dl = 0; dl + 1 < eng.lv.size(); dl++)
                 for (uint32_t i = eng.lv[dl]; i < eng.lv[dl + 1]; i++) {
                     const QEnt& e = q[i];

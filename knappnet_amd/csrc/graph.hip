@@ -453,7 +453,7 @@ void build_device_graph(Graph& g) {
     std::vector<uint8_t> flag;
     std::vector<uint32_t> hub_list;
     std::vector<HubTask> cut;
-    cut_hub_tasks(ioff.data(), g.n, kHubIn, kHubChunk, flag, hub_list, cut);
+    d.max_in = cut_hub_tasks(ioff.data(), g.n, kHubIn, kHubChunk, flag, hub_list, cut);
     std::vector<uint4> task(cut.size());
     for (size_t i = 0; i < cut.size(); i++)
         task[i] = make_uint4(cut[i].hub, cut[i].first, cut[i].end, 0);

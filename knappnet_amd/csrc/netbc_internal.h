@@ -126,6 +126,7 @@ struct DeviceGraph {
     DevBuf<uint8_t> heavy_flag;        // per node
     uint32_t heavy = 0, max_chunks = 1;
     uint64_t max_out = 0;              // largest out-degree
+    uint64_t max_in = 0;               // largest in-degree
     // Landmarks for batching sources: the nodes of largest in-degree
     // (ties by id), at most kLandmarks of them.
     std::vector<uint32_t> landmarks;

@@ -22,20 +22,24 @@ struct HubTask {
 
 // Nodes with more than `hub_in` in-arcs (flag[v] = 1, listed ascending in
 // `hubs`) and their in-rows cut into tasks of `chunk` arcs, in node order.
-inline void cut_hub_tasks(const uint32_t* in_off, uint64_t n, uint32_t hub_in, uint32_t chunk,
+// Returns the largest in-degree.
+inline uint64_t cut_hub_tasks(const uint32_t* in_off, uint64_t n, uint32_t hub_in, uint32_t chunk,
                           std::vector<uint8_t>& flag, std::vector<uint32_t>& hubs,
                           std::vector<HubTask>& tasks) {
     flag.assign(std::max<uint64_t>(n, 1), 0);
     hubs.clear();
     tasks.clear();
+    uint64_t maxdeg = 0;
     for (uint64_t v = 0; v < n; v++) {
         const uint64_t a0 = in_off[v], a1 = in_off[v + 1];
+        maxdeg = std::max(maxdeg, a1 - a0);
         if (a1 - a0 <= hub_in) continue;
         flag[v] = 1;
         hubs.push_back((uint32_t)v);
         for (uint64_t a = a0; a < a1; a += chunk)
             tasks.push_back(HubTask{(uint32_t)v, (uint32_t)a, (uint32_t)std::min(a + chunk, a1)});
     }
+    return maxdeg;
 }
 
 // Heavy out-rows (more than `heavy_out` arcs: flag[v] = 1); returns the
