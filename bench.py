@@ -14,8 +14,14 @@ The printed JSON line carries:
   level; 32 B per reached node: row offsets, sigma read, coefficient and
   dependency writes; summed over sources);
 - ``cpu_baseline``: the CPU oracle (the reference's algorithm restated in
-  C, all host cores) timed on a strided sample of the same sources, and a
-  bit-exact check of the device result on that sample.
+  C, all host cores) timed on a strided sample of the same sources, its
+  time extrapolated to every source by reached arcs, and a check of the
+  device result on that sample;
+- ``configs``: one timed run of each other workload (G(2000, 10000) from
+  seed 0; Barabasi-Albert n = 200,000 with integer weights 1-16; 4,096
+  Watts-Strogatz graphs in one call; undirected R-MAT scale 22 with 8,192
+  sources from seed 7, FP64 and FP32 dependencies), each with its own
+  roofline, CPU baseline and check (``--no-configs`` skips them).
 
 ``--edge`` times edge betweenness (``netbc_edge_betweenness_sources``)
 on the same workload instead; its roofline keeps the node sweep's byte
@@ -112,17 +118,19 @@ def run_ws_batch(args) -> None:
     cpu = None
     if args.cpu_seconds > 0:
         from oracle.oracle import OracleGraph, available_cores
-        sel = els[:: max(1, len(els) // 64)]
+        step = max(1, len(els) // 64)
+        sel = list(range(0, len(els), step))
         t = time.perf_counter()
-        ok = True
-        for e in sel:
-            r = OracleGraph(e.n, e.src, e.tgt, None, False).betweenness(threads=0)
+        want = [OracleGraph(els[i].n, els[i].src, els[i].tgt, None, False).betweenness(threads=0)
+                for i in sel]
         ct = time.perf_counter() - t
-        cw = sum(e.n * e.m for e in sel)
+        differ = [i for i, w in zip(sel, want) if not np.array_equal(parts[i], w)]
+        cw = sum(els[i].n * els[i].m for i in sel)
         cpu = {"value": round(cw / ct / 1e9, 4), "unit": "GTEPS", "cores": available_cores(),
                "kind": "port", "seconds": round(ct, 3),
-               "sample": f"{len(sel)} of 4096 graphs (every 64th), one at a time, all host "
-                         f"threads, CPU: {cpu_model()}"}
+               "sample": f"{len(sel)} of 4096 graphs (every {step}th), one at a time, all host "
+                         f"threads, CPU: {cpu_model()}",
+               "check": "bit-exact" if not differ else f"{len(differ)} graphs differ"}
     line = {
         "metric": METRIC, "value": round(work / sec / 1e9, 3), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),
@@ -136,6 +144,182 @@ def run_ws_batch(args) -> None:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def sweep_roofline(kernel: str, diag: dict, nbytes: float) -> dict:
+    """roofline of a run's timed kernel: `nbytes` algorithmic bytes over the
+    summed durations of its launches (HIP events on the engine stream)."""
+    sec = diag["dominant_ms"] / 1e3
+    achieved = nbytes / sec / 1e9 if sec > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None, "launches": diag["dominant_launches"],
+            "avg_launch_us": round(diag["dominant_ms"] * 1e3 / max(diag["dominant_launches"], 1),
+                                   2),
+            "algorithmic_bytes_per_run": int(nbytes)}
+
+
+def cpu_leg(orun, drun, pool, m: int, k: int, full_arcs: int, target_s: float):
+    """The oracle on a strided sample of `pool` (grown until it takes about
+    target_s / 4), checked against the device on the same sample; the full
+    run's CPU time is extrapolated by reached arcs (the device counts them
+    for the sample and for the full run)."""
+    from oracle.oracle import available_cores
+    take = min(len(pool), 16)
+    while True:
+        stride = max(1, len(pool) // take)
+        sample = np.asarray(pool)[::stride][:take]
+        t = time.perf_counter()
+        want = orun(sample)
+        ct = time.perf_counter() - t
+        if ct >= target_s / 4 or take >= len(pool):
+            break
+        take = min(len(pool), int(take * max(2.0, target_s / 2 / max(ct, 1e-3))))
+    got, d = drun(sample)
+    exact = bool(np.array_equal(got, want))
+    nz = (got != 0) | (want != 0)
+    rel = float(np.max(np.abs(got - want)[nz] / np.maximum(np.abs(want[nz]), 1e-300))) \
+        if nz.any() else 0.0
+    full_s = ct * full_arcs / max(d["reached_arcs"], 1)
+    return {"value": round(k * m / full_s / 1e9, 4), "unit": "GTEPS",
+            "cores": available_cores(), "kind": "port", "seconds": round(ct, 3),
+            "sample": f"{len(sample)} of {k} sources (every {stride}th), all host threads, "
+                      f"CPU time extrapolated to every source by reached arcs, CPU: "
+                      f"{cpu_model()}",
+            "check": "bit-exact" if exact else f"max rel err {rel:.3e}"}
+
+
+def extra_configs(cpu_seconds: float) -> dict:
+    """One timed run of each other workload, with its own roofline, CPU
+    baseline and parity check: G(2000, 10000) from seed 0 (all sources),
+    Barabasi-Albert n = 200,000 with integer weights 1-16 (all sources),
+    4,096 Watts-Strogatz graphs in one call, and undirected R-MAT scale 22
+    with the 8,192 sources drawn from seed 7 (FP64, and FP32 dependencies
+    against it)."""
+    from knappnet_amd import generate as gen
+    from knappnet_amd.engine import Graph, betweenness_batch
+    from oracle.oracle import OracleGraph, available_cores
+    out = {}
+    target = max(cpu_seconds / 3, 1.0) if cpu_seconds > 0 else 0.0
+
+    def med(run, steps):
+        ts, last = [], None
+        for _ in range(steps):
+            t = time.perf_counter()
+            last = run()
+            ts.append(time.perf_counter() - t)
+        return last, float(np.median(ts))
+
+    # G(2000, 10000), every node a source (the oracle runs it whole)
+    el = gen.er_2000()
+    g = Graph(el.n, el.src, el.tgt, None, False)
+    g.betweenness_sources()
+    (res, d), sec = med(lambda: g.betweenness_sources(), 7)
+    cpu = None
+    if target:
+        t = time.perf_counter()
+        want = OracleGraph(el.n, el.src, el.tgt, None, False).betweenness(threads=0)
+        ct = time.perf_counter() - t
+        cpu = {"value": round(el.n * el.m / ct / 1e9, 4), "unit": "GTEPS",
+               "cores": available_cores(), "kind": "port", "seconds": round(ct, 4),
+               "sample": f"all {el.n} sources, all host threads, CPU: {cpu_model()}",
+               "check": "bit-exact" if np.array_equal(res, want) else "differs"}
+    out["er_gnm_2000_10000_all_sources"] = {
+        "value": round(el.n * el.m / sec / 1e9, 3), "unit": "GTEPS", "seconds": round(sec, 5),
+        "steps": 7, "dtype": "f64", "n": el.n, "m": el.m, "sources": el.n,
+        "roofline": sweep_roofline("bfs_backward_kernel<double, false, false>", d,
+                                   8 * d["reached_arcs"] + 32 * d["reached_nodes"]),
+        "cpu_baseline": cpu}
+    print(f"# config er2000: {sec * 1e3:.2f} ms", flush=True)
+
+    # Barabasi-Albert n = 200,000, integer weights 1-16, every node a source
+    el = gen.ba_weighted()
+    g = Graph(el.n, el.src, el.tgt, el.weights, False)
+    g.betweenness_sources(np.arange(64), weighted=True)
+    (res, d), sec = med(lambda: g.betweenness_sources(weighted=True), 1)
+    cpu = None
+    if target:
+        o = OracleGraph(el.n, el.src, el.tgt, el.weights, False)
+        cpu = cpu_leg(lambda smp: o.betweenness_sources(smp, weighted=True, threads=0),
+                      lambda smp: g.betweenness_sources(smp, weighted=True),
+                      np.arange(el.n), el.m, el.n, d["reached_arcs"], target)
+    out["ba_200000_m4_int_weights_1_16_all_sources"] = {
+        "value": round(el.n * el.m / sec / 1e9, 3), "unit": "GTEPS", "seconds": round(sec, 3),
+        "steps": 1, "dtype": "f64", "n": el.n, "m": el.m, "sources": el.n,
+        "levels": d["levels"],
+        "roofline": sweep_roofline("dial_backward_kernel<double, false, IntPath, false>", d,
+                                   12 * d["reached_arcs"] + 32 * d["reached_nodes"]),
+        "cpu_baseline": cpu}
+    print(f"# config ba200k: {sec:.2f} s", flush=True)
+
+    # 4,096 Watts-Strogatz graphs in one call
+    els = gen.ws_batch(4096, 11)
+    graphs = [(e.n, e.src, e.tgt) for e in els]
+    work = sum(e.n * e.m for e in els)
+    betweenness_batch(graphs)
+    (res, d), sec = med(lambda: betweenness_batch(graphs), 2)
+    parts = res
+    cpu = None
+    if target:
+        step = 64
+        sel = list(range(0, len(els), step))
+        t = time.perf_counter()
+        want = [OracleGraph(els[i].n, els[i].src, els[i].tgt, None, False).betweenness(threads=0)
+                for i in sel]
+        ct = time.perf_counter() - t
+        differ = sum(not np.array_equal(parts[i], w) for i, w in zip(sel, want))
+        cw = sum(els[i].n * els[i].m for i in sel)
+        cpu = {"value": round(cw / ct / 1e9, 4), "unit": "GTEPS", "cores": available_cores(),
+               "kind": "port", "seconds": round(ct, 3),
+               "sample": f"{len(sel)} of 4096 graphs (every {step}th), one at a time, all "
+                         f"host threads, CPU: {cpu_model()}",
+               "check": "bit-exact" if not differ else f"{differ} graphs differ"}
+    k1 = 128
+    t = time.perf_counter()
+    one = [Graph(e.n, e.src, e.tgt, None, False).betweenness() for e in els[:k1]]
+    per_graph_s = (time.perf_counter() - t) / sum(e.n * e.m for e in els[:k1]) * work
+    out["watts_strogatz_4096_graphs_all_sources"] = {
+        "value": round(work / sec / 1e9, 3), "unit": "GTEPS", "seconds": round(sec, 3),
+        "steps": 2, "dtype": "f64", "graphs_per_s": round(4096 / sec, 1),
+        "one_graph_per_launch": {"seconds_extrapolated": round(per_graph_s, 3),
+                                 "timed_graphs": k1, "speedup": round(per_graph_s / sec, 2),
+                                 "bit_identical": all(np.array_equal(a, b)
+                                                      for a, b in zip(one, parts[:k1]))},
+        "roofline": sweep_roofline("small_batch_kernel<double>", d,
+                                   16 * d["reached_arcs"] + 60 * d["reached_nodes"]),
+        "cpu_baseline": cpu}
+    print(f"# config ws4096: {sec:.3f} s", flush=True)
+
+    # undirected R-MAT scale 22, the 8,192 sources drawn from seed 7
+    el = gen.rmat(22, 16, directed=False, seed=1)
+    src = gen.sample_sources(el.n, 8192, 7)
+    g = Graph(el.n, el.src, el.tgt, None, False)
+    g.betweenness_sources(src[:64])
+    (r64, d), sec = med(lambda: g.betweenness_sources(src), 1)
+    (r32, d32), sec32 = med(lambda: g.betweenness_sources(src, precision=1), 1)
+    nz = r64 != 0
+    err = np.abs(r32[nz] - r64[nz]) / np.abs(r64[nz])
+    cpu = None
+    if target:
+        o = OracleGraph(el.n, el.src, el.tgt, None, False)
+        cpu = cpu_leg(lambda smp: o.betweenness_sources(smp, threads=0),
+                      lambda smp: g.betweenness_sources(smp), src, el.m, len(src),
+                      d["reached_arcs"], target)
+        del o
+    out["rmat22_undirected_8192_sampled_sources"] = {
+        "value": round(len(src) * el.m / sec / 1e9, 3), "unit": "GTEPS",
+        "seconds": round(sec, 3), "steps": 1, "dtype": "f64", "n": el.n, "m": el.m,
+        "sources": len(src),
+        "fp32_delta": {"seconds": round(sec32, 3),
+                       "gteps": round(len(src) * el.m / sec32 / 1e9, 3),
+                       "speedup_vs_fp64": round(sec / sec32, 3),
+                       "max_rel_err_vs_fp64": float(err.max()) if nz.any() else 0.0,
+                       "mean_rel_err_vs_fp64": float(err.mean()) if nz.any() else 0.0},
+        "roofline": sweep_roofline("bfs_backward_kernel<double, false, false>", d,
+                                   8 * d["reached_arcs"] + 32 * d["reached_nodes"]),
+        "cpu_baseline": cpu}
+    print(f"# config rmat22: {sec:.3f} s (FP32 dependencies {sec32:.3f} s)", flush=True)
+    return out
 
 
 def calibration_copy(nbytes: int = 1 << 30) -> None:
@@ -204,6 +388,8 @@ def main() -> None:
                     help="betweenness for sources and targets (every third node a target)")
     ap.add_argument("--closeness", action="store_true",
                     help="closeness of every node (net_closeness, normalized) instead")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the timed runs of the other workloads (headline only)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -310,6 +496,11 @@ def main() -> None:
     if tr:
         roofline["traffic"] = tr.get("bytes_per_launch")
         roofline["traffic_source"] = tr.get("source")
+        # the same launches priced by their measured HBM traffic
+        if roofline["traffic"] and d["dominant_ms"] > 0:
+            meas = roofline["traffic"] * d["dominant_launches"] / (d["dominant_ms"] / 1e3) / 1e9
+            roofline["achieved_measured"] = round(meas, 2)
+            roofline["frac_measured"] = round(meas / HBM_PEAK_GBS, 5)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -341,20 +532,29 @@ def main() -> None:
         t = time.perf_counter()
         orun(sample, weighted=weighted, threads=0)
         ct = min(ct, time.perf_counter() - t)
+        arcs_scale = None  # CPU time per reached arc, from the device's counts
         if args.closeness:
             got = np.nan_to_num(res[sample], nan=-1.0)
             want = np.nan_to_num(want, nan=-1.0)
         else:
-            got, _ = run(sample, weighted=weighted, precision=args.precision)
+            got, dsample = run(sample, weighted=weighted, precision=args.precision)
+            if dsample["reached_arcs"] > 0:
+                arcs_scale = d["reached_arcs"] / dsample["reached_arcs"]
         exact = bool(np.array_equal(got, want))
         nz = (got != 0) | (want != 0)
         rel = float(np.max(np.abs(got - want)[nz] / np.maximum(np.abs(want[nz]), 1e-300))) \
             if nz.any() else 0.0
-        cpu_gteps = len(sample) * el.m / ct / 1e9
+        if arcs_scale:
+            # the whole run's CPU time, scaled by reached arcs (sources differ in reach)
+            cpu_gteps = k * el.m / (ct * arcs_scale) / 1e9
+            how = "CPU time extrapolated to every source by reached arcs"
+        else:
+            cpu_gteps = len(sample) * el.m / ct / 1e9
+            how = "rate per source"
         cpu = {"value": round(cpu_gteps, 4), "unit": "GTEPS", "cores": cores,
                "kind": "port", "seconds": round(ct, 3),
                "sample": f"{len(sample)} of {k} sources (every {stride}th), {cores} host "
-                         f"thread{'s' if cores > 1 else ''}, CPU: {cpu_model()}",
+                         f"thread{'s' if cores > 1 else ''}, {how}, CPU: {cpu_model()}",
                "check": "bit-exact" if exact else f"max rel err {rel:.3e}"}
         print(f"# cpu sample: {len(sample)} sources in {ct:.2f}s -> {cpu_gteps:.3f} GTEPS; "
               f"device vs oracle on sample: {cpu['check']}", flush=True)
@@ -412,6 +612,9 @@ def main() -> None:
         "sigma_overflow": d["sigma_overflow"],
         "search": {"levels": d["levels"], "bottom_up_steps": d["bottom_up_levels"]},
     }
+    if (args.workload == "rmat18" and not (args.edge or args.subset or args.closeness)
+            and args.precision == 0 and not args.max_sources and not args.no_configs):
+        line["configs"] = extra_configs(args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 
