@@ -84,6 +84,75 @@ def build_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def med(run, steps):
+    """The last result of `steps` runs and their median wall time (s)."""
+    ts, last = [], None
+    for _ in range(steps):
+        t = time.perf_counter()
+        last = run()
+        ts.append(time.perf_counter() - t)
+    return last, float(np.median(ts))
+
+
+def plan_legs(els, graphs, parts) -> dict:
+    """Repeated batches through netbc_batch_plan_*: the 4,096-graph batch
+    replayed from a captured HIP graph and launched kernel by kernel
+    (seconds per batch, bit-identity with the batch entry's scores), and
+    the conditional-uniform-graph pattern of one call per replicate
+    (r/R/cug.R:42-47): 64 Watts-Strogatz replicates of one shape (n = 1000,
+    k = 6, so m = 3000) through the batch entry one graph per call, a
+    one-graph plan captured and uncaptured, and all 64 in one captured
+    plan (graphs per second each)."""
+    from knappnet_amd import generate as gen
+    from knappnet_amd.engine import BatchPlan, betweenness_batch
+    out = {}
+    ns = [e.n for e in els]
+    ms = [e.m for e in els]
+    for capture in (True, False):
+        t = time.perf_counter()
+        plan = BatchPlan(ns, ms, capture=capture)
+        made = time.perf_counter() - t
+        got, _ = plan.run(graphs)
+        (res, _), sec = med(lambda: plan.run(graphs), 2)
+        same = all(np.array_equal(a, b) for a, b in zip(res, parts)) and \
+            all(np.array_equal(a, b) for a, b in zip(got, parts))
+        out["captured" if capture else "uncaptured"] = {
+            "seconds": round(sec, 3), "graphs_per_s": round(len(els) / sec, 1),
+            "plan_setup_s": round(made, 3), "bit_identical_to_batch_call": same}
+        plan.close()
+    reps = [gen.small_world(1000, 6, 0.1, 5000 + i, 0) for i in range(64)]
+    rg = [(e.n, e.src, e.tgt) for e in reps]
+    want, _ = betweenness_batch(rg)
+    legs = {}
+    t = time.perf_counter()
+    one = [betweenness_batch([g])[0][0] for g in rg]
+    legs["batch_call_per_replicate"] = time.perf_counter() - t
+    ok = all(np.array_equal(a, b) for a, b in zip(one, want))
+    for capture in (True, False):
+        plan = BatchPlan([1000], [3000], capture=capture)
+        plan.run([rg[0]])
+        t = time.perf_counter()
+        one = [plan.run([g])[0][0] for g in rg]
+        legs[("captured" if capture else "uncaptured") + "_plan_per_replicate"] = \
+            time.perf_counter() - t
+        ok = ok and all(np.array_equal(a, b) for a, b in zip(one, want))
+        plan.close()
+    plan = BatchPlan([1000] * 64, [3000] * 64)
+    plan.run(rg)
+    t = time.perf_counter()
+    res, _ = plan.run(rg)
+    legs["captured_plan_all_64"] = time.perf_counter() - t
+    ok = ok and all(np.array_equal(a, b) for a, b in zip(res, want))
+    plan.close()
+    out["replicates_n1000_m3000"] = {
+        **{k: {"seconds": round(v, 4), "graphs_per_s": round(64 / v, 1)} for k, v in legs.items()},
+        "bit_identical": ok}
+    print("# plans: " + ", ".join(f"{k} {v['graphs_per_s']}/s" for k, v in
+                                  out["replicates_n1000_m3000"].items() if isinstance(v, dict)),
+          flush=True)
+    return out
+
+
 def run_ws_batch(args) -> None:
     """4,096 Watts-Strogatz graphs (n uniform in [50, 2000], k = 6, p = 0.1)
     in one netbc_betweenness_batch call, against one net_betweenness call
@@ -141,6 +210,7 @@ def run_ws_batch(args) -> None:
         "one_graph_per_launch": {"seconds_extrapolated": round(per_graph_s, 3),
                                  "timed_graphs": k1, "speedup": round(per_graph_s / sec, 2),
                                  "bit_identical": same},
+        "plans": plan_legs(els, graphs, parts),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -201,14 +271,6 @@ def extra_configs(cpu_seconds: float) -> dict:
     from oracle.oracle import OracleGraph, available_cores
     out = {}
     target = max(cpu_seconds / 3, 1.0) if cpu_seconds > 0 else 0.0
-
-    def med(run, steps):
-        ts, last = [], None
-        for _ in range(steps):
-            t = time.perf_counter()
-            last = run()
-            ts.append(time.perf_counter() - t)
-        return last, float(np.median(ts))
 
     # G(2000, 10000), every node a source (the oracle runs it whole)
     el = gen.er_2000()
@@ -285,6 +347,7 @@ def extra_configs(cpu_seconds: float) -> dict:
                                  "timed_graphs": k1, "speedup": round(per_graph_s / sec, 2),
                                  "bit_identical": all(np.array_equal(a, b)
                                                       for a, b in zip(one, parts[:k1]))},
+        "plans": plan_legs(els, graphs, parts),
         "roofline": sweep_roofline("small_batch_kernel<double>", d,
                                    16 * d["reached_arcs"] + 60 * d["reached_nodes"]),
         "cpu_baseline": cpu}

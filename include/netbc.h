@@ -258,6 +258,38 @@ int32_t netbc_betweenness_batch(int64_t graphs, const int64_t *node_counts,
                                 netbc_diagnostics *diag, char *err,
                                 size_t err_len);
 
+/* A plan for repeated netbc_betweenness_batch calls on one batch shape:
+ * the same graph count, node counts and edge counts every time, as the
+ * replicate batches of a conditional uniform graph test have (r/R/cug.R:
+ * 25-47 draws `reps` graphs of the observed size and edge count per call).
+ * netbc_batch_plan_new allocates every device buffer the shape needs and,
+ * with capture != 0, records the whole device side of a call (building
+ * the union graph's CSR from its edges, the persistent launch over the
+ * small graphs and the ordered reduction) into a HIP graph; with
+ * capture == 0 the same kernels are launched one by one on every run.
+ * Unweighted graphs only; `opt` may be NULL. No counterpart in netcapi.h. */
+typedef struct netbc_batch_plan netbc_batch_plan;
+int32_t netbc_batch_plan_new(int64_t graphs, const int64_t *node_counts,
+                             const int64_t *edge_counts, int32_t directed,
+                             int32_t normalized, int32_t capture,
+                             const netbc_options *opt, netbc_batch_plan **plan,
+                             char *err, size_t err_len);
+
+/* One batch through a plan: src/tgt hold the edges of every graph in the
+ * layout netbc_betweenness_batch takes (the plan's edge counts, node ids
+ * local to each graph); `out` receives the scores, equal bit for bit to
+ * netbc_betweenness_batch's on the same graphs (weighted = 0). Graphs the
+ * persistent launch does not take (more than 2048 nodes, searches past 254
+ * levels) are computed one at a time after the replay. Calls on one plan
+ * are serialised; `diag` may be NULL. */
+int32_t netbc_batch_plan_run(netbc_batch_plan *plan, const int64_t *src,
+                             const int64_t *tgt, double *out,
+                             netbc_diagnostics *diag, char *err,
+                             size_t err_len);
+
+/* Releases a plan (NULL is a no-op). */
+void netbc_batch_plan_free(netbc_batch_plan *plan);
+
 /* netbc_betweenness_batch without device allocation. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

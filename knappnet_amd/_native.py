@@ -116,6 +116,16 @@ def device() -> ctypes.CDLL:
             c_int64, POINTER(c_int64), c_int32, c_int32, c_void_p, c_void_p, c_void_p, c_void_p,
             c_int64, c_void_p, c_void_p, c_char_p, c_size_t]
         lib.netbc_betweenness_batch_noalloc.restype = c_int32
+        lib.netbc_batch_plan_new.argtypes = [
+            c_int64, POINTER(c_int64), POINTER(c_int64), c_int32, c_int32, c_int32,
+            POINTER(Options), POINTER(c_void_p), c_char_p, c_size_t]
+        lib.netbc_batch_plan_new.restype = c_int32
+        lib.netbc_batch_plan_run.argtypes = [
+            c_void_p, POINTER(c_int64), POINTER(c_int64), POINTER(c_double),
+            POINTER(Diagnostics), c_char_p, c_size_t]
+        lib.netbc_batch_plan_run.restype = c_int32
+        lib.netbc_batch_plan_free.argtypes = [c_void_p]
+        lib.netbc_batch_plan_free.restype = None
         lib.netbc_state_dump.argtypes = [c_void_p, c_int32, c_int32, c_int32, POINTER(c_int64),
                                          c_int64, POINTER(c_uint64), POINTER(c_double),
                                          POINTER(c_double), c_char_p, c_size_t]

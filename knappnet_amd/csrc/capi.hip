@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "../../include/netbc.h"
@@ -1028,6 +1029,70 @@ NETBC_EXPORT int32_t netbc_device_info(char* buf, size_t len) {
     });
 }
 
+namespace {
+
+// The one-workgroup engine's layout of a batch (small.h): a descriptor per
+// graph (source-group size and count as parallel.rs:437-440, scale as
+// betweenness.rs:100-113, group sums one graph after another) and the
+// graph of every union node. Returns whether the engine takes any graph
+// (unweighted, 1..2048 nodes).
+bool small_layout(int64_t graphs, const int64_t* node_counts, const std::vector<uint64_t>& node0,
+                  bool directed, bool normalized, bool weighted,
+                  std::vector<netbc::SmallGraph>& descs, std::vector<uint32_t>& node_graph) {
+    descs.assign((size_t)graphs, netbc::SmallGraph{});
+    node_graph.assign(node0[graphs], 0);
+    uint64_t gsum_off = 0;
+    bool any_device = false;
+    for (int64_t i = 0; i < graphs; i++) {
+        netbc::SmallGraph& G = descs[i];
+        uint64_t n = (uint64_t)node_counts[i];
+        G.n = (uint32_t)n;
+        G.node0 = (uint32_t)node0[i];
+        G.gs = n ? (uint32_t)((n + 63) / 64) : 1;
+        G.groups = n ? (uint32_t)((n + G.gs - 1) / G.gs) : 0;
+        G.scale = graph_scale(n, directed, normalized);
+        G.device = (!weighted && n > 0 && n <= netbc::kSmallMaxNodes) ? 1 : 0;
+        G.gsum_off = gsum_off;
+        G.pad = 0;
+        if (G.device) {
+            gsum_off += (uint64_t)G.groups * n;
+            any_device = true;
+        } else {
+            G.groups = 0;
+        }
+        for (uint64_t v = 0; v < n; v++) node_graph[node0[i] + v] = (uint32_t)i;
+    }
+    return any_device;
+}
+
+// Graphs the one-workgroup engine did not take (or whose search went too
+// deep): the batched engine, one graph at a time.
+void run_left_over(int64_t graphs, const int64_t* node_counts, const int64_t* edge_counts,
+                   const std::vector<uint64_t>& node0, const std::vector<uint64_t>& edge0,
+                   const int64_t* src, const int64_t* tgt, const double* weights,
+                   int32_t directed, int32_t weighted, int32_t weights_as, int32_t normalized,
+                   const netbc_options* opt, const std::vector<netbc::SmallGraph>& descs,
+                   const std::vector<uint32_t>& too_deep, double* out, netbc::RunDiag& total) {
+    for (int64_t i = 0; i < graphs; i++) {
+        if (descs[i].device && !too_deep[i]) continue;
+        if (node_counts[i] == 0) continue;
+        netbc::Graph one;
+        uint64_t e0 = edge0[i];
+        fill_graph(one, node_counts[i], src + e0, tgt + e0, weights ? weights + e0 : nullptr,
+                   edge_counts[i], directed != 0, "");
+        netbc::RunDiag d;
+        try {
+            d = run_graph(one, nullptr, node_counts[i], weighted, weights_as, normalized, opt,
+                          out + node0[i], nullptr);
+        } catch (const netbc::Error& e) {
+            throw netbc::Error("graph " + std::to_string(i) + ": " + e.what());
+        }
+        merge_diag(total, d);
+    }
+}
+
+}  // namespace
+
 NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node_counts,
                                              const int64_t* edge_counts, const int64_t* src,
                                              const int64_t* tgt, const double* weights,
@@ -1075,52 +1140,18 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
         }
         netbc::RunDiag total;
         netbc::RunOptions o = options(opt);
-        std::vector<netbc::SmallGraph> descs((size_t)graphs);
-        std::vector<uint32_t> node_graph(U.n);
-        uint64_t gsum_off = 0;
-        bool any_device = false;
-        for (int64_t i = 0; i < graphs; i++) {
-            netbc::SmallGraph& G = descs[i];
-            uint64_t n = (uint64_t)node_counts[i];
-            G.n = (uint32_t)n;
-            G.node0 = (uint32_t)node0[i];
-            G.gs = n ? (uint32_t)((n + 63) / 64) : 1;
-            G.groups = n ? (uint32_t)((n + G.gs - 1) / G.gs) : 0;
-            G.scale = graph_scale(n, directed != 0, normalized != 0);
-            G.device = (!weighted && n > 0 && n <= netbc::kSmallMaxNodes) ? 1 : 0;
-            G.gsum_off = gsum_off;
-            G.pad = 0;
-            if (G.device) {
-                gsum_off += (uint64_t)G.groups * n;
-                any_device = true;
-            } else {
-                G.groups = 0;
-            }
-            for (uint64_t v = 0; v < n; v++) node_graph[node0[i] + v] = (uint32_t)i;
-        }
+        std::vector<netbc::SmallGraph> descs;
+        std::vector<uint32_t> node_graph;
+        const bool any_device = small_layout(graphs, node_counts, node0, directed != 0,
+                                             normalized != 0, weighted != 0, descs, node_graph);
         std::vector<uint32_t> too_deep((size_t)graphs, 0);
         if (any_device) {
             prepare(U);
             netbc::run_small_batch(U, descs, node_graph, o, out, too_deep, total);
         }
-        // Graphs the one-workgroup engine does not take: the batched engine,
-        // one graph at a time.
-        for (int64_t i = 0; i < graphs; i++) {
-            if (descs[i].device && !too_deep[i]) continue;
-            if (node_counts[i] == 0) continue;
-            netbc::Graph one;
-            uint64_t e0 = edge0[i];
-            fill_graph(one, node_counts[i], src + e0, tgt + e0, weights ? weights + e0 : nullptr,
-                       edge_counts[i], directed != 0, "");
-            netbc::RunDiag d;
-            try {
-                d = run_graph(one, nullptr, node_counts[i], weighted, weights_as, normalized, opt,
-                              out + node0[i], nullptr);
-            } catch (const netbc::Error& e) {
-                throw netbc::Error("graph " + std::to_string(i) + ": " + e.what());
-            }
-            merge_diag(total, d);
-        }
+        run_left_over(graphs, node_counts, edge_counts, node0, edge0, src, tgt, weights,
+                      directed, weighted, weights_as, normalized, opt, descs, too_deep, out,
+                      total);
         fill_diag(diag, total, 0.0);
     });
 }
@@ -1138,6 +1169,198 @@ NETBC_EXPORT int32_t netbc_betweenness_batch_noalloc(
                                        static_cast<hipStream_t>(stream), d);
     });
 }
+
+// ---- batch plans: one batch shape, replayed ------------------------------
+
+struct netbc_batch_plan {
+    int device = 0;
+    hipStream_t st = nullptr;
+    bool directed = false, normalized = false, captured = false, any_device = false;
+    int64_t graphs = 0;
+    bool has_opt = false;
+    netbc_options opt{};
+    std::vector<int64_t> node_counts, edge_counts;
+    std::vector<uint64_t> node0, edge0;
+    std::vector<netbc::SmallGraph> descs;
+    netbc::FixedCsr csr;
+    netbc::SmallRun run;
+    uint32_t* h_src = nullptr;  // pinned staging of the union's edges
+    uint32_t* h_tgt = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::mutex mu;
+
+    ~netbc_batch_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (h_src) (void)hipHostFree(h_src);
+        if (h_tgt) (void)hipHostFree(h_tgt);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    const netbc_options* options_p() const { return has_opt ? &opt : nullptr; }
+    // the device side of one run: the union CSR from the staged edges, then
+    // the persistent launch and the reduction
+    void enqueue() {
+        netbc::fixed_csr_enqueue(csr, st);
+        netbc::small_enqueue(run, csr.out_off.p, csr.out_idx.p, csr.in_off_p(), csr.in_idx_p(),
+                             st);
+    }
+};
+
+namespace {
+// Makes `dev` current for the scope and restores the caller's device.
+struct DeviceScope {
+    int prev = -1;
+    explicit DeviceScope(int dev) {
+        NETBC_HIP(hipGetDevice(&prev));
+        if (prev != dev) NETBC_HIP(hipSetDevice(dev));
+    }
+    ~DeviceScope() {
+        int cur = -1;
+        if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+}  // namespace
+
+NETBC_EXPORT int32_t netbc_batch_plan_new(int64_t graphs, const int64_t* node_counts,
+                                          const int64_t* edge_counts, int32_t directed,
+                                          int32_t normalized, int32_t capture,
+                                          const netbc_options* opt, netbc_batch_plan** plan,
+                                          char* err, size_t err_len) {
+    return guard(err, err_len, [&] {
+        if (!plan) throw netbc::Error("null plan pointer");
+        *plan = nullptr;
+        ensure_device();
+        if (graphs < 0) throw netbc::Error("negative graph count");
+        if (graphs > 0 && (!node_counts || !edge_counts))
+            throw netbc::Error("null node or edge counts");
+        check_options(opt);
+        auto P = std::make_unique<netbc_batch_plan>();
+        NETBC_HIP(hipGetDevice(&P->device));
+        P->graphs = graphs;
+        P->directed = directed != 0;
+        P->normalized = normalized != 0;
+        if (opt) {
+            P->has_opt = true;
+            P->opt = *opt;
+        }
+        P->node_counts.assign(node_counts, node_counts + graphs);
+        P->edge_counts.assign(edge_counts, edge_counts + graphs);
+        P->node0.assign((size_t)graphs + 1, 0);
+        P->edge0.assign((size_t)graphs + 1, 0);
+        for (int64_t i = 0; i < graphs; i++) {
+            const std::string prefix = "graph " + std::to_string(i) + ": ";
+            if (node_counts[i] < 0 || edge_counts[i] < 0)
+                throw netbc::Error(prefix + "negative node or edge count");
+            if ((uint64_t)node_counts[i] > 0xFFFFFFFFull)
+                throw netbc::Error(prefix + "node count (" + std::to_string(node_counts[i]) +
+                                   ") exceeds the supported maximum");
+            if ((uint64_t)edge_counts[i] > 0xFFFFFFFFull / 2)
+                throw netbc::Error(prefix + "edge count (" + std::to_string(edge_counts[i]) +
+                                   ") exceeds the supported maximum");
+            P->node0[i + 1] = P->node0[i] + (uint64_t)node_counts[i];
+            P->edge0[i + 1] = P->edge0[i] + (uint64_t)edge_counts[i];
+        }
+        const uint64_t n = P->node0[graphs], m = P->edge0[graphs];
+        if (n > 0xFFFFFFFFull || m > 0xFFFFFFFFull / 2)
+            throw netbc::Error("the batch exceeds 2^32 nodes or 2^31 edges in total");
+        std::vector<uint32_t> node_graph;
+        P->any_device = small_layout(graphs, node_counts, P->node0, P->directed, P->normalized,
+                                     false, P->descs, node_graph);
+        if (!P->any_device) {
+            *plan = P.release();
+            return;
+        }
+        NETBC_HIP(hipStreamCreateWithFlags(&P->st, hipStreamNonBlocking));
+        NETBC_HIP(hipEventCreate(&P->e0));
+        NETBC_HIP(hipEventCreate(&P->e1));
+        NETBC_HIP(hipHostMalloc((void**)&P->h_src, std::max<uint64_t>(m, 1) * 4));
+        NETBC_HIP(hipHostMalloc((void**)&P->h_tgt, std::max<uint64_t>(m, 1) * 4));
+        netbc::fixed_csr_alloc(P->csr, n, m, P->directed);
+        netbc::small_setup(P->run, P->descs, node_graph, options(opt), P->st);
+        if (capture) {
+            NETBC_HIP(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
+            try {
+                P->enqueue();
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(P->st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            NETBC_HIP(hipStreamEndCapture(P->st, &P->graph));
+            NETBC_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+            P->captured = true;
+        }
+        *plan = P.release();
+    });
+}
+
+NETBC_EXPORT int32_t netbc_batch_plan_run(netbc_batch_plan* plan, const int64_t* src,
+                                          const int64_t* tgt, double* out,
+                                          netbc_diagnostics* diag, char* err, size_t err_len) {
+    return guard(err, err_len, [&] {
+        if (!plan) throw netbc::Error("plan is null");
+        netbc_batch_plan& P = *plan;
+        std::lock_guard<std::mutex> lock(P.mu);
+        ensure_device();
+        const uint64_t m = P.edge0[P.graphs];
+        if (m && (!src || !tgt)) throw netbc::Error("null edge arrays");
+        if (P.node0[P.graphs] && !out) throw netbc::Error("null score array");
+        netbc::RunDiag total;
+        std::vector<uint32_t> too_deep((size_t)P.graphs, 0);
+        // validate as net_graph_new does and stage as union node ids
+        for (int64_t i = 0; i < P.graphs; i++) {
+            const int64_t n = P.node_counts[i];
+            const uint64_t e0 = P.edge0[i];
+            const uint32_t base = (uint32_t)P.node0[i];
+            for (int64_t e = 0; e < P.edge_counts[i]; e++) {
+                const int64_t s = src[e0 + e], t = tgt[e0 + e];
+                const int64_t ends[2] = {s, t};
+                for (int64_t v : ends)
+                    if (v < 0 || v >= n)
+                        throw netbc::Error("graph " + std::to_string(i) + ": edge " +
+                                           std::to_string(e) + " refers to node index " +
+                                           std::to_string(v) + ", but valid indices are 0.." +
+                                           std::to_string(n));
+                if (P.h_src) {
+                    P.h_src[e0 + e] = (uint32_t)s + base;
+                    P.h_tgt[e0 + e] = (uint32_t)t + base;
+                }
+            }
+        }
+        if (P.any_device) {
+            DeviceScope scope(P.device);
+            if (m) {
+                NETBC_HIP(hipMemcpyAsync(P.csr.src.p, P.h_src, m * 4, hipMemcpyHostToDevice,
+                                         P.st));
+                NETBC_HIP(hipMemcpyAsync(P.csr.tgt.p, P.h_tgt, m * 4, hipMemcpyHostToDevice,
+                                         P.st));
+            }
+            NETBC_HIP(hipEventRecord(P.e0, P.st));
+            if (P.captured) NETBC_HIP(hipGraphLaunch(P.exec, P.st));
+            else P.enqueue();
+            NETBC_HIP(hipEventRecord(P.e1, P.st));
+            unsigned long long hs[2] = {0, 0};
+            netbc::small_fetch(P.run, out, too_deep, hs, P.st);
+            float ms = 0.0f;
+            NETBC_HIP(hipEventElapsedTime(&ms, P.e0, P.e1));
+            total.kernel_ms += ms;
+            total.reached_nodes += (int64_t)hs[0];
+            total.reached_arcs += (int64_t)hs[1];
+            total.slots = (int32_t)P.run.grid;
+        }
+        run_left_over(P.graphs, P.node_counts.data(), P.edge_counts.data(), P.node0, P.edge0,
+                      src, tgt, nullptr, P.directed ? 1 : 0, 0, 0, P.normalized ? 1 : 0,
+                      P.options_p(), P.descs, too_deep, out, total);
+        fill_diag(diag, total, 0.0);
+    });
+}
+
+NETBC_EXPORT void netbc_batch_plan_free(netbc_batch_plan* plan) { delete plan; }
 
 namespace {
 // A plain streaming copy (16 bytes per thread per iteration).

@@ -567,4 +567,163 @@ void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L) {
     NETBC_HIP(hipStreamSynchronize(st));
 }
 
+// ---- fixed-shape build (FixedCsr) -------------------------------------------
+
+namespace {
+
+constexpr uint64_t kPadKey = ~0ull;  // an unused arc slot: sorts last, names no row
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, size_t count, uint32_t value) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = value;
+}
+
+// Arc keys (row << b | column) at fixed slots: directed, slot e is edge e
+// (reversed for the in-CSR); undirected, slots 2e and 2e + 1 are (s, t)
+// and (t, s), or (s, s) and an unused slot for a loop. Slot order is the
+// reference's arc order before sorting (graph.rs:201-223), so the stable
+// sort keeps parallel arcs in input order. Values are the edge index.
+__global__ void fixed_arcs_kernel(const uint32_t* __restrict__ s, const uint32_t* __restrict__ t,
+                                  size_t m, int directed, int reverse, int b,
+                                  uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    uint64_t a = s[e], c = t[e];
+    if (directed) {
+        if (reverse) {
+            const uint64_t x = a;
+            a = c;
+            c = x;
+        }
+        key[e] = a << b | c;
+        val[e] = (uint32_t)e;
+        return;
+    }
+    key[2 * e] = a << b | c;
+    val[2 * e] = (uint32_t)e;
+    key[2 * e + 1] = a != c ? (c << b | a) : kPadKey;
+    val[2 * e + 1] = (uint32_t)e;
+}
+
+// Arcs per row, unused slots (rows >= n) skipped.
+__global__ void fixed_rows_kernel(const uint64_t* __restrict__ key, size_t slots, int b,
+                                  uint64_t n, uint32_t* __restrict__ deg) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint64_t row = key[i] >> b;
+    if (row < n) atomicAdd(&deg[row], 1u);
+}
+
+__global__ void fixed_targets_kernel(const uint64_t* __restrict__ key, size_t slots,
+                                     uint64_t mask, uint32_t* __restrict__ idx) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < slots) idx[i] = (uint32_t)(key[i] & mask);
+}
+
+// Exclusive scan with chunk sums in the preallocated buffers c.scan[level..]
+// (no allocation, no wait).
+void scan_fixed(const uint32_t* in, uint32_t* out, size_t len, FixedCsr& c, int level,
+                hipStream_t st) {
+    if (len == 0) return;
+    const size_t blocks = (len + kScanTile - 1) / kScanTile;
+    if (blocks == 1) {
+        scan_apply_kernel<<<1, kBlock, 0, st>>>(in, out, len, nullptr);
+        NETBC_HIP(hipGetLastError());
+        return;
+    }
+    if (level >= 4 || c.scan[level].n < blocks) throw Error("fixed scan beyond its buffers");
+    uint32_t* partial = c.scan[level].p;
+    scan_reduce_kernel<<<blocks, kBlock, 0, st>>>(in, len, partial);
+    NETBC_HIP(hipGetLastError());
+    scan_fixed(partial, partial, blocks, c, level + 1, st);
+    scan_apply_kernel<<<blocks, kBlock, 0, st>>>(in, out, len, partial);
+    NETBC_HIP(hipGetLastError());
+}
+
+// Sorts c.cap (key, value) slots on the low 2 * c.bits key bits; returns
+// which pair of buffers holds the result.
+int sort_fixed(FixedCsr& c, hipStream_t st) {
+    const size_t len = c.cap;
+    const int bits = 2 * c.bits;
+    if (len <= 1) return 0;
+    const uint32_t tiles = (uint32_t)((len + kSortTile - 1) / kSortTile);
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        uint64_t* ki = cur ? c.k1.p : c.k0.p;
+        uint32_t* vi = cur ? c.v1.p : c.v0.p;
+        uint64_t* ko = cur ? c.k0.p : c.k1.p;
+        uint32_t* vo = cur ? c.v0.p : c.v1.p;
+        radix_hist_kernel<<<tiles, kBlock, 0, st>>>(ki, len, shift, c.hist.p, tiles);
+        NETBC_HIP(hipGetLastError());
+        scan_fixed(c.hist.p, c.hist.p, (size_t)256 * tiles, c, 0, st);
+        radix_scatter_kernel<<<tiles, kBlock, 0, st>>>(ki, vi, ko, vo, len, shift, c.hist.p,
+                                                       tiles);
+        NETBC_HIP(hipGetLastError());
+        cur ^= 1;
+    }
+    return cur;
+}
+
+// One direction: keys, sort, row offsets (n + 1) and targets (cap slots).
+void fixed_one(FixedCsr& c, bool reverse, uint32_t* off, uint32_t* idx, hipStream_t st) {
+    const unsigned eg = grid_for(std::max<uint64_t>(c.m, 1));
+    const unsigned sg = grid_for(std::max<uint64_t>(c.cap, 1));
+    if (c.m) {
+        fixed_arcs_kernel<<<eg, kBlock, 0, st>>>(c.src.p, c.tgt.p, c.m, c.directed ? 1 : 0,
+                                                 reverse ? 1 : 0, c.bits, c.k0.p, c.v0.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    const int which = sort_fixed(c, st);
+    const uint64_t* key = which ? c.k1.p : c.k0.p;
+    fill_u32_kernel<<<grid_for(c.n + 1), kBlock, 0, st>>>(off, c.n + 1, 0u);
+    NETBC_HIP(hipGetLastError());
+    if (c.cap) {
+        fixed_rows_kernel<<<sg, kBlock, 0, st>>>(key, c.cap, c.bits, c.n, off);
+        NETBC_HIP(hipGetLastError());
+        fixed_targets_kernel<<<sg, kBlock, 0, st>>>(key, c.cap, (1ull << c.bits) - 1, idx);
+        NETBC_HIP(hipGetLastError());
+    }
+    scan_fixed(off, off, c.n + 1, c, 0, st);
+}
+
+}  // namespace
+
+void fixed_csr_alloc(FixedCsr& c, uint64_t n, uint64_t m, bool directed) {
+    if (n >= 0xFFFFFFFFull) throw Error("the batch exceeds 2^32 - 1 nodes in total");
+    c.n = n;
+    c.m = m;
+    c.directed = directed;
+    c.cap = directed ? m : 2 * m;
+    if (c.cap > 0xFFFFFFFEull) throw Error("the batch exceeds 2^32 - 2 arcs in total");
+    c.bits = bits_for(n + 1);  // ids 0..n, so the unused slots' row (all ones) is no node
+    c.src.alloc(m);
+    c.tgt.alloc(m);
+    c.k0.alloc(c.cap);
+    c.k1.alloc(c.cap);
+    c.v0.alloc(c.cap);
+    c.v1.alloc(c.cap);
+    const size_t tiles = (c.cap + kSortTile - 1) / kSortTile;
+    c.hist.alloc(256 * std::max<size_t>(tiles, 1));
+    // chunk sums for the longest scan (the histogram or the offsets)
+    size_t len = std::max<size_t>(256 * std::max<size_t>(tiles, 1), n + 1);
+    for (int level = 0; level < 4; level++) {
+        len = (len + kScanTile - 1) / kScanTile;
+        c.scan[level].alloc(len);
+        if (len <= 1) break;
+    }
+    c.out_off.alloc(n + 1);
+    c.out_idx.alloc(c.cap);
+    if (directed) {
+        c.in_off.alloc(n + 1);
+        c.in_idx.alloc(c.cap);
+    }
+}
+
+void fixed_csr_enqueue(FixedCsr& c, hipStream_t st) {
+    // every key slot is written by the arc kernel (an undirected loop's
+    // unused slot gets the pad key), so nothing needs clearing
+    fixed_one(c, false, c.out_off.p, c.out_idx.p, st);
+    if (c.directed) fixed_one(c, true, c.in_off.p, c.in_idx.p, st);
+}
+
 }  // namespace netbc

@@ -193,6 +193,29 @@ struct RunDiag {
 // Builds the device CSR/CSC of g (graph.hip). Caller holds g.mu.
 void build_device_graph(Graph& g);
 
+// The CSR/CSC of an unweighted graph of one fixed shape (n nodes, m edges,
+// directed or not), rebuilt from new edges in place: fixed_csr_alloc
+// allocates every buffer once, and fixed_csr_enqueue only launches kernels
+// on the stream (no allocation, no memset, no wait on the host), so a
+// stream capture can record it and a graph replay rebuild the arrays. The
+// caller writes the edges (node ids, u32) into src/tgt first. The arrays
+// equal build_device_graph's on rows [0, n); the target arrays hold `cap`
+// slots, of which the rows use the first off[n].
+struct FixedCsr {
+    uint64_t n = 0, m = 0, cap = 0;  // cap: arc slots, m directed, 2m undirected
+    bool directed = false;
+    int bits = 0;                    // node-id bits of a key (ids 0..n are distinct)
+    DevBuf<uint32_t> src, tgt;
+    DevBuf<uint64_t> k0, k1;
+    DevBuf<uint32_t> v0, v1, hist;
+    DevBuf<uint32_t> scan[4];        // chunk sums of the scans, one buffer per level
+    DevBuf<uint32_t> out_off, out_idx, in_off, in_idx;
+    const uint32_t* in_off_p() const { return directed ? in_off.p : out_off.p; }
+    const uint32_t* in_idx_p() const { return directed ? in_idx.p : out_idx.p; }
+};
+void fixed_csr_alloc(FixedCsr& c, uint64_t n, uint64_t m, bool directed);
+void fixed_csr_enqueue(FixedCsr& c, hipStream_t st);
+
 // Classifies lengths on the device, as path_lengths (weights.rs:89-131).
 // Throws Error with the reference's message on bad weights.
 void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L);

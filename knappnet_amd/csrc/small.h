@@ -39,6 +39,35 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
                      const std::vector<uint32_t>& node_graph, const RunOptions& opt,
                      double* out_host, std::vector<uint32_t>& too_deep, RunDiag& diag);
 
+// The same engine split for repeated runs on one batch shape: small_setup
+// allocates and uploads what the node counts fix (descriptors, work items,
+// rows); small_enqueue launches the run on a union CSR (fill kernels, the
+// persistent kernel, the ordered reduction: no allocation, no memset, no
+// wait, so a stream capture can record it); small_fetch copies the scores,
+// the too-deep flags and the reached counts back and waits.
+struct SmallRun {
+    size_t total = 0;       // union nodes
+    size_t ngraphs = 0;
+    uint32_t nitems = 0;
+    uint32_t nmax = 1;
+    size_t grid = 0;
+    int precision = 0;
+    DevBuf<SmallGraph> graphs;
+    DevBuf<uint32_t> node_graph, too_deep, counter;
+    DevBuf<SmallItem> items;
+    DevBuf<double> gsum, out, sig, coef64;
+    DevBuf<float> coef32;
+    DevBuf<uint8_t> lvl;
+    DevBuf<unsigned long long> stats;
+};
+void small_setup(SmallRun& r, const std::vector<SmallGraph>& graphs,
+                 const std::vector<uint32_t>& node_graph, const RunOptions& opt, hipStream_t st);
+void small_enqueue(SmallRun& r, const uint32_t* off, const uint32_t* idx, const uint32_t* in_off,
+                   const uint32_t* in_idx, hipStream_t st, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr);
+void small_fetch(SmallRun& r, double* out_host, std::vector<uint32_t>& too_deep,
+                 unsigned long long stats[2], hipStream_t st);
+
 // Betweenness of many small unweighted graphs whose union is already on the
 // device (out-CSR off/idx and, directed, in-CSR in_off/in_idx: `arcs`
 // targets each, node ids consecutive across graphs), with no device

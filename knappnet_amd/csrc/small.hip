@@ -541,12 +541,11 @@ ScratchDevice scratch_on_current_device() {
 
 }  // namespace
 
-void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
-                     const std::vector<uint32_t>& node_graph, const RunOptions& opt,
-                     double* out_host, std::vector<uint32_t>& too_deep, RunDiag& diag) {
-    hipStream_t st = g.stream;
-    const DeviceGraph& d = g.dev;
-    size_t total = g.n;
+void small_setup(SmallRun& r, const std::vector<SmallGraph>& graphs,
+                 const std::vector<uint32_t>& node_graph, const RunOptions& opt, hipStream_t st) {
+    r.total = node_graph.size();
+    r.ngraphs = graphs.size();
+    r.precision = opt.precision;
     // Work items, largest graphs first.
     std::vector<uint32_t> order(graphs.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (uint32_t)i;
@@ -555,7 +554,8 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
     std::vector<SmallItem> items;
     size_t gsum_total = 0;
     uint32_t nmax = 1;
-    for (const SmallGraph& G : graphs) gsum_total = std::max<size_t>(gsum_total, G.gsum_off + (size_t)G.groups * G.n);
+    for (const SmallGraph& G : graphs)
+        gsum_total = std::max<size_t>(gsum_total, G.gsum_off + (size_t)G.groups * G.n);
     for (uint32_t gi : order) {
         const SmallGraph& G = graphs[gi];
         if (G.n == 0 || !G.device) continue;
@@ -564,29 +564,23 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
         for (uint32_t j = 0; j < G.groups; j += per)
             items.push_back(SmallItem{gi, j, std::min(per, G.groups - j), 0});
     }
-    DevBuf<SmallGraph> dgraphs;
-    DevBuf<uint32_t> dnode_graph, dtoo_deep, counter;
-    DevBuf<SmallItem> ditems;
-    DevBuf<double> gsum, dout;
-    DevBuf<unsigned long long> stats;
-    dgraphs.alloc(graphs.size());
-    dnode_graph.alloc(total);
-    dtoo_deep.alloc(graphs.size());
-    counter.alloc(1);
-    ditems.alloc(items.size());
-    gsum.alloc(gsum_total);
-    dout.alloc(total);
-    stats.alloc(2);
-    NETBC_HIP(hipMemcpyAsync(dgraphs.p, graphs.data(), graphs.size() * sizeof(SmallGraph),
+    r.nitems = (uint32_t)items.size();
+    r.nmax = nmax;
+    r.graphs.alloc(graphs.size());
+    r.node_graph.alloc(r.total);
+    r.too_deep.alloc(graphs.size());
+    r.counter.alloc(1);
+    r.items.alloc(items.size());
+    r.gsum.alloc(gsum_total);
+    r.out.alloc(r.total);
+    r.stats.alloc(2);
+    NETBC_HIP(hipMemcpyAsync(r.graphs.p, graphs.data(), graphs.size() * sizeof(SmallGraph),
                              hipMemcpyHostToDevice, st));
-    NETBC_HIP(hipMemcpyAsync(dnode_graph.p, node_graph.data(), total * 4, hipMemcpyHostToDevice,
-                             st));
+    NETBC_HIP(hipMemcpyAsync(r.node_graph.p, node_graph.data(), r.total * 4,
+                             hipMemcpyHostToDevice, st));
     if (!items.empty())
-        NETBC_HIP(hipMemcpyAsync(ditems.p, items.data(), items.size() * sizeof(SmallItem),
+        NETBC_HIP(hipMemcpyAsync(r.items.p, items.data(), items.size() * sizeof(SmallItem),
                                  hipMemcpyHostToDevice, st));
-    NETBC_HIP(hipMemsetAsync(dtoo_deep.p, 0, graphs.size() * 4, st));
-    NETBC_HIP(hipMemsetAsync(stats.p, 0, 16, st));
-    NETBC_HIP(hipMemsetAsync(gsum.p, 0, gsum_total * 8, st));
 
     int dev = 0;
     NETBC_HIP(hipGetDevice(&dev));
@@ -603,45 +597,80 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
     size_t grid = std::min<size_t>((size_t)prop.multiProcessorCount * per_cu,
                                    std::max<size_t>(items.size(), 1));
     if (opt.slots > 0) grid = std::min<size_t>(grid, (size_t)opt.slots);
-    DevBuf<double> sig;
-    DevBuf<double> coef64;
-    DevBuf<float> coef32;
-    DevBuf<uint8_t> lvl;
-    size_t per = (size_t)nmax * kLanes;
-    sig.alloc(grid * per);
-    lvl.alloc(grid * per);
-    if (opt.precision) coef32.alloc(grid * per);
-    else coef64.alloc(grid * per);
+    r.grid = grid;
+    const size_t per = (size_t)nmax * kLanes;
+    r.sig.alloc(grid * per);
+    r.lvl.alloc(grid * per);
+    if (opt.precision) r.coef32.alloc(grid * per);
+    else r.coef64.alloc(grid * per);
+    // the uploads read host vectors that end with this call
+    NETBC_HIP(hipStreamSynchronize(st));
+}
 
+void small_enqueue(SmallRun& r, const uint32_t* off, const uint32_t* idx, const uint32_t* in_off,
+                   const uint32_t* in_idx, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    // Clearing by kernels rather than memsets, so that a captured graph
+    // clears on every replay. gsum needs none: every group sum of a graph
+    // the kernel takes is written, and the scores of the other graphs
+    // (too deep, or not taken) are recomputed by the caller.
+    zero_u32_kernel<<<1, 64, 0, st>>>(r.counter.p, 1);
+    NETBC_HIP(hipGetLastError());
+    zero_u32_kernel<<<(unsigned)((r.ngraphs + 255) / 256 + 1), 256, 0, st>>>(r.too_deep.p,
+                                                                           r.ngraphs);
+    NETBC_HIP(hipGetLastError());
+    zero_u32_kernel<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(r.stats.p), 4);
+    NETBC_HIP(hipGetLastError());
+    if (e0) NETBC_HIP(hipEventRecord(e0, st));
+    if (r.nitems) {
+        if (r.precision)
+            small_batch_kernel<float><<<(unsigned)r.grid, kSmallBlock, 0, st>>>(
+                r.items.p, r.nitems, r.counter.p, r.graphs.p, off, idx, in_off, in_idx, r.sig.p,
+                r.coef32.p, r.lvl.p, r.nmax, r.gsum.p, r.too_deep.p, r.stats.p);
+        else
+            small_batch_kernel<double><<<(unsigned)r.grid, kSmallBlock, 0, st>>>(
+                r.items.p, r.nitems, r.counter.p, r.graphs.p, off, idx, in_off, in_idx, r.sig.p,
+                r.coef64.p, r.lvl.p, r.nmax, r.gsum.p, r.too_deep.p, r.stats.p);
+        NETBC_HIP(hipGetLastError());
+    }
+    if (e1) NETBC_HIP(hipEventRecord(e1, st));
+    if (r.total) {
+        small_reduce_kernel<<<(unsigned)((r.total + 255) / 256), 256, 0, st>>>(
+            r.graphs.p, r.node_graph.p, r.gsum.p, r.total, r.out.p);
+        NETBC_HIP(hipGetLastError());
+    }
+}
+
+void small_fetch(SmallRun& r, double* out_host, std::vector<uint32_t>& too_deep,
+                 unsigned long long stats[2], hipStream_t st) {
+    if (r.total)
+        NETBC_HIP(hipMemcpyAsync(out_host, r.out.p, r.total * 8, hipMemcpyDeviceToHost, st));
+    too_deep.assign(r.ngraphs, 0);
+    if (r.ngraphs)
+        NETBC_HIP(hipMemcpyAsync(too_deep.data(), r.too_deep.p, r.ngraphs * 4,
+                                 hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipMemcpyAsync(stats, r.stats.p, 16, hipMemcpyDeviceToHost, st));
+    NETBC_HIP(hipStreamSynchronize(st));
+}
+
+void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
+                     const std::vector<uint32_t>& node_graph, const RunOptions& opt,
+                     double* out_host, std::vector<uint32_t>& too_deep, RunDiag& diag) {
+    hipStream_t st = g.stream;
+    const DeviceGraph& d = g.dev;
+    SmallRun r;
+    small_setup(r, graphs, node_graph, opt, st);
     hipEvent_t e0, e1;
     NETBC_HIP(hipEventCreate(&e0));
     NETBC_HIP(hipEventCreate(&e1));
-    zero_u32_kernel<<<1, 64, 0, st>>>(counter.p, 1);
-    NETBC_HIP(hipEventRecord(e0, st));
-    if (!items.empty()) {
-        if (opt.precision)
-            small_batch_kernel<float><<<(unsigned)grid, kSmallBlock, 0, st>>>(
-                ditems.p, (uint32_t)items.size(), counter.p, dgraphs.p, d.out_off.p,
-                d.out_idx.p, d.in_off_p(), d.in_idx_p(), sig.p, coef32.p, lvl.p, nmax, gsum.p,
-                dtoo_deep.p, stats.p);
-        else
-            small_batch_kernel<double><<<(unsigned)grid, kSmallBlock, 0, st>>>(
-                ditems.p, (uint32_t)items.size(), counter.p, dgraphs.p, d.out_off.p,
-                d.out_idx.p, d.in_off_p(), d.in_idx_p(), sig.p, coef64.p, lvl.p, nmax, gsum.p,
-                dtoo_deep.p, stats.p);
-        NETBC_HIP(hipGetLastError());
+    unsigned long long hs[2] = {0, 0};
+    try {
+        small_enqueue(r, d.out_off.p, d.out_idx.p, d.in_off_p(), d.in_idx_p(), st, e0, e1);
+        small_fetch(r, out_host, too_deep, hs, st);
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
     }
-    NETBC_HIP(hipEventRecord(e1, st));
-    small_reduce_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(
-        dgraphs.p, dnode_graph.p, gsum.p, total, dout.p);
-    NETBC_HIP(hipGetLastError());
-    NETBC_HIP(hipMemcpyAsync(out_host, dout.p, total * 8, hipMemcpyDeviceToHost, st));
-    too_deep.assign(graphs.size(), 0);
-    NETBC_HIP(hipMemcpyAsync(too_deep.data(), dtoo_deep.p, graphs.size() * 4,
-                             hipMemcpyDeviceToHost, st));
-    unsigned long long hs[2];
-    NETBC_HIP(hipMemcpyAsync(hs, stats.p, 16, hipMemcpyDeviceToHost, st));
-    NETBC_HIP(hipStreamSynchronize(st));
     float ms = 0.0f;
     NETBC_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
@@ -651,7 +680,7 @@ void run_small_batch(Graph& g, const std::vector<SmallGraph>& graphs,
     diag.dominant_launches += 1;
     diag.reached_nodes += (int64_t)hs[0];
     diag.reached_arcs += (int64_t)hs[1];
-    diag.slots = (int32_t)grid;
+    diag.slots = (int32_t)r.grid;
 }
 
 }  // namespace netbc

@@ -281,6 +281,62 @@ def betweenness_batch(graphs, directed: bool = False, weighted: bool = False,
     return parts, diag.as_dict()
 
 
+class BatchPlan:
+    """``netbc_batch_plan_*``: repeated betweenness batches of one shape
+    (graph count, node counts, edge counts), the device side recorded once
+    into a HIP graph (``capture=True``) or launched kernel by kernel."""
+
+    def __init__(self, node_counts, edge_counts, directed: bool = False,
+                 normalized: bool = False, capture: bool = True, precision: int = 0,
+                 slots: int = 0):
+        self.lib = _native.device()
+        self.ns = _i64(np.asarray(node_counts, dtype=np.int64))
+        self.ms = _i64(np.asarray(edge_counts, dtype=np.int64))
+        if len(self.ns) != len(self.ms):
+            raise ValueError("node and edge counts differ in length")
+        self.handle = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(ERR_LEN)
+        opt = _native.Options(precision, slots, 0, 0, 0.0)
+        if self.lib.netbc_batch_plan_new(len(self.ns), _ptr(self.ns, c_int64),
+                                         _ptr(self.ms, c_int64), int(bool(directed)),
+                                         int(bool(normalized)), int(bool(capture)),
+                                         ctypes.byref(opt), ctypes.byref(self.handle), err,
+                                         ERR_LEN):
+            raise EngineError(err.value.decode())
+
+    def run(self, graphs):
+        """Scores of a list of (n, src, tgt) graphs of the plan's shape;
+        returns (list of per-graph score arrays, diagnostics dict)."""
+        if len(graphs) != len(self.ns):
+            raise ValueError("graph count differs from the plan's")
+        src = _i64(np.concatenate([np.asarray(g[1], dtype=np.int64) for g in graphs])
+                   if graphs else np.zeros(0, np.int64))
+        tgt = _i64(np.concatenate([np.asarray(g[2], dtype=np.int64) for g in graphs])
+                   if graphs else np.zeros(0, np.int64))
+        total = int(self.ns.sum())
+        if len(src) != int(self.ms.sum()) or len(tgt) != len(src):
+            raise ValueError("edge count differs from the plan's")
+        out = np.zeros(max(total, 1), dtype=np.float64)
+        err = ctypes.create_string_buffer(ERR_LEN)
+        diag = _native.Diagnostics()
+        if self.lib.netbc_batch_plan_run(self.handle, _ptr(src, c_int64), _ptr(tgt, c_int64),
+                                         _ptr(out, c_double), ctypes.byref(diag), err, ERR_LEN):
+            raise EngineError(err.value.decode())
+        parts = np.split(out[:total], np.cumsum(self.ns)[:-1]) if len(self.ns) else []
+        return parts, diag.as_dict()
+
+    def close(self):
+        if self.handle:
+            self.lib.netbc_batch_plan_free(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def device_info() -> str:
     lib = _native.device()
     buf = ctypes.create_string_buffer(256)
