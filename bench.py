@@ -398,9 +398,14 @@ def calibration_copy(nbytes: int = 1 << 30) -> None:
     err = ctypes.create_string_buffer(256)
     if _native.device().netbc_calibration_copy(b, a, nbytes, err, 256):
         raise RuntimeError(err.value.decode())
+    print(f"# calibration copy: {nbytes} bytes by netbc_copy_kernel", flush=True)
+    # random 8-byte reads, one per 128-byte line of the source
+    if _native.device().netbc_calibration_gather(b, a, nbytes, 128, err, 256):
+        raise RuntimeError(err.value.decode())
+    print(f"# calibration gather: {nbytes // 128} random 8-byte reads (one per 128-byte line) "
+          f"by netbc_gather_kernel", flush=True)
     hip.hipFree(a)
     hip.hipFree(b)
-    print(f"# calibration copy: {nbytes} bytes by netbc_copy_kernel", flush=True)
 
 
 def load_traffic(workload: str, precision: int):

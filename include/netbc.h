@@ -348,6 +348,14 @@ int32_t netbc_graph_csr(const GraphHandle *h, int32_t incoming,
 int32_t netbc_calibration_copy(void *dst, const void *src, size_t bytes,
                                char *err, size_t err_len);
 
+/* Diagnostics: reads one 8-byte word from each `line`-byte line of `src`
+ * (`bytes` / `line` lines, a power of two, every line once, in a scattered
+ * order) with netbc_gather_kernel, writing one 8-byte sum per thread
+ * (524,288 threads: `dst` holds 4 MiB). Counters read for it say how many
+ * bytes a random 8-byte read costs. No counterpart in netcapi.h. */
+int32_t netbc_calibration_gather(void *dst, const void *src, size_t bytes,
+                                 size_t line, char *err, size_t err_len);
+
 /* Name of the device the library runs on (e.g. "AMD Instinct MI355X,
  * gfx950, 256 CUs"). */
 int32_t netbc_device_info(char *buf, size_t len);

@@ -137,6 +137,9 @@ def device() -> ctypes.CDLL:
         lib.netbc_graph_csr.restype = c_int32
         lib.netbc_calibration_copy.argtypes = [c_void_p, c_void_p, c_size_t, c_char_p, c_size_t]
         lib.netbc_calibration_copy.restype = c_int32
+        lib.netbc_calibration_gather.argtypes = [c_void_p, c_void_p, c_size_t, c_size_t, c_char_p,
+                                                 c_size_t]
+        lib.netbc_calibration_gather.restype = c_int32
         lib.netbc_device_info.argtypes = [c_char_p, c_size_t]
         lib.netbc_device_info.restype = c_int32
         _device = lib

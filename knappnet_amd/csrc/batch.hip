@@ -1306,7 +1306,9 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
             } else {
                 const double delta = s * dvv;
                 coef[r] = Dep<C>::coef(target_of(target, v), delta, s);
-                sig[r] = delta;
+                // with bsum the dependency goes into the (slot, node) sum
+                // alone; nothing reads the row afterwards
+                if (!bsum) sig[r] = delta;
                 dep = delta;
             }
             smax = fmax(smax, s);
@@ -1505,7 +1507,7 @@ __global__ void bfs_combine_kernel(const QEnt* __restrict__ q, const uint32_t* _
             double s = sig[r];
             double delta = s * dv;
             coef[r] = Dep<C>::coef(target_of(target, en.v), delta, s);
-            sig[r] = delta;
+            if (!bsum) sig[r] = delta;
             dep = delta;
             smax = fmax(smax, s);
         }
@@ -1715,6 +1717,9 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                 uint64_t bal = __ballot(mm != 0);
                 while (bal) {
                     // up to kAhead predecessors loaded before they are added
+                    // (distances and path counts together: loading the counts
+                    // of tying lanes only, after the distances, measured
+                    // slower: one more round trip per batch of hits)
                     Dist du[kAhead];
                     double su[kAhead];
                     Len lu[kAhead];
@@ -1870,7 +1875,12 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                 C val[kAhead];
                 bool use[kAhead];
                 uint32_t ak[kAhead], bk[kAhead];
+                Dist dx[kAhead];
+                Len lx[kAhead];
+                size_t rx[kAhead];
                 [[maybe_unused]] bool tk[kAhead];
+                // the heads' distances first, then the coefficients of the
+                // heads the arc leads to on a shortest path
 #pragma unroll
                 for (int u = 0; u < kAhead; u++) {
                     use[u] = false;
@@ -1878,22 +1888,28 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                     ak[u] = kNone;
                     bk[u] = en.b;
                     tk[u] = true;
+                    dx[u] = Dist(0);
+                    lx[u] = Len(0);
+                    rx[u] = 0;
                     if (bal) {
                         const int j = __ffsll((unsigned long long)bal) - 1;
                         bal &= bal - 1;
                         const uint32_t xx = rl32(x, j);
-                        const Len ll = rl(len, j);
+                        lx[u] = rl(len, j);
                         const uint64_t m2 = rl64(mm, j);
                         ak[u] = (uint32_t)(c + (uint32_t)j);
                         if constexpr (kQuirk) tk[u] = target_of(target, xx) != 0.0;
                         if ((m2 >> lane) & 1) {
-                            const size_t r = (base + xx) * kLanes + lane;
-                            if (P::tie(P::add(dv0, ll), dist[r])) {
-                                use[u] = true;
-                                val[u] = coef[r];
-                            }
+                            rx[u] = (base + xx) * kLanes + lane;
+                            use[u] = true;
+                            dx[u] = dist[rx[u]];
                         }
                     }
+                }
+#pragma unroll
+                for (int u = 0; u < kAhead; u++) {
+                    use[u] = use[u] && P::tie(P::add(dv0, lx[u]), dx[u]);
+                    if (use[u]) val[u] = coef[rx[u]];
                 }
                 [[maybe_unused]] double cv[kAhead];
 #pragma unroll

@@ -1370,6 +1370,24 @@ __global__ void netbc_copy_kernel(uint4* __restrict__ dst, const uint4* __restri
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < count; i += stride) dst[i] = src[i];
 }
+
+// Random 8-byte reads: one word at the start of each of `lines` lines of
+// `line` bytes, every line once, in the scattered order i * 0x9E3779B1
+// mod lines (a bijection: lines is a power of two and the factor odd).
+// Each thread writes its sum to dst[thread], so the kernel's writes are
+// grid-sized and its reads are the gathers alone.
+__global__ void netbc_gather_kernel(unsigned long long* __restrict__ dst,
+                                    const unsigned char* __restrict__ src, size_t lines,
+                                    size_t line) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    unsigned long long s = 0;
+    for (size_t i = tid; i < lines; i += stride) {
+        const size_t l = (i * 0x9E3779B1ull) & (lines - 1);
+        s += *reinterpret_cast<const unsigned long long*>(src + l * line);
+    }
+    dst[tid] = s;
+}
 }  // namespace
 
 NETBC_EXPORT int32_t netbc_calibration_copy(void* dst, const void* src, size_t bytes, char* err,
@@ -1381,6 +1399,25 @@ NETBC_EXPORT int32_t netbc_calibration_copy(void* dst, const void* src, size_t b
         netbc::check_device_range(src, bytes, "source");
         size_t count = bytes / 16;
         netbc_copy_kernel<<<2048, 256>>>((uint4*)dst, (const uint4*)src, count);
+        NETBC_HIP(hipGetLastError());
+        NETBC_HIP(hipDeviceSynchronize());
+    });
+}
+
+NETBC_EXPORT int32_t netbc_calibration_gather(void* dst, const void* src, size_t bytes,
+                                              size_t line, char* err, size_t err_len) {
+    return guard(err, err_len, [&] {
+        ensure_device();
+        if (line < 8 || (line & (line - 1)))
+            throw netbc::Error("line must be a power of two of at least 8 bytes");
+        if (bytes < line || bytes % line) throw netbc::Error("bytes must be a multiple of line");
+        const size_t lines = bytes / line;
+        if (lines & (lines - 1)) throw netbc::Error("bytes / line must be a power of two");
+        constexpr unsigned kGrid = 2048, kThreads = 256;
+        netbc::check_device_range(dst, (size_t)kGrid * kThreads * 8, "destination");
+        netbc::check_device_range(src, bytes, "source");
+        netbc_gather_kernel<<<kGrid, kThreads>>>((unsigned long long*)dst,
+                                                 (const unsigned char*)src, lines, line);
         NETBC_HIP(hipGetLastError());
         NETBC_HIP(hipDeviceSynchronize());
     });

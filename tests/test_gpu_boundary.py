@@ -151,3 +151,30 @@ def test_calibration_copy_checks_its_buffers():
     finally:
         hip.hipFree(a)
         hip.hipFree(b)
+
+
+def test_calibration_gather_checks_its_arguments():
+    """The gather calibration reads one word per line of a device buffer:
+    it refuses line sizes and buffer sizes it cannot cover and buffers that
+    are not device memory of the stated size, and runs on a valid one."""
+    lib = _native.device()
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(1 << 20)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(4 << 20)) == 0
+    err = ctypes.create_string_buffer(256)
+    try:
+        assert lib.netbc_calibration_gather(b, a, 1 << 20, 128, err, 256) == 0, err.value
+        assert lib.netbc_calibration_gather(b, a, 1 << 20, 96, err, 256) == 1
+        assert err.value == b"line must be a power of two of at least 8 bytes"
+        assert lib.netbc_calibration_gather(b, a, 3 << 18, 128, err, 256) == 1
+        assert err.value == b"bytes / line must be a power of two"
+        assert lib.netbc_calibration_gather(b, a, 2 << 20, 128, err, 256) == 1
+        assert err.value == b"the source allocation is smaller than 2097152 bytes"
+        assert lib.netbc_calibration_gather(a, b, 1 << 20, 128, err, 256) == 1
+        assert err.value == b"the destination allocation is smaller than 4194304 bytes"
+        g = Graph(4, [0, 1, 2], [1, 2, 3], None, False)
+        assert list(g.betweenness()) == [0.0, 2.0, 2.0, 0.0]
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
