@@ -111,6 +111,24 @@ constexpr double kTwo53 = 9007199254740992.0;
 #define NETBC_AHEAD 8
 #endif
 constexpr int kAhead = NETBC_AHEAD;
+// The same for the bucketed searches: hits per batch of the step
+// (dial_step_kernel) and of the sweep (dial_backward_kernel), whose
+// registers are capped for 7 wavefronts per SIMD. Measured on BA n=200,000
+// with lengths 1-16 (16,384 sources): the sweep with 4 hits per batch at 7
+// wavefronts per SIMD 443 -> 373 ms against 8 hits at 5 (its register
+// use); the step is fastest with 8 hits and no cap (4 hits, or a cap of 6
+// wavefronts: 10-15% slower).
+#ifndef NETBC_DIAL_AHEAD
+#define NETBC_DIAL_AHEAD 8
+#endif
+constexpr int kDialAhead = NETBC_DIAL_AHEAD;
+#ifndef NETBC_DIAL_BACK_AHEAD
+#define NETBC_DIAL_BACK_AHEAD 4
+#endif
+constexpr int kDialBackAhead = NETBC_DIAL_BACK_AHEAD;
+#ifndef NETBC_DIAL_BACK_WAVES
+#define NETBC_DIAL_BACK_WAVES 7
+#endif
 // Predecessor rows loaded at once by the path-count gathers (gather_counts).
 #ifndef NETBC_GATHER_AHEAD
 #define NETBC_GATHER_AHEAD 8
@@ -1716,16 +1734,16 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                 const uint64_t mm = ok ? (vis[base + u] & M) : 0;
                 uint64_t bal = __ballot(mm != 0);
                 while (bal) {
-                    // up to kAhead predecessors loaded before they are added
+                    // up to kDialAhead predecessors loaded before they are added
                     // (distances and path counts together: loading the counts
                     // of tying lanes only, after the distances, measured
                     // slower: one more round trip per batch of hits)
-                    Dist du[kAhead];
-                    double su[kAhead];
-                    Len lu[kAhead];
-                    bool use[kAhead];
+                    Dist du[kDialAhead];
+                    double su[kDialAhead];
+                    Len lu[kDialAhead];
+                    bool use[kDialAhead];
 #pragma unroll
-                    for (int h = 0; h < kAhead; h++) {
+                    for (int h = 0; h < kDialAhead; h++) {
                         use[h] = false;
                         su[h] = 0.0;
                         du[h] = Dist(0);
@@ -1745,7 +1763,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                         }
                     }
 #pragma unroll
-                    for (int h = 0; h < kAhead; h++)
+                    for (int h = 0; h < kDialAhead; h++)
                         if (use[h] && P::tie(P::add(du[h], lu[h]), dx)) {
                             acc += su[h];
                             npred++;
@@ -1772,13 +1790,13 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             NETBC_COUNT(c3, mm != 0);
             uint64_t bal = __ballot(mm != 0);
             while (bal) {
-                // up to kAhead heads: their distances loaded first, then offered
-                uint32_t yh[kAhead];
-                Dist cur[kAhead];
-                typename P::Sum nd[kAhead];
-                bool use[kAhead];
+                // up to kDialAhead heads: their distances loaded first, then offered
+                uint32_t yh[kDialAhead];
+                Dist cur[kDialAhead];
+                typename P::Sum nd[kDialAhead];
+                bool use[kDialAhead];
 #pragma unroll
-                for (int h = 0; h < kAhead; h++) {
+                for (int h = 0; h < kDialAhead; h++) {
                     yh[h] = kNone;
                     use[h] = false;
                     cur[h] = Dist(0);
@@ -1797,7 +1815,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
                     }
                 }
 #pragma unroll
-                for (int h = 0; h < kAhead; h++) {
+                for (int h = 0; h < kDialAhead; h++) {
                     if (yh[h] == kNone) continue;
                     bool offered = false;
                     uint32_t kk = 0;
@@ -1840,10 +1858,11 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
 // RealPath): a wavefront per entry walks its row 64 arcs at a time; for
 // every arc whose head some lane of the entry reached, the lanes whose
 // offer dist[v] + length ties the head's distance add its coefficient, in
-// CSR order (up to kAhead rows loaded before they are added). With kEdge
+// CSR order (up to kDialBackAhead rows loaded before they are added). With kEdge
 // every arc's credit also goes to eacc (edge_flush).
 template <class C, bool kEdge, class P, bool kQuirk = false>
-__global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
+__global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
+    dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                      const uint32_t* __restrict__ off,
                                      const uint32_t* __restrict__ idx,
                                      const typename P::Len* __restrict__ len_arr,
@@ -1872,17 +1891,17 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
             uint64_t mm = ok ? (vis[base + x] & M) : 0;
             uint64_t bal = __ballot(mm != 0);
             while (bal) {
-                C val[kAhead];
-                bool use[kAhead];
-                uint32_t ak[kAhead], bk[kAhead];
-                Dist dx[kAhead];
-                Len lx[kAhead];
-                size_t rx[kAhead];
-                [[maybe_unused]] bool tk[kAhead];
+                C val[kDialBackAhead];
+                bool use[kDialBackAhead];
+                uint32_t ak[kDialBackAhead], bk[kDialBackAhead];
+                Dist dx[kDialBackAhead];
+                Len lx[kDialBackAhead];
+                size_t rx[kDialBackAhead];
+                [[maybe_unused]] bool tk[kDialBackAhead];
                 // the heads' distances first, then the coefficients of the
                 // heads the arc leads to on a shortest path
 #pragma unroll
-                for (int u = 0; u < kAhead; u++) {
+                for (int u = 0; u < kDialBackAhead; u++) {
                     use[u] = false;
                     val[u] = 0;
                     ak[u] = kNone;
@@ -1907,13 +1926,13 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < kAhead; u++) {
+                for (int u = 0; u < kDialBackAhead; u++) {
                     use[u] = use[u] && P::tie(P::add(dv0, lx[u]), dx[u]);
                     if (use[u]) val[u] = coef[rx[u]];
                 }
-                [[maybe_unused]] double cv[kAhead];
+                [[maybe_unused]] double cv[kDialBackAhead];
 #pragma unroll
-                for (int u = 0; u < kAhead; u++) {
+                for (int u = 0; u < kDialBackAhead; u++) {
                     cv[u] = 0.0;
                     if (!use[u]) continue;
                     if (!kQuirk || tk[u]) {
@@ -1924,7 +1943,7 @@ __global__ void dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, 
                         cv[u] = (double)val[u];
                     }
                 }
-                if constexpr (kEdge) edge_flush<kAhead>(cv, ak, bk, eacc, arcs);
+                if constexpr (kEdge) edge_flush<kDialBackAhead>(cv, ak, bk, eacc, arcs);
             }
         }
         if (mine) {
