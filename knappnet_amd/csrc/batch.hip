@@ -3994,16 +3994,12 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     ws.slots = slots;
     ws.n = g.n;
     ws.vis.alloc(nm);
-    double* before = ws.sig.p;
-    ws.sig.alloc(nm * kLanes);
-    if (ws.sig.p != before) ws.rows_dirty = true;  // rows must start at zero
+    if (ws.sig.alloc(nm * kLanes)) ws.rows_dirty = true;  // rows must start at zero
     if (L.kind == kUniform) {
         ws.cand.alloc(nm);
         ws.succ.alloc(nm);
         ws.flanes.alloc((size_t)2 * slots);
-        uint64_t* lv_before = ws.levels.p;
-        ws.levels.alloc(nm * kLevelBufs);
-        if (ws.levels.p != lv_before) ws.zero_words = ws.levels.n;
+        if (ws.levels.alloc(nm * kLevelBufs)) ws.zero_words = ws.levels.n;
         ws.touched.alloc(nm);
         ws.bsum.alloc(nm);
     }
@@ -4025,11 +4021,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
         const size_t W = (size_t)L.max + 1;
         ws.succ.alloc(nm);
         ws.bsum.alloc(nm);
-        uint64_t* r0 = ws.ring.p;
-        uint64_t* c0 = ws.cring.p;
-        ws.ring.alloc(nm * W);
-        ws.cring.alloc(nm * W);
-        if (ws.ring.p != r0 || ws.cring.p != c0) ws.ring_dirty = true;
+        const bool fresh_ring = ws.ring.alloc(nm * W);
+        if (ws.cring.alloc(nm * W) || fresh_ring) ws.ring_dirty = true;
         if (ws.lvl_cap == 0) {
             ws.lvl_cap = 4096;
             ws.lvl_q.alloc(ws.lvl_cap);

@@ -67,8 +67,10 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
-    void alloc(size_t count) {
-        if (count <= n && p) return;
+    // Returns true when it made a new allocation (whose contents are
+    // undefined, even where the address equals the old one's).
+    bool alloc(size_t count) {
+        if (count <= n && p) return false;
         release();
         size_t bytes = (count ? count : 1) * sizeof(T);
         hipError_t e = alloc_refused(bytes) ? hipErrorOutOfMemory : hipMalloc(&p, bytes);
@@ -78,6 +80,7 @@ struct DevBuf {
                         std::to_string(bytes) + " bytes");
         }
         n = count;
+        return true;
     }
 };
 
