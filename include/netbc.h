@@ -164,7 +164,20 @@ typedef struct netbc_options {
     double alpha;      /* direction switch factor; 0 = tuned default */
     int32_t sweep_group; /* entries per dependency-sweep task at most (1 to
                             64); 0 = tuned default */
-    int32_t reserved;  /* must be 0 */
+    int32_t sweep_block; /* threads per workgroup of the dependency sweep
+                            (64 to 1024, a multiple of 64); 0 = tuned */
+    int32_t step_block;  /* threads per workgroup of the bottom-up step and
+                            the hub tasks (64 to 1024, a multiple of 64);
+                            0 = tuned */
+    int32_t hub_in;      /* in-degree above which a node's path counts are
+                            gathered by hub tasks (1 to 2^30); 0 = tuned */
+    int32_t hub_chunk;   /* in-arcs per hub task (64 to 65536, a multiple of
+                            64); 0 = tuned */
+    int32_t bucket_width; /* integer lengths: distances per bucket of the
+                             bucketed search (1 to the shortest arc, so that
+                             every bucket is settled at once; wider values
+                             are cut to the shortest arc); 0 = tuned */
+    int32_t reserved;    /* must be 0 */
 } netbc_options;
 
 /* What a run saw. */

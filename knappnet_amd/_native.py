@@ -26,7 +26,12 @@ class Options(Structure):
     """netbc_options (include/netbc.h)."""
     _fields_ = [("precision", c_int32), ("slots", c_int32), ("direction", c_int32),
                 ("exact", c_int32), ("alpha", c_double), ("sweep_group", c_int32),
-                ("reserved", c_int32)]
+                ("sweep_block", c_int32), ("step_block", c_int32), ("hub_in", c_int32),
+                ("hub_chunk", c_int32), ("bucket_width", c_int32), ("reserved", c_int32)]
+
+    # the engine settings an autotuner may set (0 = the library's tuned value)
+    TUNABLE = ("slots", "alpha", "sweep_group", "sweep_block", "step_block", "hub_in",
+               "hub_chunk", "bucket_width")
 
 
 class Diagnostics(Structure):

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <type_traits>
 #include <unordered_map>
@@ -1550,7 +1551,10 @@ struct IntPath {
     [[maybe_unused]] static constexpr uint64_t kInfBits = kNone;
     __device__ static Sum add(Dist d, Len l) { return (Sum)d + l; }
     __device__ static bool reached(Dist d) { return d != kNone; }
-    __device__ static uint64_t bucket(Sum x, double) { return x; }
+    // buckets of `width` distances (the tuned bucket width; 1: a bucket per distance)
+    __device__ static uint64_t bucket(Sum x, double width) {
+        return width == 1.0 ? x : x / (uint64_t)width;
+    }
     __device__ static bool tie(Sum via, Dist d) { return via == (Sum)d; }
     __device__ static void offer(Dist* p, Sum v) { atomicMin(p, (uint32_t)v); }
 };
@@ -3093,26 +3097,30 @@ inline unsigned blocks_for_waves(size_t waves) { return blocks_for_threads(waves
 // than the device holds at once, so waves stay resident and loop over
 // their work instead of being launched one per item.
 std::mutex g_resident_mu;
-std::unordered_map<const void*, unsigned> g_resident;
+std::map<std::pair<const void*, int>, unsigned> g_resident;
 
+// (blocks of `block` threads; dynamic LDS per block `lds` bytes)
 template <class K>
-unsigned resident_blocks(K kernel, size_t waves) {
+unsigned resident_blocks(K kernel, size_t waves, int block = kBlock, size_t lds = 0) {
     unsigned cap;
     {
         std::lock_guard<std::mutex> lock(g_resident_mu);
-        auto it = g_resident.find((const void*)kernel);
+        const auto key = std::make_pair((const void*)kernel, block);
+        auto it = g_resident.find(key);
         if (it == g_resident.end()) {
             int dev = 0, cus = 0, per = 0;
             NETBC_HIP(hipGetDevice(&dev));
             NETBC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-            NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0));
+            NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, lds));
             cap = (unsigned)std::max(1, cus * std::max(per, 1));
-            g_resident.emplace((const void*)kernel, cap);
+            g_resident.emplace(key, cap);
         } else {
             cap = it->second;
         }
     }
-    return std::min(blocks_for_waves(waves), cap);
+    const size_t wpb = (size_t)std::max(block, 64) / 64;
+    const size_t b = (waves + wpb - 1) / wpb;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>({b, (size_t)1 << 20, (size_t)cap}));
 }
 
 // Grid of a kernel that splits its work by XCD (xcd_wave): a multiple of 8.
@@ -3129,9 +3137,13 @@ inline unsigned ticket_grid(unsigned blocks) {
 
 // Wavefronts of `kernel` that fit on the device at once.
 template <class K>
-size_t resident_waves(K kernel) {
-    return (size_t)resident_blocks(kernel, ~(size_t)0 >> 8) * (kBlock / 64);
+size_t resident_waves(K kernel, int block = kBlock, size_t lds = 0) {
+    return (size_t)resident_blocks(kernel, ~(size_t)0 >> 8, block, lds) * (block / 64);
 }
+
+// Dynamic LDS of the kernels that stage queue entries per wavefront
+// (QueueStage): kPullBuf 16-byte entries per wavefront of a `block`.
+inline size_t pull_lds_bytes(int block) { return (size_t)(block / 64) * kPullBuf * 16; }
 
 // Durations of one kernel's launches from pairs of events recorded around
 // each launch on the stream it runs on; read back only once the stream has
@@ -3266,10 +3278,15 @@ bool real_bucketed(const Lengths& L, uint64_t n) {
     return window <= kRealWindowMax && last < 4.0e9;
 }
 
-Buckets buckets_for(const Lengths& L) {
+// (width: integer lengths' distances per bucket, at most the shortest arc)
+Buckets buckets_for(const Lengths& L, int width = 1) {
     Buckets b;
     if (L.kind == kInteger) {
-        b.W = (int)L.max + 1;
+        // bucket k holds distances [k w, k w + w); offers from it reach at
+        // most k w + w - 1 + max, i.e. (w - 1 + max) / w buckets ahead
+        const uint64_t w = (uint64_t)std::max(width, 1);
+        b.W = (int)((L.max + w - 1) / w + 1);
+        b.delta = (double)w;
     } else if (L.kind == kReal) {
         b.delta = L.rmin / 2.0;
         b.W = (int)(std::floor(L.rmax / b.delta) + 2.0);
@@ -3338,6 +3355,14 @@ RunOptions tuned_options(const Graph& g, const Lengths& L, const RunOptions& opt
         o.slots = t ? t->slots : (L.kind == kReal && !real_bucketed(L, g.n) ? 4 : 16);
     if (!(o.alpha > 0.0)) o.alpha = t ? t->alpha : kBottomUpAlpha;
     if (o.sweep_group <= 0) o.sweep_group = t ? t->sweep_group : (int)kSweepGroup;
+    if (o.sweep_block <= 0) o.sweep_block = t ? t->sweep_block : kBlock;
+    if (o.step_block <= 0) o.step_block = t ? t->step_block : kBlock;
+    if (o.hub_in <= 0) o.hub_in = t ? t->hub_in : (int)kHubIn;
+    if (o.hub_chunk <= 0) o.hub_chunk = t ? t->hub_chunk : (int)kHubChunk;
+    if (o.bucket_width <= 0) o.bucket_width = t ? t->bucket_width : 1;
+    // buckets no wider than the shortest arc are settled whole
+    if (L.kind == kInteger) o.bucket_width = (int)std::min<uint32_t>(o.bucket_width, L.min);
+    else o.bucket_width = 1;
     return o;
 }
 
@@ -3453,6 +3478,9 @@ struct Engine {
 
     bool run_bfs(int64_t item0) {
         const DeviceGraph& d = g.dev;
+        // a super-batch repeated after a queue overflow counts its levels once
+        const int64_t levels0 = diag.levels;
+        const int32_t bottom_up0 = diag.bottom_up_levels;
         reset_slots();
         if (lane_buf) {
             const size_t words = (size_t)5 * slots * kLanes;
@@ -3509,8 +3537,9 @@ struct Engine {
                     ws.sweep.mark(st);
                     stale_bytes += 24ull * (ze - zb);
                 }
-                pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64)), kBlock, kPullLds,
-                       st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
+                const int pb = opt.step_block;
+                pull<<<xcd_grid(resident_blocks(pull, (pairs + 63) / 64, pb, pull_lds_bytes(pb))),
+                       pb, pull_lds_bytes(pb), st>>>(ws.q.p, zb, ze, d.in_off_p(), d.in_idx_p(), d.out_off.p,
                                         ws.vis.p, cur, nxt, ws.succ.p, ws.sig.p,
                                         d.hub_flag.p, d.heavy_flag.p, ws.q.p, (uint32_t)ws.qcap,
                                         ws.hq.p, ws.ctrl.p, front[(dl + 1) & 1], front[dl & 1],
@@ -3538,10 +3567,10 @@ struct Engine {
             } else if (d.hubs && !reach_only) {
                 const size_t hub_tasks = (size_t)d.hub_chunks * slots;
                 auto hub = npr ? bfs_hub_kernel<true> : bfs_hub_kernel<false>;
-                const uint32_t gsz = group_size(hub_tasks, resident_waves(hub));
-                hub<<<xcd_grid(resident_blocks(hub,
-                                                          (hub_tasks + gsz - 1) / gsz)),
-                                 kBlock, 0, st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
+                const int hb = opt.step_block;
+                const uint32_t gsz = group_size(hub_tasks, resident_waves(hub, hb));
+                hub<<<xcd_grid(resident_blocks(hub, (hub_tasks + gsz - 1) / gsz, hb)), hb, 0,
+                      st>>>(d.hub_task.p, d.hub_chunks, slots, d.in_idx_p(),
                                                   cur, ws.cand.p, ws.sig.p, bottom_up ? 1 : 0,
                                                   ws.vis.p, ws.succ.p, ws.touched.p,
                                                   &ws.ctrl.p->tlen[dl & 1],
@@ -3563,6 +3592,8 @@ struct Engine {
             read_ctrl(ws, st);
             if (ws.host_ctrl->qoverflow) {
                 ws.zero_words = ws.levels.n;
+                diag.levels = levels0;
+                diag.bottom_up_levels = bottom_up0;
                 return false;
             }
             uint32_t ql = ws.host_ctrl->qlen;
@@ -3685,8 +3716,9 @@ struct Engine {
             uint32_t ab = rebuild && dl + 2 <= D ? lv[dl + 2] : 0;
             uint32_t ae = rebuild && dl + 2 <= D ? lv[dl + 3] : 0;
             bool heavy = split && dl < D && he > hb;
+            const int sb = opt.sweep_block;
             uint32_t gsz =
-                group_size(qe - qb, resident_waves(bfs_backward_kernel<C, kEdge, kQuirk>),
+                group_size(qe - qb, resident_waves(bfs_backward_kernel<C, kEdge, kQuirk>, sb),
                            (uint32_t)std::min(std::max(opt.sweep_group, 1), 64));
             // A task walks its entries' rows as one concatenation indexed in
             // 32 bits: with whole rows (no split) of up to 2^32 - 2 arcs,
@@ -3700,8 +3732,8 @@ struct Engine {
             }
             ws.sweep.mark(st);
             bfs_backward_kernel<C, kEdge, kQuirk>
-                <<<ticket_grid(resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves)), kBlock,
-                   0, st>>>(
+                <<<ticket_grid(resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves, sb)),
+                   sb, 0, st>>>(
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     LevelBits{level_bits(dl + 1)}, level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
@@ -3723,8 +3755,9 @@ struct Engine {
     template <class P>
     bool run_bucketed(int64_t item0) {
         const DeviceGraph& d = g.dev;
+        const int64_t levels0 = diag.levels;  // counted once if the super-batch repeats
         reset_slots();
-        const Buckets bk = buckets_for(L);
+        const Buckets bk = buckets_for(L, opt.bucket_width);
         const int W = bk.W;
         const int words = bk.words;
         typename P::Dist* dist = dist_rows<P>();
@@ -3762,7 +3795,10 @@ struct Engine {
                 &ws.ctrl.p->next_bucket[p], n, npr);
             launch_check();
             read_ctrl(ws, st);
-            if (ws.host_ctrl->qoverflow) return false;
+            if (ws.host_ctrl->qoverflow) {
+                diag.levels = levels0;
+                return false;
+            }
             const uint32_t ql = ws.host_ctrl->qlen;
             if (ql > lv.back()) {
                 lv.push_back(ql);
@@ -4175,6 +4211,8 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     if (edge && g.m) NETBC_HIP(hipMemsetAsync(d_eb, 0, g.m * sizeof(double), st));
     if (k == 0) return;
     Workspace& ws = get_workspace(g);
+    // hub tables cut for this run's thresholds; rows of new hubs start at zero
+    if (ensure_hub_tables(g, (uint32_t)opt.hub_in, (uint32_t)opt.hub_chunk)) ws.rows_dirty = true;
     const bool ring = ring_path(g, L);
     int slots = choose_slots(g, L, k, opt, edge, ring);
     diag.slots = slots;
@@ -4418,6 +4456,8 @@ void run_closeness(Graph& g, const Lengths& L, const RunOptions& opt_in, const C
     if (n == 0) return;
     const int64_t k = (int64_t)n;
     Workspace& ws = get_workspace(g);
+    // hub tables cut for this run's thresholds; rows of new hubs start at zero
+    if (ensure_hub_tables(g, (uint32_t)opt.hub_in, (uint32_t)opt.hub_chunk)) ws.rows_dirty = true;
     const bool per_thread = L.kind == kReal && !real_bucketed(L, n);
     const int slots = choose_slots(g, L, k, opt, false);
     diag.slots = slots;
@@ -4581,6 +4621,8 @@ void run_dump(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
     if (L.kind == kReal) throw Error("the state dump covers breadth-first and Dial searches");
     Workspace& ws = get_workspace(g);
     RunOptions o = tuned_options(g, L, opt);
+    // hub tables cut for this run's thresholds; rows of new hubs start at zero
+    if (ensure_hub_tables(g, (uint32_t)o.hub_in, (uint32_t)o.hub_chunk)) ws.rows_dirty = true;
     const bool ring = ring_path(g, L);
     int slots = choose_slots(g, L, k, o, false, ring);
     size_t qcap = std::max<size_t>(queue_capacity(slots, n), ws.qcap);

@@ -108,6 +108,19 @@ void check_options(const netbc_options* opt) {
         throw netbc::Error("sweep group " + std::to_string(opt->sweep_group) +
                            " is outside 0..64");
     if (!(opt->alpha >= 0.0)) throw netbc::Error("negative or NaN direction factor");
+    for (const auto& [v, what] : {std::pair<int32_t, const char*>{opt->sweep_block, "sweep"},
+                                  {opt->step_block, "step"}})
+        if (v < 0 || v > 1024 || v % 64)
+            throw netbc::Error(std::string(what) + " block " + std::to_string(v) +
+                               " is not 0 or a multiple of 64 up to 1024");
+    if (opt->hub_in < 0 || opt->hub_in > (1 << 30))
+        throw netbc::Error("hub in-degree " + std::to_string(opt->hub_in) +
+                           " is outside 0..2^30");
+    if (opt->hub_chunk < 0 || opt->hub_chunk > 65536 || opt->hub_chunk % 64)
+        throw netbc::Error("hub chunk " + std::to_string(opt->hub_chunk) +
+                           " is not 0 or a multiple of 64 up to 65536");
+    if (opt->bucket_width < 0) throw netbc::Error("negative bucket width");
+    if (opt->reserved != 0) throw netbc::Error("the reserved option field must be 0");
 }
 
 netbc::RunOptions options(const netbc_options* opt) {
@@ -119,6 +132,11 @@ netbc::RunOptions options(const netbc_options* opt) {
         o.alpha = opt->alpha;
         o.exact = opt->exact;
         o.sweep_group = opt->sweep_group;
+        o.sweep_block = opt->sweep_block;
+        o.step_block = opt->step_block;
+        o.hub_in = opt->hub_in;
+        o.hub_chunk = opt->hub_chunk;
+        o.bucket_width = opt->bucket_width;
     }
     return o;
 }

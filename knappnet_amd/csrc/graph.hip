@@ -294,7 +294,7 @@ struct ClassifyOut {
     uint32_t not_uniform;
     uint32_t not_integer;
     uint32_t max;
-    uint32_t pad;
+    uint32_t min;  // shortest integer length
 };
 
 __global__ void first_nonpositive_kernel(const double* __restrict__ w, size_t m,
@@ -314,10 +314,12 @@ __global__ void classify_kernel(const double* __restrict__ w, size_t arcs, int s
     double x = arc_length(w[a], strength);
     double x0 = arc_length(w[0], strength);
     if (!(x == x0)) out->not_uniform = 1;
-    if (x - trunc(x) == 0.0 && x >= 1.0 && x <= 4294967295.0)
+    if (x - trunc(x) == 0.0 && x >= 1.0 && x <= 4294967295.0) {
         atomicMax(&out->max, (uint32_t)x);
-    else
+        atomicMin(&out->min, (uint32_t)x);
+    } else {
         out->not_integer = 1;
+    }
 }
 
 __global__ void int_lengths_kernel(const double* __restrict__ w, size_t arcs, int strength,
@@ -404,6 +406,46 @@ void ensure_stream(Graph& g) {
     g.device = dev;
 }
 
+// Hub flags, list and tasks of g's device graph for the given thresholds
+// (in_off: the in-row offsets on the host); sets max_in.
+void cut_hub_tables(Graph& g, const uint32_t* ioff, uint32_t hub_in, uint32_t hub_chunk) {
+    DeviceGraph& d = g.dev;
+    std::vector<uint8_t> flag;
+    std::vector<uint32_t> hub_list;
+    std::vector<HubTask> cut;
+    d.max_in = cut_hub_tasks(ioff, g.n, hub_in, hub_chunk, flag, hub_list, cut);
+    std::vector<uint4> task(cut.size());
+    for (size_t i = 0; i < cut.size(); i++)
+        task[i] = make_uint4(cut[i].hub, cut[i].first, cut[i].end, 0);
+    d.hubs = (uint32_t)hub_list.size();
+    d.hub_list.alloc(hub_list.size());
+    if (!hub_list.empty())
+        NETBC_HIP(hipMemcpy(d.hub_list.p, hub_list.data(), hub_list.size() * 4,
+                            hipMemcpyHostToDevice));
+    d.hub_chunks = (uint32_t)task.size();
+    d.hub_flag.alloc(flag.size());
+    NETBC_HIP(hipMemcpy(d.hub_flag.p, flag.data(), flag.size(), hipMemcpyHostToDevice));
+    d.hub_task.alloc(task.size());
+    if (!task.empty())
+        NETBC_HIP(hipMemcpy(d.hub_task.p, task.data(), task.size() * sizeof(uint4),
+                            hipMemcpyHostToDevice));
+    d.hub_in = hub_in;
+    d.hub_chunk = hub_chunk;
+}
+
+bool ensure_hub_tables(Graph& g, uint32_t hub_in, uint32_t hub_chunk) {
+    if (hub_in == 0 || hub_chunk < 64 || hub_chunk % 64)
+        throw Error("hub thresholds out of range: in-degree " + std::to_string(hub_in) +
+                    ", chunk " + std::to_string(hub_chunk));
+    DeviceGraph& d = g.dev;
+    if (d.hub_in == hub_in && d.hub_chunk == hub_chunk) return false;
+    NETBC_HIP(hipStreamSynchronize(g.stream));  // the old tables may be in use
+    std::vector<uint32_t> ioff(g.n + 1);
+    NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));
+    cut_hub_tables(g, ioff.data(), hub_in, hub_chunk);
+    return true;
+}
+
 void build_device_graph(Graph& g) {
     if (g.dev_ready) return;
     ensure_stream(g);
@@ -450,25 +492,7 @@ void build_device_graph(Graph& g) {
     // Hub tables from the in-degrees.
     std::vector<uint32_t> ioff(g.n + 1);
     NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));
-    std::vector<uint8_t> flag;
-    std::vector<uint32_t> hub_list;
-    std::vector<HubTask> cut;
-    d.max_in = cut_hub_tasks(ioff.data(), g.n, kHubIn, kHubChunk, flag, hub_list, cut);
-    std::vector<uint4> task(cut.size());
-    for (size_t i = 0; i < cut.size(); i++)
-        task[i] = make_uint4(cut[i].hub, cut[i].first, cut[i].end, 0);
-    d.hubs = (uint32_t)hub_list.size();
-    d.hub_list.alloc(hub_list.size());
-    if (!hub_list.empty())
-        NETBC_HIP(hipMemcpy(d.hub_list.p, hub_list.data(), hub_list.size() * 4,
-                            hipMemcpyHostToDevice));
-    d.hub_chunks = (uint32_t)task.size();
-    d.hub_flag.alloc(flag.size());
-    NETBC_HIP(hipMemcpy(d.hub_flag.p, flag.data(), flag.size(), hipMemcpyHostToDevice));
-    d.hub_task.alloc(task.size());
-    if (!task.empty())
-        NETBC_HIP(hipMemcpy(d.hub_task.p, task.data(), task.size() * sizeof(uint4),
-                            hipMemcpyHostToDevice));
+    cut_hub_tables(g, ioff.data(), kHubIn, kHubChunk);
     // Heavy rows from the out-degrees.
     std::vector<uint32_t> ooff(g.n + 1);
     NETBC_HIP(hipMemcpy(ooff.data(), d.out_off.p, (g.n + 1) * 4, hipMemcpyDeviceToHost));
@@ -501,7 +525,7 @@ void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L) {
     DeviceGraph& d = g.dev;
     DevBuf<ClassifyOut> res;
     res.alloc(1);
-    ClassifyOut init{~0ull, 0, 0, 0, 0};
+    ClassifyOut init{~0ull, 0, 0, 0, ~0u};
     NETBC_HIP(hipMemcpyAsync(res.p, &init, sizeof init, hipMemcpyHostToDevice, st));
     // The raw weights in input order are the out-CSR weights permuted; the
     // first offending edge index is found on the host copy's order by
@@ -531,6 +555,7 @@ void classify_lengths(Graph& g, bool weighted, int weights_as, Lengths& L) {
     if (!out.not_integer && out.max <= 1024u) {
         L.kind = kInteger;
         L.max = out.max;
+        L.min = out.min;
         L.out_len.alloc(d.arcs);
         int_lengths_kernel<<<grid_for(d.arcs), kBlock, 0, st>>>(d.out_w.p, d.arcs, strength,
                                                                 L.out_len.p);

@@ -124,6 +124,7 @@ struct DeviceGraph {
     DevBuf<uint32_t> hub_list;         // the hubs, ascending
     DevBuf<uint4> hub_task;            // per chunk: (hub, first in-arc, end in-arc, 0)
     uint32_t hubs = 0, hub_chunks = 0;
+    uint32_t hub_in = 0, hub_chunk = 0;  // the thresholds the hub tables were cut with
     // Heavy rows: out-degree above kHeavyOut, expanded and swept in tasks
     // of kChunkArcs arcs.
     DevBuf<uint8_t> heavy_flag;        // per node
@@ -161,6 +162,7 @@ struct Graph {
 struct Lengths {
     LengthKind kind = kUniform;
     uint32_t max = 0;
+    uint32_t min = 0;  // integer lengths: the shortest arc
     DevBuf<uint32_t> out_len, in_len;  // integer lengths, CSR / CSC order
     DevBuf<double> out_real, in_real;  // real lengths, CSR / CSC order
     double rmin = 0.0, rmax = 0.0;     // real lengths: shortest and longest arc
@@ -175,6 +177,11 @@ struct RunOptions {
     int keep_sigma = 0;     // debug: keep per-source sigma and levels
     int exact = 0;          // 1: sum every row's successors in CSR order (no heavy-row split)
     int sweep_group = 0;    // entries per sweep task at most (0 = tuned default)
+    int sweep_block = 0;    // threads per workgroup of the dependency sweep (0 = tuned)
+    int step_block = 0;     // threads per workgroup of the bottom-up step and hub tasks
+    int hub_in = 0;         // in-degree above which a node is a hub (0 = tuned)
+    int hub_chunk = 0;      // in-arcs per hub task (0 = tuned)
+    int bucket_width = 0;   // integer lengths: distances per bucket (0 = tuned)
 };
 
 struct RunDiag {
@@ -195,6 +202,9 @@ struct RunDiag {
 
 // Builds the device CSR/CSC of g (graph.hip). Caller holds g.mu.
 void build_device_graph(Graph& g);
+// Re-cuts the hub tables for other thresholds (hub_in: in-degree above
+// which a node is a hub; hub_chunk: in-arcs per task); true if they changed.
+bool ensure_hub_tables(Graph& g, uint32_t hub_in, uint32_t hub_chunk);
 
 // The CSR/CSC of an unweighted graph of one fixed shape (n nodes, m edges,
 // directed or not), rebuilt from new edges in place: fixed_csr_alloc

@@ -52,6 +52,7 @@ struct SmallRun {
     uint32_t nmax = 1;
     size_t grid = 0;
     int precision = 0;
+    int block = 512;        // threads per workgroup (256, 512 or 1024)
     DevBuf<SmallGraph> graphs;
     DevBuf<uint32_t> node_graph, too_deep, counter;
     DevBuf<SmallItem> items;

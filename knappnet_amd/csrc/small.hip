@@ -20,9 +20,11 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "netbc_internal.h"
 #include "small.h"
+#include "tuning.h"
 
 namespace netbc {
 namespace {
@@ -109,7 +111,7 @@ __device__ __forceinline__ uint32_t arc_end(uint32_t u, uint32_t base, uint32_t 
 // frontier bitmaps vis/fa/fb in LDS), then the backward dependency sweep.
 // Leaves delta_s(v) of lane l in sig[v * 64 + l] for the reached lanes of
 // vis[v]. Returns false (every thread) when the search passes 254 levels.
-template <class C, bool kChecked>
+template <class C, bool kChecked, int kB = kSmallBlock>
 __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx,
                                const uint32_t* __restrict__ in_off,
@@ -122,7 +124,7 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
     const uint32_t nl = min(I.ngroups * G.gs, n - first);
     const uint64_t active = nl >= 64 ? ~0ull : ((1ull << nl) - 1);
 
-    for (uint32_t v = tid; v < n; v += kSmallBlock) {
+    for (uint32_t v = tid; v < n; v += kB) {
         vis[v] = 0;
         fa[v] = 0;
     }
@@ -144,7 +146,7 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
     int D = 0;
     for (int d = 0;; d++) {
         int any = 0;
-        for (uint32_t w = wave; w < n; w += kSmallWaves) {
+        for (uint32_t w = wave; w < n; w += kB / 64) {
             const uint64_t U = active & ~vis[w];
             uint64_t N = 0;
             if (U) {
@@ -189,7 +191,7 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
 
     // Backward: levels D..1. fa = lanes at level d, fb = lanes at d+1.
     for (int d = D; d >= 1; d--) {
-        for (uint32_t v = wave; v < n; v += kSmallWaves) {
+        for (uint32_t v = wave; v < n; v += kB / 64) {
             const uint8_t lv = lvl[(size_t)v * kLanes + lane];
             const uint64_t r = vis[v];
             const uint64_t Mv = __ballot(lv == d) & r;
@@ -200,7 +202,7 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
             }
         }
         __syncthreads();
-        for (uint32_t v = wave; v < n; v += kSmallWaves) {
+        for (uint32_t v = wave; v < n; v += kB / 64) {
             const uint64_t M = fa[v];
             if (!M) continue;
             SDep<C> dv;
@@ -251,8 +253,8 @@ __device__ __forceinline__ double group_sum(const SmallItem& I, const SmallGraph
     return acc;
 }
 
-template <class C>
-__global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
+template <class C, int kB>
+__global__ __launch_bounds__(kB) void small_batch_kernel(
     const SmallItem* __restrict__ items, uint32_t nitems, uint32_t* __restrict__ counter,
     const SmallGraph* __restrict__ graphs, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ idx, const uint32_t* __restrict__ in_off,
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
         if (it >= nitems) break;
         const SmallItem I = items[it];
         const SmallGraph G = graphs[I.g];
-        if (!run_small_item<C, false>(I, G, off, idx, in_off, in_idx, sig, coef, lvl, vis, fa,
+        if (!run_small_item<C, false, kB>(I, G, off, idx, in_off, in_idx, sig, coef, lvl, vis, fa,
                                       fb, chk)) {
             if (tid == 0) too_deep[I.g] = 1;
             __syncthreads();
@@ -287,7 +289,7 @@ __global__ __launch_bounds__(kSmallBlock) void small_batch_kernel(
         // Group sums: for each whole group of the batch, in source order.
         const uint32_t first = I.j0 * G.gs;
         const uint32_t nl = min(I.ngroups * G.gs, G.n - first);
-        for (uint32_t v = tid; v < G.n; v += kSmallBlock) {
+        for (uint32_t v = tid; v < G.n; v += kB) {
             const uint64_t r = vis[v];
             reached += (unsigned long long)__popcll(r);
             reached_arcs += (unsigned long long)__popcll(r) *
@@ -539,6 +541,30 @@ ScratchDevice scratch_on_current_device() {
     return s;
 }
 
+// The coefficient type of a small_batch_kernel instance (for dispatch).
+template <class C>
+C* kernel_coef(void (*)(const SmallItem*, uint32_t, uint32_t*, const SmallGraph*,
+                        const uint32_t*, const uint32_t*, const uint32_t*, const uint32_t*,
+                        double*, C*, uint8_t*, size_t, double*, uint32_t*,
+                        unsigned long long*)) {
+    return nullptr;
+}
+
+// Calls f with the small_batch_kernel instance for the precision and
+// workgroup size (256, 512 or 1024 threads).
+template <class F>
+void with_small_kernel(int precision, int block, F&& f) {
+    if (precision) {
+        if (block == 256) f(small_batch_kernel<float, 256>);
+        else if (block == 1024) f(small_batch_kernel<float, 1024>);
+        else f(small_batch_kernel<float, 512>);
+    } else {
+        if (block == 256) f(small_batch_kernel<double, 256>);
+        else if (block == 1024) f(small_batch_kernel<double, 1024>);
+        else f(small_batch_kernel<double, 512>);
+    }
+}
+
 }  // namespace
 
 void small_setup(SmallRun& r, const std::vector<SmallGraph>& graphs,
@@ -586,13 +612,22 @@ void small_setup(SmallRun& r, const std::vector<SmallGraph>& graphs,
     NETBC_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     NETBC_HIP(hipGetDeviceProperties(&prop, dev));
+    // workgroup size: the caller's, else the tuning table's class 2 (many
+    // small unweighted graphs), else kSmallBlock
+    r.block = kSmallBlock;
+    for (const TuneEntry& e : kTuning)
+        if (e.kind == 2) {
+            r.block = e.sweep_block;
+            break;
+        }
+    if (opt.sweep_block > 0) r.block = opt.sweep_block;
+    if (r.block != 256 && r.block != 512 && r.block != 1024)
+        throw Error("the one-workgroup engine runs workgroups of 256, 512 or 1024 threads, not " +
+                    std::to_string(r.block));
     int per_cu = 0;
-    if (opt.precision)
-        NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, small_batch_kernel<float>, kSmallBlock, 0));
-    else
-        NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, small_batch_kernel<double>, kSmallBlock, 0));
+    with_small_kernel(r.precision, r.block, [&](auto kernel) {
+        NETBC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, r.block, 0));
+    });
     per_cu = std::max(1, per_cu);
     size_t grid = std::min<size_t>((size_t)prop.multiProcessorCount * per_cu,
                                    std::max<size_t>(items.size(), 1));
@@ -622,14 +657,15 @@ void small_enqueue(SmallRun& r, const uint32_t* off, const uint32_t* idx, const 
     NETBC_HIP(hipGetLastError());
     if (e0) NETBC_HIP(hipEventRecord(e0, st));
     if (r.nitems) {
-        if (r.precision)
-            small_batch_kernel<float><<<(unsigned)r.grid, kSmallBlock, 0, st>>>(
+        with_small_kernel(r.precision, r.block, [&](auto kernel) {
+            using C = std::remove_pointer_t<decltype(kernel_coef(kernel))>;
+            C* coef = nullptr;
+            if constexpr (std::is_same_v<C, float>) coef = r.coef32.p;
+            else coef = r.coef64.p;
+            kernel<<<(unsigned)r.grid, r.block, 0, st>>>(
                 r.items.p, r.nitems, r.counter.p, r.graphs.p, off, idx, in_off, in_idx, r.sig.p,
-                r.coef32.p, r.lvl.p, r.nmax, r.gsum.p, r.too_deep.p, r.stats.p);
-        else
-            small_batch_kernel<double><<<(unsigned)r.grid, kSmallBlock, 0, st>>>(
-                r.items.p, r.nitems, r.counter.p, r.graphs.p, off, idx, in_off, in_idx, r.sig.p,
-                r.coef64.p, r.lvl.p, r.nmax, r.gsum.p, r.too_deep.p, r.stats.p);
+                coef, r.lvl.p, r.nmax, r.gsum.p, r.too_deep.p, r.stats.p);
+        });
         NETBC_HIP(hipGetLastError());
     }
     if (e1) NETBC_HIP(hipEventRecord(e1, st));

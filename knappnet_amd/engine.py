@@ -27,6 +27,14 @@ def _i64(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
 
 
+def _options(precision=0, slots=0, direction=0, exact=False, alpha=0.0, sweep_group=0,
+             sweep_block=0, step_block=0, hub_in=0, hub_chunk=0, bucket_width=0):
+    """netbc_options from keyword settings (0: the library's tuned value)."""
+    return _native.Options(int(precision), int(slots), int(direction), int(bool(exact)),
+                           float(alpha), int(sweep_group), int(sweep_block), int(step_block),
+                           int(hub_in), int(hub_chunk), int(bucket_width), 0)
+
+
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(POINTER(ctype))
 
@@ -119,10 +127,7 @@ class Graph:
             if self.lib.net_betweenness_subset(*args, 0, *outs, err, ERR_LEN):
                 raise EngineError(err.value.decode())
             return nodes[: self.n], None if ev is None else ev[:m]
-        o = dict(precision=0, slots=0, direction=0, exact=False, alpha=0.0, sweep_group=0)
-        o.update(options)
-        opt = _native.Options(o["precision"], o["slots"], o["direction"], int(bool(o["exact"])),
-                              o["alpha"], o["sweep_group"], 0)
+        opt = _options(**options)
         diag = _native.Diagnostics()
         if self.lib.netbc_betweenness_subset_ex(*args, ctypes.byref(opt), *outs,
                                                 ctypes.byref(diag), err, ERR_LEN):
@@ -145,10 +150,7 @@ class Graph:
             if self.lib.net_closeness(*args, 0, _ptr(out, c_double), err, ERR_LEN):
                 raise EngineError(err.value.decode())
             return out[: self.n]
-        o = dict(precision=0, slots=0, direction=0, exact=False, alpha=0.0, sweep_group=0)
-        o.update(options)
-        opt = _native.Options(o["precision"], o["slots"], o["direction"], int(bool(o["exact"])),
-                              o["alpha"], o["sweep_group"], 0)
+        opt = _options(**options)
         diag = _native.Diagnostics()
         if self.lib.netbc_closeness_ex(*args, ctypes.byref(opt), _ptr(out, c_double),
                                        ctypes.byref(diag), err, ERR_LEN):
@@ -183,12 +185,10 @@ class Graph:
             raise EngineError(err.value.decode())
         return out[: self.n]
 
-    def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, precision,
-                      slots, direction, alpha, exact, sweep_group):
+    def _sources_call(self, fn, length, sources, weighted, normalized, weights_as, options):
         out = np.zeros(max(length, 1), dtype=np.float64)
         err = ctypes.create_string_buffer(ERR_LEN)
-        opt = _native.Options(precision, slots, direction, int(bool(exact)), alpha,
-                              sweep_group, 0)
+        opt = _options(**options)
         diag = _native.Diagnostics()
         kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
         if sources is None:
@@ -202,22 +202,20 @@ class Graph:
         return out[:length], diag.as_dict()
 
     def betweenness_sources(self, sources=None, weighted: bool = False,
-                            normalized: bool = False, weights_as: str = "distance",
-                            precision: int = 0, slots: int = 0, direction: int = 0,
-                            alpha: float = 0.0, exact: bool = False, sweep_group: int = 0):
-        """``netbc_betweenness_sources``: returns (scores, diagnostics dict)."""
+                            normalized: bool = False, weights_as: str = "distance", **options):
+        """``netbc_betweenness_sources``: returns (scores, diagnostics dict).
+        ``options``: the fields of netbc_options (precision, slots,
+        direction, exact, alpha, sweep_group, sweep_block, step_block,
+        hub_in, hub_chunk, bucket_width)."""
         return self._sources_call(self.lib.netbc_betweenness_sources, self.n, sources,
-                                  weighted, normalized, weights_as, precision, slots, direction,
-                                  alpha, exact, sweep_group)
+                                  weighted, normalized, weights_as, options)
 
     def edge_betweenness_sources(self, sources=None, weighted: bool = False,
                                  normalized: bool = False, weights_as: str = "distance",
-                                 precision: int = 0, slots: int = 0, direction: int = 0,
-                                 alpha: float = 0.0, exact: bool = False, sweep_group: int = 0):
+                                 **options):
         """``netbc_edge_betweenness_sources``: returns (edge scores, diagnostics dict)."""
         return self._sources_call(self.lib.netbc_edge_betweenness_sources, self.edge_count(),
-                                  sources, weighted, normalized, weights_as, precision, slots,
-                                  direction, alpha, exact, sweep_group)
+                                  sources, weighted, normalized, weights_as, options)
 
     def state_dump(self, sources, weighted: bool = False, weights_as: str = "distance",
                    precision: int = 0):
@@ -251,9 +249,11 @@ class Graph:
 
 def betweenness_batch(graphs, directed: bool = False, weighted: bool = False,
                       normalized: bool = False, weights_as: str = "distance",
-                      precision: int = 0, slots: int = 0):
+                      precision: int = 0, slots: int = 0, sweep_block: int = 0):
     """``netbc_betweenness_batch`` over a list of (n, src, tgt[, weights])
-    graphs; returns (list of per-graph score arrays, diagnostics dict)."""
+    graphs; returns (list of per-graph score arrays, diagnostics dict).
+    sweep_block: the one-workgroup engine's workgroup size (256, 512 or
+    1024; 0: the tuned size)."""
     lib = _native.device()
     ns = np.array([int(g[0]) for g in graphs], dtype=np.int64)
     ms = np.array([len(g[1]) for g in graphs], dtype=np.int64)
@@ -267,7 +267,7 @@ def betweenness_batch(graphs, directed: bool = False, weighted: bool = False,
                                                  for g in graphs]))
     out = np.zeros(max(int(ns.sum()), 1), dtype=np.float64)
     err = ctypes.create_string_buffer(ERR_LEN)
-    opt = _native.Options(precision, slots, 0, 0, 0.0)
+    opt = _options(precision=precision, slots=slots, sweep_block=sweep_block)
     diag = _native.Diagnostics()
     kind = KIND[weights_as] if isinstance(weights_as, str) else int(weights_as)
     if lib.netbc_betweenness_batch(len(graphs), _ptr(ns, c_int64), _ptr(ms, c_int64),
@@ -296,7 +296,7 @@ class BatchPlan:
             raise ValueError("node and edge counts differ in length")
         self.handle = ctypes.c_void_p()
         err = ctypes.create_string_buffer(ERR_LEN)
-        opt = _native.Options(precision, slots, 0, 0, 0.0)
+        opt = _options(precision=precision, slots=slots)
         if self.lib.netbc_batch_plan_new(len(self.ns), _ptr(self.ns, c_int64),
                                          _ptr(self.ms, c_int64), int(bool(directed)),
                                          int(bool(normalized)), int(bool(capture)),
