@@ -33,7 +33,6 @@ namespace {
 #define NETBC_SMALL_BLOCK 512
 #endif
 constexpr int kSmallBlock = NETBC_SMALL_BLOCK;
-constexpr int kSmallWaves = kSmallBlock / 64;
 
 __device__ __forceinline__ uint32_t srl32(uint32_t x, int j) {
     return (uint32_t)__builtin_amdgcn_readlane((int)x, j);
@@ -106,11 +105,46 @@ __device__ __forceinline__ uint32_t arc_end(uint32_t u, uint32_t base, uint32_t 
     return x;
 }
 
+// Exclusive prefix of per-lane counts over the wavefront (total: their sum)
+// and the lane whose items hold position t of the lanes' concatenation
+// (the last lane c with pre[c] <= t). Idx: uint32_t where the rows of a
+// graph are disjoint (their lengths sum to at most its 2^32 - 2 arcs),
+// uint64_t for rows the host has not validated, which may overlap.
+template <class Idx>
+__device__ __forceinline__ Idx swave_prefix(Idx v, Idx& total) {
+    const int lane = threadIdx.x & 63;
+    Idx incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const Idx y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    total = __shfl(incl, 63, 64);
+    return incl - v;
+}
+template <class Idx>
+__device__ __forceinline__ int sowner_lane(Idx pre, Idx t) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+        const Idx pm = __shfl(pre, lo + step, 64);
+        if (pm <= t) lo += step;
+    }
+    return lo;
+}
+
+// Rows loaded at once by the walks below.
+constexpr int kSmallAhead = 8;
+
 // One batch of sources (work item I of graph G) from start to finish in
-// one workgroup: the breadth-first levels (dense pulls over the graph,
-// frontier bitmaps vis/fa/fb in LDS), then the backward dependency sweep.
-// Leaves delta_s(v) of lane l in sig[v * 64 + l] for the reached lanes of
-// vis[v]. Returns false (every thread) when the search passes 254 levels.
+// one workgroup: the breadth-first levels (pulls over the nodes with lanes
+// still unreached, frontier bitmaps vis/fa/fb in LDS), then the backward
+// dependency sweep. A wavefront takes 64 nodes at a time (lane c: node
+// w0 + c) and walks their rows as one concatenation, 64 arcs per step, so
+// that the memory accesses of 64 nodes overlap; hits are taken in arc
+// order, so each node's sums run over its row in CSR order. Leaves
+// delta_s(v) of lane l in sig[v * 64 + l] for the reached lanes of vis[v].
+// Returns false (every thread) when the search passes 254 levels.
 template <class C, bool kChecked, int kB = kSmallBlock>
 __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
                                const uint32_t* __restrict__ off, const uint32_t* __restrict__ idx,
@@ -118,6 +152,7 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
                                const uint32_t* __restrict__ in_idx, double* __restrict__ sig,
                                C* __restrict__ coef, uint8_t* __restrict__ lvl, uint64_t* vis,
                                uint64_t* fa, uint64_t* fb, const ArcCheck& chk) {
+    using Idx = std::conditional_t<kChecked, uint64_t, uint32_t>;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t n = G.n, base = G.node0;
     const uint32_t first = I.j0 * G.gs;
@@ -140,44 +175,82 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
 
     // Forward: level d -> d+1 by a pull over every node with unreached
     // lanes; sigma[w][l] sums sigma[u][l] over in-arcs u->w with u at
-    // level d in lane l (in CSC order; exact integers).
+    // level d in lane l (exact integers).
     uint64_t* cur = fa;
     uint64_t* nxt = fb;
     int D = 0;
     for (int d = 0;; d++) {
         int any = 0;
-        for (uint32_t w = wave; w < n; w += kB / 64) {
-            const uint64_t U = active & ~vis[w];
-            uint64_t N = 0;
-            if (U) {
-                double acc = 0.0;
-                uint64_t a0, a1;
-                row_bounds<kChecked>(in_off, base + w, I.g, chk, a0, a1);
-                for (uint64_t c = a0; c < a1; c += 64) {
-                    const uint64_t a = c + lane;
-                    const bool ok = a < a1;
-                    const uint32_t u = ok ? arc_end<kChecked>(in_idx[a], base, n, I.g, chk) : n;
-                    const uint64_t mm = u < n ? (cur[u] & U) : 0;
-                    uint64_t bal = __ballot(mm != 0);
-                    while (bal) {
-                        const int j = __ffsll((unsigned long long)bal) - 1;
-                        bal &= bal - 1;
-                        const uint32_t uu = srl32(u, j);
-                        const uint64_t m2 = srl64(mm, j);
-                        N |= m2;
-                        if ((m2 >> lane) & 1) acc += sig[(size_t)uu * kLanes + lane];
+        for (uint32_t w0 = (uint32_t)wave * 64; w0 < n; w0 += kB) {
+            const uint32_t w = w0 + (uint32_t)lane;
+            const uint64_t U = w < n ? (active & ~vis[w]) : 0;
+            uint64_t a0 = 0, a1 = 0;
+            if (U) row_bounds<kChecked>(in_off, base + w, I.g, chk, a0, a1);
+            Idx T;
+            const Idx pre = swave_prefix((Idx)(a1 - a0), T);
+            int curc = -1;  // node whose hits are being summed (wave-uniform)
+            uint64_t ncur = 0, mine = 0;
+            double acc = 0.0;
+            auto close_pair = [&]() {
+                if (curc < 0) return;
+                const uint32_t xs = w0 + (uint32_t)curc;
+                if ((ncur >> lane) & 1) {
+                    sig[(size_t)xs * kLanes + lane] = acc;
+                    lvl[(size_t)xs * kLanes + lane] = (uint8_t)(d + 1);
+                }
+                if (lane == curc) mine = ncur;
+                acc = 0.0;
+                ncur = 0;
+            };
+            for (Idx t0 = 0; t0 < T; t0 += 64) {
+                const Idx t = t0 + (Idx)lane;
+                const int c = sowner_lane(pre, t);
+                const uint64_t ac = __shfl(a0, c, 64);
+                const Idx pc = __shfl(pre, c, 64);
+                const uint64_t Uc = __shfl(U, c, 64);
+                uint32_t u = n;
+                uint64_t m = 0;
+                if (t < T) {
+                    u = arc_end<kChecked>(in_idx[ac + (t - pc)], base, n, I.g, chk);
+                    if (u < n) m = cur[u] & Uc;
+                }
+                uint64_t hits = __ballot(m != 0);
+                while (hits) {
+                    double val[kSmallAhead];
+                    int hc[kSmallAhead];
+                    uint64_t hm[kSmallAhead];
+#pragma unroll
+                    for (int k = 0; k < kSmallAhead; k++) {
+                        val[k] = 0.0;
+                        hc[k] = -1;
+                        hm[k] = 0;
+                        if (hits) {
+                            const int j = __ffsll((unsigned long long)hits) - 1;
+                            hits &= hits - 1;
+                            const uint32_t uu = srl32(u, j);
+                            hm[k] = srl64(m, j);
+                            hc[k] = (int)srl32((uint32_t)c, j);
+                            if ((hm[k] >> lane) & 1) val[k] = sig[(size_t)uu * kLanes + lane];
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kSmallAhead; k++) {
+                        if (hc[k] < 0) break;
+                        if (hc[k] != curc) {
+                            close_pair();
+                            curc = hc[k];
+                        }
+                        ncur |= hm[k];
+                        acc += val[k];
                     }
                 }
-                if ((N >> lane) & 1) {
-                    sig[(size_t)w * kLanes + lane] = acc;
-                    lvl[(size_t)w * kLanes + lane] = (uint8_t)(d + 1);
-                }
             }
-            if (lane == 0) {
-                nxt[w] = N;
-                if (N) vis[w] |= N;
+            close_pair();
+            if (w < n) {
+                nxt[w] = mine;
+                if (mine) vis[w] |= mine;
             }
-            any |= N != 0;
+            any |= mine != 0;
         }
         if (!__syncthreads_or(any)) {
             D = d;
@@ -191,45 +264,110 @@ __device__ bool run_small_item(const SmallItem& I, const SmallGraph& G,
 
     // Backward: levels D..1. fa = lanes at level d, fb = lanes at d+1.
     for (int d = D; d >= 1; d--) {
-        for (uint32_t v = wave; v < n; v += kB / 64) {
-            const uint8_t lv = lvl[(size_t)v * kLanes + lane];
-            const uint64_t r = vis[v];
-            const uint64_t Mv = __ballot(lv == d) & r;
-            const uint64_t Sv = __ballot(lv == d + 1) & r;
-            if (lane == 0) {
-                fa[v] = Mv;
-                fb[v] = Sv;
+        // a wavefront reads 4 nodes' level rows at once (byte per lane)
+        for (uint32_t v0 = (uint32_t)wave * 4; v0 < n; v0 += kB / 16) {
+            uint8_t lv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                lv[q] = v0 + q < n ? lvl[(size_t)(v0 + q) * kLanes + lane] : (uint8_t)0xFF;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t v = v0 + (uint32_t)q;
+                if (v >= n) break;
+                const uint64_t r = vis[v];
+                const uint64_t Mv = __ballot(lv[q] == d) & r;
+                const uint64_t Sv = __ballot(lv[q] == d + 1) & r;
+                if (lane == 0) {
+                    fa[v] = Mv;
+                    fb[v] = Sv;
+                }
             }
         }
         __syncthreads();
-        for (uint32_t v = wave; v < n; v += kB / 64) {
-            const uint64_t M = fa[v];
-            if (!M) continue;
+        for (uint32_t v0 = (uint32_t)wave * 64; v0 < n; v0 += kB) {
+            const uint32_t v = v0 + (uint32_t)lane;
+            const uint64_t M = v < n ? fa[v] : 0;
+            uint64_t a0 = 0, a1 = 0;
+            if (M && d < D) row_bounds<kChecked>(off, base + v, I.g, chk, a0, a1);
+            Idx T;
+            const Idx pre = swave_prefix((Idx)(a1 - a0), T);
+            bool done = M == 0;
+            int cur_e = -1;  // node whose successors are being summed (wave-uniform)
+            uint64_t Me = 0;
+            double sv = 0.0;
             SDep<C> dv;
-            if (d < D) {
-                uint64_t a0, a1;
-                row_bounds<kChecked>(off, base + v, I.g, chk, a0, a1);
-                for (uint64_t c = a0; c < a1; c += 64) {
-                    const uint64_t a = c + lane;
-                    const bool ok = a < a1;
-                    const uint32_t x = ok ? arc_end<kChecked>(idx[a], base, n, I.g, chk) : n;
-                    const uint64_t mm = x < n ? (fb[x] & M) : 0;
-                    uint64_t bal = __ballot(mm != 0);
-                    while (bal) {
-                        const int j = __ffsll((unsigned long long)bal) - 1;
-                        bal &= bal - 1;
-                        const uint32_t xx = srl32(x, j);
-                        const uint64_t m2 = srl64(mm, j);
-                        if ((m2 >> lane) & 1) dv.add(coef[(size_t)xx * kLanes + lane]);
+            auto finish = [&](uint32_t vv, uint64_t Mv, double s, double dvv) {
+                if ((Mv >> lane) & 1) {
+                    const size_t r = (size_t)vv * kLanes + lane;
+                    const double delta = s * dvv;
+                    coef[r] = SDep<C>::coef(delta, s);
+                    sig[r] = delta;
+                }
+            };
+            auto close_entry = [&]() {
+                if (cur_e < 0) return;
+                finish(v0 + (uint32_t)cur_e, Me, sv, dv.value());
+                if (lane == cur_e) done = true;
+                dv = SDep<C>();
+            };
+            auto open_entry = [&](int c) {
+                cur_e = c;
+                Me = srl64(M, c);
+                sv = ((Me >> lane) & 1) ? sig[(size_t)(v0 + (uint32_t)c) * kLanes + lane] : 0.0;
+            };
+            for (Idx t0 = 0; t0 < T; t0 += 64) {
+                const Idx t = t0 + (Idx)lane;
+                const int c = sowner_lane(pre, t);
+                const uint64_t ac = __shfl(a0, c, 64);
+                const Idx pc = __shfl(pre, c, 64);
+                const uint64_t Mc = __shfl(M, c, 64);
+                uint32_t x = n;
+                uint64_t mm = 0;
+                if (t < T) {
+                    x = arc_end<kChecked>(idx[ac + (t - pc)], base, n, I.g, chk);
+                    if (x < n) mm = fb[x] & Mc;
+                }
+                uint64_t hits = __ballot(mm != 0);
+                while (hits) {
+                    C val[kSmallAhead];
+                    int hc[kSmallAhead];
+                    bool use[kSmallAhead];
+#pragma unroll
+                    for (int k = 0; k < kSmallAhead; k++) {
+                        val[k] = 0;
+                        hc[k] = -1;
+                        use[k] = false;
+                        if (hits) {
+                            const int j = __ffsll((unsigned long long)hits) - 1;
+                            hits &= hits - 1;
+                            const uint32_t xx = srl32(x, j);
+                            const uint64_t m2 = srl64(mm, j);
+                            hc[k] = (int)srl32((uint32_t)c, j);
+                            use[k] = (m2 >> lane) & 1;
+                            if (use[k]) val[k] = coef[(size_t)xx * kLanes + lane];
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kSmallAhead; k++) {
+                        if (hc[k] < 0) break;
+                        if (hc[k] != cur_e) {
+                            close_entry();
+                            open_entry(hc[k]);
+                        }
+                        if (use[k]) dv.add(val[k]);
                     }
                 }
             }
-            if ((M >> lane) & 1) {
-                const size_t r = (size_t)v * kLanes + lane;
-                const double s = sig[r];
-                const double delta = s * dv.value();
-                coef[r] = SDep<C>::coef(delta, s);
-                sig[r] = delta;
+            close_entry();
+            // entries without successors in their lanes: delta = 0
+            uint64_t rest = __ballot(!done);
+            while (rest) {
+                const int j = __ffsll((unsigned long long)rest) - 1;
+                rest &= rest - 1;
+                const uint64_t Mj = srl64(M, j);
+                const uint32_t vj = v0 + (uint32_t)j;
+                const double s = ((Mj >> lane) & 1) ? sig[(size_t)vj * kLanes + lane] : 0.0;
+                finish(vj, Mj, s, SDep<C>().value());
             }
         }
         __syncthreads();
