@@ -111,12 +111,18 @@ def test_tuning_table_writer_compiles(tmp_path):
     import subprocess
     from knappnet_amd.autotune import write_table
     out = tmp_path / "tuning.h"
-    write_table(out, [((0, 1), "R-MAT", {"slots": 64, "alpha": 1.0, "sweep_group": 32}),
-                      ((1, -1), "BA", {"slots": 16, "alpha": 0.5, "sweep_group": 16})], "gfx950")
+    s0 = dict(slots=64, alpha=1.0, sweep_group=32, sweep_block=256, step_block=512, hub_in=64,
+              hub_chunk=512, bucket_width=1)
+    write_table(out, [(0, 1, "R-MAT", s0),
+                      (1, -1, "BA", dict(s0, slots=16, alpha=0.5, bucket_width=2)),
+                      (2, 0, "graphs", dict(s0, sweep_block=1024))], "gfx950")
     src = tmp_path / "use.cpp"
     src.write_text('#include "tuning.h"\n'
-                   'static_assert(sizeof(netbc::kTuning) / sizeof(netbc::kTuning[0]) == 2, "");\n'
-                   'int main() { return netbc::kTuning[1].slots == 16 ? 0 : 1; }\n')
+                   'static_assert(sizeof(netbc::kTuning) / sizeof(netbc::kTuning[0]) == 3, "");\n'
+                   'int main() { return netbc::kTuning[1].slots == 16 &&\n'
+                   '    netbc::kTuning[1].bucket_width == 2 && netbc::kTuning[0].step_block == 512 &&\n'
+                   '    netbc::kTuning[2].kind == 2 && netbc::kTuning[2].sweep_block == 1024 &&\n'
+                   '    netbc::kTuning[2].slots == 0 ? 0 : 1; }\n')
     exe = tmp_path / "use"
     subprocess.run(["g++", "-std=c++17", "-I", str(tmp_path), str(src), "-o", str(exe)], check=True)
     assert subprocess.run([str(exe)]).returncode == 0
