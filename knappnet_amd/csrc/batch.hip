@@ -402,6 +402,9 @@ __device__ __forceinline__ uint32_t xcd_slots(uint32_t k, int slots) {
     return (uint32_t)slots > k ? ((uint32_t)slots - k + kXcds - 1) / kXcds : 0;
 }
 
+// Levels of at least this many entries are swept in slot order.
+constexpr uint32_t kSlotOrderMin = 1u << 16;
+
 // Ticket classes of the dependency sweep (a multiple of kXcds).
 constexpr uint32_t kTicketClasses = 32;
 
@@ -1290,7 +1293,8 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
                                     double* __restrict__ sig, C* __restrict__ coef, Ctrl* ctrl,
                                     uint32_t gsz, size_t n, double* __restrict__ eacc,
                                     size_t arcs, const uint8_t* __restrict__ target,
-                                    uint32_t* __restrict__ npr, double* __restrict__ bsum) {
+                                    uint32_t* __restrict__ npr, double* __restrict__ bsum,
+                                    const uint32_t* __restrict__ perm) {
     const int lane = lane_id();
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c4 = 0, c5 = 0, c6 = 0, c7 = 0;
@@ -1367,7 +1371,7 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
         bool live = false;  // this lane's entry is finished by this wave
         uint32_t a0 = 0, deg = 0;
         if ((uint32_t)lane < gsz && i < qend) {
-            en = q[i];
+            en = q[perm ? perm[i] : i];
             live = !(split && heavy_flag[en.v]);
             // rows without a successor in any of their lanes add nothing
             if (live && !last_level && (succ[(size_t)en.b * n + en.v] & en.m)) {
@@ -1498,6 +1502,73 @@ __global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, u
     NETBC_FLUSH(ctrl, 5, c5);
     NETBC_FLUSH(ctrl, 6, c6);
     NETBC_FLUSH(ctrl, 7, c7);
+}
+
+// Slot order of a level's queue entries for the dependency sweep: perm
+// lists the positions [qb, qe) of the level's entries grouped by slot (in
+// no particular order within a slot; the sweep's results do not depend on
+// the order of a level's entries). The sweep hands out its tasks in
+// increasing order, so the wavefronts at work at any time share one or two
+// slots and their successor lookups (random reads of the next level's
+// words, 2 MB per slot at n = 262,144) stay in each XCD's L2.
+// Counts per slot (cnt zeroed before).
+__global__ void slot_count_kernel(const QEnt* __restrict__ q, uint32_t qb, uint32_t qe, int slots,
+                                  uint32_t* __restrict__ cnt) {
+    extern __shared__ uint32_t hist[];
+    for (int b = threadIdx.x; b < slots; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = qb + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < qe; i += stride)
+        atomicAdd(&hist[q[i].b], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < slots; b += blockDim.x)
+        if (hist[b]) atomicAdd(&cnt[b], hist[b]);
+}
+
+// cnt[b] = the first position of slot b (exclusive prefix over slots; one
+// workgroup of 1024 threads, slots <= 1024).
+__global__ void __launch_bounds__(1024) slot_scan_kernel(uint32_t* __restrict__ cnt, int slots) {
+    __shared__ uint32_t wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t v = t < slots ? cnt[t] : 0;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int k = 0; k < w; k++) before += wsum[k];
+    if (t < slots) cnt[t] = before + incl - v;
+}
+
+// perm[qb + cursor[b]++] = i for every entry i of [qb, qe) in slot b; each
+// workgroup reserves one range per slot per round of blockDim entries.
+__global__ void slot_scatter_kernel(const QEnt* __restrict__ q, uint32_t qb, uint32_t qe,
+                                    int slots, uint32_t* __restrict__ cursor,
+                                    uint32_t* __restrict__ perm) {
+    extern __shared__ uint32_t lds[];
+    uint32_t* hist = lds;          // entries per slot this round
+    uint32_t* base = lds + slots;  // their first position
+    const size_t per_round = (size_t)gridDim.x * blockDim.x;
+    for (size_t r0 = qb + (size_t)blockIdx.x * blockDim.x; r0 < qe; r0 += per_round) {
+        for (int b = threadIdx.x; b < slots; b += blockDim.x) hist[b] = 0;
+        __syncthreads();
+        const size_t i = r0 + threadIdx.x;
+        uint32_t b = 0, rank = 0;
+        if (i < qe) {
+            b = q[i].b;
+            rank = atomicAdd(&hist[b], 1u);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < slots; s += blockDim.x)
+            if (hist[s]) base[s] = atomicAdd(&cursor[s], hist[s]);
+        __syncthreads();
+        if (i < qe) perm[qb + base[b] + rank] = (uint32_t)i;
+        __syncthreads();
+    }
 }
 
 // Finishes the heavy rows of a level: dv = the chunk partial sums added in
@@ -3199,6 +3270,8 @@ struct Workspace {
     DevBuf<QEnt> q;
     DevBuf<uint32_t> hq;    // positions of heavy-row entries in q
     DevBuf<uint32_t> hpre;  // chunk-task prefix over hq (per level)
+    DevBuf<uint32_t> perm;  // a level's entry positions in slot order (sweep)
+    DevBuf<uint32_t> slot_cnt;  // entries per slot, then their first positions
     DevBuf<double> part;  // per-chunk partial sums of heavy rows
     // Edge runs: per-arc credit, one row of arcs per slot (per thread for
     // real lengths), summed over the rows at the end (fold_arcs_kernel).
@@ -3701,6 +3774,20 @@ struct Engine {
         else return g.dev.directed ? L.in_real.p : L.out_real.p;
     }
 
+    // perm[qb, qe) = the level's entry positions grouped by slot
+    // (slot_count / slot_scan / slot_scatter kernels).
+    void order_by_slot(uint32_t qb, uint32_t qe) {
+        const size_t cnt = qe - qb;
+        const unsigned blocks = blocks_for_threads(std::min<size_t>(cnt, (size_t)1 << 18));
+        fill_u32_kernel<<<1, 256, 0, st>>>(ws.slot_cnt.p, (size_t)slots, 0);
+        slot_count_kernel<<<blocks, kBlock, (size_t)slots * 4, st>>>(ws.q.p, qb, qe, slots,
+                                                                    ws.slot_cnt.p);
+        slot_scan_kernel<<<1, 1024, 0, st>>>(ws.slot_cnt.p, slots);
+        slot_scatter_kernel<<<blocks, kBlock, (size_t)slots * 8, st>>>(ws.q.p, qb, qe, slots,
+                                                                      ws.slot_cnt.p, ws.perm.p);
+        launch_check();
+    }
+
     template <class C, bool kEdge, bool kQuirk = false>
     void backward_bfs(C* coef) {
         const DeviceGraph& d = g.dev;
@@ -3730,6 +3817,12 @@ struct Engine {
                 NETBC_HIP(hipStreamSynchronize(st));  // the old buffer may be in use
                 ws.part.alloc((size_t)hc[dl] * kLanes * 2);
             }
+            // large levels are swept slot by slot (entries with successors)
+            const uint32_t* perm = nullptr;
+            if (dl < D && slots > 1 && qe - qb >= kSlotOrderMin) {
+                order_by_slot(qb, qe);
+                perm = ws.perm.p;
+            }
             ws.sweep.mark(st);
             bfs_backward_kernel<C, kEdge, kQuirk>
                 <<<ticket_grid(resident_blocks(bfs_backward_kernel<C, kEdge, kQuirk>, waves, sb)),
@@ -3737,7 +3830,7 @@ struct Engine {
                     ws.q.p, qb, qe, ab, ae, dl == D, rebuild, ws.hq.p, hb, he, ws.hpre.p, hc[dl],
                     split, d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                     LevelBits{level_bits(dl + 1)}, level_bits(dl), ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz,
-                    n, ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr);
+                    n, ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr, perm);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0, st>>>(
@@ -3979,7 +4072,7 @@ struct Engine {
                 d.heavy_flag.p, ws.part.p, d.out_off.p, d.out_idx.p,
                 RingBits<false>{ws.ring.p, stride, out_lengths<IntPath>(), d0, W},
                 ws.ring.p + (size_t)d0 * stride, ws.succ.p, ws.sig.p, coef, ws.ctrl.p, gsz, n,
-                ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr);
+                ws.eacc.p, d.arcs, target, npr, dl > 0 ? bsum : nullptr, nullptr);
             launch_check();
             if (heavy) {
                 bfs_combine_kernel<C><<<resident_blocks(bfs_combine_kernel<C>, he - hb), kBlock, 0,
@@ -4020,6 +4113,8 @@ void grow_queue(Workspace& ws) {
     ws.hq.alloc(nq);
     ws.hpre.release();
     ws.hpre.alloc(nq + 1);
+    ws.perm.release();
+    ws.perm.alloc(nq);
     ws.qcap = nq;
 }
 
@@ -4050,6 +4145,8 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
             ws.hq.alloc(qcap);
             ws.hpre.release();
             ws.hpre.alloc(qcap + 1);
+            ws.perm.release();
+            ws.perm.alloc(qcap);
             ws.qcap = qcap;
         }
     }
@@ -4074,6 +4171,7 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
     ws.acc.alloc(n);
     ws.gcur.alloc(n);
     ws.ctrl.alloc(1);
+    ws.slot_cnt.alloc((size_t)std::max(slots, 1));
 }
 
 // Weighted distances from every node to the first kWeightedLandmarks
