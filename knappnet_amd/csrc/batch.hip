@@ -429,6 +429,26 @@ __device__ __forceinline__ void retire_tickets(Ctrl* ctrl, uint32_t xc) {
     }
 }
 
+// Work tickets per XCD class k (blockIdx.x % kXcds) for kernels whose work
+// is split by XCD (the first kXcds counter pairs of xticket; the sweep's
+// retire leaves them at zero): the wavefronts of class k take the class's
+// task groups in order from one counter, so the slow groups do not hold up
+// a static share. retire_xcd_tickets: once per wavefront after its last
+// ticket; the class's last wavefront resets the pair. The grid must be a
+// multiple of kXcds (xcd_grid).
+__device__ __forceinline__ uint32_t next_xcd_ticket(Ctrl* ctrl, uint32_t k) {
+    uint32_t t = 0;
+    if (lane_id() == 0) t = atomicAdd(&ctrl->xticket[k].ticket, 1u);
+    return rl32(t, 0);
+}
+__device__ __forceinline__ void retire_xcd_tickets(Ctrl* ctrl, uint32_t k) {
+    const uint32_t waves = (gridDim.x / kXcds) * (blockDim.x >> 6);
+    if (lane_id() == 0 && atomicAdd(&ctrl->xticket[k].exits, 1u) == waves - 1) {
+        ctrl->xticket[k].ticket = 0;
+        ctrl->xticket[k].exits = 0;
+    }
+}
+
 // Exclusive prefix of per-lane counts over the wavefront; total = their sum.
 __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t& total) {
     const int lane = lane_id();
@@ -782,7 +802,8 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
 // ORed into cand[x], x enters the touched list with its first lanes, and
 // the in-neighbours record their successor lanes (succ). A wavefront takes
 // gsz tasks of one slot and walks the arcs of those with N != 0 as one
-// concatenation (gather_counts); slots are spread over the XCDs (xcd_wave).
+// concatenation (gather_counts); slots are spread over the XCDs (xcd_wave)
+// and each XCD class takes its task groups in order by ticket.
 // kNpr: predecessor-arc counts as well (npr).
 template <bool kNpr>
 __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chunks, int slots,
@@ -799,7 +820,8 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
     const XcdWave xw = xcd_wave();
     const size_t gps = (chunks + gsz - 1) / gsz;  // task groups per slot
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
-    for (size_t g = xw.w; g < groups; g += xw.stride) {
+    for (size_t g = next_xcd_ticket(ctrl, xw.xcd); g < groups;
+         g = next_xcd_ticket(ctrl, xw.xcd)) {
         const uint32_t bs = xw.xcd + kXcds * (uint32_t)(g / gps);
         const uint32_t c = (uint32_t)(g % gps) * gsz + lane;
         uint32_t b = 0;
@@ -830,6 +852,7 @@ __global__ void bfs_hub_kernel(const uint4* __restrict__ hub_task, uint32_t chun
             }
         }
     }
+    retire_xcd_tickets(ctrl, xw.xcd);
     NETBC_FLUSH(ctrl, 12, c4);
     NETBC_FLUSH(ctrl, 13, c5);
     NETBC_FLUSH(ctrl, 14, c6);
@@ -979,9 +1002,11 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
 // successor lanes in succ); the pairs reached join level d+1 here. A
 // wavefront takes 64 consecutive pairs of one slot; slots are spread over
 // the XCDs (xcd_wave). Hubs are left to bfs_hub_kernel and
-// bfs_gather_kernel. Threads also clear the level d-1 entries of the next
-// bitmap (AND-NOT, commuting with the OR of the new lanes: a node's lanes
-// at levels d-1 and d+1 are disjoint). Without kCount (reachability only)
+// bfs_gather_kernel. Each XCD class takes its 64-pair groups in order by
+// ticket (groups differ in cost: their in-rows and hits). Threads also
+// clear the level d-1 entries of the next bitmap (AND-NOT, commuting with
+// the OR of the new lanes: a node's lanes at levels d-1 and d+1 are
+// disjoint). Without kCount (reachability only)
 // each pair walks its own in-row and stops at the first in-neighbours that
 // cover every lane sought. kNpr: predecessor-arc counts as well (npr).
 template <bool kCount, bool kNpr = false>
@@ -1014,7 +1039,8 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     const XcdWave xw = xcd_wave();
     const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
-    for (size_t g = xw.w; g < groups; g += xw.stride) {
+    for (size_t g = next_xcd_ticket(ctrl, xw.xcd); g < groups;
+         g = next_xcd_ticket(ctrl, xw.xcd)) {
         const uint32_t b = xw.xcd + kXcds * (uint32_t)(g / gps);
         const uint64_t F = flanes[b];  // lanes still searching in this slot
         if (!F) continue;
@@ -1061,6 +1087,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         stage.add(mine != 0, x, b, mine, ctrl, q, qcap, hq, heavy_flag);
     }
     if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
+    retire_xcd_tickets(ctrl, xw.xcd);
     NETBC_FLUSH(ctrl, 8, c4);
     NETBC_FLUSH(ctrl, 9, c5);
     NETBC_FLUSH(ctrl, 15, c6);
