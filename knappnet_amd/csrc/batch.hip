@@ -1811,7 +1811,9 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     const uint32_t T = *llen;
-    for (size_t t = global_wave(); t < T; t += wave_count()) {
+    // entries by ticket (their rows differ widely in length)
+    const uint32_t xc = blockIdx.x % kTicketClasses;
+    for (uint32_t t = next_ticket(ctrl, xc); t < T; t = next_ticket(ctrl, xc)) {
         const uint2 te = list[t];
         const uint32_t x = te.x;
         const size_t base = (size_t)te.y * n;
@@ -1942,6 +1944,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             }
         }
     }
+    retire_tickets(ctrl, xc);
     if (cnt) flush();
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long other = (unsigned long long)__shfl_xor((long long)least, o, 64);
@@ -1972,11 +1975,13 @@ __global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
                                      const uint64_t* __restrict__ vis, double* __restrict__ sig,
                                      C* __restrict__ coef, size_t n, double* __restrict__ eacc,
                                      size_t arcs, const uint8_t* __restrict__ target,
-                                     uint32_t* __restrict__ npr) {
+                                     uint32_t* __restrict__ npr, Ctrl* ctrl) {
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     const int lane = lane_id();
-    for (size_t e = qbeg + global_wave(); e < qend; e += wave_count()) {
+    // entries by ticket (their rows differ widely in length)
+    const uint32_t xc = blockIdx.x % kTicketClasses;
+    for (size_t e = qbeg + next_ticket(ctrl, xc); e < qend; e = qbeg + next_ticket(ctrl, xc)) {
         QEnt en = q[e];
         size_t base = (size_t)en.b * n;
         uint64_t M = en.m;
@@ -2064,6 +2069,7 @@ __global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
             }
         }
     }
+    retire_tickets(ctrl, xc);
 }
 
 // ---- integer lengths: the distance ring ------------------------------------
@@ -3908,7 +3914,7 @@ struct Engine {
                 &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p], &ws.ctrl.p->left[p ^ 1],
                 &ws.ctrl.p->next_bucket[p], &ws.ctrl.p->next_bucket[p ^ 1], n);
             launch_check();
-            step<<<step_blocks, kBlock, 0, st>>>(
+            step<<<ticket_grid(step_blocks), kBlock, 0, st>>>(
                 ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
                 d.in_off_p(), d.in_idx_p(), in_lengths<P>(), d.out_off.p, d.out_idx.p,
                 out_lengths<P>(), dist, ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p,
@@ -3942,10 +3948,11 @@ struct Engine {
             uint32_t qb = lv[dl], qe = lv[dl + 1];
             ws.sweep.mark(st);
             dial_backward_kernel<C, kEdge, P, kQuirk>
-                <<<resident_blocks(dial_backward_kernel<C, kEdge, P, kQuirk>, qe - qb), kBlock, 0,
-                   st>>>(ws.q.p, qb, qe, d.out_off.p, d.out_idx.p, out_lengths<P>(),
-                         dist_rows<P>(), ws.vis.p, ws.sig.p, coef, n, ws.eacc.p, d.arcs, target,
-                         npr);
+                <<<ticket_grid(resident_blocks(dial_backward_kernel<C, kEdge, P, kQuirk>,
+                                               qe - qb)),
+                   kBlock, 0, st>>>(ws.q.p, qb, qe, d.out_off.p, d.out_idx.p, out_lengths<P>(),
+                                    dist_rows<P>(), ws.vis.p, ws.sig.p, coef, n, ws.eacc.p,
+                                    d.arcs, target, npr, ws.ctrl.p);
             launch_check();
             ws.sweep.mark(st);
         }
