@@ -714,7 +714,9 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
     const size_t nq = qend - qbeg;
     const size_t groups = (nq + gsz - 1) / gsz;
     const size_t tasks = groups + hchunks;
-    for (size_t w = global_wave(); w < tasks; w += wave_count()) {
+    // tasks by ticket (entries' rows differ in length)
+    const uint32_t xc = blockIdx.x % kTicketClasses;
+    for (size_t w = next_ticket(ctrl, xc); w < tasks; w = next_ticket(ctrl, xc)) {
         if (w < groups) {
             // gsz light entries, their rows as one concatenation.
             const size_t i = qbeg + w * gsz + lane;
@@ -785,6 +787,7 @@ __global__ void bfs_expand_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uin
                 atomicOr((unsigned long long*)&succ[base + en.v], (unsigned long long)offered);
         }
     }
+    retire_tickets(ctrl, xc);
     if (tcnt) flush();
     NETBC_FLUSH(ctrl, 0, c0);
     NETBC_FLUSH(ctrl, 1, c1);
@@ -957,7 +960,10 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
     // new entries staged in dynamic LDS (kPullLds bytes per block)
     extern __shared__ unsigned long long pull_lds[];
     QueueStage stage{reinterpret_cast<QEnt*>(pull_lds) + (threadIdx.x >> 6) * kPullBuf, 0};
-    for (size_t t0 = global_wave() * gsz; t0 < T; t0 += wave_count() * gsz) {
+    // groups of touched pairs by ticket (their in-rows differ in length)
+    const uint32_t xc = blockIdx.x % kTicketClasses;
+    for (size_t k = next_ticket(ctrl, xc); k * gsz < T; k = next_ticket(ctrl, xc)) {
+        const size_t t0 = k * gsz;
         const size_t t = t0 + lane;
         const bool ok = (uint32_t)lane < gsz && t < T;
         const uint2 te = ok ? touched[t] : make_uint2(0, 0);
@@ -983,6 +989,7 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
         }
         stage.add(ok, te.x, te.y, N, ctrl, q, qcap, hq, heavy_flag);
     }
+    retire_tickets(ctrl, xc);
     if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
     NETBC_FLUSH(ctrl, 10, c4);
     NETBC_FLUSH(ctrl, 11, c5);
@@ -3657,7 +3664,8 @@ struct Engine {
                     group_size(qe - qb, resident_waves(bfs_expand_kernel));
                 size_t tasks = (size_t)(qe - qb + gsz - 1) / gsz + hc[dl];
                 size_t waves = std::max<size_t>(tasks, (ze - zb + 63) / 64);
-                bfs_expand_kernel<<<resident_blocks(bfs_expand_kernel, waves), kBlock, 0, st>>>(
+                bfs_expand_kernel<<<ticket_grid(resident_blocks(bfs_expand_kernel, waves)), kBlock,
+                                    0, st>>>(
                     ws.q.p, qb, qe, zb, ze, ws.hq.p, hb, he, ws.hpre.p, hc[dl], d.out_off.p,
                     d.out_idx.p, ws.vis.p, ws.cand.p, nxt, reach_only ? nullptr : ws.succ.p,
                     ws.touched.p,
@@ -3686,7 +3694,8 @@ struct Engine {
             auto gather = reach_only ? bfs_gather_kernel<false>
                           : npr      ? bfs_gather_kernel<true, true>
                                      : bfs_gather_kernel<true, false>;
-            gather<<<resident_blocks(gather, (pairs + 63) / 64), kBlock, kPullLds, st>>>(
+            gather<<<ticket_grid(resident_blocks(gather, (pairs + 63) / 64)), kBlock, kPullLds,
+                     st>>>(
                 ws.touched.p, &ws.ctrl.p->tlen[dl & 1],
                                       &ws.ctrl.p->tlen[(dl + 1) & 1], d.in_off_p(), d.in_idx_p(),
                                       d.out_off.p, cur, ws.cand.p, ws.vis.p, nxt,
