@@ -1783,8 +1783,13 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
 // out-arc x->y offers dist[x] + len to the lanes of y not yet settled, by
 // atomic minimum, and marks the offer's bucket mod W at y. Offers land in
 // later buckets, so settled distances never change here. Without kCount
-// (distances only) no path counts are pulled.
-template <class P, bool kCount = true>
+// (distances only) no path counts are pulled. With kSym (undirected: the
+// in-row of x is its out-row) the row is walked once: per neighbour y, the
+// lanes of M already settled at y are the predecessor candidates and the
+// others take offers, so one visited word and one distance row of y serve
+// both (lanes at y settled in this same bucket are neither: every length is
+// longer than a bucket).
+template <class P, bool kCount = true, bool kSym = false>
 __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t* __restrict__ llen,
                                  uint32_t D, int W, int words, int first_step, double delta,
                                  uint64_t* __restrict__ pend,
@@ -1833,6 +1838,95 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             NETBC_COUNT(c2, off[x + 1] - off[x]);
         }
         const bool mine = (M >> lane) & 1;
+        if constexpr (kSym) {
+            double acc = 0.0;
+            uint32_t npred = 0;  // predecessor arcs in this lane
+            const uint64_t a0 = off[x], a1 = off[x + 1];
+            for (uint64_t c = a0; c < a1; c += 64) {
+                const uint64_t a = c + lane;
+                const bool ok = a < a1;
+                const uint32_t y = ok ? idx[a] : 0;
+                const Len ln = ok ? len_arr[a] : Len(0);
+                const uint64_t vw = ok ? vis[base + y] : 0;
+                const uint64_t pm = ok && kCount && !first_step ? (vw & M) : 0;
+                const uint64_t hm = ok ? (M & ~vw) : 0;
+                NETBC_COUNT(c3, hm != 0);
+                uint64_t bal = __ballot((pm | hm) != 0);
+                while (bal) {
+                    // up to kDialAhead neighbours: distance rows (and the
+                    // predecessors' path counts) loaded first, then used
+                    uint32_t yh[kDialAhead];
+                    Dist du[kDialAhead];
+                    double su[kDialAhead];
+                    Len lu[kDialAhead];
+                    bool pred[kDialAhead], head[kDialAhead];
+#pragma unroll
+                    for (int h = 0; h < kDialAhead; h++) {
+                        yh[h] = kNone;
+                        pred[h] = head[h] = false;
+                        du[h] = Dist(0);
+                        su[h] = 0.0;
+                        lu[h] = Len(0);
+                        if (bal) {
+                            const int j = __ffsll((unsigned long long)bal) - 1;
+                            bal &= bal - 1;
+                            yh[h] = rl32(y, j);
+                            lu[h] = rl(ln, j);  // a cross-lane read: every lane takes part
+                            pred[h] = (rl64(pm, j) >> lane) & 1;
+                            head[h] = (rl64(hm, j) >> lane) & 1;
+                            if (pred[h] || head[h]) {
+                                const size_t r = (base + yh[h]) * kLanes + lane;
+                                du[h] = dist[r];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int h = 0; h < kDialAhead; h++) {
+                        pred[h] = pred[h] && P::tie(P::add(du[h], lu[h]), dx);
+                        if (pred[h]) su[h] = sig[(base + yh[h]) * kLanes + lane];
+                    }
+#pragma unroll
+                    for (int h = 0; h < kDialAhead; h++) {
+                        if (yh[h] == kNone) continue;
+                        if (pred[h]) {
+                            acc += su[h];
+                            npred++;
+                        }
+                        bool offered = false;
+                        uint32_t kk = 0;
+                        const typename P::Sum nd = P::add(dx, lu[h]);
+                        if (head[h] && nd < (typename P::Sum)du[h]) {
+                            P::offer(&dist[(base + yh[h]) * kLanes + lane], nd);
+                            offered = true;
+                            const uint64_t bnd = P::bucket(nd, delta);
+                            kk = (uint32_t)(bnd % (uint64_t)W);
+                            least = min(least, (unsigned long long)bnd);
+                        }
+                        uint64_t pending = __ballot(offered);
+                        while (pending) {
+                            const int f = __ffsll((unsigned long long)pending) - 1;
+                            const uint32_t kf = rl32(kk, f);
+                            pending &= ~__ballot(offered && kk == kf);
+                            if (lane == f)
+                                atomicOr((unsigned long long*)&pend[(base + yh[h]) * words + (kf >> 6)],
+                                         1ull << (kf & 63));
+                        }
+                    }
+                }
+            }
+            if (!first_step) {
+                if (kCount && mine) {
+                    sig[(base + x) * kLanes + lane] = acc;
+                    if (npr) npr[(base + x) * kLanes + lane] = npred;
+                    smax = fmax(smax, acc);
+                }
+                if (lane == 0) vis[base + x] |= M;
+                if (cnt + 1 > kTouchBuf / 2) flush();
+                if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
+                cnt += 1;
+            }
+            continue;
+        }
         if (!first_step) {
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
@@ -3909,7 +4003,10 @@ struct Engine {
         lv.push_back(ws.host_ctrl->qlen);
         const size_t pairs = (size_t)slots * n;
         const unsigned scan_blocks = resident_blocks(dial_scan_kernel, (pairs + 63) / 64);
-        auto step = reach_only ? dial_step_kernel<P, false> : dial_step_kernel<P, true>;
+        auto step = d.directed ? (reach_only ? dial_step_kernel<P, false, false>
+                                             : dial_step_kernel<P, true, false>)
+                               : (reach_only ? dial_step_kernel<P, false, true>
+                                             : dial_step_kernel<P, true, true>);
         const unsigned step_blocks = resident_blocks(step, pairs);
         // the first step's bucket minimum starts empty; later ones are reset
         // by the scan before them
