@@ -960,9 +960,12 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
     // new entries staged in dynamic LDS (kPullLds bytes per block)
     extern __shared__ unsigned long long pull_lds[];
     QueueStage stage{reinterpret_cast<QEnt*>(pull_lds) + (threadIdx.x >> 6) * kPullBuf, 0};
-    // groups of touched pairs by ticket (their in-rows differ in length)
+    // groups of touched pairs by ticket when counting (their in-rows differ
+    // in length); reachability-only groups are cheap, and a static share
+    // costs less than a ticket atomic per group
     const uint32_t xc = blockIdx.x % kTicketClasses;
-    for (size_t k = next_ticket(ctrl, xc); k * gsz < T; k = next_ticket(ctrl, xc)) {
+    for (size_t k = kCount ? next_ticket(ctrl, xc) : global_wave(); k * gsz < T;
+         k = kCount ? next_ticket(ctrl, xc) : k + wave_count()) {
         const size_t t0 = k * gsz;
         const size_t t = t0 + lane;
         const bool ok = (uint32_t)lane < gsz && t < T;
@@ -989,7 +992,7 @@ __global__ void bfs_gather_kernel(const uint2* __restrict__ touched,
         }
         stage.add(ok, te.x, te.y, N, ctrl, q, qcap, hq, heavy_flag);
     }
-    retire_tickets(ctrl, xc);
+    if (kCount) retire_tickets(ctrl, xc);
     if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
     NETBC_FLUSH(ctrl, 10, c4);
     NETBC_FLUSH(ctrl, 11, c5);
@@ -1046,8 +1049,11 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     const XcdWave xw = xcd_wave();
     const size_t gps = (n + 63) / 64;  // 64-pair groups per slot
     const size_t groups = (size_t)xcd_slots(xw.xcd, slots) * gps;
-    for (size_t g = next_xcd_ticket(ctrl, xw.xcd); g < groups;
-         g = next_xcd_ticket(ctrl, xw.xcd)) {
+    // the counting step takes its groups by ticket (they differ in cost);
+    // reachability-only groups are cheap, and a static share costs less
+    // than a ticket atomic per group
+    for (size_t g = kCount ? next_xcd_ticket(ctrl, xw.xcd) : xw.w; g < groups;
+         g = kCount ? next_xcd_ticket(ctrl, xw.xcd) : g + xw.stride) {
         const uint32_t b = xw.xcd + kXcds * (uint32_t)(g / gps);
         const uint64_t F = flanes[b];  // lanes still searching in this slot
         if (!F) continue;
@@ -1094,7 +1100,7 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
         stage.add(mine != 0, x, b, mine, ctrl, q, qcap, hq, heavy_flag);
     }
     if (stage.cnt) stage.flush(ctrl, q, qcap, hq, heavy_flag);
-    retire_xcd_tickets(ctrl, xw.xcd);
+    if (kCount) retire_xcd_tickets(ctrl, xw.xcd);
     NETBC_FLUSH(ctrl, 8, c4);
     NETBC_FLUSH(ctrl, 9, c5);
     NETBC_FLUSH(ctrl, 15, c6);
@@ -1823,9 +1829,12 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
     const uint32_t T = *llen;
-    // entries by ticket (their rows differ widely in length)
+    // entries by ticket when counting (their rows differ widely in length);
+    // distances-only entries are cheap, and a static share costs less than
+    // a ticket atomic per entry
     const uint32_t xc = blockIdx.x % kTicketClasses;
-    for (uint32_t t = next_ticket(ctrl, xc); t < T; t = next_ticket(ctrl, xc)) {
+    for (size_t t = kCount ? next_ticket(ctrl, xc) : global_wave(); t < T;
+         t = kCount ? next_ticket(ctrl, xc) : t + wave_count()) {
         const uint2 te = list[t];
         const uint32_t x = te.x;
         const size_t base = (size_t)te.y * n;
@@ -2045,7 +2054,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             }
         }
     }
-    retire_tickets(ctrl, xc);
+    if (kCount) retire_tickets(ctrl, xc);
     if (cnt) flush();
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long other = (unsigned long long)__shfl_xor((long long)least, o, 64);
