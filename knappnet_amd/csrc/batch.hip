@@ -3543,7 +3543,10 @@ bool ring_path(const Graph& g, const Lengths& L) {
 // list, about four queue entries with their heavy-row slots, and the
 // per-lane rows.
 size_t slot_bytes_per_node(const Lengths& L, int precision, uint64_t n, bool ring = false) {
-    size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 8) + kLanes * 8 + kLanes * (precision ? 4 : 8);
+    // visited/candidate/successor words, touched list, 4 queue entries per
+    // pair (entry, heavy-row position, chunk prefix, slot-order position),
+    // path-count row, coefficient row
+    size_t b = 3 * 8 + 8 + 4 * (sizeof(QEnt) + 12) + kLanes * 8 + kLanes * (precision ? 4 : 8);
     if (L.kind == kUniform) b += kLevelBufs * 8 + 8;  // level bitmaps, dependency sums (bsum)
     if (L.kind == kInteger && ring) b += 2 * (size_t)(L.max + 1) * 8 + 8;  // rings, bsum
     if (L.kind == kInteger && !ring) b += kLanes * 4 + 8 * buckets_for(L).words + 8;
