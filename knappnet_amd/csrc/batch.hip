@@ -1712,7 +1712,15 @@ struct RealPathExact : RealPath {
 // clears it, records whether any bucket is still set anywhere, and lowers
 // next_bucket to the first bucket after D still set at some pair (the
 // pending buckets of a pair lie in (D, D + W), so a residue r names bucket
-// D + ((r - D) mod W)); the step adds the buckets it offers to.
+// D + ((r - D) mod W)); the step adds the buckets it offers to. It runs
+// one workgroup per CU (with a workgroup per resident slot the wavefronts'
+// appends to the list's one counter made it 3x slower), each lane loading
+// kScanUnroll bucket words at once. The wavefronts advance through the
+// pairs together and append as they go, so the list, and the level queue
+// built from it, stay near slot-major order, which the step and the sweep
+// rely on for L2 hits (one append per workgroup at its end measured 2%
+// slower overall for that reason).
+constexpr int kScanUnroll = 4;
 __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, uint32_t D,
                                  size_t count, uint2* __restrict__ list,
                                  uint32_t* __restrict__ llen, uint32_t* __restrict__ llen_next,
@@ -1744,36 +1752,47 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
     const uint64_t below = bit - 1;
     int any = 0;
     uint32_t ahead = kNone;  // least (r - kb) mod W over pending residues r
-    for (size_t w0 = global_wave() * 64; w0 < count; w0 += wave_count() * 64) {
-        size_t i = w0 + lane;
-        bool hit = false;
-        if (i < count) {
-            uint64_t* p = &pend[i * words];
-            uint64_t word = p[kw];
-            hit = (word & bit) != 0;
-            if (hit) p[kw] = word & ~bit;
-            word &= ~bit;
-            uint32_t hi = kNone, lo = kNone;  // first residue above kb / at all below it
-            if (word & above) hi = (uint32_t)(kw * 64 + __ffsll((unsigned long long)(word & above)) - 1);
-            if (word & below) lo = (uint32_t)(kw * 64 + __ffsll((unsigned long long)(word & below)) - 1);
-            for (int j = 0; j < words; j++) {
-                if (j == kw) continue;
-                const uint64_t wj = p[j];
-                if (!wj) continue;
-                const uint32_t r = (uint32_t)(j * 64 + __ffsll((unsigned long long)wj) - 1);
-                if (j > kw) hi = min(hi, r);
-                else lo = min(lo, r);
-            }
-            if (hi != kNone) ahead = min(ahead, hi - (uint32_t)kb);
-            else if (lo != kNone) ahead = min(ahead, lo + (uint32_t)W - (uint32_t)kb);
-            any |= (hi != kNone || lo != kNone);
+    for (size_t w0 = global_wave() * 64 * kScanUnroll; w0 < count;
+         w0 += wave_count() * 64 * kScanUnroll) {
+        // the bucket words of kScanUnroll pairs per lane loaded together
+        uint64_t wk[kScanUnroll];
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; u++) {
+            const size_t i = w0 + (size_t)u * 64 + lane;
+            wk[u] = i < count ? pend[i * words + kw] : 0ull;
         }
-        uint64_t bal = __ballot(hit);
-        if (bal) {
-            uint32_t k = (uint32_t)__popcll(bal);
-            if (cnt + k > kTouchBuf) flush();
-            if (hit) mybuf[cnt + __popcll(bal & lt)] = make_uint2((uint32_t)(i % n), (uint32_t)(i / n));
-            cnt += k;
+#pragma unroll
+        for (int u = 0; u < kScanUnroll; u++) {
+            const size_t i = w0 + (size_t)u * 64 + lane;
+            bool hit = false;
+            if (i < count) {
+                uint64_t* p = &pend[i * words];
+                uint64_t word = wk[u];
+                hit = (word & bit) != 0;
+                if (hit) p[kw] = word & ~bit;
+                word &= ~bit;
+                uint32_t hi = kNone, lo = kNone;  // first residue above kb / at all below it
+                if (word & above) hi = (uint32_t)(kw * 64 + __ffsll((unsigned long long)(word & above)) - 1);
+                if (word & below) lo = (uint32_t)(kw * 64 + __ffsll((unsigned long long)(word & below)) - 1);
+                for (int j = 0; j < words; j++) {
+                    if (j == kw) continue;
+                    const uint64_t wj = p[j];
+                    if (!wj) continue;
+                    const uint32_t r = (uint32_t)(j * 64 + __ffsll((unsigned long long)wj) - 1);
+                    if (j > kw) hi = min(hi, r);
+                    else lo = min(lo, r);
+                }
+                if (hi != kNone) ahead = min(ahead, hi - (uint32_t)kb);
+                else if (lo != kNone) ahead = min(ahead, lo + (uint32_t)W - (uint32_t)kb);
+                any |= (hi != kNone || lo != kNone);
+            }
+            uint64_t bal = __ballot(hit);
+            if (bal) {
+                uint32_t k = (uint32_t)__popcll(bal);
+                if (cnt + k > kTouchBuf) flush();
+                if (hit) mybuf[cnt + __popcll(bal & lt)] = make_uint2((uint32_t)(i % n), (uint32_t)(i / n));
+                cnt += k;
+            }
         }
     }
     if (cnt) flush();
@@ -3346,6 +3365,14 @@ unsigned resident_blocks(K kernel, size_t waves, int block = kBlock, size_t lds 
     return (unsigned)std::max<size_t>(1, std::min<size_t>({b, (size_t)1 << 20, (size_t)cap}));
 }
 
+// Compute units of the current device.
+inline unsigned cu_count() {
+    int dev = 0, cus = 0;
+    NETBC_HIP(hipGetDevice(&dev));
+    NETBC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return (unsigned)std::max(cus, 1);
+}
+
 // Grid of a kernel that splits its work by XCD (xcd_wave): a multiple of 8.
 inline unsigned xcd_grid(unsigned blocks) {
     return std::max<unsigned>(kXcds, (blocks + kXcds - 1) / kXcds * kXcds);
@@ -4023,7 +4050,9 @@ struct Engine {
         lv.assign(1, 0);
         lv.push_back(ws.host_ctrl->qlen);
         const size_t pairs = (size_t)slots * n;
-        const unsigned scan_blocks = resident_blocks(dial_scan_kernel, (pairs + 63) / 64);
+        const unsigned scan_blocks = std::min(
+            resident_blocks(dial_scan_kernel, (pairs + 64 * kScanUnroll - 1) / (64 * kScanUnroll)),
+            cu_count());
         auto step = d.directed ? (reach_only ? dial_step_kernel<P, false, false>
                                              : dial_step_kernel<P, true, false>)
                                : (reach_only ? dial_step_kernel<P, false, true>
