@@ -1708,8 +1708,9 @@ struct RealPathExact : RealPath {
 // length + 1) in which some lane holds a tentative distance; all tentative
 // distances lie in [D, D + W), so bucket D mod W holds exactly distance D.
 //
-// dial_scan_kernel lists the (slot, node) pairs with bucket D mod W set and
-// clears it, records whether any bucket is still set anywhere, and lowers
+// dial_scan_kernel lists the (slot, node) pairs with bucket D mod W set,
+// with the node's out-row bounds (the step starts on the row while the
+// pair's distance row is in flight), and clears it, records whether any bucket is still set anywhere, and lowers
 // next_bucket to the first bucket after D still set at some pair (the
 // pending buckets of a pair lie in (D, D + W), so a residue r names bucket
 // D + ((r - D) mod W)); the step adds the buckets it offers to. It runs
@@ -1722,20 +1723,21 @@ struct RealPathExact : RealPath {
 // slower overall for that reason).
 constexpr int kScanUnroll = 4;
 __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, uint32_t D,
-                                 size_t count, uint2* __restrict__ list,
+                                 size_t count, const uint32_t* __restrict__ off,
+                                 uint4* __restrict__ list,
                                  uint32_t* __restrict__ llen, uint32_t* __restrict__ llen_next,
                                  uint32_t* __restrict__ left, uint32_t* __restrict__ left_next,
                                  unsigned long long* __restrict__ next_bucket,
                                  unsigned long long* __restrict__ next_bucket_next, size_t n) {
     const int kb = (int)(D % (uint32_t)W);
-    __shared__ uint2 sbuf[kBlock / 64][kTouchBuf];
+    __shared__ uint4 sbuf[kBlock / 64][kTouchBuf];
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *llen_next = 0;
         *left_next = 0;
         *next_bucket_next = ~0ull;
     }
-    uint2* mybuf = sbuf[threadIdx.x >> 6];
+    uint4* mybuf = sbuf[threadIdx.x >> 6];
     uint32_t cnt = 0;
     auto flush = [&]() {
         uint32_t at = 0;
@@ -1790,7 +1792,11 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
             if (bal) {
                 uint32_t k = (uint32_t)__popcll(bal);
                 if (cnt + k > kTouchBuf) flush();
-                if (hit) mybuf[cnt + __popcll(bal & lt)] = make_uint2((uint32_t)(i % n), (uint32_t)(i / n));
+                if (hit) {
+                    const uint32_t x = (uint32_t)(i % n);
+                    mybuf[cnt + __popcll(bal & lt)] =
+                        make_uint4(x, (uint32_t)(i / n), off[x], off[x + 1]);
+                }
                 cnt += k;
             }
         }
@@ -1821,7 +1827,7 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
 // both (lanes at y settled in this same bucket are neither: every length is
 // longer than a bucket).
 template <class P, bool kCount = true, bool kSym = false>
-__global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t* __restrict__ llen,
+__global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t* __restrict__ llen,
                                  uint32_t D, int W, int words, int first_step, double delta,
                                  uint64_t* __restrict__ pend,
                                  const uint32_t* __restrict__ in_off,
@@ -1863,10 +1869,25 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
          t = !kCount                      ? t + wave_count()
              : (t + 1) % kDialGroup != 0 ? t + 1
                                          : (size_t)next_ticket(ctrl, xc) * kDialGroup) {
-        const uint2 te = list[t];
+        const uint4 te = list[t];  // node, slot, out-row bounds
         const uint32_t x = te.x;
         const size_t base = (size_t)te.y * n;
+        // undirected: the row's first 64 arcs and their visited words are
+        // loaded while the distance row is in flight
+        uint32_t y0 = 0;
+        Len ln0 = Len(0);
+        uint64_t vw0 = 0;
+        const bool ok0 = (uint64_t)te.z + lane < te.w;
+        if constexpr (kSym) {
+            if (ok0) {
+                y0 = idx[te.z + lane];
+                ln0 = len_arr[te.z + lane];
+            }
+        }
         const Dist dx = dist[(base + x) * kLanes + lane];
+        if constexpr (kSym) {
+            if (ok0) vw0 = vis[base + y0];
+        }
         const uint64_t M = __ballot(P::reached(dx) && P::bucket(dx, delta) == D);
         if (M == 0) continue;
         if (lane == 0) {
@@ -1878,13 +1899,18 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
         if constexpr (kSym) {
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
-            const uint64_t a0 = off[x], a1 = off[x + 1];
+            const uint64_t a0 = te.z, a1 = te.w;
             for (uint64_t c = a0; c < a1; c += 64) {
                 const uint64_t a = c + lane;
                 const bool ok = a < a1;
-                const uint32_t y = ok ? idx[a] : 0;
-                const Len ln = ok ? len_arr[a] : Len(0);
-                const uint64_t vw = ok ? vis[base + y] : 0;
+                uint32_t y = y0;
+                Len ln = ln0;
+                uint64_t vw = vw0;
+                if (c != a0) {
+                    y = ok ? idx[a] : 0;
+                    ln = ok ? len_arr[a] : Len(0);
+                    vw = ok ? vis[base + y] : 0;
+                }
                 const uint64_t pm = ok && kCount && !first_step ? (vw & M) : 0;
                 const uint64_t hm = ok ? (M & ~vw) : 0;
                 NETBC_COUNT(c3, hm != 0);
@@ -2022,7 +2048,7 @@ __global__ void dial_step_kernel(const uint2* __restrict__ list, const uint32_t*
             if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
             cnt += 1;
         }
-        const uint64_t a0 = off[x], a1 = off[x + 1];
+        const uint64_t a0 = te.z, a1 = te.w;
         for (uint64_t c = a0; c < a1; c += 64) {
             const uint64_t a = c + lane;
             const bool ok = a < a1;
@@ -3466,6 +3492,7 @@ struct Workspace {
     size_t lvl_cap = 0;
     size_t qcap = 0;
     DevBuf<uint2> touched;
+    DevBuf<uint4> dlist;  // bucketed searches: listed (node, slot, out-row bounds)
     DevBuf<Ctrl> ctrl;
     Ctrl* host_ctrl = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -4066,12 +4093,12 @@ struct Engine {
         for (uint32_t it = 0;; it++) {
             const int p = it & 1;
             dial_scan_kernel<<<scan_blocks, kBlock, 0, st>>>(
-                ws.inb.p, words, W, D, pairs, ws.touched.p, &ws.ctrl.p->tlen[p],
+                ws.inb.p, words, W, D, pairs, d.out_off.p, ws.dlist.p, &ws.ctrl.p->tlen[p],
                 &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p], &ws.ctrl.p->left[p ^ 1],
                 &ws.ctrl.p->next_bucket[p], &ws.ctrl.p->next_bucket[p ^ 1], n);
             launch_check();
             step<<<ticket_grid(step_blocks), kBlock, 0, st>>>(
-                ws.touched.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
+                ws.dlist.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
                 d.in_off_p(), d.in_idx_p(), in_lengths<P>(), d.out_off.p, d.out_idx.p,
                 out_lengths<P>(), dist, ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p,
                 &ws.ctrl.p->next_bucket[p], n, npr);
@@ -4356,7 +4383,7 @@ void ensure_workspace(Workspace& ws, const Graph& g, const Lengths& L, int slots
         if (L.kind == kInteger) ws.dist.alloc(nm * kLanes);
         else ws.rdist.alloc(nm * kLanes);
         ws.inb.alloc(nm * buckets_for(L).words);
-        ws.touched.alloc(nm);
+        ws.dlist.alloc(nm);
     }
     ws.acc.alloc(n);
     ws.gcur.alloc(n);
