@@ -391,8 +391,15 @@ __device__ __forceinline__ double group_sum(const SmallItem& I, const SmallGraph
     return acc;
 }
 
+// Wavefronts per SIMD the one-workgroup kernel's registers are capped for:
+// as many as its 49 KB of LDS per workgroup lets a CU hold (3 workgroups
+// of 256 or 512 threads, 1 of 1024). At 512 threads the cap (85 -> 80
+// VGPRs) takes it from 2 to 3 workgroups per CU: 4,096 Watts-Strogatz
+// graphs 0.651 -> 0.548 s.
+constexpr int small_waves(int block) { return block == 256 ? 3 : block == 512 ? 6 : 4; }
+
 template <class C, int kB>
-__global__ __launch_bounds__(kB) void small_batch_kernel(
+__global__ __launch_bounds__(kB, small_waves(kB)) void small_batch_kernel(
     const SmallItem* __restrict__ items, uint32_t nitems, uint32_t* __restrict__ counter,
     const SmallGraph* __restrict__ graphs, const uint32_t* __restrict__ off,
     const uint32_t* __restrict__ idx, const uint32_t* __restrict__ in_off,
