@@ -1315,6 +1315,12 @@ __device__ __forceinline__ void add_lane_sum(double* dst, double x) {
     if (lane_id() == 0 && x != 0.0) unsafeAtomicAdd(dst, x);
 }
 
+// Wavefronts per SIMD the sweep's registers are capped for: 8 (64 VGPRs,
+// up to 16 bytes per lane spilled) for node scores, 587 -> 566 ms per
+// R-MAT-18 run; edge runs keep their 65 (7 waves: capped, they spill 9-14
+// VGPRs and run 2% slower).
+constexpr int sweep_waves(bool edge) { return edge ? 1 : 8; }
+
 // Dependencies of the level-d entries; `nx` gives the lanes in which an
 // arc's head is a successor (breadth-first: the level d+1 bitmap; integer
 // lengths: the bitmap of distance D + len in the distance ring). With
@@ -1337,7 +1343,7 @@ __device__ __forceinline__ void add_lane_sum(double* dst, double x) {
 // lanes' dependencies to bsum[b][v] (add_lane_sum; bfs_combine_kernel does
 // the same for heavy rows).
 template <class C, bool kEdge, bool kQuirk = false, class Next = LevelBits>
-__global__ void bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
+__global__ void __launch_bounds__(1024, sweep_waves(kEdge)) bfs_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
                                     uint32_t abeg, uint32_t aend, int last_level, int rebuild,
                                     const uint32_t* __restrict__ hq, uint32_t hbeg, uint32_t hend,
                                     const uint32_t* __restrict__ hpre, uint32_t hchunks, int split,
