@@ -411,9 +411,11 @@ __device__ __forceinline__ uint32_t xcd_slots(uint32_t k, int slots) {
 // Levels of at least this many entries are swept in slot order.
 constexpr uint32_t kSlotOrderMin = 1u << 16;
 
-// Consecutive 64-pair groups per ticket of the counting bottom-up step
-// (R-MAT 18 all sources: 1.42 s at 1, 1.40 at 2, 1.38 at 4 and 8, 1.42 at
-// 16; same-address atomics serialise, so fewer tickets wait less).
+// Consecutive 64-pair groups per ticket of the counting bottom-up step on
+// large searches (R-MAT 18 all sources: 1.42 s at 1, 1.40 at 2, 1.38 at 4
+// and 8, 1.42 at 16; same-address atomics serialise, so fewer tickets wait
+// less; on G(2000, 10000), with a few groups per wavefront, 4 per ticket
+// took the run from 1.8 to 2.7 ms). A power of two.
 #ifndef NETBC_PULL_GROUP
 #define NETBC_PULL_GROUP 4
 #endif
@@ -1064,12 +1066,14 @@ __global__ void bfs_pull_kernel(const QEnt* qz, uint32_t zbeg, uint32_t zend,
     // the counting step takes its groups by ticket (they differ in cost);
     // reachability-only groups are cheap, and a static share costs less
     // than a ticket atomic per group
-    for (size_t g = kCount ? (size_t)next_xcd_ticket(ctrl, xw.xcd) * NETBC_PULL_GROUP : xw.w;
-         g < groups;
-         g = !kCount                             ? g + xw.stride
-             : (g + 1) % NETBC_PULL_GROUP != 0 ? g + 1
-                                               : (size_t)next_xcd_ticket(ctrl, xw.xcd) *
-                                                     NETBC_PULL_GROUP) {
+    // counting: NETBC_PULL_GROUP consecutive groups per ticket when there
+    // are at least 8 groups per wavefront (small searches keep one, so that
+    // their few groups spread over the wavefronts)
+    const size_t G = groups >= xw.stride * 8 ? NETBC_PULL_GROUP : 1;
+    for (size_t g = kCount ? (size_t)next_xcd_ticket(ctrl, xw.xcd) * G : xw.w; g < groups;
+         g = !kCount                ? g + xw.stride
+             : ((g + 1) & (G - 1)) ? g + 1
+                                   : (size_t)next_xcd_ticket(ctrl, xw.xcd) * G) {
         const uint32_t b = xw.xcd + kXcds * (uint32_t)(g / gps);
         const uint64_t F = flanes[b];  // lanes still searching in this slot
         if (!F) continue;
