@@ -130,8 +130,9 @@ constexpr int kDialBackAhead = NETBC_DIAL_BACK_AHEAD;
 #ifndef NETBC_DIAL_GROUP
 #define NETBC_DIAL_GROUP 2
 #endif
-// Entries per work ticket in the Dial step (one per ticket measured 5%
-// slower on BA n = 200,000, four 6% slower); the sweep takes one.
+// Entries per work ticket in the Dial step on long bucket lists (one per
+// ticket measured 5% slower on BA n = 200,000, four 6% slower); the sweep
+// takes one. A power of two.
 constexpr uint32_t kDialGroup = NETBC_DIAL_GROUP;
 #ifndef NETBC_DIAL_BACK_WAVES
 #define NETBC_DIAL_BACK_WAVES 7
@@ -1884,11 +1885,13 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
     // distances-only entries are cheap, and a static share costs less than
     // a ticket atomic per entry
     const uint32_t xc = blockIdx.x % kTicketClasses;
-    // counting: kDialGroup consecutive entries per ticket
-    for (size_t t = kCount ? (size_t)next_ticket(ctrl, xc) * kDialGroup : global_wave(); t < T;
-         t = !kCount                      ? t + wave_count()
-             : (t + 1) % kDialGroup != 0 ? t + 1
-                                         : (size_t)next_ticket(ctrl, xc) * kDialGroup) {
+    // counting: kDialGroup consecutive entries per ticket when there are at
+    // least 8 entries per wavefront
+    const size_t G = T >= wave_count() * 8 ? kDialGroup : 1;
+    for (size_t t = kCount ? (size_t)next_ticket(ctrl, xc) * G : global_wave(); t < T;
+         t = !kCount                ? t + wave_count()
+             : ((t + 1) & (G - 1)) ? t + 1
+                                   : (size_t)next_ticket(ctrl, xc) * G) {
         const uint4 te = list[t];  // node, slot, out-row bounds
         const uint32_t x = te.x;
         const size_t base = (size_t)te.y * n;
