@@ -27,7 +27,7 @@ struct TuneEntry {
 
 constexpr TuneEntry kTuning[] = {
     {0, 1, 0, 40, 0.0, 1e30, 64, 2.0, 4, 256, 256, 64, 512, 1, "R-MAT scale 18 directed, first 32,768 sources"},
-    {0, 0, 0, 40, 0.0, 1e30, 64, 2.0, 32, 256, 512, 64, 512, 1, "R-MAT scale 22 undirected, 4,096 sampled sources"},
+    {0, 0, 0, 40, 0.0, 1e30, 64, 4.0, 32, 256, 512, 64, 512, 1, "R-MAT scale 22 undirected, 4,096 sampled sources"},
     {1, -1, 0, 40, 0.0, 1e30, 32, 1.0, 32, 256, 256, 64, 512, 1, "BA n=200,000 integer weights 1-16, first 8,192 sources"},
     {2, 0, 0, 40, 0.0, 1e30, 0, 1.0, 32, 512, 256, 64, 512, 1, "1,024 Watts-Strogatz graphs, n 50-2,000, k 6, p 0.1"},
 };
