@@ -141,3 +141,24 @@ def test_many_graphs_workgroup_sizes():
     with pytest.raises(EngineError, match=r"^the one-workgroup engine runs workgroups of 256, "
                                           r"512 or 1024 threads, not 128$"):
         betweenness_batch(graphs, sweep_block=128)
+
+
+def test_slots_even_out_over_super_batches():
+    """65 batches of 64 sources with 64 slots requested run as 2 super-
+    batches of 33 slots (not 64 and 1): the diagnostics report 33, exact
+    order is the oracle's bits, free order within 1e-12."""
+    el = gen.rmat(12, 16, directed=True, seed=4)
+    g = Graph(el.n, el.src, el.tgt, None, True)
+    src = np.arange(65 * 64) % el.n
+    want = OracleGraph(el.n, el.src, el.tgt, None, True).betweenness_sources(src, threads=0)
+    got, diag = g.betweenness_sources(src, exact=True, slots=64)
+    assert diag["slots"] == 33
+    assert_same(got, want, "exact order, 33 slots")
+    free, diag = g.betweenness_sources(src, slots=64)
+    assert diag["slots"] == 33
+    rel = np.abs(free - want) / np.maximum(np.abs(want), 1e-300)
+    rel[(free == 0) & (want == 0)] = 0.0
+    assert rel.max() <= 1e-12
+    # a request within one super-batch is kept
+    _, diag = g.betweenness_sources(src[:40 * 64], slots=64)
+    assert diag["slots"] == 40
