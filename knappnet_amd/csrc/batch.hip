@@ -1833,6 +1833,139 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
         atomicOr(left, 1u);
 }
 
+// Packing of (neighbour, lane) pairs for entries of at most kPackLanes
+// lanes (dial_step_packed, dial_sweep_packed).
+#ifndef NETBC_PACK_LANES
+#define NETBC_PACK_LANES 16
+#endif
+constexpr uint32_t kPackLanes = NETBC_PACK_LANES;
+// 64-pair loads per packed window.
+#ifndef NETBC_PACK_ROUNDS
+#define NETBC_PACK_ROUNDS 2
+#endif
+constexpr int kPackRounds = NETBC_PACK_ROUNDS;
+
+// The k-th lane of M in lane k < popcount(M) (0 elsewhere).
+__device__ __forceinline__ uint32_t select_lanes(uint64_t M) {
+    const int lane = lane_id();
+    uint32_t sel = 0;
+    if ((uint64_t)lane < (uint64_t)__popcll(M)) {
+        uint64_t m = M;
+        for (int k = 0; k < lane; k++) m &= m - 1;
+        sel = (uint32_t)__ffsll((unsigned long long)m) - 1;
+    }
+    return sel;
+}
+
+// The undirected Dial step's row walk for an entry of lanes M (|M| at most
+// kPackLanes) with its (neighbour, lane) pairs packed 64 to a load: each
+// load covers 64 / |M| whole neighbours, kPackRounds loads a window. A
+// pair whose lane is already settled at the neighbour is a predecessor
+// candidate (its path count is loaded when its distance ties, with
+// `settle`), any other takes the offer dist[x] + length by atomic minimum
+// and marks the offer's bucket (one atomic per distinct neighbour and
+// bucket). Each lane of M then adds its predecessors' path counts in CSR
+// order, as the unpacked walk does. The first 64 arcs (y0, ln0, vw0) are
+// loaded by the caller. Every lane calls.
+template <class P, bool kCount>
+__device__ __forceinline__ void dial_step_packed(
+    uint64_t a0, uint64_t a1, uint32_t y0, typename P::Len ln0, uint64_t vw0, uint64_t M,
+    size_t base, typename P::Dist dx, int W, int words, double delta, bool settle,
+    const uint32_t* __restrict__ idx, const typename P::Len* __restrict__ len_arr,
+    const uint64_t* __restrict__ vis, typename P::Dist* __restrict__ dist,
+    const double* __restrict__ sig, uint64_t* __restrict__ pend, double& acc, uint32_t& npred,
+    unsigned long long& least) {
+    using Dist = typename P::Dist;
+    using Len = typename P::Len;
+    using Sum = typename P::Sum;
+    const int lane = lane_id();
+    const uint32_t mc = (uint32_t)__popcll(M);
+    const uint32_t sel = select_lanes(M);
+    const bool mine = (M >> lane) & 1;
+    const uint32_t rank = (uint32_t)__popcll(M & lanemask_lt(lane));
+    const uint32_t A = 64 / mc;  // whole neighbours per 64-pair load
+    const uint32_t grp = (uint32_t)lane / mc, gi = (uint32_t)lane - grp * mc;
+    const uint32_t ls = (uint32_t)__shfl((int)sel, (int)gi, 64);  // this pair's lane
+    const Dist d0 = __shfl(dx, (int)ls, 64);                      // its distance at x
+    for (uint64_t c = a0; c < a1; c += 64) {
+        const uint64_t a = c + lane;
+        const bool ok = a < a1;
+        uint32_t y = y0;
+        Len ln = ln0;
+        uint64_t vw = vw0;
+        if (c != a0) {
+            y = ok ? idx[a] : 0;
+            ln = ok ? len_arr[a] : Len(0);
+            vw = ok ? vis[base + y] : 0;
+        }
+        const uint32_t na = (uint32_t)min<uint64_t>(64, a1 - c);
+        for (uint32_t w0 = 0; w0 < na; w0 += A * kPackRounds) {
+            Dist du[kPackRounds];
+            bool pr[kPackRounds], hd[kPackRounds];
+            size_t row[kPackRounds];
+            Len lp[kPackRounds];
+            uint32_t yr[kPackRounds];
+#pragma unroll
+            for (int r = 0; r < kPackRounds; r++) {
+                const uint32_t j = w0 + (uint32_t)r * A + grp;  // neighbour in this chunk
+                const bool valid = grp < A && j < na;
+                const int jj = (int)min(j, 63u);
+                yr[r] = (uint32_t)__shfl((int)y, jj, 64);
+                lp[r] = __shfl(ln, jj, 64);
+                const uint64_t vj = __shfl(vw, jj, 64);
+                const bool settled = (vj >> ls) & 1;
+                pr[r] = valid && kCount && settle && settled;
+                hd[r] = valid && !settled;
+                row[r] = (base + yr[r]) * kLanes + ls;
+                du[r] = valid ? dist[row[r]] : Dist(0);
+            }
+            double su[kPackRounds];
+            uint64_t tm[kPackRounds];
+#pragma unroll
+            for (int r = 0; r < kPackRounds; r++) {
+                const bool tie = pr[r] && P::tie(P::add(du[r], lp[r]), d0);
+                su[r] = tie ? sig[row[r]] : 0.0;
+                tm[r] = __ballot(tie);
+                bool offered = false;
+                uint32_t kk = 0;
+                const Sum nd = P::add(d0, lp[r]);
+                if (hd[r] && nd < (Sum)du[r]) {
+                    P::offer(&dist[row[r]], nd);
+                    offered = true;
+                    const uint64_t bnd = P::bucket(nd, delta);
+                    kk = (uint32_t)(bnd % (uint64_t)W);
+                    least = min(least, (unsigned long long)bnd);
+                }
+                // mark the offered buckets: one atomic per distinct
+                // (neighbour, bucket) of this load
+                uint64_t pending = __ballot(offered);
+                while (pending) {
+                    const int f = __ffsll((unsigned long long)pending) - 1;
+                    const uint32_t kf = rl32(kk, f);
+                    const uint32_t gf = (uint32_t)f / mc;
+                    pending &= ~__ballot(offered && kk == kf && grp == gf);
+                    if (lane == f)
+                        atomicOr((unsigned long long*)&pend[(base + yr[r]) * words + (kf >> 6)],
+                                 1ull << (kf & 63));
+                }
+            }
+            // each lane of M adds its predecessors' path counts in CSR order
+#pragma unroll
+            for (int r = 0; r < kPackRounds; r++) {
+                for (uint32_t g = 0; g < A; g++) {
+                    if (w0 + (uint32_t)r * A + g >= na) break;
+                    const int src = (int)(g * mc + rank) & 63;
+                    const double v = __shfl(su[r], src, 64);
+                    if (mine && ((tm[r] >> src) & 1)) {
+                        acc += v;
+                        npred++;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Settles and relaxes the listed pairs of bucket D (P: IntPath or
 // RealPath). The lanes of x whose tentative distance lies in bucket D are
 // final (M). Their path counts are pulled over in-arcs u->x of length len
@@ -1923,7 +2056,14 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
             const uint64_t a0 = te.z, a1 = te.w;
-            for (uint64_t c = a0; c < a1; c += 64) {
+            bool packed = false;
+            if ((uint32_t)__popcll(M) <= kPackLanes) {
+                dial_step_packed<P, kCount>(a0, a1, y0, ln0, vw0, M, base, dx, W, words, delta,
+                                            !first_step, idx, len_arr, vis, dist, sig, pend, acc,
+                                            npred, least);
+                packed = true;
+            }
+            for (uint64_t c = packed ? a1 : a0; c < a1; c += 64) {
                 const uint64_t a = c + lane;
                 const bool ok = a < a1;
                 uint32_t y = y0;
@@ -2146,12 +2286,95 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
     NETBC_FLUSH(ctrl, 3, c3);
 }
 
+// The Dial sweep of one entry (lanes M, base of its slot, own distances
+// dv0 in its lanes) whose row is [a0, a1), with the (head, lane) pairs of
+// up to 128 / |M| heads at a time packed into two wave-wide loads: pair p
+// holds head p / |M| and the (p mod |M|)-th lane of M. All the pairs'
+// distances are loaded, then the coefficients of the pairs on a shortest
+// path; each lane of M then adds its tying heads' coefficients in CSR
+// order, as the unpacked walk does (one load per head there, with 9 of 64
+// lanes in use on BA n = 200,000). Every lane calls.
+template <class C, class P>
+__device__ __forceinline__ void dial_sweep_packed(uint64_t a0, uint64_t a1, uint64_t M,
+                                                  size_t base, typename P::Dist dv0,
+                                                  const uint32_t* __restrict__ idx,
+                                                  const typename P::Len* __restrict__ len_arr,
+                                                  const typename P::Dist* __restrict__ dist,
+                                                  const uint64_t* __restrict__ vis,
+                                                  const C* __restrict__ coef, Dep<C>& dv) {
+    using Dist = typename P::Dist;
+    using Len = typename P::Len;
+    const int lane = lane_id();
+    const uint32_t mc = (uint32_t)__popcll(M);
+    const uint32_t sel = select_lanes(M);
+    const bool mine = (M >> lane) & 1;
+    const uint32_t rank = (uint32_t)__popcll(M & lanemask_lt(lane));
+    const uint32_t A = 64 * kPackRounds / mc;  // heads per packed window
+    for (uint64_t c = a0; c < a1; c += 64) {
+        const uint64_t a = c + lane;
+        const bool ok = a < a1;
+        const uint32_t x = ok ? idx[a] : 0;
+        const Len len = ok ? len_arr[a] : Len(0);
+        const uint64_t mm = ok ? (vis[base + x] & M) : 0;
+        const uint32_t na = (uint32_t)min<uint64_t>(64, a1 - c);
+        for (uint32_t w0 = 0; w0 < na; w0 += A) {
+            const uint32_t wa = min(A, na - w0);
+            const uint32_t pairs = wa * mc;
+            Dist dxp[kPackRounds];
+            C cv[kPackRounds];
+            bool use[kPackRounds];
+            size_t row[kPackRounds];
+            Len lp[kPackRounds];
+            uint32_t lsrc[kPackRounds];
+#pragma unroll
+            for (int r = 0; r < kPackRounds; r++) {
+                const uint32_t p = (uint32_t)r * 64 + (uint32_t)lane;
+                const uint32_t ai = p / mc;
+                const uint32_t ri = p - ai * mc;
+                const uint32_t head = min(w0 + ai, 63u);
+                lsrc[r] = (uint32_t)__shfl((int)sel, (int)min(ri, 63u), 64);
+                const uint32_t xa = (uint32_t)__shfl((int)x, (int)head, 64);
+                lp[r] = __shfl(len, (int)head, 64);
+                const uint64_t ma = __shfl(mm, (int)head, 64);
+                use[r] = p < pairs && ((ma >> lsrc[r]) & 1);
+                row[r] = (base + xa) * kLanes + lsrc[r];
+                dxp[r] = use[r] ? dist[row[r]] : Dist(0);
+            }
+            uint64_t tm[kPackRounds];
+#pragma unroll
+            for (int r = 0; r < kPackRounds; r++) {
+                const Dist d0 = __shfl(dv0, (int)lsrc[r], 64);
+                const bool tie = use[r] && P::tie(P::add(d0, lp[r]), dxp[r]);
+                cv[r] = tie ? coef[row[r]] : C(0);
+                tm[r] = __ballot(tie);
+            }
+            // each lane of M adds its tying heads in CSR order
+            for (uint32_t j = 0; j < wa; j++) {
+                const uint32_t qp = j * mc + rank;
+                const int src = (int)(qp & 63);
+                C v = C(0);
+                uint64_t t = 0;
+#pragma unroll
+                for (int r = 0; r < kPackRounds; r++) {
+                    const C vr = __shfl(cv[r], src, 64);
+                    if ((qp >> 6) == (uint32_t)r) {
+                        v = vr;
+                        t = tm[r];
+                    }
+                }
+                if (mine && ((t >> src) & 1)) dv.add(v);
+            }
+        }
+    }
+}
+
 // Dependencies of the entries settled in one bucket (P: IntPath or
 // RealPath): a wavefront per entry walks its row 64 arcs at a time; for
 // every arc whose head some lane of the entry reached, the lanes whose
 // offer dist[v] + length ties the head's distance add its coefficient, in
-// CSR order (up to kDialBackAhead rows loaded before they are added). With kEdge
-// every arc's credit also goes to eacc (edge_flush).
+// CSR order (up to kDialBackAhead rows loaded before they are added; node
+// scores of entries with at most kPackLanes lanes: dial_sweep_packed). With
+// kEdge every arc's credit also goes to eacc (edge_flush).
 template <class C, bool kEdge, class P, bool kQuirk = false>
 __global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
     dial_backward_kernel(const QEnt* __restrict__ q, uint32_t qbeg, uint32_t qend,
@@ -2177,7 +2400,14 @@ __global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
         const Dist dv0 = mine ? dist[(base + en.v) * kLanes + lane] : Dist(0);
         Dep<C> dv, dq;  // dq: kQuirk (as bfs_backward_kernel), non-target successors
         const uint64_t a0 = off[en.v], a1 = off[en.v + 1];
-        for (uint64_t c = a0; c < a1; c += 64) {
+        bool packed = false;
+        if constexpr (!kEdge && !kQuirk) {
+            if ((uint32_t)__popcll(M) <= kPackLanes) {
+                dial_sweep_packed<C, P>(a0, a1, M, base, dv0, idx, len_arr, dist, vis, coef, dv);
+                packed = true;
+            }
+        }
+        for (uint64_t c = packed ? a1 : a0; c < a1; c += 64) {
             const uint64_t a = c + lane;
             bool ok = a < a1;
             uint32_t x = ok ? idx[a] : 0;
