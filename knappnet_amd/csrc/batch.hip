@@ -1833,8 +1833,12 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
         atomicOr(left, 1u);
 }
 
-// Packing of (neighbour, lane) pairs for entries of at most kPackLanes
-// lanes (dial_step_packed, dial_sweep_packed).
+// The Dial sweep packs the (head, lane) pairs of entries of at most
+// kPackLanes lanes (dial_sweep_packed; wider entries measured faster
+// unpacked), the undirected step those of every entry (dial_step_packed:
+// BA n = 200,000, 16,384 sources 0.935 s with 16 lanes, 0.922 s with 32,
+// 0.88 s with all; its unpacked walk loads one neighbour, about 9 of 64
+// lanes, per load).
 #ifndef NETBC_PACK_LANES
 #define NETBC_PACK_LANES 16
 #endif
@@ -1857,8 +1861,8 @@ __device__ __forceinline__ uint32_t select_lanes(uint64_t M) {
     return sel;
 }
 
-// The undirected Dial step's row walk for an entry of lanes M (|M| at most
-// kPackLanes) with its (neighbour, lane) pairs packed 64 to a load: each
+// The undirected Dial step's row walk for an entry of lanes M with its
+// (neighbour, lane) pairs packed 64 to a load: each
 // load covers 64 / |M| whole neighbours, kPackRounds loads a window. A
 // pair whose lane is already settled at the neighbour is a predecessor
 // candidate (its path count is loaded when its distance ties, with
@@ -1975,11 +1979,13 @@ __device__ __forceinline__ void dial_step_packed(
 // atomic minimum, and marks the offer's bucket mod W at y. Offers land in
 // later buckets, so settled distances never change here. Without kCount
 // (distances only) no path counts are pulled. With kSym (undirected: the
-// in-row of x is its out-row) the row is walked once: per neighbour y, the
-// lanes of M already settled at y are the predecessor candidates and the
-// others take offers, so one visited word and one distance row of y serve
-// both (lanes at y settled in this same bucket are neither: every length is
-// longer than a bucket).
+// in-row of x is its out-row) the row is walked once, its (neighbour, lane)
+// pairs packed 64 to a load (dial_step_packed): per neighbour y, the lanes
+// of M already settled at y are the predecessor candidates and the others
+// take offers, so one visited word and one distance row of y serve both
+// (lanes at y settled in this same bucket are neither: every length is
+// longer than a bucket). Directed graphs walk the in-row, then the
+// out-row, kDialAhead neighbours per batch.
 template <class P, bool kCount = true, bool kSym = false>
 __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t* __restrict__ llen,
                                  uint32_t D, int W, int words, int first_step, double delta,
@@ -2056,90 +2062,9 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
             const uint64_t a0 = te.z, a1 = te.w;
-            bool packed = false;
-            if ((uint32_t)__popcll(M) <= kPackLanes) {
-                dial_step_packed<P, kCount>(a0, a1, y0, ln0, vw0, M, base, dx, W, words, delta,
-                                            !first_step, idx, len_arr, vis, dist, sig, pend, acc,
-                                            npred, least);
-                packed = true;
-            }
-            for (uint64_t c = packed ? a1 : a0; c < a1; c += 64) {
-                const uint64_t a = c + lane;
-                const bool ok = a < a1;
-                uint32_t y = y0;
-                Len ln = ln0;
-                uint64_t vw = vw0;
-                if (c != a0) {
-                    y = ok ? idx[a] : 0;
-                    ln = ok ? len_arr[a] : Len(0);
-                    vw = ok ? vis[base + y] : 0;
-                }
-                const uint64_t pm = ok && kCount && !first_step ? (vw & M) : 0;
-                const uint64_t hm = ok ? (M & ~vw) : 0;
-                NETBC_COUNT(c3, hm != 0);
-                uint64_t bal = __ballot((pm | hm) != 0);
-                while (bal) {
-                    // up to kDialAhead neighbours: distance rows (and the
-                    // predecessors' path counts) loaded first, then used
-                    uint32_t yh[kDialAhead];
-                    Dist du[kDialAhead];
-                    double su[kDialAhead];
-                    Len lu[kDialAhead];
-                    bool pred[kDialAhead], head[kDialAhead];
-#pragma unroll
-                    for (int h = 0; h < kDialAhead; h++) {
-                        yh[h] = kNone;
-                        pred[h] = head[h] = false;
-                        du[h] = Dist(0);
-                        su[h] = 0.0;
-                        lu[h] = Len(0);
-                        if (bal) {
-                            const int j = __ffsll((unsigned long long)bal) - 1;
-                            bal &= bal - 1;
-                            yh[h] = rl32(y, j);
-                            lu[h] = rl(ln, j);  // a cross-lane read: every lane takes part
-                            pred[h] = (rl64(pm, j) >> lane) & 1;
-                            head[h] = (rl64(hm, j) >> lane) & 1;
-                            if (pred[h] || head[h]) {
-                                const size_t r = (base + yh[h]) * kLanes + lane;
-                                du[h] = dist[r];
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int h = 0; h < kDialAhead; h++) {
-                        pred[h] = pred[h] && P::tie(P::add(du[h], lu[h]), dx);
-                        if (pred[h]) su[h] = sig[(base + yh[h]) * kLanes + lane];
-                    }
-#pragma unroll
-                    for (int h = 0; h < kDialAhead; h++) {
-                        if (yh[h] == kNone) continue;
-                        if (pred[h]) {
-                            acc += su[h];
-                            npred++;
-                        }
-                        bool offered = false;
-                        uint32_t kk = 0;
-                        const typename P::Sum nd = P::add(dx, lu[h]);
-                        if (head[h] && nd < (typename P::Sum)du[h]) {
-                            P::offer(&dist[(base + yh[h]) * kLanes + lane], nd);
-                            offered = true;
-                            const uint64_t bnd = P::bucket(nd, delta);
-                            kk = (uint32_t)(bnd % (uint64_t)W);
-                            least = min(least, (unsigned long long)bnd);
-                        }
-                        uint64_t pending = __ballot(offered);
-                        while (pending) {
-                            const int f = __ffsll((unsigned long long)pending) - 1;
-                            const uint32_t kf = rl32(kk, f);
-                            pending &= ~__ballot(offered && kk == kf);
-                            if (lane == f)
-                                atomicOr((unsigned long long*)&pend[(base + yh[h]) * words + (kf >> 6)],
-                                         1ull << (kf & 63));
-                        }
-                    }
-                }
-            }
+            dial_step_packed<P, kCount>(a0, a1, y0, ln0, vw0, M, base, dx, W, words, delta,
+                                        !first_step, idx, len_arr, vis, dist, sig, pend, acc,
+                                        npred, least);
             if (!first_step) {
                 if (kCount && mine) {
                     sig[(base + x) * kLanes + lane] = acc;
