@@ -1146,14 +1146,37 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
             U.has_weights = true;
             U.w.assign(weights, weights + U.m);
         }
+        // one pass over the edges: each graph's checks (fill_graph's, with
+        // its messages) and its arcs shifted to its node range
         for (int64_t i = 0; i < graphs; i++) {
-            netbc::Graph one;
-            uint64_t e0 = edge0[i];
-            fill_graph(one, node_counts[i], src + e0, tgt + e0, weights ? weights + e0 : nullptr,
-                       edge_counts[i], directed != 0, "graph " + std::to_string(i) + ": ");
-            for (uint64_t e = 0; e < one.m; e++) {
-                U.src[e0 + e] = one.src[e] + (uint32_t)node0[i];
-                U.tgt[e0 + e] = one.tgt[e] + (uint32_t)node0[i];
+            const uint64_t e0 = edge0[i], m = (uint64_t)edge_counts[i];
+            const uint64_t n = (uint64_t)node_counts[i];
+            const uint32_t shift = (uint32_t)node0[i];
+            auto fail = [&](const std::string& what) {
+                throw netbc::Error("graph " + std::to_string(i) + ": " + what);
+            };
+            if (n > 0xFFFFFFFFull)
+                fail("node count (" + std::to_string(n) + ") exceeds the supported maximum");
+            if (weights)
+                for (uint64_t e = 0; e < m; e++)
+                    if (!std::isfinite(weights[e0 + e]))
+                        fail("edge " + std::to_string(e) +
+                             " has a missing, NaN, or infinite weight");
+            if (m > 0xFFFFFFFFull / 2)
+                fail("edge count (" + std::to_string(m) + ") exceeds the supported maximum");
+            const int64_t* si = src + e0;
+            const int64_t* ti = tgt + e0;
+            uint32_t* su = U.src.data() + e0;
+            uint32_t* tu = U.tgt.data() + e0;
+            for (uint64_t e = 0; e < m; e++) {
+                const int64_t a = si[e], b = ti[e];
+                if ((uint64_t)a >= n || (uint64_t)b >= n) {
+                    const int64_t v = (uint64_t)a >= n ? a : b;
+                    fail("edge " + std::to_string(e) + " refers to node index " +
+                         std::to_string(v) + ", but valid indices are 0.." + std::to_string(n));
+                }
+                su[e] = (uint32_t)a + shift;
+                tu[e] = (uint32_t)b + shift;
             }
         }
         netbc::RunDiag total;
@@ -1164,7 +1187,8 @@ NETBC_EXPORT int32_t netbc_betweenness_batch(int64_t graphs, const int64_t* node
                                              normalized != 0, weighted != 0, descs, node_graph);
         std::vector<uint32_t> too_deep((size_t)graphs, 0);
         if (any_device) {
-            prepare(U);
+            ensure_device();
+            netbc::build_device_csr(U);
             netbc::run_small_batch(U, descs, node_graph, o, out, too_deep, total);
         }
         run_left_over(graphs, node_counts, edge_counts, node0, edge0, src, tgt, weights,

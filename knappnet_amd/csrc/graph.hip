@@ -446,8 +446,8 @@ bool ensure_hub_tables(Graph& g, uint32_t hub_in, uint32_t hub_chunk) {
     return true;
 }
 
-void build_device_graph(Graph& g) {
-    if (g.dev_ready) return;
+void build_device_csr(Graph& g) {
+    if (g.dev_csr) return;
     ensure_stream(g);
     hipStream_t st = g.stream;
     DeviceGraph& d = g.dev;
@@ -489,6 +489,13 @@ void build_device_graph(Graph& g) {
         build_one(g.n, g.m, true, true, ds.p, dt.p, g.has_weights ? dw.p : nullptr, nullptr,
                   arcs, d.in_off, d.in_idx, d.in_w, nullptr, st);
     NETBC_HIP(hipStreamSynchronize(st));
+    g.dev_csr = true;
+}
+
+void build_device_graph(Graph& g) {
+    if (g.dev_ready) return;
+    build_device_csr(g);
+    DeviceGraph& d = g.dev;
     // Hub tables from the in-degrees.
     std::vector<uint32_t> ioff(g.n + 1);
     NETBC_HIP(hipMemcpy(ioff.data(), d.in_off_p(), (g.n + 1) * 4, hipMemcpyDeviceToHost));

@@ -149,7 +149,8 @@ struct Graph {
     bool has_weights = false;
 
     std::mutex mu;  // guards dev, the stream and the cached workspace
-    bool dev_ready = false;
+    bool dev_ready = false;  // device CSR and the batched engine's tables built
+    bool dev_csr = false;    // device CSR built
     DeviceGraph dev;
     hipStream_t stream = nullptr;
     int device = -1;
@@ -202,6 +203,8 @@ struct RunDiag {
 
 // Builds the device CSR/CSC of g (graph.hip). Caller holds g.mu.
 void build_device_graph(Graph& g);
+// The device CSR alone (the many-small-graphs engine reads nothing else).
+void build_device_csr(Graph& g);
 // Re-cuts the hub tables for other thresholds (hub_in: in-degree above
 // which a node is a hub; hub_chunk: in-arcs per task); true if they changed.
 bool ensure_hub_tables(Graph& g, uint32_t hub_in, uint32_t hub_chunk);

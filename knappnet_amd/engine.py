@@ -247,6 +247,17 @@ class Graph:
         return off, tgt[:arcs], None if w is None else w[:arcs]
 
 
+def _cat_i64(parts) -> np.ndarray:
+    """The arrays concatenated as one contiguous int64 array (one
+    conversion when they share an integer type)."""
+    arrs = [np.asarray(p) for p in parts]
+    if not arrs:
+        return np.zeros(0, np.int64)
+    if all(a.dtype == arrs[0].dtype for a in arrs) and arrs[0].dtype.kind in "iu":
+        return np.ascontiguousarray(np.concatenate(arrs).astype(np.int64, copy=False))
+    return _i64(np.concatenate([np.asarray(a, dtype=np.int64) for a in arrs]))
+
+
 def betweenness_batch(graphs, directed: bool = False, weighted: bool = False,
                       normalized: bool = False, weights_as: str = "distance",
                       precision: int = 0, slots: int = 0, sweep_block: int = 0):
@@ -257,10 +268,8 @@ def betweenness_batch(graphs, directed: bool = False, weighted: bool = False,
     lib = _native.device()
     ns = np.array([int(g[0]) for g in graphs], dtype=np.int64)
     ms = np.array([len(g[1]) for g in graphs], dtype=np.int64)
-    src = _i64(np.concatenate([np.asarray(g[1], dtype=np.int64) for g in graphs])
-               if graphs else np.zeros(0, np.int64))
-    tgt = _i64(np.concatenate([np.asarray(g[2], dtype=np.int64) for g in graphs])
-               if graphs else np.zeros(0, np.int64))
+    src = _cat_i64([g[1] for g in graphs])
+    tgt = _cat_i64([g[2] for g in graphs])
     w = None
     if weighted:
         w = np.ascontiguousarray(np.concatenate([np.asarray(g[3], dtype=np.float64)
