@@ -4623,46 +4623,59 @@ void landmark_distances(const DeviceGraph& dg, const T* len, int lanes, double u
 // counts). Writes the list to ws.list2.
 void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hipStream_t st,
                    const Lengths* L = nullptr) {
-    const DeviceGraph& dg = g.dev;
+    DeviceGraph& dg = g.dev;
     const size_t n = g.n;
     const int lanes = (int)dg.landmarks.size();
-    DevBuf<uint8_t> dist;
-    dist.alloc(n * kLandmarks);
-    NETBC_HIP(hipMemsetAsync(dist.p, 15, n * kLandmarks, st));
-    DevBuf<uint32_t> cur, vis, nxt;
-    cur.alloc(n);
-    vis.alloc(n);
-    nxt.alloc(n);
-    NETBC_HIP(hipMemsetAsync(cur.p, 0, n * 4, st));
-    NETBC_HIP(hipMemsetAsync(nxt.p, 0, n * 4, st));
-    if (lanes > 0) {
-        // level 0: the landmarks themselves
-        std::vector<uint32_t> seed(n, 0);
-        for (int j = 0; j < lanes; j++) seed[dg.landmarks[j]] |= 1u << j;
-        NETBC_HIP(hipMemcpyAsync(cur.p, seed.data(), n * 4, hipMemcpyHostToDevice, st));
-        NETBC_HIP(hipMemcpyAsync(vis.p, seed.data(), n * 4, hipMemcpyHostToDevice, st));
-        for (int j = 0; j < lanes; j++)
-            NETBC_HIP(hipMemsetAsync(dist.p + (size_t)dg.landmarks[j] * kLandmarks + j, 0, 1, st));
-        const unsigned offer_blocks = resident_blocks(landmark_offer_kernel, (n + 63) / 64);
-        const unsigned settle_blocks = blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22));
-        for (int level = 1; level < 15; level++) {
-            landmark_offer_kernel<<<offer_blocks, kBlock, 0, st>>>(dg.in_off_p(), dg.in_idx_p(),
-                                                                   cur.p, vis.p, nxt.p, n);
-            NETBC_HIP(hipGetLastError());
-            landmark_settle_kernel<<<settle_blocks, kBlock, 0, st>>>(cur.p, vis.p, nxt.p, dist.p,
-                                                                     level, lanes, n);
-            NETBC_HIP(hipGetLastError());
+    if (!dg.lm_hops_ready) {
+        DevBuf<uint8_t>& dist = dg.lm_hops;
+        dist.alloc(n * kLandmarks);
+        NETBC_HIP(hipMemsetAsync(dist.p, 15, n * kLandmarks, st));
+        DevBuf<uint32_t> cur, vis, nxt;
+        cur.alloc(n);
+        vis.alloc(n);
+        nxt.alloc(n);
+        NETBC_HIP(hipMemsetAsync(cur.p, 0, n * 4, st));
+        NETBC_HIP(hipMemsetAsync(nxt.p, 0, n * 4, st));
+        if (lanes > 0) {
+            // level 0: the landmarks themselves
+            std::vector<uint32_t> seed(n, 0);
+            for (int j = 0; j < lanes; j++) seed[dg.landmarks[j]] |= 1u << j;
+            NETBC_HIP(hipMemcpyAsync(cur.p, seed.data(), n * 4, hipMemcpyHostToDevice, st));
+            NETBC_HIP(hipMemcpyAsync(vis.p, seed.data(), n * 4, hipMemcpyHostToDevice, st));
+            for (int j = 0; j < lanes; j++)
+                NETBC_HIP(hipMemsetAsync(dist.p + (size_t)dg.landmarks[j] * kLandmarks + j, 0, 1,
+                                         st));
+            const unsigned offer_blocks = resident_blocks(landmark_offer_kernel, (n + 63) / 64);
+            const unsigned settle_blocks =
+                blocks_for_threads(std::min<size_t>(n, (size_t)1 << 22));
+            for (int level = 1; level < 15; level++) {
+                landmark_offer_kernel<<<offer_blocks, kBlock, 0, st>>>(
+                    dg.in_off_p(), dg.in_idx_p(), cur.p, vis.p, nxt.p, n);
+                NETBC_HIP(hipGetLastError());
+                landmark_settle_kernel<<<settle_blocks, kBlock, 0, st>>>(
+                    cur.p, vis.p, nxt.p, dist.p, level, lanes, n);
+                NETBC_HIP(hipGetLastError());
+            }
+            // the seed buffer must outlive the copies
+            NETBC_HIP(hipStreamSynchronize(st));
         }
-        // the seed buffer must outlive the copies
-        NETBC_HIP(hipStreamSynchronize(st));
+        dg.lm_hops_ready = true;
     }
+    const uint8_t* table = dg.lm_hops.p;
+    // weighted runs: this run's weighted distances in the first columns of
+    // a copy of the table
+    DevBuf<uint8_t> wdist;
     const bool weighted =
         L && lanes > 0 && (L->kind == kInteger || (L->kind == kReal && real_bucketed(*L, n)));
     if (weighted) {
+        wdist.alloc(n * kLandmarks);
+        NETBC_HIP(hipMemcpyAsync(wdist.p, dg.lm_hops.p, n * kLandmarks, hipMemcpyDeviceToDevice,
+                                 st));
         if (L->kind == kInteger)
-            landmark_distances<uint32_t>(dg, L->out_len.p, lanes, 1.0, dist.p, st);
+            landmark_distances<uint32_t>(dg, L->out_len.p, lanes, 1.0, wdist.p, st);
         else
-            landmark_distances<double>(dg, L->out_real.p, lanes, L->rmin, dist.p, st);
+            landmark_distances<double>(dg, L->out_real.p, lanes, L->rmin, wdist.p, st);
+        table = wdist.p;
     }
     DevBuf<uint64_t> k0, k1;
     DevBuf<uint32_t> v0, v1;
@@ -4680,7 +4693,7 @@ void order_sources(Graph& g, Workspace& ws, const int64_t* d_list, int64_t k, hi
     for (int which = 0; which < 3; which++) {
         const int kind = which == 2 && weighted ? 3 : which;
         source_key_kernel<<<blocks_for_threads(k), kBlock, 0, st>>>(
-            d_list, k, perm, dg.out_off.p, dg.out_idx.p, dist.p, (uint32_t)n, kind, key);
+            d_list, k, perm, dg.out_off.p, dg.out_idx.p, table, (uint32_t)n, kind, key);
         NETBC_HIP(hipGetLastError());
         const int bits = which < 2 ? b : weighted ? 8 * kWeightedLandmarks : 4 * kLandmarks;
         if (sort_pairs_u64(key, perm, key_tmp, perm_tmp, k, bits, st)) {

@@ -134,6 +134,11 @@ struct DeviceGraph {
     // Landmarks for batching sources: the nodes of largest in-degree
     // (ties by id), at most kLandmarks of them.
     std::vector<uint32_t> landmarks;
+    // Hop distances from every node to the landmarks (a byte each, capped
+    // at 15; n x kLandmarks), computed by the first run that batches
+    // sources and kept: they depend on the graph alone.
+    DevBuf<uint8_t> lm_hops;
+    bool lm_hops_ready = false;
     const uint32_t* in_off_p() const { return directed ? in_off.p : out_off.p; }
     const uint32_t* in_idx_p() const { return directed ? in_idx.p : out_idx.p; }
     const double* in_w_p() const { return directed ? in_w.p : out_w.p; }
