@@ -112,17 +112,11 @@ constexpr double kTwo53 = 9007199254740992.0;
 #define NETBC_AHEAD 8
 #endif
 constexpr int kAhead = NETBC_AHEAD;
-// The same for the bucketed searches: hits per batch of the step
-// (dial_step_kernel) and of the sweep (dial_backward_kernel), whose
-// registers are capped for 7 wavefronts per SIMD. Measured on BA n=200,000
-// with lengths 1-16 (16,384 sources): the sweep with 4 hits per batch at 7
-// wavefronts per SIMD 443 -> 373 ms against 8 hits at 5 (its register
-// use); the step is fastest with 8 hits and no cap (4 hits, or a cap of 6
-// wavefronts: 10-15% slower).
-#ifndef NETBC_DIAL_AHEAD
-#define NETBC_DIAL_AHEAD 8
-#endif
-constexpr int kDialAhead = NETBC_DIAL_AHEAD;
+// The same for the bucketed sweep's unpacked walk (dial_backward_kernel,
+// entries wider than kPackLanes), whose registers are capped for 7
+// wavefronts per SIMD. Measured on BA n=200,000 with lengths 1-16 (16,384
+// sources): 4 hits per batch at 7 wavefronts per SIMD 443 -> 373 ms against
+// 8 hits at 5 (its register use).
 #ifndef NETBC_DIAL_BACK_AHEAD
 #define NETBC_DIAL_BACK_AHEAD 4
 #endif
@@ -1869,11 +1863,14 @@ __device__ __forceinline__ uint32_t select_lanes(uint64_t M) {
 // `settle`), any other takes the offer dist[x] + length by atomic minimum
 // and marks the offer's bucket (one atomic per distinct neighbour and
 // bucket). Each lane of M then adds its predecessors' path counts in CSR
-// order, as the unpacked walk does. The first 64 arcs (y0, ln0, vw0) are
-// loaded by the caller. Every lane calls.
-template <class P, bool kCount>
+// order, as the unpacked walk does. With `preloaded` the first 64 arcs
+// (y0, ln0, vw0) are loaded by the caller. kWalk 0: the undirected row
+// (both); 1: an in-row (predecessors only); 2: an out-row (offers only).
+// Every lane calls.
+template <class P, bool kCount, int kWalk = 0>
 __device__ __forceinline__ void dial_step_packed(
-    uint64_t a0, uint64_t a1, uint32_t y0, typename P::Len ln0, uint64_t vw0, uint64_t M,
+    uint64_t a0, uint64_t a1, bool preloaded, uint32_t y0, typename P::Len ln0, uint64_t vw0,
+    uint64_t M,
     size_t base, typename P::Dist dx, int W, int words, double delta, bool settle,
     const uint32_t* __restrict__ idx, const typename P::Len* __restrict__ len_arr,
     const uint64_t* __restrict__ vis, typename P::Dist* __restrict__ dist,
@@ -1897,7 +1894,7 @@ __device__ __forceinline__ void dial_step_packed(
         uint32_t y = y0;
         Len ln = ln0;
         uint64_t vw = vw0;
-        if (c != a0) {
+        if (c != a0 || !preloaded) {
             y = ok ? idx[a] : 0;
             ln = ok ? len_arr[a] : Len(0);
             vw = ok ? vis[base + y] : 0;
@@ -1918,10 +1915,10 @@ __device__ __forceinline__ void dial_step_packed(
                 lp[r] = __shfl(ln, jj, 64);
                 const uint64_t vj = __shfl(vw, jj, 64);
                 const bool settled = (vj >> ls) & 1;
-                pr[r] = valid && kCount && settle && settled;
-                hd[r] = valid && !settled;
+                pr[r] = kWalk != 2 && valid && kCount && settle && settled;
+                hd[r] = kWalk != 1 && valid && !settled;
                 row[r] = (base + yr[r]) * kLanes + ls;
-                du[r] = valid ? dist[row[r]] : Dist(0);
+                du[r] = pr[r] || hd[r] ? dist[row[r]] : Dist(0);
             }
             double su[kPackRounds];
             uint64_t tm[kPackRounds];
@@ -1984,8 +1981,8 @@ __device__ __forceinline__ void dial_step_packed(
 // of M already settled at y are the predecessor candidates and the others
 // take offers, so one visited word and one distance row of y serve both
 // (lanes at y settled in this same bucket are neither: every length is
-// longer than a bucket). Directed graphs walk the in-row, then the
-// out-row, kDialAhead neighbours per batch.
+// longer than a bucket). Directed graphs walk the in-row (predecessors),
+// then the out-row (offers), with the pairs packed the same way.
 template <class P, bool kCount = true, bool kSym = false>
 __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t* __restrict__ llen,
                                  uint32_t D, int W, int words, int first_step, double delta,
@@ -2062,7 +2059,7 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
             const uint64_t a0 = te.z, a1 = te.w;
-            dial_step_packed<P, kCount>(a0, a1, y0, ln0, vw0, M, base, dx, W, words, delta,
+            dial_step_packed<P, kCount>(a0, a1, true, y0, ln0, vw0, M, base, dx, W, words, delta,
                                         !first_step, idx, len_arr, vis, dist, sig, pend, acc,
                                         npred, least);
             if (!first_step) {
@@ -2081,51 +2078,11 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
         if (!first_step) {
             double acc = 0.0;
             uint32_t npred = 0;  // predecessor arcs in this lane
-            const uint64_t a0 = in_off[x], a1 = kCount ? in_off[x + 1] : a0;
-            for (uint64_t c = a0; c < a1; c += 64) {
-                const uint64_t a = c + lane;
-                const bool ok = a < a1;
-                const uint32_t u = ok ? in_idx[a] : 0;
-                const Len ln = ok ? in_len[a] : Len(0);
-                const uint64_t mm = ok ? (vis[base + u] & M) : 0;
-                uint64_t bal = __ballot(mm != 0);
-                while (bal) {
-                    // up to kDialAhead predecessors loaded before they are added
-                    // (distances and path counts together: loading the counts
-                    // of tying lanes only, after the distances, measured
-                    // slower: one more round trip per batch of hits)
-                    Dist du[kDialAhead];
-                    double su[kDialAhead];
-                    Len lu[kDialAhead];
-                    bool use[kDialAhead];
-#pragma unroll
-                    for (int h = 0; h < kDialAhead; h++) {
-                        use[h] = false;
-                        su[h] = 0.0;
-                        du[h] = Dist(0);
-                        lu[h] = Len(0);
-                        if (bal) {
-                            const int j = __ffsll((unsigned long long)bal) - 1;
-                            bal &= bal - 1;
-                            const uint32_t uu = rl32(u, j);
-                            lu[h] = rl(ln, j);
-                            const uint64_t m2 = rl64(mm, j);
-                            if ((m2 >> lane) & 1) {
-                                const size_t r = (base + uu) * kLanes + lane;
-                                use[h] = true;
-                                du[h] = dist[r];
-                                su[h] = sig[r];
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int h = 0; h < kDialAhead; h++)
-                        if (use[h] && P::tie(P::add(du[h], lu[h]), dx)) {
-                            acc += su[h];
-                            npred++;
-                        }
-                }
-            }
+            // predecessors over the in-row (pairs packed as for undirected rows)
+            if (kCount)
+                dial_step_packed<P, kCount, 1>(in_off[x], in_off[x + 1], false, 0, Len(0), 0, M,
+                                               base, dx, W, words, delta, true, in_idx, in_len,
+                                               vis, dist, sig, pend, acc, npred, least);
             if (kCount && mine) {
                 sig[(base + x) * kLanes + lane] = acc;
                 if (npr) npr[(base + x) * kLanes + lane] = npred;
@@ -2136,65 +2093,12 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
             cnt += 1;
         }
-        const uint64_t a0 = te.z, a1 = te.w;
-        for (uint64_t c = a0; c < a1; c += 64) {
-            const uint64_t a = c + lane;
-            const bool ok = a < a1;
-            const uint32_t y = ok ? idx[a] : 0;
-            const Len ln = ok ? len_arr[a] : Len(0);
-            const uint64_t mm = ok ? (M & ~vis[base + y]) : 0;
-            NETBC_COUNT(c3, mm != 0);
-            uint64_t bal = __ballot(mm != 0);
-            while (bal) {
-                // up to kDialAhead heads: their distances loaded first, then offered
-                uint32_t yh[kDialAhead];
-                Dist cur[kDialAhead];
-                typename P::Sum nd[kDialAhead];
-                bool use[kDialAhead];
-#pragma unroll
-                for (int h = 0; h < kDialAhead; h++) {
-                    yh[h] = kNone;
-                    use[h] = false;
-                    cur[h] = Dist(0);
-                    nd[h] = typename P::Sum(0);
-                    if (bal) {
-                        const int j = __ffsll((unsigned long long)bal) - 1;
-                        bal &= bal - 1;
-                        yh[h] = rl32(y, j);
-                        const uint64_t m2 = rl64(mm, j);
-                        const Len lh = rl(ln, j);  // a cross-lane read: every lane takes part
-                        if ((m2 >> lane) & 1) {
-                            use[h] = true;
-                            nd[h] = P::add(dx, lh);
-                            cur[h] = dist[(base + yh[h]) * kLanes + lane];
-                        }
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < kDialAhead; h++) {
-                    if (yh[h] == kNone) continue;
-                    bool offered = false;
-                    uint32_t kk = 0;
-                    if (use[h] && nd[h] < (typename P::Sum)cur[h]) {
-                        P::offer(&dist[(base + yh[h]) * kLanes + lane], nd[h]);
-                        offered = true;
-                        const uint64_t bnd = P::bucket(nd[h], delta);
-                        kk = (uint32_t)(bnd % (uint64_t)W);
-                        least = min(least, (unsigned long long)bnd);
-                    }
-                    // mark the offered buckets (one atomic per distinct bucket)
-                    uint64_t pending = __ballot(offered);
-                    while (pending) {
-                        const int f = __ffsll((unsigned long long)pending) - 1;
-                        const uint32_t kf = rl32(kk, f);
-                        pending &= ~__ballot(offered && kk == kf);
-                        if (lane == f)
-                            atomicOr((unsigned long long*)&pend[(base + yh[h]) * words + (kf >> 6)],
-                                     1ull << (kf & 63));
-                    }
-                }
-            }
-        }
+        // offers over the out-row
+        double acc_unused = 0.0;
+        uint32_t npred_unused = 0;
+        dial_step_packed<P, kCount, 2>(te.z, te.w, false, 0, Len(0), 0, M, base, dx, W, words,
+                                       delta, false, idx, len_arr, vis, dist, sig, pend, acc_unused,
+                                       npred_unused, least);
     }
     if (kCount) retire_tickets(ctrl, xc);
     if (cnt) flush();
