@@ -2904,6 +2904,76 @@ __global__ void accumulate_kernel(const uint64_t* __restrict__ vis, double* __re
     }
 }
 
+// The same sums with a wavefront per node, for super-batches that span
+// many groups (few sources per group): lane j takes group gA + j (gA, gB:
+// the first and last group of this super-batch's items), adds its items'
+// dependencies in item order from 0.0 (group gA from the carried partial
+// sum when it continues one), and the group sums are then added into bc in
+// group order; the last group's sum is carried. Groups are summed in the
+// same order and each group's items in the same order as accumulate_kernel
+// (a group without contributions adds +0.0, which changes nothing), so the
+// bits are the same.
+__global__ void accumulate_groups_kernel(const uint64_t* __restrict__ vis,
+                                         double* __restrict__ sig, int slots, size_t n,
+                                         int64_t item0, int64_t k, int64_t gsize,
+                                         const int64_t* __restrict__ list,
+                                         double* __restrict__ acc, double* __restrict__ bc,
+                                         int64_t* __restrict__ gcur,
+                                         const uint32_t* __restrict__ off, Ctrl* ctrl) {
+    const int lane = lane_id();
+    const int64_t iend = min(k, item0 + (int64_t)slots * kLanes);
+    const int64_t gA = item0 / gsize, gB = (iend - 1) / gsize;
+    const int active = (int)((iend - item0 + kLanes - 1) / kLanes);  // slots with items
+    unsigned long long rn = 0, ra = 0;
+    for (size_t v = global_wave(); v < n; v += wave_count()) {
+        double a = acc[v], B = bc[v];
+        const int64_t g = gcur[v];
+        if (g < gA) {  // the carried group ends before this super-batch
+            B += a;
+            a = 0.0;
+        }
+        for (int64_t g0 = gA; g0 <= gB; g0 += kLanes) {
+            const int64_t gj = g0 + lane;
+            double sum = gj == g ? a : 0.0;
+            if (gj <= gB) {
+                const int64_t i0 = max(gj * gsize, item0), i1 = min((gj + 1) * gsize, iend);
+                for (int64_t i = i0; i < i1; i++) {
+                    const int64_t rel = i - item0;
+                    const size_t p = (size_t)(rel >> 6) * n + v;
+                    const int l = (int)(rel & 63);
+                    if ((vis[p] >> l) & 1) {
+                        double* x = &sig[p * kLanes + l];
+                        const double d = *x;
+                        *x = 0.0;  // rows start at zero for the next super-batch
+                        const int64_t src = list ? list[i] : i;
+                        if ((size_t)src != v) sum += d;
+                    }
+                }
+            }
+            for (int j = 0; j < kLanes && g0 + j <= gB; j++) {
+                const double sj = __shfl(sum, j, 64);
+                if (g0 + j < gB) B += sj;
+                else a = sj;
+            }
+        }
+        unsigned long long r = 0;
+        for (int b = lane; b < active; b += kLanes)
+            r += (unsigned long long)__popcll(vis[(size_t)b * n + v]);
+        const unsigned long long rv = wave_sum_u64(r);
+        if (lane == 0) {
+            acc[v] = a;
+            bc[v] = B;
+            gcur[v] = gB;
+            rn += rv;
+            ra += rv * (unsigned long long)(off[v + 1] - off[v]);
+        }
+    }
+    if (lane == 0 && (rn || ra)) {
+        atomicAdd(&ctrl->reached_nodes[blockIdx.x % kSpread], rn);
+        atomicAdd(&ctrl->reached_arcs[blockIdx.x % kSpread], ra);
+    }
+}
+
 // Zeroes the per-lane rows of every reached (node, lane) of the slots, so
 // the path-count rows start at zero for the next super-batch.
 __global__ void clear_rows_kernel(const uint64_t* __restrict__ vis, double* __restrict__ sig,
@@ -4772,6 +4842,12 @@ void run_batched(Graph& g, const Lengths& L, const int64_t* sources, int64_t k,
                                        kBlock, 0, st>>>(ws.vis.p, ws.sig.p, g.dev.hub_list.p,
                                                         g.dev.hubs, slots, n);
             }
+        } else if (std::min<int64_t>(k - item0, (int64_t)slots * kLanes) >= 16 * gsize) {
+            // many groups per super-batch: a wavefront per node, a lane per group
+            accumulate_groups_kernel<<<resident_blocks(accumulate_groups_kernel, n), kBlock, 0,
+                                       st>>>(ws.vis.p, ws.sig.p, slots, n, item0, k, gsize,
+                                             d_list, ws.acc.p, d_bc, ws.gcur.p, g.dev.out_off.p,
+                                             ws.ctrl.p);
         } else
             accumulate_kernel<<<blocks_for_threads(n), kBlock, 0, st>>>(
                 ws.vis.p, ws.sig.p, slots, n, item0, k, gsize, d_list, ws.acc.p, d_bc, ws.gcur.p,
