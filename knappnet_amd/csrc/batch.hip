@@ -52,6 +52,9 @@ struct Ctrl {
     uint32_t hlen;       // heavy-row entries listed (positions in the level queue)
     uint32_t left[2];    // bucketed searches: tentative distances still pending
     uint32_t hchunks;    // chunk tasks of the heavy rows listed last
+    // Bucketed searches: entries with long rows, listed from the end of the
+    // bucket list (alternating by step parity like tlen).
+    uint32_t tlong[2];
     // Direction choice: out-arcs of the entries of a level (alternating by
     // level parity) and in-arcs of the (slot, node) pairs whose active lanes
     // are all reached.
@@ -1728,7 +1731,8 @@ struct RealPathExact : RealPath {
 // pair's distance row is in flight), and clears it, records whether any bucket is still set anywhere, and lowers
 // next_bucket to the first bucket after D still set at some pair (the
 // pending buckets of a pair lie in (D, D + W), so a residue r names bucket
-// D + ((r - D) mod W)); the step adds the buckets it offers to. It runs
+// D + ((r - D) mod W)); the step adds the buckets it offers to. Pairs of
+// rows longer than kLongRow are listed from the list's end instead. It runs
 // one workgroup per CU (with a workgroup per resident slot the wavefronts'
 // appends to the list's one counter made it 3x slower), each lane loading
 // kScanUnroll bucket words at once. The wavefronts advance through the
@@ -1737,10 +1741,21 @@ struct RealPathExact : RealPath {
 // rely on for L2 hits (one append per workgroup at its end measured 2%
 // slower overall for that reason).
 constexpr int kScanUnroll = 4;
+// Pairs whose node has more out-arcs than this are listed from the end of
+// the list (count in tlong), and the step takes them first: one wavefront
+// walks a row, so a long row taken last would end the launch alone (BA
+// n = 200,000, 32,768 sources: 1.844 s with no such rows, 1.80 s from 512
+// arcs; from 128 or 256 the sweep, fed in the step's order, slowed more
+// than the step gained).
+#ifndef NETBC_LONG_ROW
+#define NETBC_LONG_ROW 512
+#endif
+constexpr uint32_t kLongRow = NETBC_LONG_ROW;
 __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, uint32_t D,
                                  size_t count, const uint32_t* __restrict__ off,
                                  uint4* __restrict__ list,
                                  uint32_t* __restrict__ llen, uint32_t* __restrict__ llen_next,
+                                 uint32_t* __restrict__ tlong, uint32_t* __restrict__ tlong_next,
                                  uint32_t* __restrict__ left, uint32_t* __restrict__ left_next,
                                  unsigned long long* __restrict__ next_bucket,
                                  unsigned long long* __restrict__ next_bucket_next, size_t n) {
@@ -1749,6 +1764,7 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
     const int lane = lane_id();
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *llen_next = 0;
+        *tlong_next = 0;
         *left_next = 0;
         *next_bucket_next = ~0ull;
     }
@@ -1805,14 +1821,31 @@ __global__ void dial_scan_kernel(uint64_t* __restrict__ pend, int words, int W, 
             }
             uint64_t bal = __ballot(hit);
             if (bal) {
-                uint32_t k = (uint32_t)__popcll(bal);
-                if (cnt + k > kTouchBuf) flush();
+                uint32_t x = 0, o0 = 0, o1 = 0;
                 if (hit) {
-                    const uint32_t x = (uint32_t)(i % n);
-                    mybuf[cnt + __popcll(bal & lt)] =
-                        make_uint4(x, (uint32_t)(i / n), off[x], off[x + 1]);
+                    x = (uint32_t)(i % n);
+                    o0 = off[x];
+                    o1 = off[x + 1];
                 }
-                cnt += k;
+                const bool lng = hit && o1 - o0 > kLongRow;
+                const uint64_t lb = __ballot(lng);
+                if (lb) {
+                    uint32_t at = 0;
+                    if (lane == 0) at = atomicAdd(tlong, (uint32_t)__popcll(lb));
+                    at = rl32(at, 0);
+                    if (lng)
+                        list[count - 1 - (at + (uint32_t)__popcll(lb & lt))] =
+                            make_uint4(x, (uint32_t)(i / n), o0, o1);
+                    bal &= ~lb;
+                }
+                const uint32_t k = (uint32_t)__popcll(bal);
+                if (k) {
+                    if (cnt + k > kTouchBuf) flush();
+                    if (hit && !lng)
+                        mybuf[cnt + __popcll(bal & lt)] =
+                            make_uint4(x, (uint32_t)(i / n), o0, o1);
+                    cnt += k;
+                }
             }
         }
     }
@@ -1985,7 +2018,7 @@ __device__ __forceinline__ void dial_step_packed(
 // then the out-row (offers), with the pairs packed the same way.
 template <class P, bool kCount = true, bool kSym = false>
 __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t* __restrict__ llen,
-                                 uint32_t D, int W, int words, int first_step, double delta,
+                                 const uint32_t* __restrict__ tlong, size_t cap, uint32_t D, int W, int words, int first_step, double delta,
                                  uint64_t* __restrict__ pend,
                                  const uint32_t* __restrict__ in_off,
                                  const uint32_t* __restrict__ in_idx,
@@ -2016,7 +2049,9 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
     };
     double smax = 0.0;
     [[maybe_unused]] unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
-    const uint32_t T = *llen;
+    // long rows (listed from the end) first
+    const uint32_t TL = *tlong;
+    const uint32_t T = *llen + TL;
     // entries by ticket when counting (their rows differ widely in length);
     // distances-only entries are cheap, and a static share costs less than
     // a ticket atomic per entry
@@ -2028,7 +2063,7 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
          t = !kCount                ? t + wave_count()
              : ((t + 1) & (G - 1)) ? t + 1
                                    : (size_t)next_ticket(ctrl, xc) * G) {
-        const uint4 te = list[t];  // node, slot, out-row bounds
+        const uint4 te = t < TL ? list[cap - 1 - t] : list[t - TL];  // node, slot, out-row bounds
         const uint32_t x = te.x;
         const size_t base = (size_t)te.y * n;
         // undirected: the row's first 64 arcs and their visited words are
@@ -4252,11 +4287,12 @@ struct Engine {
             const int p = it & 1;
             dial_scan_kernel<<<scan_blocks, kBlock, 0, st>>>(
                 ws.inb.p, words, W, D, pairs, d.out_off.p, ws.dlist.p, &ws.ctrl.p->tlen[p],
-                &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->left[p], &ws.ctrl.p->left[p ^ 1],
+                &ws.ctrl.p->tlen[p ^ 1], &ws.ctrl.p->tlong[p], &ws.ctrl.p->tlong[p ^ 1],
+                &ws.ctrl.p->left[p], &ws.ctrl.p->left[p ^ 1],
                 &ws.ctrl.p->next_bucket[p], &ws.ctrl.p->next_bucket[p ^ 1], n);
             launch_check();
             step<<<ticket_grid(step_blocks), kBlock, 0, st>>>(
-                ws.dlist.p, &ws.ctrl.p->tlen[p], D, W, words, D == 0, bk.delta, ws.inb.p,
+                ws.dlist.p, &ws.ctrl.p->tlen[p], &ws.ctrl.p->tlong[p], pairs, D, W, words, D == 0, bk.delta, ws.inb.p,
                 d.in_off_p(), d.in_idx_p(), in_lengths<P>(), d.out_off.p, d.out_idx.p,
                 out_lengths<P>(), dist, ws.vis.p, ws.sig.p, ws.q.p, (uint32_t)ws.qcap, ws.ctrl.p,
                 &ws.ctrl.p->next_bucket[p], n, npr);
@@ -4272,7 +4308,9 @@ struct Engine {
                 diag.levels++;
             }
             // Done when this step listed nothing and no bucket is pending.
-            if (ws.host_ctrl->tlen[p] == 0 && ws.host_ctrl->left[p] == 0) break;
+            if (ws.host_ctrl->tlen[p] == 0 && ws.host_ctrl->tlong[p] == 0 &&
+                ws.host_ctrl->left[p] == 0)
+                break;
             // otherwise on to the lowest pending bucket (empty ones skipped)
             const unsigned long long nb = ws.host_ctrl->next_bucket[p];
             if (nb > 0xFFFFFFFFull && nb != ~0ull) throw Error("bucket index exceeds 2^32");

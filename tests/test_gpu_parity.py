@@ -265,6 +265,25 @@ def test_dial_ba_bitexact():
     assert_same(got, o.betweenness(weighted=True, threads=0), "ba3000")
 
 
+def test_dial_long_rows_first():
+    # nodes with more than 512 arcs are listed at the end of each bucket's
+    # list and stepped first; two hubs of 1,400 and 900 arcs on a BA graph,
+    # undirected and directed, bit-exact in the reference's order
+    el = gen.ba_weighted(n=3000, m=4, seed=8)
+    rng = np.random.default_rng(9)
+    hub_a = rng.choice(np.arange(2, el.n), 1400, replace=False)
+    hub_b = rng.choice(np.arange(2, el.n), 900, replace=False)
+    s = np.concatenate([el.src, np.zeros(1400, np.int64), np.ones(900, np.int64)])
+    t = np.concatenate([el.tgt, hub_a, hub_b])
+    w = np.concatenate([el.weights, rng.integers(1, 17, 2300).astype(np.float64)])
+    for directed in (False, True):
+        say("ba3000 + hubs, directed", directed)
+        g, o = pair(el.n, s, t, w, directed)
+        got, diag = g.betweenness_sources(weighted=True, exact=True)
+        assert diag["path_kind"] == 1
+        assert_same(got, o.betweenness(weighted=True, threads=0), f"hubs directed={directed}")
+
+
 def test_dial_free_order_weighted_landmarks():
     # 5,000 sources: batched by weighted distances to the landmarks (integer
     # lengths, undirected); only the association of the FP64 sums changes
