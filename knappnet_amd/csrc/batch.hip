@@ -2107,6 +2107,9 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
                 if (cnt + 1 > kTouchBuf / 2) flush();
                 if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
                 cnt += 1;
+                // a long row's entry goes to the queue at once (the sweep
+                // takes it first too)
+                if (te.w - te.z > kLongRow) flush();
             }
             continue;
         }
@@ -2127,6 +2130,7 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             if (cnt + 1 > kTouchBuf / 2) flush();
             if (lane == 0) mybuf[cnt] = QEnt{x, te.y, (unsigned long long)M};
             cnt += 1;
+            if (te.w - te.z > kLongRow) flush();
         }
         // offers over the out-row
         double acc_unused = 0.0;
