@@ -1896,7 +1896,9 @@ __device__ __forceinline__ uint32_t select_lanes(uint64_t M) {
 // `settle`), any other takes the offer dist[x] + length by atomic minimum
 // and marks the offer's bucket (one atomic per distinct neighbour and
 // bucket). Each lane of M then adds its predecessors' path counts in CSR
-// order, as the unpacked walk does. With `preloaded` the first 64 arcs
+// order, as the unpacked walk does, reading them from the window's values
+// staged in the wavefront's LDS row `stage` (64 * kPackRounds doubles; one
+// LDS read per value instead of two lane shuffles). With `preloaded` the first 64 arcs
 // (y0, ln0, vw0) are loaded by the caller. kWalk 0: the undirected row
 // (both); 1: an in-row (predecessors only); 2: an out-row (offers only).
 // Every lane calls.
@@ -1908,7 +1910,7 @@ __device__ __forceinline__ void dial_step_packed(
     const uint32_t* __restrict__ idx, const typename P::Len* __restrict__ len_arr,
     const uint64_t* __restrict__ vis, typename P::Dist* __restrict__ dist,
     const double* __restrict__ sig, uint64_t* __restrict__ pend, double& acc, uint32_t& npred,
-    unsigned long long& least) {
+    unsigned long long& least, double* __restrict__ stage) {
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     using Sum = typename P::Sum;
@@ -1984,15 +1986,26 @@ __device__ __forceinline__ void dial_step_packed(
                 }
             }
             // each lane of M adds its predecessors' path counts in CSR order
+            uint64_t any_tie = 0;
 #pragma unroll
-            for (int r = 0; r < kPackRounds; r++) {
-                for (uint32_t g = 0; g < A; g++) {
-                    if (w0 + (uint32_t)r * A + g >= na) break;
-                    const int src = (int)(g * mc + rank) & 63;
-                    const double v = __shfl(su[r], src, 64);
-                    if (mine && ((tm[r] >> src) & 1)) {
-                        acc += v;
-                        npred++;
+            for (int r = 0; r < kPackRounds; r++) any_tie |= tm[r];
+            if (any_tie) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int r = 0; r < kPackRounds; r++) stage[r * 64 + lane] = su[r];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                for (int r = 0; r < kPackRounds; r++) {
+                    for (uint32_t g = 0; g < A; g++) {
+                        if (w0 + (uint32_t)r * A + g >= na) break;
+                        const int src = (int)(g * mc + rank) & 63;
+                        if (mine && ((tm[r] >> src) & 1)) {
+                            acc += stage[r * 64 + src];
+                            npred++;
+                        }
                     }
                 }
             }
@@ -2033,6 +2046,9 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     __shared__ QEnt qbuf[kBlock / 64][kTouchBuf / 2];
+    // per wavefront: the packed path counts of a window (dial_step_packed)
+    __shared__ double stage_rows[kBlock / 64][64 * kPackRounds];
+    double* stage = stage_rows[threadIdx.x >> 6];
     unsigned long long least = ~0ull;  // least bucket offered by this lane
     const int lane = lane_id();
     QEnt* mybuf = qbuf[threadIdx.x >> 6];
@@ -2096,7 +2112,7 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             const uint64_t a0 = te.z, a1 = te.w;
             dial_step_packed<P, kCount>(a0, a1, true, y0, ln0, vw0, M, base, dx, W, words, delta,
                                         !first_step, idx, len_arr, vis, dist, sig, pend, acc,
-                                        npred, least);
+                                        npred, least, stage);
             if (!first_step) {
                 if (kCount && mine) {
                     sig[(base + x) * kLanes + lane] = acc;
@@ -2120,7 +2136,7 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
             if (kCount)
                 dial_step_packed<P, kCount, 1>(in_off[x], in_off[x + 1], false, 0, Len(0), 0, M,
                                                base, dx, W, words, delta, true, in_idx, in_len,
-                                               vis, dist, sig, pend, acc, npred, least);
+                                               vis, dist, sig, pend, acc, npred, least, stage);
             if (kCount && mine) {
                 sig[(base + x) * kLanes + lane] = acc;
                 if (npr) npr[(base + x) * kLanes + lane] = npred;
@@ -2137,7 +2153,7 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
         uint32_t npred_unused = 0;
         dial_step_packed<P, kCount, 2>(te.z, te.w, false, 0, Len(0), 0, M, base, dx, W, words,
                                        delta, false, idx, len_arr, vis, dist, sig, pend, acc_unused,
-                                       npred_unused, least);
+                                       npred_unused, least, stage);
     }
     if (kCount) retire_tickets(ctrl, xc);
     if (cnt) flush();
@@ -2161,7 +2177,9 @@ __global__ void dial_step_kernel(const uint4* __restrict__ list, const uint32_t*
 // distances are loaded, then the coefficients of the pairs on a shortest
 // path; each lane of M then adds its tying heads' coefficients in CSR
 // order, as the unpacked walk does (one load per head there, with 9 of 64
-// lanes in use on BA n = 200,000). Every lane calls.
+// lanes in use on BA n = 200,000), reading them from the window's
+// coefficients staged in the wavefront's LDS row `stage` (64 * kPackRounds
+// values). Every lane calls.
 template <class C, class P>
 __device__ __forceinline__ void dial_sweep_packed(uint64_t a0, uint64_t a1, uint64_t M,
                                                   size_t base, typename P::Dist dv0,
@@ -2169,7 +2187,8 @@ __device__ __forceinline__ void dial_sweep_packed(uint64_t a0, uint64_t a1, uint
                                                   const typename P::Len* __restrict__ len_arr,
                                                   const typename P::Dist* __restrict__ dist,
                                                   const uint64_t* __restrict__ vis,
-                                                  const C* __restrict__ coef, Dep<C>& dv) {
+                                                  const C* __restrict__ coef, Dep<C>& dv,
+                                                  C* __restrict__ stage) {
     using Dist = typename P::Dist;
     using Len = typename P::Len;
     const int lane = lane_id();
@@ -2217,20 +2236,21 @@ __device__ __forceinline__ void dial_sweep_packed(uint64_t a0, uint64_t a1, uint
                 tm[r] = __ballot(tie);
             }
             // each lane of M adds its tying heads in CSR order
+            // the packed coefficients staged in this wavefront's LDS row
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int r = 0; r < kPackRounds; r++) stage[r * 64 + lane] = cv[r];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (uint32_t j = 0; j < wa; j++) {
                 const uint32_t qp = j * mc + rank;
-                const int src = (int)(qp & 63);
-                C v = C(0);
-                uint64_t t = 0;
+                uint64_t t = tm[0];
 #pragma unroll
-                for (int r = 0; r < kPackRounds; r++) {
-                    const C vr = __shfl(cv[r], src, 64);
-                    if ((qp >> 6) == (uint32_t)r) {
-                        v = vr;
-                        t = tm[r];
-                    }
-                }
-                if (mine && ((t >> src) & 1)) dv.add(v);
+                for (int r = 1; r < kPackRounds; r++)
+                    if ((qp >> 6) == (uint32_t)r) t = tm[r];
+                if (mine && ((t >> (qp & 63)) & 1)) dv.add(stage[min(qp, 64u * kPackRounds - 1)]);
             }
         }
     }
@@ -2259,6 +2279,9 @@ __global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
     const int lane = lane_id();
     // entries by ticket (their rows differ widely in length)
     const uint32_t xc = blockIdx.x % kTicketClasses;
+    // per wavefront: the packed coefficients of a window (dial_sweep_packed)
+    __shared__ C stage_rows[kBlock / 64][64 * kPackRounds];
+    C* stage = stage_rows[threadIdx.x >> 6];
     for (size_t e = qbeg + next_ticket(ctrl, xc); e < qend; e = qbeg + next_ticket(ctrl, xc)) {
         QEnt en = q[e];
         size_t base = (size_t)en.b * n;
@@ -2271,7 +2294,8 @@ __global__ void __launch_bounds__(kBlock, NETBC_DIAL_BACK_WAVES)
         bool packed = false;
         if constexpr (!kEdge && !kQuirk) {
             if ((uint32_t)__popcll(M) <= kPackLanes) {
-                dial_sweep_packed<C, P>(a0, a1, M, base, dv0, idx, len_arr, dist, vis, coef, dv);
+                dial_sweep_packed<C, P>(a0, a1, M, base, dv0, idx, len_arr, dist, vis, coef, dv,
+                                        stage);
                 packed = true;
             }
         }
